@@ -1,0 +1,147 @@
+"""Benchmark: 4D-STEM patterns/s of the multislice forward on one MI355X.
+
+Workload (default): 256^2 complex64 windows cropped from a 1024^2 slab,
+64 slices of 1 Angstrom, 300 kV / 25 mrad / C1 = 50 A / Cs = 1 um probe,
+64 x 64 = 4096 scan positions, float32 patterns.  One step = one pass of the
+forward over all 4096 positions with inputs resident in HBM.
+
+Prints ONE JSON line.  ``roofline`` uses the algorithmic bytes per pattern
+B = 8 S n^2 + 8 n^2 + 4 n^2 (transmission window per slice, incident probe,
+float32 pattern) over the device time of the forward measured with HIP
+events on the launch stream.  ``cpu_baseline`` times the oracle's numpy /
+pocketfft complex128 restatement of the reference algorithm on a bounded
+sample of the same workload (rank 0, N = 1 only).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def algorithmic_bytes(n: int, s: int) -> int:
+    return 8 * s * n * n + 8 * n * n + 4 * n * n
+
+
+def algorithmic_flops(n: int, s: int) -> float:
+    return 2 * s * 5 * n * n * np.log2(n * n) + 6 * n * n * 2 * s + 3 * n * n
+
+
+def cpu_baseline(w, sample: int, cores: int):
+    """Oracle (numpy/pocketfft, complex128, `cores` workers) on `sample` positions."""
+    from oracle import ms_oracle as o
+    prop = o.propagation_func(w.n, w.n, w.slice_thickness, w.voltage_kv, w.calib)
+    sel = np.linspace(0, len(w.origins) - 1, sample).astype(int)
+    slab = w.slab_t  # complex64 input, arithmetic promoted to complex128 inside
+    o.window_stem_batched(slab, w.probe, w.origins[sel[:2]], w.shifts_px[sel[:2]], prop, workers=cores)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.window_stem_batched(slab, w.probe, w.origins[sel], w.shifts_px[sel], prop, workers=cores)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": round(sample / t, 3), "unit": "patterns/s", "cores": cores, "kind": "port",
+            "sample": f"{sample} of {len(w.origins)} positions (every {len(w.origins) // sample}th), complex128, "
+                      f"median of 3 after 1 warm-up; reference-algorithm CPU restatement (numpy/pocketfft)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-waves", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    import torch
+
+    from ptyrodactyl_amd import engine, physics, synth
+
+    t0 = time.perf_counter()
+    w = synth.config2()
+    print(f"# workload built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda", 0)
+    slab = torch.from_numpy(w.slab_t).to(dev)
+    pk = torch.fft.fft2(torch.from_numpy(w.probe[None]).to(dev)).to(torch.complex64).contiguous()
+    weights = torch.ones(1, dtype=torch.float32, device=dev)
+    origins = torch.from_numpy(w.origins).to(dev)
+    shifts = torch.from_numpy(w.shifts_px).to(dev)
+    prop = physics.propagator_factors(w.n, w.slice_thickness, w.voltage_kv, w.calib, device=dev)
+    p_count = len(w.origins)
+    out = torch.empty((p_count, w.n, w.n), dtype=torch.float32, device=dev)
+
+    def step():
+        engine.multislice_forward(slab, pk, weights, origins, shifts, prop, patterns_out=out,
+                                  max_batch_waves=args.batch_waves)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record()
+        step()
+        ends[i].record()
+        print(f"# step {i} issued", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    dev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms_per_step = wall * 1000.0 / args.steps
+    value = p_count * args.steps / wall
+    kernel_ms = float(np.mean(dev_ms))
+    bytes_step = algorithmic_bytes(w.n, w.num_slices) * p_count
+    achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
+    flops_step = algorithmic_flops(w.n, w.num_slices) * p_count
+    result = {
+        "metric": "4D-STEM patterns/s at 256^2 x 64 slices",
+        "value": round(value, 2),
+        "unit": "patterns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64",
+        "data": "synthetic (seeded perovskite-like crystal, generator in ptyrodactyl_amd/synth.py)",
+        "config": {"workload": w.name, "window": w.n, "slices": w.num_slices, "positions": p_count,
+                   "slab": list(w.slab_t.shape[1:]), "voltage_kv": w.voltage_kv, "aperture_mrad": w.aperture_mrad,
+                   "defocus_ang": w.defocus_ang, "cs_um": 1.0},
+        "latency_ms": {"p50": round(float(np.percentile(dev_ms, 50)), 3),
+                       "p99": round(float(np.percentile(dev_ms, 99)), 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "multislice forward (all launches of one step)", "kernel_ms": round(kernel_ms, 3),
+                     "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
+                     "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+    }
+    if not args.no_cpu_baseline:
+        cores = len(os.sched_getaffinity(0))
+        result["cpu_baseline"] = cpu_baseline(w, args.cpu_sample, cores)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,71 @@
+/*
+ * ptyrodactyl_amd -- C-ABI of the MI355X (gfx950) multislice 4D-STEM library.
+ *
+ * All array arguments are DEVICE pointers (complex64 = two float32, row
+ * major); `stream` is the caller's hipStream_t (NULL = legacy default
+ * stream).  Calls are stream-ordered and never synchronise the device.
+ * Every entry point returns PTY_OK or an error code and leaves a message in
+ * pty_last_error() (thread local).  The library never frees caller memory.
+ *
+ * The reference (ptyrodactyl, pure JAX) has no FFI of its own; the entry
+ * points below replace the XLA executables of the Python functions cited
+ * beside each one (paths relative to the reference repository root).
+ */
+#ifndef PTYRODACTYL_AMD_H
+#define PTYRODACTYL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTY_ABI_VERSION 1
+
+#define PTY_OK 0
+#define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
+#define PTY_ERR_DEVICE 2  /* HIP runtime / launch failure: maps to RuntimeError */
+
+int pty_abi_version(void);
+const char* pty_last_error(void);
+
+/*
+ * Workspace (device bytes) that pty_forward needs to process `num_positions`
+ * windows of n x n (n in {32, 64, 128, 256, 512}) with `num_modes` probe modes in one batch, capped at
+ * `max_batch_waves` waves per batch (0 = no cap).  Windows n <= 128 run
+ * LDS-resident and need none (returns 0).
+ */
+size_t pty_forward_workspace_bytes(int n, long long num_positions, int num_modes, long long max_batch_waves);
+
+/*
+ * Multislice forward over a batch of scan positions.
+ *
+ * Replaces the per-position scan of `_cbed_amplitude_from_slice_provider`
+ * (src/ptyrodactyl/multislice/simulations.py:501-603) as mapped over
+ * positions by `stem_4d` (simulations.py:1330-1443, via `cbed_image`
+ * :1130-1233 and `shift_beam_fourier` :1236-1327) and reduced by
+ * `apply_distribution` (src/ptyrodactyl/multislice/reduce.py:29-89).
+ *
+ *   slab        complex64 [num_slices][slab_h][slab_w] transmission exp(i sigma V)
+ *   probe_k     complex64 [num_modes][n][n] fft2 of each real-space probe mode
+ *   mode_weights float32 [num_modes] incoherent weights (already normalised)
+ *   origins     int32 [num_positions][2] window origin (row, col) in the slab;
+ *               the window wraps periodically
+ *   shifts_px   float64 [num_positions][2] probe shift (row, col) in pixels,
+ *               applied as a Fourier phase ramp exactly as shift_beam_fourier
+ *   prop_y/x    complex64 [n] separable propagator exp(-i pi lambda dz q^2)/n
+ *   patterns    float32 [num_positions][n][n] fftshifted sum_m w_m |A_m|^2 (or NULL)
+ *   amplitude   complex64 [num_positions][num_modes][n][n] fftshifted A_m (or NULL)
+ *   workspace   device scratch of workspace_bytes (see pty_forward_workspace_bytes)
+ */
+int pty_forward(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
+                long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+                const double* shifts_px, const void* prop_y, const void* prop_x, float* patterns, void* amplitude,
+                void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTYRODACTYL_AMD_H */

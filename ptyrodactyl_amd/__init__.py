@@ -1,0 +1,25 @@
+"""MI355X-native multislice 4D-STEM forward model and gradient.
+
+Drop-in for the multislice path of ptyrodactyl: the public names below keep
+the reference's signatures (``src/ptyrodactyl/multislice/simulations.py``,
+``src/ptyrodactyl/types/electron_types.py``); the per-position multislice
+runs in hand-written gfx950 HIP kernels behind the C-ABI declared in
+``include/ptyrodactyl_amd.h``.
+"""
+
+from .physics import interaction_sigma, phase_interaction_parameter, relativistic_wavelength_ang
+from .simulations import (aberration, cbed_amplitude, cbed_image, make_probe, propagation_func,
+                          shift_beam_fourier, stem_4d, stem_4d_windows, transmission_func)
+from .types import (STEM4D, CalibratedArray, DetectorConfig, EnsembleAxes, MicroscopeConfig, PotentialSlices,
+                    ProbeModes, RuntimeCheckError, create_calibrated_array, create_detector_config,
+                    create_ensemble_axes, create_microscope_config, create_potential_slices, create_probe_modes,
+                    create_stem4d)
+
+__all__ = [
+    "aberration", "cbed_amplitude", "cbed_image", "make_probe", "propagation_func", "shift_beam_fourier",
+    "stem_4d", "stem_4d_windows", "transmission_func", "interaction_sigma", "phase_interaction_parameter",
+    "relativistic_wavelength_ang", "STEM4D", "CalibratedArray", "DetectorConfig", "EnsembleAxes",
+    "MicroscopeConfig", "PotentialSlices", "ProbeModes", "RuntimeCheckError", "create_calibrated_array",
+    "create_detector_config", "create_ensemble_axes", "create_microscope_config", "create_potential_slices",
+    "create_probe_modes", "create_stem4d",
+]

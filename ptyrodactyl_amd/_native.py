@@ -1,0 +1,87 @@
+"""ctypes binding of the C-ABI in ``include/ptyrodactyl_amd.h``.
+
+The HIP library is the only compute path for the multislice kernels: if it
+is missing this module raises on first use (there is no CPU fallback).
+torch is imported first so that the library binds to the HIP runtime torch
+already loaded (both carry the soname ``libamdhip64.so.7``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede loading the HIP library)
+
+from .build import LIB_PATH
+
+PTY_OK = 0
+PTY_ERR_INVALID = 1
+PTY_ERR_DEVICE = 2
+ABI_VERSION = 1
+
+_lock = threading.Lock()
+_lib = None
+
+c_int = ctypes.c_int
+c_ll = ctypes.c_longlong
+c_size = ctypes.c_size_t
+c_vp = ctypes.c_void_p
+c_dbl = ctypes.c_double
+
+# name -> (restype, argtypes); mirrors include/ptyrodactyl_amd.h
+SIGNATURES = {
+    "pty_abi_version": (c_int, []),
+    "pty_last_error": (ctypes.c_char_p, []),
+    "pty_forward_workspace_bytes": (c_size, [c_int, c_ll, c_int, c_ll]),
+    "pty_forward": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
+                            c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+}
+
+
+class NativeLibraryMissing(RuntimeError):
+    """The HIP library has not been built (run ``__graft_entry__.build()``)."""
+
+
+def load(path: Path | None = None):
+    """Load (once) and return the ctypes handle of the HIP library."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = Path(path) if path is not None else LIB_PATH
+        if not p.exists():
+            raise NativeLibraryMissing(
+                f"{p} not found: build it with `python -m ptyrodactyl_amd.build`")
+        lib = ctypes.CDLL(str(p), mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            if not hasattr(lib, name):
+                continue
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if lib.pty_abi_version() != ABI_VERSION:
+            raise NativeLibraryMissing("HIP library ABI version mismatch; rebuild it")
+        _lib = lib
+        return lib
+
+
+def check(status: int, what: str) -> None:
+    """Map a C-ABI status to the reference's error conventions."""
+    if status == PTY_OK:
+        return
+    msg = load().pty_last_error().decode(errors="replace")
+    if status == PTY_ERR_INVALID:
+        raise ValueError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")
+
+
+def ptr(t) -> int:
+    """Device address of a tensor (0 for None)."""
+    return 0 if t is None else int(t.data_ptr())
+
+
+def stream_handle(device=None) -> int:
+    """hipStream_t of torch's current stream on ``device``."""
+    return int(torch.cuda.current_stream(device).cuda_stream)

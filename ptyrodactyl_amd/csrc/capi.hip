@@ -1,0 +1,103 @@
+// C-ABI boundary of the library: plain pointers, sizes and the caller's
+// stream; no framework types cross this line.  Every entry point returns a
+// status (PTY_OK, PTY_ERR_INVALID, PTY_ERR_DEVICE) and records a message
+// readable with pty_last_error() on the calling thread.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/ptyrodactyl_amd.h"
+#include "forward.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int device_status(hipError_t e, const char* what) {
+  if (e == hipSuccess) return PTY_OK;
+  return fail(PTY_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+bool valid_n(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+
+int check_common(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                 long long slab_h, long long slab_w, const void* probe_k, const float* weights,
+                 const int32_t* origins, const double* shifts, const void* prop_y, const void* prop_x) {
+  if (!valid_n(n)) return fail(PTY_ERR_INVALID, "window size n=%d is not one of 32, 64, 128, 256, 512", n);
+  if (num_slices < 1) return fail(PTY_ERR_INVALID, "num_slices must be >= 1 (got %d)", num_slices);
+  if (num_positions < 1 || num_positions > (1LL << 30))
+    return fail(PTY_ERR_INVALID, "num_positions must be in [1, 2^30] (got %lld)", num_positions);
+  if (num_modes < 1 || num_modes > 4096) return fail(PTY_ERR_INVALID, "num_modes must be in [1, 4096]");
+  if (slab_h < n || slab_w < n || slab_h > (1LL << 30) || slab_w > (1LL << 30))
+    return fail(PTY_ERR_INVALID, "slab extent %lldx%lld must be at least the window %dx%d", slab_h, slab_w, n, n);
+  if (!slab || !probe_k || !weights || !origins || !shifts || !prop_y || !prop_x)
+    return fail(PTY_ERR_INVALID, "null input pointer");
+  return PTY_OK;
+}
+
+pty::FwdParams make_params(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                           long long slab_h, long long slab_w, const void* probe_k, const float* weights,
+                           const int32_t* origins, const double* shifts, const void* prop_y,
+                           const void* prop_x) {
+  pty::FwdParams a;
+  a.n = n;
+  a.S = num_slices;
+  a.M = num_modes;
+  a.P = num_positions;
+  a.slab = (const float2*)slab;
+  a.slab_h = slab_h;
+  a.slab_w = slab_w;
+  a.slice_stride = slab_h * slab_w;
+  a.probe_k = (const float2*)probe_k;
+  a.weights = weights;
+  a.origins = origins;
+  a.shifts = shifts;
+  a.prop_y = (const float2*)prop_y;
+  a.prop_x = (const float2*)prop_x;
+  a.patterns = nullptr;
+  a.amplitude = nullptr;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pty_abi_version(void) { return PTY_ABI_VERSION; }
+
+const char* pty_last_error(void) { return g_err; }
+
+size_t pty_forward_workspace_bytes(int n, long long num_positions, int num_modes, long long max_batch_waves) {
+  if (!valid_n(n) || num_positions < 1 || num_modes < 1) return 0;
+  return pty::forward_workspace_bytes(n, num_positions, num_modes, max_batch_waves);
+}
+
+int pty_forward(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
+                long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+                const double* shifts_px, const void* prop_y, const void* prop_x, float* patterns, void* amplitude,
+                void* workspace, size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                        origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (!patterns && !amplitude) return fail(PTY_ERR_INVALID, "nothing to compute: patterns and amplitude are null");
+  const size_t need = pty::forward_workspace_bytes(n, 1, num_modes, 0);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
+                need);
+  pty::FwdParams a = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k,
+                                 mode_weights, origins, shifts_px, prop_y, prop_x);
+  a.patterns = patterns;
+  a.amplitude = (float2*)amplitude;
+  return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_forward");
+}
+
+}  // extern "C"

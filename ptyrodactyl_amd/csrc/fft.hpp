@@ -1,0 +1,251 @@
+// Register/LDS FFT building blocks for gfx950 (64-wide wavefronts).
+//
+// Data distribution used throughout: a group of T threads transforms one
+// line of L = T*E complex64 points; thread t holds v[q] = x[t + T*q] for
+// q in [0, E).  The forward and inverse transforms leave the output in the
+// same distribution (v[q] = X[t + T*q]), so a pointwise multiply between a
+// forward and an inverse transform needs no data exchange.  Internal passes
+// are Stockham radix-R passes that exchange through an LDS scratch owned by
+// the group; every stage is written for this build (no vendor FFT library).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define PTY_HD __host__ __device__ __forceinline__
+
+namespace pty {
+
+PTY_HD float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+PTY_HD float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+PTY_HD float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+PTY_HD float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+PTY_HD float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+PTY_HD float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+PTY_HD float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+// a * (-i) and a * (+i)
+PTY_HD float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+PTY_HD float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
+
+// Multiply by W16^E = exp(-2 pi i E / 16) (forward) or its conjugate.
+template <int E, bool INV>
+PTY_HD float2 tw16(float2 z) {
+  constexpr int e = ((E % 16) + 16) % 16;
+  constexpr float C1 = 0.92387953251128675613f;  // cos(pi/8)
+  constexpr float S1 = 0.38268343236508977173f;  // sin(pi/8)
+  constexpr float R2 = 0.70710678118654752440f;  // 1/sqrt(2)
+  // (cr, ci) = exp(-2 pi i e/16)
+  constexpr float tab[16][2] = {
+      {1.f, 0.f},  {C1, -S1},  {R2, -R2},  {S1, -C1},  {0.f, -1.f}, {-S1, -C1}, {-R2, -R2}, {-C1, -S1},
+      {-1.f, 0.f}, {-C1, S1},  {-R2, R2},  {-S1, C1},  {0.f, 1.f},  {S1, C1},   {R2, R2},   {C1, S1}};
+  if constexpr (e == 0) {
+    return z;
+  } else if constexpr (e == 4) {
+    return INV ? mul_pi(z) : mul_mi(z);
+  } else if constexpr (e == 8) {
+    return make_float2(-z.x, -z.y);
+  } else if constexpr (e == 12) {
+    return INV ? mul_mi(z) : mul_pi(z);
+  } else {
+    constexpr float cr = tab[e][0];
+    constexpr float ci = tab[e][1];
+    return INV ? cmul(z, make_float2(cr, -ci)) : cmul(z, make_float2(cr, ci));
+  }
+}
+
+template <bool INV>
+PTY_HD void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+PTY_HD void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
+  t3 = INV ? mul_pi(t3) : mul_mi(t3);
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// Natural-order in-register DFT of size R (R in {2, 4, 8, 16}).
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+  PTY_HD static void run(float2* a) { dft2<INV>(a[0], a[1]); }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+  PTY_HD static void run(float2* a) { dft4<INV>(a[0], a[1], a[2], a[3]); }
+};
+
+// 8 = 2 x 4 (decimation in time): X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k].
+template <bool INV>
+struct Dft<8, INV> {
+  PTY_HD static void run(float2* a) {
+    float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    o1 = tw16<2, INV>(o1);
+    o2 = tw16<4, INV>(o2);
+    o3 = tw16<6, INV>(o3);
+    a[0] = cadd(e0, o0);
+    a[4] = csub(e0, o0);
+    a[1] = cadd(e1, o1);
+    a[5] = csub(e1, o1);
+    a[2] = cadd(e2, o2);
+    a[6] = csub(e2, o2);
+    a[3] = cadd(e3, o3);
+    a[7] = csub(e3, o3);
+  }
+};
+
+// 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.
+template <bool INV>
+struct Dft<16, INV> {
+  PTY_HD static void run(float2* a) {
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) dft4<INV>(a[n2], a[4 + n2], a[8 + n2], a[12 + n2]);
+    // a[4*k1 + n2] now holds the inner DFT; apply W16^(n2*k1)
+    a[5] = tw16<1, INV>(a[5]);
+    a[6] = tw16<2, INV>(a[6]);
+    a[7] = tw16<3, INV>(a[7]);
+    a[9] = tw16<2, INV>(a[9]);
+    a[10] = tw16<4, INV>(a[10]);
+    a[11] = tw16<6, INV>(a[11]);
+    a[13] = tw16<3, INV>(a[13]);
+    a[14] = tw16<6, INV>(a[14]);
+    a[15] = tw16<9, INV>(a[15]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4<INV>(a[4 * k1], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
+    // a[4*k1 + k2] = X[k1 + 4*k2]: transpose the 4x4 index grid into natural order
+    float2 t[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t[i] = a[i];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = t[4 * k1 + k2];
+  }
+};
+
+// Radix schedule for a line of length L with E = 16 points per thread.
+template <int L>
+struct Plan;
+template <>
+struct Plan<32> {
+  static constexpr int NP = 2;
+  static constexpr int R[3] = {16, 2, 1};
+};
+template <>
+struct Plan<64> {
+  static constexpr int NP = 2;
+  static constexpr int R[3] = {16, 4, 1};
+};
+template <>
+struct Plan<128> {
+  static constexpr int NP = 2;
+  static constexpr int R[3] = {16, 8, 1};
+};
+template <>
+struct Plan<256> {
+  static constexpr int NP = 2;
+  static constexpr int R[3] = {16, 16, 1};
+};
+template <>
+struct Plan<512> {
+  static constexpr int NP = 3;
+  static constexpr int R[3] = {16, 16, 2};
+};
+
+// One Stockham pass.  NS = product of the radices of earlier passes.
+// Buf provides ld(e), st(e, v) and sync() for the group's scratch; TW(m)
+// returns W_L^m = exp(-2 pi i m / L).
+template <int L, int E, int R, int NS, bool INV, bool FIRST, bool LAST, class Buf, class TW>
+__device__ __forceinline__ void stockham_pass(float2 (&v)[E], int t, Buf& buf, const TW& tw) {
+  constexpr int T = L / E;
+  constexpr int NB = E / R;
+  float2 x[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + T * b;
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[b][r] = FIRST ? v[b + NB * r] : buf.ld(j + (L / R) * r);
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + T * b;
+    if constexpr (!FIRST) {
+      const int jm = j % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw(jm * r * (L / (NS * R)));
+        x[b][r] = INV ? cmulc(x[b][r], w) : cmul(x[b][r], w);
+      }
+    }
+    Dft<R, INV>::run(x[b]);
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + NB * r] = x[b][r];
+  } else {
+    buf.sync();  // every read of the previous contents is done
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + T * b;
+      const int idx = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf.st(idx + NS * r, x[b][r]);
+    }
+    buf.sync();  // writes visible to the group
+  }
+}
+
+// Full line FFT (forward when INV == false, unnormalised inverse otherwise).
+template <int L, bool INV, class Buf, class TW>
+__device__ __forceinline__ void line_fft(float2 (&v)[16], int t, Buf& buf, const TW& tw) {
+  using P = Plan<L>;
+  if constexpr (P::NP == 2) {
+    stockham_pass<L, 16, P::R[0], 1, INV, true, false>(v, t, buf, tw);
+    stockham_pass<L, 16, P::R[1], P::R[0], INV, false, true>(v, t, buf, tw);
+  } else {
+    stockham_pass<L, 16, P::R[0], 1, INV, true, false>(v, t, buf, tw);
+    stockham_pass<L, 16, P::R[1], P::R[0], INV, false, false>(v, t, buf, tw);
+    stockham_pass<L, 16, P::R[2], P::R[0] * P::R[1], INV, false, true>(v, t, buf, tw);
+  }
+}
+
+// Twiddle lookup from an LDS table of W_L^m.
+struct TwTable {
+  const float2* tab;
+  __device__ __forceinline__ float2 operator()(int m) const { return tab[m]; }
+};
+
+// Fill an LDS table with W_L^m = exp(-2 pi i m / L), m in [0, L).
+__device__ __forceinline__ void fill_twiddles(float2* tab, int L, int tid, int nthreads) {
+  for (int m = tid; m < L; m += nthreads) {
+    float s, c;
+    sincospif(-2.0f * (float)m / (float)L, &s, &c);
+    tab[m] = make_float2(c, s);
+  }
+}
+
+// Wavefront-local synchronisation for groups that never span two waves.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace pty

@@ -1,0 +1,390 @@
+// Multislice 4D-STEM forward kernels for gfx950.
+//
+// One "wave" below is one probe mode at one scan position.  For every wave
+// the kernels compute, in complex64 with FP32 accumulation,
+//   psi_0     = ifft2(fft2(probe) * ramp(shift))
+//   phi_s     = psi_s * T_s(window at origin)         s = 0..S-1
+//   psi_{s+1} = ifft2(fft2(phi_s) * P)                s < S-1
+//   I        += w_m |fftshift(fft2(phi_{S-1}))|^2
+// which is the scan body of the reference's _cbed_amplitude_from_slice_provider
+// (src/ptyrodactyl/multislice/simulations.py:543-602) applied to the shifted
+// probe of shift_beam_fourier (simulations.py:1287-1321) and reduced as in
+// apply_distribution (multislice/reduce.py:81-86).
+//
+// The Fresnel propagator exp(-i pi lambda dz (qx^2 + qy^2)) factors into
+// Py(qy) Px(qx), so fft2 -> *P -> ifft2 is applied as a row pass
+// (fft_x, *Px, ifft_x) followed by a column pass (fft_y, *Py, ifft_y); the
+// transmission multiply is fused into the row pass and |.|^2 into the last
+// column pass.
+//
+// Two execution strategies:
+//  * n <= 128 (32, 64, 128): one workgroup per scan position keeps the whole wave resident
+//    in LDS across all slices (128^2 c64 = 128 KiB of the 160 KiB LDS) and
+//    streams only the transmission window from HBM.
+//  * n >= 256: the wave exceeds LDS, so each slice is one row-pass launch
+//    and one column-pass launch over a batch of waves held in a device
+//    workspace sized to stay in the L2/Infinity-Cache working set.
+#include "common.hpp"
+#include "forward.hpp"
+
+namespace pty {
+
+// ---------------------------------------------------------------------------
+// LDS-resident kernel (n = 64, 128)
+// ---------------------------------------------------------------------------
+
+// Column x of the resident wave used as FFT exchange scratch; element e sits
+// in row e ^ ((e >> 4) & 3) so the strided exchange spreads over banks.
+struct ColBufResident {
+  float2* col;
+  int rs;
+  __device__ __forceinline__ float2 ld(int e) const { return col[(e ^ ((e >> 4) & 3)) * rs]; }
+  __device__ __forceinline__ void st(int e, float2 v) const { col[(e ^ ((e >> 4) & 3)) * rs] = v; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
+};
+
+template <int N>
+__global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
+  constexpr int T = N / 16;       // threads per line
+  constexpr int NT = N * N / 16;  // threads per workgroup
+  constexpr int RS = N + N / 16;  // padded row stride (complex elements)
+  constexpr int G = 64 / T;       // lines per wavefront
+  extern __shared__ float2 lds[];
+  float2* wave = lds;
+  float2* tw = wave + N * RS;
+  float2* py = tw + N;
+  float2* px = py + N;
+  float2* ramp_y = px + N;
+  float2* ramp_x = ramp_y + N;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  const int p = blockIdx.x;
+  const WaveGeom g = load_geom(a.origins, a.shifts, p, N, a.slab_h, a.slab_w);
+  fill_twiddles(tw, N, tid, NT);
+  for (int i = tid; i < N; i += NT) {
+    py[i] = a.prop_y[i];
+    px[i] = a.prop_x[i];
+    ramp_y[i] = shift_ramp(i, N, g.my, g.dy);
+    ramp_x[i] = shift_ramp(i, N, g.mx, g.dx);
+  }
+  const TwTable twt{tw};
+  __syncthreads();
+
+  for (int m = 0; m < a.M; ++m) {
+    for (int s = -1; s < a.S; ++s) {
+      // Thread coordinates are re-derived every iteration through an opaque
+      // copy so the compiler keeps per-thread LDS addresses local to each
+      // phase instead of hoisting dozens of them across the loop (spills).
+      int tid_ = tid;
+      asm volatile("" : "+v"(tid_));
+      const int lane_ = tid_ & 63, wid_ = tid_ >> 6;
+      if (s >= 0) {  // row phase of slice s
+        const int rt = lane_ % T, ry = wid_ * G + lane_ / T;
+        const RowBufWave rb{wave + ry * RS};
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = rb.row[pad16(rt + T * q)];
+        if (s == 0) line_fft<N, true>(v, rt, rb, twt);  // finish the shifted probe: ifft along x
+        int gy = g.oy + ry;
+        if (gy >= a.slab_h) gy -= (int)a.slab_h;
+        const float2* trow = a.slab + (long long)s * a.slice_stride + (long long)gy * a.slab_w;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          int x = g.ox + rt + T * q;
+          if (x >= a.slab_w) x -= (int)a.slab_w;
+          v[q] = cmul(v[q], trow[x]);
+        }
+        line_fft<N, false>(v, rt, rb, twt);
+        if (s < a.S - 1) {
+#pragma unroll
+          for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], px[rt + T * q]);
+          line_fft<N, true>(v, rt, rb, twt);
+        }
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rb.row[pad16(rt + T * q)] = v[q];
+        __syncthreads();
+      }
+      {  // column phase
+        const int ct = lane_ / G, cx = wid_ * G + lane_ % G;
+        const ColBufResident cb{wave + pad16(cx), RS};
+        float2* col = wave + pad16(cx);
+        float2 v[16];
+        if (s < 0) {  // probe_k column * shift ramp, inverse FFT along y
+          const float2 rx = ramp_x[cx];
+          const float2* pk = a.probe_k + (size_t)m * N * N + cx;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int ky = ct + T * q;
+            v[q] = cmul(cmul(pk[(size_t)ky * N], ramp_y[ky]), rx);
+          }
+          line_fft<N, true>(v, ct, cb, twt);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 16; ++q) v[q] = col[(ct + T * q) * RS];
+          line_fft<N, false>(v, ct, cb, twt);
+          if (s < a.S - 1) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], py[ct + T * q]);
+            line_fft<N, true>(v, ct, cb, twt);
+          }
+        }
+        if (s < a.S - 1) {
+          wave_sync();
+#pragma unroll
+          for (int q = 0; q < 16; ++q) col[(ct + T * q) * RS] = v[q];
+        } else {
+          const int xs = (cx + N / 2) & (N - 1);
+          const float wm = a.weights[m];
+          if (a.patterns) {
+            float* out = a.patterns + (size_t)p * N * N + xs;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+              float* o = out + (size_t)((ct + T * q + N / 2) & (N - 1)) * N;
+              *o = (m == 0 ? 0.f : *o) + wm * cabs2(v[q]);  // same thread owns the pixel for every mode
+            }
+          }
+          if (a.amplitude) {
+            float2* amp = a.amplitude + ((size_t)p * a.M + m) * N * N + xs;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) amp[(size_t)((ct + T * q + N / 2) & (N - 1)) * N] = v[q];
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+template <int N>
+static size_t resident_lds_bytes() {
+  return sizeof(float2) * ((size_t)N * (N + N / 16) + 5 * N);
+}
+
+// ---------------------------------------------------------------------------
+// Streaming row / column passes (n = 256, 512)
+// ---------------------------------------------------------------------------
+
+enum : int { ROW_FIRST = 1, ROW_LAST = 2 };
+
+template <int N, int FLAGS>
+__global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0, int nb, int s) {
+  constexpr int T = N / 16;
+  constexpr int RPB = 256 / T;  // rows per workgroup
+  constexpr int RS = N + N / 16;
+  __shared__ float2 rows[RPB * RS];
+  __shared__ float2 tw[N];
+  __shared__ float2 px[N];
+  const int tid = threadIdx.x;
+  fill_twiddles(tw, N, tid, 256);
+  if (!(FLAGS & ROW_LAST))
+    for (int i = tid; i < N; i += 256) px[i] = a.prop_x[i];
+  const int wl = blockIdx.x % nb;  // consecutive blocks -> different waves (XCD round robin)
+  const int rblk = blockIdx.x / nb;
+  const int w = w0 + wl;
+  const int p = w / a.M;
+  const int t = tid % T, lr = tid / T;
+  const int y = rblk * RPB + lr;
+  const WaveGeom g = load_geom(a.origins, a.shifts, p, N, a.slab_h, a.slab_w);
+  const TwTable twt{tw};
+  const RowBufWave rb{rows + lr * RS};
+  __syncthreads();
+
+  float2* wrow = waves + ((size_t)wl * N + y) * N;
+  float2 v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = wrow[t + T * q];
+  if (FLAGS & ROW_FIRST) line_fft<N, true>(v, t, rb, twt);
+  int gy = g.oy + y;
+  if (gy >= a.slab_h) gy -= (int)a.slab_h;
+  const float2* trow = a.slab + (long long)s * a.slice_stride + (long long)gy * a.slab_w;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    int x = g.ox + t + T * q;
+    if (x >= a.slab_w) x -= (int)a.slab_w;
+    v[q] = cmul(v[q], trow[x]);
+  }
+  line_fft<N, false>(v, t, rb, twt);
+  if (!(FLAGS & ROW_LAST)) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], px[t + T * q]);
+    line_fft<N, true>(v, t, rb, twt);
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) wrow[t + T * q] = v[q];
+}
+
+// Column-tile scratch: element e of column g at pad16(e) * NC + g.
+template <int NC>
+struct ColBufBlock {
+  float2* base;
+  __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
+  __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+};
+
+enum : int { COL_INIT = 0, COL_MID = 1, COL_FINAL = 2 };
+
+template <int N, int MODE>
+__global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0, int nb) {
+  constexpr int T = N / 16;
+  constexpr int NC = 256 / T;  // columns per workgroup
+  constexpr int PS = N + N / 16;
+  __shared__ float2 buf[PS * NC];
+  __shared__ float2 tw[N];
+  __shared__ float2 py[N];
+  const int tid = threadIdx.x;
+  fill_twiddles(tw, N, tid, 256);
+  if (MODE == COL_MID)
+    for (int i = tid; i < N; i += 256) py[i] = a.prop_y[i];
+  const int gc = tid % NC, t = tid / NC;
+  const int unit = blockIdx.x % nb;  // wave (INIT/MID) or position (FINAL) inside the batch
+  const int cblk = blockIdx.x / nb;
+  const int x = cblk * NC + gc;
+  const TwTable twt{tw};
+  const ColBufBlock<NC> cb{buf + gc};
+  __syncthreads();
+
+  if constexpr (MODE == COL_INIT) {
+    const int w = w0 + unit;
+    const int p = w / a.M, m = w % a.M;
+    const WaveGeom g = load_geom(a.origins, a.shifts, p, N, a.slab_h, a.slab_w);
+    const float2 rx = shift_ramp(x, N, g.mx, g.dx);
+    const float2* pk = a.probe_k + (size_t)m * N * N + x;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int ky = t + T * q;
+      v[q] = cmul(cmul(pk[(size_t)ky * N], shift_ramp(ky, N, g.my, g.dy)), rx);
+    }
+    line_fft<N, true>(v, t, cb, twt);
+    float2* wcol = waves + (size_t)unit * N * N + x;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
+  } else if constexpr (MODE == COL_MID) {
+    float2* wcol = waves + (size_t)unit * N * N + x;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
+    line_fft<N, false>(v, t, cb, twt);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], py[t + T * q]);
+    line_fft<N, true>(v, t, cb, twt);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
+  } else {
+    const int pl = unit;  // position inside the batch
+    const int p = w0 / a.M + pl;
+    float acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+    const int xs = (x + N / 2) & (N - 1);
+    for (int m = 0; m < a.M; ++m) {
+      const float2* wcol = waves + ((size_t)pl * a.M + m) * N * N + x;
+      float2 v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
+      line_fft<N, false>(v, t, cb, twt);
+      const float wm = a.weights[m];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] += wm * cabs2(v[q]);
+      if (a.amplitude) {
+        float2* amp = a.amplitude + ((size_t)p * a.M + m) * N * N;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) amp[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = v[q];
+      }
+    }
+    if (a.patterns) {
+      float* out = a.patterns + (size_t)p * N * N;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) out[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = acc[q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+template <int N>
+static hipError_t launch_resident(const FwdParams& a, hipStream_t st) {
+  const size_t lds = resident_lds_bytes<N>();
+  static bool attr_set[64] = {false};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (dev >= 0 && dev < 64 && !attr_set[dev]) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_forward_resident<N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr_set[dev] = true;
+  }
+  hipLaunchKernelGGL(k_forward_resident<N>, dim3(a.P), dim3(N * N / 16), lds, st, a);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_streaming(const FwdParams& a, float2* ws, size_t ws_bytes, hipStream_t st) {
+  constexpr int T = N / 16;
+  constexpr int RPB = 256 / T, NC = 256 / T;
+  const size_t wave_bytes = (size_t)N * N * sizeof(float2);
+  const long long max_waves = (long long)(ws_bytes / wave_bytes);
+  long long pb = max_waves / a.M;
+  if (pb < 1) return hipErrorInvalidValue;
+  if (pb > a.P) pb = a.P;
+  if (pb >= 8) pb -= pb % 8;  // keep a wave on one XCD in both passes
+  for (long long p0 = 0; p0 < a.P; p0 += pb) {
+    const int np = (int)((a.P - p0) < pb ? (a.P - p0) : pb);
+    const int nb = np * a.M;
+    const int w0 = (int)(p0 * a.M);
+    hipLaunchKernelGGL((k_cols<N, COL_INIT>), dim3(nb * (N / NC)), dim3(256), 0, st, a, ws, w0, nb);
+    for (int s = 0; s < a.S; ++s) {
+      const bool first = s == 0, last = s == a.S - 1;
+      const dim3 grid(nb * (N / RPB));
+      if (first && last)
+        hipLaunchKernelGGL((k_rows<N, ROW_FIRST | ROW_LAST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
+      else if (first)
+        hipLaunchKernelGGL((k_rows<N, ROW_FIRST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
+      else if (last)
+        hipLaunchKernelGGL((k_rows<N, ROW_LAST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
+      else
+        hipLaunchKernelGGL((k_rows<N, 0>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
+      if (!last)
+        hipLaunchKernelGGL((k_cols<N, COL_MID>), dim3(nb * (N / NC)), dim3(256), 0, st, a, ws, w0, nb);
+    }
+    hipLaunchKernelGGL((k_cols<N, COL_FINAL>), dim3(np * (N / NC)), dim3(256), 0, st, a, ws, w0, np);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+size_t forward_workspace_bytes(int n, long long num_positions, int num_modes, long long max_batch_waves) {
+  if (n <= 128) return 0;
+  long long waves = num_positions * (long long)num_modes;
+  if (max_batch_waves > 0 && waves > max_batch_waves) {
+    waves = (max_batch_waves / num_modes) * num_modes;
+    if (waves < num_modes) waves = num_modes;
+  }
+  return (size_t)waves * n * n * sizeof(float2);
+}
+
+hipError_t launch_forward(const FwdParams& a, void* ws, size_t ws_bytes, hipStream_t st) {
+  switch (a.n) {
+    case 32:
+      return launch_resident<32>(a, st);
+    case 64:
+      return launch_resident<64>(a, st);
+    case 128:
+      return launch_resident<128>(a, st);
+    case 256:
+      return launch_streaming<256>(a, (float2*)ws, ws_bytes, st);
+    case 512:
+      return launch_streaming<512>(a, (float2*)ws, ws_bytes, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace pty

@@ -1,0 +1,102 @@
+"""Device-side multislice engine: validates operands and calls the C-ABI.
+
+This is the single entry to the HIP forward kernels.  Operands are torch
+tensors that already live on the GPU; the engine checks dtype, layout,
+shape and device on the host before any launch, allocates outputs and the
+streaming workspace with torch's caching allocator, and launches on torch's
+current stream.
+"""
+
+from __future__ import annotations
+
+import os
+from typing import Optional, Tuple
+
+import torch
+
+from . import _native
+
+SUPPORTED_N = (32, 64, 128, 256, 512)
+# Waves (positions x modes) per streaming batch for n >= 256; the batch's
+# waves live in a device workspace that should stay cache-resident.
+DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "256"))
+
+
+def _require(cond: bool, msg: str) -> None:
+    if not cond:
+        raise ValueError(msg)
+
+
+def _check_cuda(t: torch.Tensor, name: str, dtype, ndim: int, device) -> None:
+    _require(isinstance(t, torch.Tensor), f"{name} must be a torch tensor")
+    _require(t.is_cuda, f"{name} must be a GPU tensor (the multislice path has no CPU fallback)")
+    _require(t.dtype == dtype, f"{name} must be {dtype}, got {t.dtype}")
+    _require(t.ndim == ndim, f"{name} must be {ndim}-D, got shape {tuple(t.shape)}")
+    _require(t.is_contiguous(), f"{name} must be contiguous")
+    _require(t.device == device, f"{name} is on {t.device}, expected {device}")
+
+
+def workspace_bytes(n: int, num_positions: int, num_modes: int, max_batch_waves: Optional[int] = None) -> int:
+    lib = _native.load()
+    cap = DEFAULT_BATCH_WAVES if max_batch_waves is None else int(max_batch_waves)
+    return int(lib.pty_forward_workspace_bytes(int(n), int(num_positions), int(num_modes), cap))
+
+
+def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: torch.Tensor,
+                       origins: torch.Tensor, shifts_px: torch.Tensor, prop: torch.Tensor, *,
+                       want_patterns: bool = True, want_amplitude: bool = False,
+                       patterns_out: Optional[torch.Tensor] = None,
+                       max_batch_waves: Optional[int] = None) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor]]:
+    """Windowed multislice forward on the GPU.
+
+    slab_t (S, Hs, Ws) complex64 transmission; probe_k (M, n, n) complex64
+    fft2 of the probe modes; weights (M,) float32; origins (P, 2) int32;
+    shifts_px (P, 2) float64; prop (n,) complex64 separable propagator
+    factor including 1/n.  Returns (patterns (P, n, n) float32 fftshifted,
+    amplitude (P, M, n, n) complex64 fftshifted) with None for outputs not
+    requested.
+    """
+    _require(isinstance(slab_t, torch.Tensor) and slab_t.is_cuda,
+             "slab must be a GPU tensor (the multislice path has no CPU fallback)")
+    dev = slab_t.device
+    _check_cuda(slab_t, "slab", torch.complex64, 3, dev)
+    _check_cuda(probe_k, "probe_k", torch.complex64, 3, dev)
+    m_count, n, n2 = probe_k.shape
+    _require(n == n2, "probe window must be square")
+    _require(n in SUPPORTED_N, f"window size {n} is not one of {SUPPORTED_N}")
+    s_count, hs, ws = slab_t.shape
+    _require(s_count >= 1, "at least one slice is required")
+    _require(hs >= n and ws >= n, f"slab {hs}x{ws} is smaller than the {n}x{n} window")
+    _check_cuda(weights, "weights", torch.float32, 1, dev)
+    _require(weights.shape[0] == m_count, "weights must have shape (M,)")
+    _check_cuda(origins, "origins", torch.int32, 2, dev)
+    _check_cuda(shifts_px, "shifts_px", torch.float64, 2, dev)
+    p_count = origins.shape[0]
+    _require(origins.shape == (p_count, 2) and shifts_px.shape == (p_count, 2),
+             "origins and shifts must both have shape (P, 2)")
+    _require(p_count >= 1, "at least one scan position is required")
+    _check_cuda(prop, "prop", torch.complex64, 1, dev)
+    _require(prop.shape[0] == n, "propagator factor must have n entries")
+    _require(want_patterns or want_amplitude, "nothing requested")
+
+    patterns = None
+    if want_patterns:
+        if patterns_out is not None:
+            _check_cuda(patterns_out, "patterns_out", torch.float32, 3, dev)
+            _require(tuple(patterns_out.shape) == (p_count, n, n), "patterns_out has the wrong shape")
+            patterns = patterns_out
+        else:
+            patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev)
+    amplitude = torch.empty((p_count, m_count, n, n), dtype=torch.complex64, device=dev) if want_amplitude else None
+    ws_bytes = workspace_bytes(n, p_count, m_count, max_batch_waves)
+    workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev) if ws_bytes else None
+
+    lib = _native.load()
+    with torch.cuda.device(dev):
+        status = lib.pty_forward(
+            n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k),
+            _native.ptr(weights), _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop),
+            _native.ptr(prop), _native.ptr(patterns), _native.ptr(amplitude), _native.ptr(workspace),
+            ws_bytes, _native.stream_handle(dev))
+    _native.check(status, "pty_forward")
+    return patterns, amplitude

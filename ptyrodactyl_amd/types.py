@@ -1,0 +1,280 @@
+"""Carriers and validated factories for the multislice path.
+
+Mirrors the reference's Equinox carriers and ``create_*`` factories
+(``src/ptyrodactyl/types/electron_types.py:103-373`` for the classes,
+``:529-1504`` for the factories) with torch tensors.  Two validation tiers
+as in the reference:
+
+* structural problems (ranks, shapes, missing fields) raise ``ValueError``
+  eagerly (e.g. ``electron_types.py:1283-1291``);
+* value problems (non-finite data, non-positive scalars) raise
+  :class:`RuntimeCheckError`, the counterpart of ``equinox.EquinoxRuntimeError``
+  raised by ``eqx.error_if`` in the reference, with the same message text;
+  setting ``PTYRODACTYL_DISABLE_RUNTIME_CHECKS=1`` turns this tier off
+  (``src/ptyrodactyl/__init__.py:89-90``).
+"""
+
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Tuple
+
+import torch
+
+
+class RuntimeCheckError(RuntimeError):
+    """A deferred value check failed (the reference raises EquinoxRuntimeError)."""
+
+
+def runtime_checks_enabled() -> bool:
+    return os.environ.get("PTYRODACTYL_DISABLE_RUNTIME_CHECKS", "0").lower() not in ("1", "true", "yes")
+
+
+def error_if(condition, message: str) -> None:
+    """Raise RuntimeCheckError(message) when ``condition`` holds (tier 2)."""
+    if not runtime_checks_enabled():
+        return
+    cond = bool(condition.item()) if isinstance(condition, torch.Tensor) else bool(condition)
+    if cond:
+        raise RuntimeCheckError(message)
+
+
+def _as_tensor(x, dtype, device=None) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(dtype=dtype, device=device if device is not None else x.device)
+    return torch.as_tensor(x, dtype=dtype, device=device)
+
+
+def _scalar(value, name: str) -> float:
+    t = torch.as_tensor(value, dtype=torch.float64)
+    if t.ndim != 0:
+        raise ValueError(f"{name} must be a scalar")
+    return float(t)
+
+
+def _checked_finite(value: float, name: str) -> float:
+    error_if(not torch.isfinite(torch.tensor(value)), f"{name} must be finite")
+    return value
+
+
+def _checked_positive(value: float, name: str) -> float:
+    _checked_finite(value, name)
+    error_if(value <= 0, f"{name} must be positive")
+    return value
+
+
+def _checked_nonnegative(value: float, name: str) -> float:
+    _checked_finite(value, name)
+    error_if(value < 0, f"{name} must be non-negative")
+    return value
+
+
+def _optional_shape(shape, name: str) -> Optional[Tuple[int, int]]:
+    if shape is None:
+        return None
+    vals = tuple(int(v) for v in (shape.tolist() if isinstance(shape, torch.Tensor) else shape))
+    if len(vals) != 2:
+        raise ValueError(f"{name} must have two entries")
+    return vals
+
+
+def _optional_positions(pos, name: str) -> Optional[torch.Tensor]:
+    if pos is None:
+        return None
+    t = _as_tensor(pos, torch.float64)
+    if t.ndim == 1 and t.shape[0] == 2:
+        t = t[None, :]
+    if t.ndim != 2:
+        raise ValueError(f"{name} must be 2D")
+    if t.shape[1] != 2:
+        raise ValueError(f"{name} must have shape (P, 2)")
+    error_if(~torch.isfinite(t).all(), f"{name} contain non-finite values")
+    return t
+
+
+@dataclass(frozen=True)
+class EnsembleAxes:
+    """Optional ensemble distributions (``electron_types.py:74-101``).
+
+    The multislice path here supports the empty carrier only; populated
+    axes are rejected by the simulators.
+    """
+
+    probe_modes: object = None
+    position_jitter: object = None
+    coherence: object = None
+
+    def is_empty(self) -> bool:
+        return self.position_jitter is None and self.coherence is None
+
+
+@dataclass(frozen=True)
+class MicroscopeConfig:
+    """``electron_types.py:103-145``."""
+
+    voltage_kv: float
+    aperture_mrad: float
+    defocus_ang: float
+    c3_ang: float
+    c5_ang: float
+    ensemble: EnsembleAxes = field(default_factory=EnsembleAxes)
+    probe_shape: Optional[Tuple[int, int]] = None
+
+
+@dataclass(frozen=True)
+class DetectorConfig:
+    """``electron_types.py:147-190``."""
+
+    real_space_calib_ang: float
+    probe_calibration_pm: float
+    collection_inner_mrad: float
+    collection_outer_mrad: float
+    scan_positions_px: Optional[torch.Tensor] = None
+    scan_positions_ang: Optional[torch.Tensor] = None
+    scan_shape: Optional[Tuple[int, int]] = None
+
+
+@dataclass(frozen=True)
+class CalibratedArray:
+    """``electron_types.py:264-293``."""
+
+    data_array: torch.Tensor
+    calib_y: float
+    calib_x: float
+    real_space: bool
+
+
+@dataclass(frozen=True)
+class ProbeModes:
+    """``electron_types.py:295-318``: modes (H, W, M) complex, weights (M,) normalised."""
+
+    modes: torch.Tensor
+    weights: torch.Tensor
+    calib: float
+
+
+@dataclass(frozen=True)
+class PotentialSlices:
+    """``electron_types.py:320-343``: slices (H, W, S) projected potential in V*Angstrom."""
+
+    slices: torch.Tensor
+    slice_thickness: float
+    calib: float
+
+
+@dataclass(frozen=True)
+class STEM4D:
+    """``electron_types.py:345-373``: data (P, H, W)."""
+
+    data: torch.Tensor
+    real_space_calib: float
+    fourier_space_calib: float
+    scan_positions: torch.Tensor
+    voltage_kv: float
+
+
+def create_ensemble_axes(probe_modes=None, position_jitter=None, coherence=None) -> EnsembleAxes:
+    """``electron_types.py:488-527``."""
+    return EnsembleAxes(probe_modes, position_jitter, coherence)
+
+
+def create_microscope_config(voltage_kv, aperture_mrad, defocus_ang=0.0, c3_ang=0.0, c5_ang=0.0,
+                             ensemble: Optional[EnsembleAxes] = None, probe_shape=None) -> MicroscopeConfig:
+    """``electron_types.py:529-614``."""
+    voltage = _checked_positive(_scalar(voltage_kv, "voltage_kv"), "voltage_kv")
+    aperture = _checked_positive(_scalar(aperture_mrad, "aperture_mrad"), "aperture_mrad")
+    defocus = _checked_finite(_scalar(defocus_ang, "defocus_ang"), "defocus_ang")
+    c3 = _checked_finite(_scalar(c3_ang, "c3_ang"), "c3_ang")
+    c5 = _checked_finite(_scalar(c5_ang, "c5_ang"), "c5_ang")
+    shape = _optional_shape(probe_shape, "probe_shape")
+    return MicroscopeConfig(voltage, aperture, defocus, c3, c5,
+                            create_ensemble_axes() if ensemble is None else ensemble, shape)
+
+
+def create_detector_config(real_space_calib_ang, probe_calibration_pm=None, collection_inner_mrad=0.0,
+                           collection_outer_mrad=0.0, scan_positions_px=None, scan_positions_ang=None,
+                           scan_shape=None) -> DetectorConfig:
+    """``electron_types.py:616-733``; probe_calibration_pm defaults to 100x the calibration."""
+    real_calib = _scalar(real_space_calib_ang, "real_space_calib_ang")
+    probe_calib = real_calib * 100.0 if probe_calibration_pm is None else _scalar(
+        probe_calibration_pm, "probe_calibration_pm")
+    inner = _scalar(collection_inner_mrad, "collection_inner_mrad")
+    outer = _scalar(collection_outer_mrad, "collection_outer_mrad")
+    pos_px = _optional_positions(scan_positions_px, "scan_positions_px")
+    pos_ang = _optional_positions(scan_positions_ang, "scan_positions_ang")
+    shape = _optional_shape(scan_shape, "scan_shape")
+    _checked_positive(real_calib, "real_space_calib_ang")
+    _checked_positive(probe_calib, "probe_calibration_pm")
+    _checked_nonnegative(inner, "collection_inner_mrad")
+    _checked_nonnegative(outer, "collection_outer_mrad")
+    error_if(outer < inner, "collection_outer_mrad must be >= collection_inner_mrad")
+    return DetectorConfig(real_calib, probe_calib, inner, outer, pos_px, pos_ang, shape)
+
+
+def create_calibrated_array(data_array, calib_y, calib_x, real_space) -> CalibratedArray:
+    """``electron_types.py:853-935``."""
+    data = data_array if isinstance(data_array, torch.Tensor) else torch.as_tensor(data_array)
+    if data.ndim != 2:
+        raise ValueError("data_array must be 2D")
+    cy = _checked_positive(_scalar(calib_y, "calib_y"), "calib_y")
+    cx = _checked_positive(_scalar(calib_x, "calib_x"), "calib_x")
+    error_if(~torch.isfinite(data).all(), "data_array contain non-finite values")
+    return CalibratedArray(data, cy, cx, bool(real_space))
+
+
+def create_probe_modes(modes, weights, calib) -> ProbeModes:
+    """``electron_types.py:1238-1325``: weights are normalised to sum to one."""
+    modes_t = modes if isinstance(modes, torch.Tensor) else torch.as_tensor(modes)
+    if not modes_t.is_complex():
+        modes_t = modes_t.to(torch.complex128)
+    weights_t = _as_tensor(weights, torch.float64, device=modes_t.device)
+    if modes_t.ndim != 3:
+        raise ValueError("modes must be 3D")
+    if weights_t.ndim != 1:
+        raise ValueError("weights must be 1D")
+    if weights_t.shape != (modes_t.shape[2],):
+        raise ValueError("weights must have shape (M,)")
+    calib_f = _scalar(calib, "calib")
+    error_if(~torch.isfinite(torch.view_as_real(modes_t)).all(), "modes contain non-finite values")
+    error_if(~torch.isfinite(weights_t).all(), "weights contain non-finite values")
+    error_if((weights_t < 0).any(), "weights must be non-negative")
+    wsum = weights_t.sum()
+    error_if(wsum <= torch.finfo(torch.float64).eps, "weights must sum to a positive value")
+    _checked_positive(calib_f, "calib")
+    return ProbeModes(modes_t, weights_t / wsum, calib_f)
+
+
+def create_potential_slices(slices, slice_thickness, calib) -> PotentialSlices:
+    """``electron_types.py:1328-1398``."""
+    slices_t = _as_tensor(slices, torch.float64) if not (isinstance(slices, torch.Tensor)
+                                                         and slices.dtype in (torch.float32, torch.float64)) else slices
+    if slices_t.ndim != 3:
+        raise ValueError("slices must be 3D")
+    thick = _scalar(slice_thickness, "slice_thickness")
+    calib_f = _scalar(calib, "calib")
+    error_if(~torch.isfinite(slices_t).all(), "slices contain non-finite values")
+    _checked_positive(thick, "slice_thickness")
+    _checked_positive(calib_f, "calib")
+    return PotentialSlices(slices_t, thick, calib_f)
+
+
+def create_stem4d(data, real_space_calib, fourier_space_calib, scan_positions, voltage_kv) -> STEM4D:
+    """``electron_types.py:1401-1504``."""
+    data_t = data if isinstance(data, torch.Tensor) else torch.as_tensor(data, dtype=torch.float64)
+    pos_t = _as_tensor(scan_positions, torch.float64, device=data_t.device)
+    if data_t.ndim != 3:
+        raise ValueError("data must be 3D")
+    if pos_t.ndim != 2:
+        raise ValueError("scan_positions must be 2D")
+    if pos_t.shape != (data_t.shape[0], 2):
+        raise ValueError("scan_positions must have shape (P, 2)")
+    rc = _scalar(real_space_calib, "real_space_calib")
+    fc = _scalar(fourier_space_calib, "fourier_space_calib")
+    kv = _scalar(voltage_kv, "voltage_kv")
+    error_if(~torch.isfinite(data_t).all(), "data contain non-finite values")
+    error_if(~torch.isfinite(pos_t).all(), "scan_positions contain non-finite values")
+    _checked_positive(rc, "real_space_calib")
+    _checked_positive(fc, "fourier_space_calib")
+    _checked_positive(kv, "voltage_kv")
+    return STEM4D(data_t, rc, fc, pos_t, kv)

@@ -1,0 +1,178 @@
+"""GPU parity of the HIP multislice forward against the complex128 oracle.
+
+Tolerance: relative L2 error of the patterns <= 1e-5 (complex64 kernels
+against the complex128 oracle), the bar the forward path is held to.
+"""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import ms_oracle as o
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "plan01_ug_reference.npz"
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _random_case(n, s_count, p_count, slab_hw, seed, modes=1):
+    rng = np.random.default_rng(seed)
+    phase = rng.uniform(0, 0.4, (s_count,) + slab_hw)
+    amp = rng.uniform(0.95, 1.0, (s_count,) + slab_hw)
+    slab = (amp * np.exp(1j * phase)).astype(np.complex64)
+    probes = [o.make_probe(300.0, 20.0 + 3 * m, 30.0 * m, 0.0, 0.0, (n, n), 10.0) for m in range(modes)]
+    origins = np.stack([rng.integers(0, slab_hw[0], p_count), rng.integers(0, slab_hw[1], p_count)], 1)
+    shifts = rng.uniform(-3.0, 3.0, (p_count, 2))
+    prop = o.propagation_func(n, n, 1.5, 300.0, 0.1)
+    return slab, np.stack(probes), origins.astype(np.int32), shifts, prop
+
+
+def _oracle_patterns(slab, probes, origins, shifts, prop, weights):
+    total = 0
+    for m in range(probes.shape[0]):
+        total = total + weights[m] * o.window_stem(slab.astype(np.complex128), probes[m], origins, shifts, prop)
+    return total
+
+
+def _run_hip(slab, probes, origins, shifts, n, dz=1.5, kv=300.0, calib=0.1, weights=None, amplitude=False):
+    from ptyrodactyl_amd import engine, physics
+    dev = torch.device("cuda", 0)
+    m_count = probes.shape[0]
+    w = np.full(m_count, 1.0 / m_count) if weights is None else np.asarray(weights) / np.sum(weights)
+    pk = torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous()
+    pat, amp = engine.multislice_forward(
+        torch.tensor(slab, device=dev), pk, torch.tensor(w, dtype=torch.float32, device=dev),
+        torch.tensor(origins, dtype=torch.int32, device=dev), torch.tensor(shifts, device=dev),
+        physics.propagator_factors(n, dz, kv, calib, device=dev), want_amplitude=amplitude)
+    torch.cuda.synchronize()
+    return pat.cpu().numpy(), (amp.cpu().numpy() if amp is not None else None), w
+
+
+def test_library_loads_on_gpu():
+    from ptyrodactyl_amd import _native
+    lib = _native.load()
+    assert lib.pty_abi_version() == _native.ABI_VERSION
+
+
+@pytest.mark.parametrize("n,s_count,p_count,slab_hw", [
+    (32, 3, 5, (40, 36)),
+    (64, 4, 7, (64, 80)),
+    (128, 5, 6, (160, 128)),
+    (256, 3, 9, (256, 300)),
+    (512, 2, 3, (512, 512)),
+])
+def test_forward_matches_oracle(n, s_count, p_count, slab_hw):
+    slab, probes, origins, shifts, prop = _random_case(n, s_count, p_count, slab_hw, seed=n + s_count)
+    print(f"forward parity n={n} S={s_count} P={p_count} slab={slab_hw}", flush=True)
+    got, _, w = _run_hip(slab, probes, origins, shifts, n)
+    ref = _oracle_patterns(slab, probes, origins, shifts, prop, w)
+    err = rel_l2(got, ref)
+    assert err < TOL, f"rel-L2 {err:.3e} at n={n}"
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_forward_single_slice_and_modes(n):
+    slab, probes, origins, shifts, prop = _random_case(n, 1, 4, (n, n + 16), seed=11, modes=2)
+    print(f"modes/single-slice n={n}", flush=True)
+    got, amp, w = _run_hip(slab, probes, origins, shifts, n, weights=[0.7, 0.3], amplitude=True)
+    ref = _oracle_patterns(slab, probes, origins, shifts, prop, w)
+    assert rel_l2(got, ref) < TOL
+    _, ref_amp = o.window_stem(slab.astype(np.complex128), probes[1], origins, shifts, prop, return_amplitude=True)
+    assert rel_l2(np.abs(amp[:, 1]), np.abs(ref_amp)) < TOL
+    assert rel_l2(amp[:, 1].real, ref_amp.real) < 10 * TOL
+
+
+def test_forward_vacuum_is_aperture_indicator():
+    n = 128
+    slab = np.ones((4, n, n), np.complex64)
+    probe = o.make_probe(200.0, 20.0, 25.0, 0.0, 0.0, (n, n), 15.0)[None]
+    got, _, _ = _run_hip(slab, probe, np.zeros((3, 2), np.int32), np.array([[0, 0], [3.3, -7.25], [60.0, 1.5]]), n,
+                         kv=200.0, calib=0.15)
+    q = o.probe_fourier_grid(n, n, 15.0)
+    ind = np.fft.fftshift((q <= 0.02 / o.wavelength_ang(200.0)).astype(float))
+    assert np.max(np.abs(got - ind[None])) < 1e-5
+
+
+def test_forward_is_deterministic():
+    n = 256
+    slab, probes, origins, shifts, _ = _random_case(n, 3, 5, (300, 256), seed=3)
+    a, _, _ = _run_hip(slab, probes, origins, shifts, n)
+    b, _, _ = _run_hip(slab, probes, origins, shifts, n)
+    assert np.array_equal(a, b)
+
+
+def test_streaming_batches_agree_with_one_batch():
+    """A batch cap that splits the positions into several launches changes nothing."""
+    from ptyrodactyl_amd import engine, physics
+    n = 256
+    slab, probes, origins, shifts, _ = _random_case(n, 2, 11, (256, 256), seed=5)
+    dev = torch.device("cuda", 0)
+    args = (torch.tensor(slab, device=dev),
+            torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous(),
+            torch.ones(1, dtype=torch.float32, device=dev), torch.tensor(origins, device=dev),
+            torch.tensor(shifts, device=dev), physics.propagator_factors(n, 1.5, 300.0, 0.1, device=dev))
+    full, _ = engine.multislice_forward(*args)
+    split, _ = engine.multislice_forward(*args, max_batch_waves=3)
+    assert torch.equal(full, split)
+
+
+def test_golden_fixture_through_hip_drop_in():
+    """The reference's golden capture reproduced through the drop-in API on the GPU."""
+    import ptyrodactyl_amd as pa
+    ref = np.load(GOLDEN)
+    pot = torch.tensor(o.jax_uniform_f64(42, (32, 32, 2)) * 10.0)
+    slices = pa.create_potential_slices(pot, 2.0, 0.2)
+    mic = pa.create_microscope_config(voltage_kv=100.0, aperture_mrad=20.0, probe_shape=(32, 32))
+    det = pa.create_detector_config(real_space_calib_ang=0.2, probe_calibration_pm=20.0)
+    probe = pa.make_probe(mic, det)
+    pm = pa.create_probe_modes(probe[..., None], torch.tensor([1.0]), 0.2)
+    cbed = pa.cbed_image(slices, pm, mic)
+    assert rel_l2(cbed.data_array.cpu().numpy(), ref["cbed"]) < TOL
+    assert cbed.calib_y == pytest.approx(1 / 6.4)
+    positions = torch.tensor([[12.0, 12.0], [12.0, 20.0], [20.0, 12.0], [20.0, 20.0]], dtype=torch.float64)
+    det2 = pa.create_detector_config(real_space_calib_ang=0.2, probe_calibration_pm=20.0, scan_positions_px=positions)
+    s4d = pa.stem_4d(slices, pm, mic, det2)
+    assert rel_l2(s4d.data.cpu().numpy(), ref["stem4d"]) < TOL
+    np.testing.assert_allclose(s4d.scan_positions.cpu().numpy(), positions.numpy() * 0.2)
+
+
+def test_cbed_amplitude_matches_oracle():
+    import ptyrodactyl_amd as pa
+    rng = np.random.default_rng(12)
+    n = 64
+    pot = rng.uniform(0, 20, (n, n, 3))
+    mic = pa.create_microscope_config(voltage_kv=200.0, aperture_mrad=22.0, defocus_ang=20.0, probe_shape=(n, n))
+    det = pa.create_detector_config(real_space_calib_ang=0.2)
+    probe = pa.make_probe(mic, det)
+    pm = pa.create_probe_modes(probe[..., None], [1.0], 0.2)
+    amp = pa.cbed_amplitude(pa.create_potential_slices(torch.tensor(pot), 1.0, 0.2), pm, mic)
+    ref = o.cbed_amplitude(pot, 1.0, 0.2, probe.numpy()[..., None], 0.2, 200.0)
+    assert amp.shape == (n, n, 1)
+    assert rel_l2(amp.cpu().numpy().real, ref.real) < 10 * TOL
+    assert rel_l2(np.abs(amp.cpu().numpy()), np.abs(ref)) < TOL
+
+
+def test_config2_subset_full_size():
+    """Headline workload (256^2 windows, 64 slices, 1024^2 slab): 8 positions against the oracle."""
+    from ptyrodactyl_amd import synth
+    w = synth.config2(scan=64)
+    sel = np.arange(0, 4096, 512)
+    probe = w.probe[None]
+    prop = o.propagation_func(w.n, w.n, w.slice_thickness, w.voltage_kv, w.calib)
+    print("config2 subset", flush=True)
+    got, _, _ = _run_hip(w.slab_t, probe, w.origins[sel], w.shifts_px[sel], w.n, w.slice_thickness, w.voltage_kv,
+                         w.calib)
+    ref = o.window_stem_batched(w.slab_t.astype(np.complex128), w.probe, w.origins[sel], w.shifts_px[sel], prop,
+                                workers=8)
+    err = rel_l2(got, ref)
+    print(f"config2 subset rel-L2 {err:.3e}", flush=True)
+    assert err < TOL

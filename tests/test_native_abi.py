@@ -1,0 +1,48 @@
+"""CPU checks of the C-ABI library: it loads and exports every declared symbol."""
+
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for header in (ROOT / "include").glob("*.h"):
+        text = header.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(pty_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert "pty_forward" in names and "pty_last_error" in names
+
+
+def test_library_exports_every_declared_symbol():
+    from ptyrodactyl_amd import _native
+    if not _native.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    lib = _native.load()
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, f"missing exports: {missing}"
+    assert set(declared_symbols()) <= set(_native.SIGNATURES), "ctypes signatures lag the header"
+    assert lib.pty_abi_version() == _native.ABI_VERSION
+
+
+def test_invalid_arguments_rejected_without_gpu():
+    """Argument validation runs on the host before any device call."""
+    from ptyrodactyl_amd import _native
+    if not _native.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    lib = _native.load()
+    rc = lib.pty_forward(100, 1, 1, 1, None, 100, 100, None, None, None, None, None, None, None, None, None, 0, None)
+    assert rc == _native.PTY_ERR_INVALID
+    assert b"window size" in lib.pty_last_error()
+    with pytest.raises(ValueError, match="window size"):
+        _native.check(rc, "pty_forward")
+    assert lib.pty_forward_workspace_bytes(256, 10, 1, 4) == 4 * 256 * 256 * 8
+    assert lib.pty_forward_workspace_bytes(128, 10, 1, 0) == 0

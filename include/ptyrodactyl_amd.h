@@ -64,6 +64,39 @@ int pty_forward(int n, int num_slices, long long num_positions, int num_modes, c
                 const double* shifts_px, const void* prop_y, const void* prop_x, float* patterns, void* amplitude,
                 void* workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * Forward that also stores the wave entering slices k, 2k, ... (< S) for the
+ * adjoint: checkpoints complex64 [num_positions * num_modes][(S-1)/k][n][n].
+ * Same multislice as pty_forward; the reference has no checkpointing (XLA
+ * keeps every slice's residuals under jax.value_and_grad,
+ * src/ptyrodactyl/multislice/multislice_recon.py:1915-1951).
+ */
+int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                             long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                             const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                             float* patterns, void* checkpoints, int ckpt_every, void* workspace,
+                             size_t workspace_bytes, void* stream);
+
+/* Device scratch pty_adjoint needs (bounded by the number of waves kept in flight). */
+size_t pty_adjoint_workspace_bytes(int n, int ckpt_every, long long num_positions, int num_modes);
+
+/*
+ * Reverse-mode multislice: given dL/dI (float32 [num_positions][n][n], the
+ * fftshifted pattern layout), ACCUMULATES into grad_slab (complex64, the
+ * slab's shape) and grad_probe_k (complex64 [num_modes][n][n], may be NULL)
+ * the gradients dL/dRe + i dL/dIm w.r.t. the transmission slab and probe_k.
+ * Replaces the VJP that jax.value_and_grad builds through stem_4d in the
+ * reconstructors (src/ptyrodactyl/multislice/multislice_recon.py:406-434,
+ * 849-885, 1400-1436, 1915-1951).  Each k-slice segment is recomputed from
+ * the checkpoints of pty_forward_checkpointed.  Accumulation uses float32
+ * atomics: results are reproducible to rounding, not bitwise.
+ */
+int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
+                long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+                const double* shifts_px, const void* prop_y, const void* prop_x, const float* grad_patterns,
+                const void* checkpoints, int ckpt_every, void* grad_slab, void* grad_probe_k, void* workspace,
+                size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,11 @@ SIGNATURES = {
     "pty_forward_workspace_bytes": (c_size, [c_int, c_ll, c_int, c_ll]),
     "pty_forward": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
                             c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "pty_forward_checkpointed": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size, c_vp]),
+    "pty_adjoint_workspace_bytes": (c_size, [c_int, c_int, c_ll, c_int]),
+    "pty_adjoint": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                            c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_size, c_vp]),
 }
 
 

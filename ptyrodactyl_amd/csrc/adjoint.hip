@@ -1,0 +1,515 @@
+// Reverse-mode (adjoint) multislice kernels for gfx950.
+//
+// Forward per wave (see forward.hip):  psi_0 = F^H(probe_k . R),
+// phi_s = psi_s T_s,  psi_{s+1} = Prop(phi_s),  A = F(phi_{S-1}),
+// I = sum_m w_m |A_m|^2 (fftshifted).  Given g = dL/dI the adjoint is
+//   gPhi      = 2 w_m g . A
+//   gphi_S-1  = F^H gPhi
+//   gT_s     += conj(psi_s) gphi_s          gpsi_s = conj(T_s) gphi_s
+//   gphi_s-1  = Prop^H(gpsi_s)              (propagator conjugated)
+//   gprobe_k += conj(R) F(gpsi_0)
+// with gradients in the dL/dRe + i dL/dIm convention.  This is the reverse
+// of the scan the reference differentiates with jax.value_and_grad
+// (src/ptyrodactyl/multislice/multislice_recon.py:1915-1951 through
+// simulations.py:543-602); the reference keeps every slice's residuals,
+// while here the wave entering every k-th slice was checkpointed by the
+// forward and each k-slice segment is recomputed before it is reversed.
+#include "adjoint.hpp"
+#include "common.hpp"
+
+namespace pty {
+
+enum : int { PRE_NONE = 0, PRE_IFFT = 1, PRE_ADJPROP = 2 };
+enum : int { POST_NONE = 0, POST_PROP = 1, POST_FFT = 2 };
+enum : int { PW_NONE = 0, PW_FWD = 1, PW_ADJ = 2 };
+enum : int { C_PROP = 0, C_ADJPROP = 1, C_IFFT_PROBE = 2, C_FINAL_ADJ = 3, C_PROBE_GRAD = 4 };
+
+// Row work shared by both strategies: v holds row y of one wave in the
+// strided distribution (thread t owns x = t + T q).
+template <int N, int PRE, int POST, class Buf>
+__device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, int pw, const AdjParams& a,
+                                        const WaveGeom& g, float2* seg_row, bool seg_store, const Buf& rb,
+                                        const TwTable& twt, const float2* px) {
+  constexpr int T = N / 16;
+  if constexpr (PRE == PRE_IFFT) line_fft<N, true>(v, t, rb, twt);
+  if constexpr (PRE == PRE_ADJPROP) {
+    line_fft<N, false>(v, t, rb, twt);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = cmulc(v[q], px[t + T * q]);
+    line_fft<N, true>(v, t, rb, twt);
+  }
+  if (seg_store) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) seg_row[t + T * q] = v[q];
+  }
+  if (pw != PW_NONE) {
+    int gy = g.oy + y;
+    if (gy >= a.f.slab_h) gy -= (int)a.f.slab_h;
+    const long long off = (long long)s * a.f.slice_stride + (long long)gy * a.f.slab_w;
+    const float2* trow = a.f.slab + off;
+    if (pw == PW_FWD) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        int x = g.ox + t + T * q;
+        if (x >= a.f.slab_w) x -= (int)a.f.slab_w;
+        v[q] = cmul(v[q], trow[x]);
+      }
+    } else {
+      float2* grow = a.grad_slab + off;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        int x = g.ox + t + T * q;
+        if (x >= a.f.slab_w) x -= (int)a.f.slab_w;
+        const float2 gt = cmulc(v[q], seg_row[t + T * q]);  // gphi * conj(psi)
+        unsafeAtomicAdd(&grow[x].x, gt.x);
+        unsafeAtomicAdd(&grow[x].y, gt.y);
+        v[q] = cmulc(v[q], trow[x]);  // conj(T) * gphi
+      }
+    }
+  }
+  if constexpr (POST == POST_PROP) {
+    line_fft<N, false>(v, t, rb, twt);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], px[t + T * q]);
+    line_fft<N, true>(v, t, rb, twt);
+  }
+  if constexpr (POST == POST_FFT) line_fft<N, false>(v, t, rb, twt);
+}
+
+// Column work: v holds column x (thread t owns ky = t + T q).  Returns false
+// when the result was consumed (probe gradient) and must not be stored.
+template <int N, int OP, class Buf>
+__device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long w, const AdjParams& a,
+                                        const WaveGeom& g, const Buf& cb, const TwTable& twt, const float2* py,
+                                        const float2* ramp_y, const float2* ramp_x) {
+  constexpr int T = N / 16;
+  const int p = (int)(w / a.f.M), m = (int)(w % a.f.M);
+  if constexpr (OP == C_IFFT_PROBE) {
+    const float2 rx = ramp_x ? ramp_x[x] : shift_ramp(x, N, g.mx, g.dx);
+    const float2* pk = a.f.probe_k + (size_t)m * N * N + x;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int ky = t + T * q;
+      const float2 ry = ramp_y ? ramp_y[ky] : shift_ramp(ky, N, g.my, g.dy);
+      v[q] = cmul(cmul(pk[(size_t)ky * N], ry), rx);
+    }
+    line_fft<N, true>(v, t, cb, twt);
+    return true;
+  } else {
+    line_fft<N, false>(v, t, cb, twt);
+    if constexpr (OP == C_PROP || OP == C_ADJPROP) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = OP == C_PROP ? cmul(v[q], py[t + T * q]) : cmulc(v[q], py[t + T * q]);
+      line_fft<N, true>(v, t, cb, twt);
+      return true;
+    } else if constexpr (OP == C_FINAL_ADJ) {
+      const float two_w = 2.0f * a.f.weights[m];
+      const float* gi = a.grad_i + (size_t)p * N * N + ((x + N / 2) & (N - 1));
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = cscale(v[q], two_w * gi[(size_t)((t + T * q + N / 2) & (N - 1)) * N]);
+      line_fft<N, true>(v, t, cb, twt);
+      return true;
+    } else {  // C_PROBE_GRAD
+      if (a.grad_probe_k) {
+        const float2 rx = ramp_x ? ramp_x[x] : shift_ramp(x, N, g.mx, g.dx);
+        float2* gp = a.grad_probe_k + (size_t)m * N * N + x;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int ky = t + T * q;
+          const float2 ry = ramp_y ? ramp_y[ky] : shift_ramp(ky, N, g.my, g.dy);
+          const float2 val = cmulc(v[q], cmul(ry, rx));
+          unsafeAtomicAdd(&gp[(size_t)ky * N].x, val.x);
+          unsafeAtomicAdd(&gp[(size_t)ky * N].y, val.y);
+        }
+      }
+      return false;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-resident adjoint (n = 32, 64, 128): one workgroup per wave, looping
+// over waves so that the global scratch is bounded by the grid size.
+// ---------------------------------------------------------------------------
+
+struct ColBufRes {
+  float2* col;
+  int rs;
+  __device__ __forceinline__ float2 ld(int e) const { return col[(e ^ ((e >> 4) & 3)) * rs]; }
+  __device__ __forceinline__ void st(int e, float2 v) const { col[(e ^ ((e >> 4) & 3)) * rs] = v; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
+};
+
+template <int N>
+struct ResShared {
+  float2* wave;
+  const float2* tw;
+  const float2* py;
+  const float2* px;
+  const float2* ramp_y;
+  const float2* ramp_x;
+};
+
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <int N, int PRE, int POST>
+__device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams& a, const WaveGeom& g, int s, int pw,
+                                        float2* seg, bool seg_store) {
+  constexpr int T = N / 16, RS = N + N / 16, G = 64 / T;
+  const int tid = opaque((int)threadIdx.x);
+  const int lane = tid & 63, wid = tid >> 6;
+  const int rt = lane % T, ry = wid * G + lane / T;
+  const RowBufWave rb{sh.wave + ry * RS};
+  const TwTable twt{sh.tw};
+  float2 v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = rb.row[pad16(rt + T * q)];
+  row_ops<N, PRE, POST>(v, rt, ry, s, pw, a, g, seg ? seg + (size_t)ry * N : nullptr, seg_store, rb, twt, sh.px);
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) rb.row[pad16(rt + T * q)] = v[q];
+  __syncthreads();
+}
+
+template <int N, int OP>
+__device__ __forceinline__ void res_col(const ResShared<N>& sh, const AdjParams& a, const WaveGeom& g, long long w) {
+  constexpr int T = N / 16, RS = N + N / 16, G = 64 / T;
+  const int tid = opaque((int)threadIdx.x);
+  const int lane = tid & 63, wid = tid >> 6;
+  const int ct = lane / G, cx = wid * G + lane % G;
+  float2* col = sh.wave + pad16(cx);
+  const ColBufRes cb{col, RS};
+  const TwTable twt{sh.tw};
+  float2 v[16];
+  if constexpr (OP != C_IFFT_PROBE) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = col[(ct + T * q) * RS];
+  }
+  if (col_ops<N, OP>(v, ct, cx, w, a, g, cb, twt, sh.py, sh.ramp_y, sh.ramp_x)) {
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) col[(ct + T * q) * RS] = v[q];
+  }
+  __syncthreads();
+}
+
+template <int N>
+__device__ __forceinline__ void res_copy(const ResShared<N>& sh, float2* gmem, bool to_lds) {
+  constexpr int T = N / 16, RS = N + N / 16, G = 64 / T;
+  const int tid = opaque((int)threadIdx.x);
+  const int lane = tid & 63, wid = tid >> 6;
+  const int rt = lane % T, ry = wid * G + lane / T;
+  float2* row = sh.wave + ry * RS;
+  float2* grow = gmem + (size_t)ry * N;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    if (to_lds)
+      row[pad16(rt + T * q)] = grow[rt + T * q];
+    else
+      grow[rt + T * q] = row[pad16(rt + T * q)];
+  }
+  __syncthreads();
+}
+
+template <int N>
+__global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, float2* scratch) {
+  constexpr int NT = N * N / 16, RS = N + N / 16;
+  extern __shared__ float2 lds[];
+  float2* wave = lds;
+  float2* tw = wave + N * RS;
+  float2* py = tw + N;
+  float2* px = py + N;
+  float2* ramp_y = px + N;
+  float2* ramp_x = ramp_y + N;
+  const int tid = threadIdx.x;
+  fill_twiddles(tw, N, tid, NT);
+  for (int i = tid; i < N; i += NT) {
+    py[i] = a.f.prop_y[i];
+    px[i] = a.f.prop_x[i];
+  }
+  const ResShared<N> sh{wave, tw, py, px, ramp_y, ramp_x};
+  const int S = a.f.S, k = a.k;
+  const int nc = num_checkpoints(S, k), C = nc + 1;
+  float2* seg = scratch + (size_t)blockIdx.x * (k + 1) * N * N;
+  float2* park = seg + (size_t)k * N * N;
+  const long long waves = a.f.P * a.f.M;
+
+  for (long long w = blockIdx.x; w < waves; w += gridDim.x) {
+    const int p = (int)(w / a.f.M);
+    const WaveGeom g = load_geom(a.f.origins, a.f.shifts, p, N, a.f.slab_h, a.f.slab_w);
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) {
+      ramp_y[i] = shift_ramp(i, N, g.my, g.dy);
+      ramp_x[i] = shift_ramp(i, N, g.mx, g.dx);
+    }
+    __syncthreads();
+    for (int c = C - 1; c >= 0; --c) {
+      const int s0 = c * k, s1 = min(S, s0 + k);
+      if (c < C - 1) res_copy<N>(sh, park, false);
+      // recompute psi_{s0} .. psi_{s1-1} into seg
+      if (c == 0)
+        res_col<N, C_IFFT_PROBE>(sh, a, g, w);
+      else
+        res_copy<N>(sh, const_cast<float2*>(a.ckpt) + ((size_t)w * nc + (c - 1)) * N * N, true);
+      for (int s = s0; s < s1; ++s) {
+        float2* sg = seg + (size_t)(s - s0) * N * N;
+        const bool more = s < s1 - 1;
+        if (s == 0) {
+          if (more)
+            res_row<N, PRE_IFFT, POST_PROP>(sh, a, g, s, PW_FWD, sg, true);
+          else
+            res_row<N, PRE_IFFT, POST_NONE>(sh, a, g, s, PW_NONE, sg, true);
+        } else {
+          if (more)
+            res_row<N, PRE_NONE, POST_PROP>(sh, a, g, s, PW_FWD, sg, true);
+          else
+            res_row<N, PRE_NONE, POST_NONE>(sh, a, g, s, PW_NONE, sg, true);
+        }
+        if (more) res_col<N, C_PROP>(sh, a, g, w);
+      }
+      int s = s1 - 1;
+      if (c == C - 1) {  // exit plane: gphi_{S-1} = F^H (2 w g A)
+        res_row<N, PRE_NONE, POST_FFT>(sh, a, g, S - 1, PW_FWD, nullptr, false);
+        res_col<N, C_FINAL_ADJ>(sh, a, g, w);
+        res_row<N, PRE_IFFT, POST_NONE>(sh, a, g, S - 1, PW_ADJ, seg + (size_t)(S - 1 - s0) * N * N, false);
+        s = S - 2;
+      } else {
+        res_copy<N>(sh, park, true);
+      }
+      for (; s >= s0; --s) {
+        res_col<N, C_ADJPROP>(sh, a, g, w);
+        res_row<N, PRE_ADJPROP, POST_NONE>(sh, a, g, s, PW_ADJ, seg + (size_t)(s - s0) * N * N, false);
+      }
+    }
+    res_row<N, PRE_NONE, POST_FFT>(sh, a, g, 0, PW_NONE, nullptr, false);
+    res_col<N, C_PROBE_GRAD>(sh, a, g, w);
+  }
+}
+
+template <int N>
+static size_t resident_adj_lds_bytes() {
+  return sizeof(float2) * ((size_t)N * (N + N / 16) + 5 * N);
+}
+
+// ---------------------------------------------------------------------------
+// Streaming adjoint (n = 256, 512): per-slice row / column passes over a
+// batch of waves in global scratch.
+// ---------------------------------------------------------------------------
+
+struct WaveBuf {  // per-wave buffers: base + wl * stride
+  float2* base;
+  long long stride;
+  __device__ __forceinline__ float2* at(int wl) const { return base + (size_t)wl * stride; }
+};
+
+template <int N, int PRE, int POST>
+__global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, WaveBuf dst, WaveBuf segb, long long w0,
+                                                  int nb, int s, int pw, int seg_idx, int seg_store) {
+  constexpr int T = N / 16, RPB = 256 / T, RS = N + N / 16;
+  __shared__ float2 rows[RPB * RS];
+  __shared__ float2 tw[N];
+  __shared__ float2 px[N];
+  const int tid = threadIdx.x;
+  fill_twiddles(tw, N, tid, 256);
+  for (int i = tid; i < N; i += 256) px[i] = a.f.prop_x[i];
+  const int wl = blockIdx.x % nb, rblk = blockIdx.x / nb;
+  const long long w = w0 + wl;
+  const int p = (int)(w / a.f.M);
+  const int t = tid % T, lr = tid / T, y = rblk * RPB + lr;
+  const WaveGeom g = load_geom(a.f.origins, a.f.shifts, p, N, a.f.slab_h, a.f.slab_w);
+  const TwTable twt{tw};
+  const RowBufWave rb{rows + lr * RS};
+  __syncthreads();
+  const float2* srow = src.at(wl) + (size_t)y * N;
+  float2 v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = srow[t + T * q];
+  float2* seg_row = segb.base ? segb.at(wl) + ((size_t)seg_idx * N + y) * N : nullptr;
+  row_ops<N, PRE, POST>(v, t, y, s, pw, a, g, seg_row, seg_store != 0, rb, twt, px);
+  float2* drow = dst.at(wl) + (size_t)y * N;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) drow[t + T * q] = v[q];
+}
+
+template <int NC>
+struct ColBufBlk {
+  float2* base;
+  __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
+  __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+};
+
+template <int N, int OP>
+__global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, WaveBuf dst, long long w0, int nb) {
+  constexpr int T = N / 16, NC = 256 / T, PS = N + N / 16;
+  __shared__ float2 buf[PS * NC];
+  __shared__ float2 tw[N];
+  __shared__ float2 py[N];
+  const int tid = threadIdx.x;
+  fill_twiddles(tw, N, tid, 256);
+  for (int i = tid; i < N; i += 256) py[i] = a.f.prop_y[i];
+  const int gc = tid % NC, t = tid / NC;
+  const int wl = blockIdx.x % nb, cblk = blockIdx.x / nb;
+  const long long w = w0 + wl;
+  const int p = (int)(w / a.f.M);
+  const int x = cblk * NC + gc;
+  const WaveGeom g = load_geom(a.f.origins, a.f.shifts, p, N, a.f.slab_h, a.f.slab_w);
+  const TwTable twt{tw};
+  const ColBufBlk<NC> cb{buf + gc};
+  __syncthreads();
+  float2 v[16];
+  if constexpr (OP != C_IFFT_PROBE) {
+    const float2* scol = src.at(wl) + x;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = scol[(size_t)(t + T * q) * N];
+  }
+  if (col_ops<N, OP>(v, t, x, w, a, g, cb, twt, py, nullptr, nullptr)) {
+    float2* dcol = dst.at(wl) + x;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dcol[(size_t)(t + T * q) * N] = v[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static int device_cus() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+  }
+  return cus > 0 ? cus : 256;
+}
+
+static long long resident_slots(int n) {
+  const int per_cu = n >= 128 ? 1 : (n == 64 ? 4 : 8);
+  return (long long)device_cus() * per_cu;
+}
+
+static long long streaming_batch(int n) { return n >= 512 ? 16 : 64; }
+
+size_t adjoint_workspace_bytes(int n, int k, long long waves) {
+  const size_t wave_bytes = (size_t)n * n * sizeof(float2);
+  if (n <= 128) {
+    long long slots = resident_slots(n);
+    if (slots > waves) slots = waves;
+    return (size_t)slots * (k + 1) * wave_bytes;
+  }
+  long long nb = streaming_batch(n);
+  if (nb > waves) nb = waves;
+  return (size_t)nb * (k + 2) * wave_bytes;  // seg[k], X, G per wave
+}
+
+template <int N>
+static hipError_t launch_adj_resident(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st) {
+  const size_t lds = resident_adj_lds_bytes<N>();
+  static bool attr_set[64] = {false};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (dev >= 0 && dev < 64 && !attr_set[dev]) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_adjoint_resident<N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr_set[dev] = true;
+  }
+  const long long waves = a.f.P * a.f.M;
+  const size_t per_slot = (size_t)(a.k + 1) * N * N * sizeof(float2);
+  long long slots = (long long)(ws_bytes / per_slot);
+  if (slots > waves) slots = waves;
+  if (slots > resident_slots(N)) slots = resident_slots(N);
+  if (slots < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_adjoint_resident<N>, dim3((unsigned)slots), dim3(N * N / 16), lds, st, a, (float2*)ws);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_adj_streaming(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st) {
+  constexpr int T = N / 16, RPB = 256 / T, NC = 256 / T;
+  const long long NN = (long long)N * N;
+  const int S = a.f.S, k = a.k, nc = num_checkpoints(S, k), C = nc + 1;
+  const long long waves = a.f.P * a.f.M;
+  long long nbmax = (long long)(ws_bytes / ((size_t)(k + 2) * NN * sizeof(float2)));
+  if (nbmax > waves) nbmax = waves;
+  if (nbmax < 1) return hipErrorInvalidValue;
+  if (nbmax >= 8) nbmax -= nbmax % 8;
+  float2* base = (float2*)ws;
+  for (long long w0 = 0; w0 < waves; w0 += nbmax) {
+    const int nb = (int)((waves - w0) < nbmax ? (waves - w0) : nbmax);
+    const WaveBuf seg{base, (long long)k * NN};
+    const WaveBuf X{base + (size_t)nb * k * NN, NN};
+    const WaveBuf Gb{base + (size_t)nb * (k + 1) * NN, NN};
+    const WaveBuf none{nullptr, 0};
+    const dim3 rg(nb * (N / RPB)), cg(nb * (N / NC)), blk(256);
+    for (int c = C - 1; c >= 0; --c) {
+      const int s0 = c * k, s1 = min(S, s0 + k);
+      WaveBuf first = X;
+      if (c == 0)
+        hipLaunchKernelGGL((k_adj_cols<N, C_IFFT_PROBE>), cg, blk, 0, st, a, none, X, w0, nb);
+      else
+        first = WaveBuf{const_cast<float2*>(a.ckpt) + ((size_t)w0 * nc + (c - 1)) * NN, (long long)nc * NN};
+      for (int s = s0; s < s1; ++s) {
+        const bool more = s < s1 - 1;
+        const WaveBuf src = s == s0 ? first : X;
+        if (s == 0) {
+          if (more)
+            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_PROP>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
+                               (int)PW_FWD, s - s0, 1);
+          else
+            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
+                               (int)PW_NONE, s - s0, 1);
+        } else {
+          if (more)
+            hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_PROP>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
+                               (int)PW_FWD, s - s0, 1);
+          else
+            hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_NONE>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
+                               (int)PW_NONE, s - s0, 1);
+        }
+        if (more) hipLaunchKernelGGL((k_adj_cols<N, C_PROP>), cg, blk, 0, st, a, X, X, w0, nb);
+      }
+      int s = s1 - 1;
+      if (c == C - 1) {
+        hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, X, X, none, w0, nb, S - 1,
+                           (int)PW_FWD, 0, 0);
+        hipLaunchKernelGGL((k_adj_cols<N, C_FINAL_ADJ>), cg, blk, 0, st, a, X, Gb, w0, nb);
+        hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, S - 1,
+                           (int)PW_ADJ, S - 1 - s0, 0);
+        s = S - 2;
+      }
+      for (; s >= s0; --s) {
+        hipLaunchKernelGGL((k_adj_cols<N, C_ADJPROP>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
+        hipLaunchKernelGGL((k_adj_rows<N, PRE_ADJPROP, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, s,
+                           (int)PW_ADJ, s - s0, 0);
+      }
+    }
+    hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, Gb, Gb, none, w0, nb, 0,
+                       (int)PW_NONE, 0, 0);
+    hipLaunchKernelGGL((k_adj_cols<N, C_PROBE_GRAD>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_adjoint(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st) {
+  switch (a.f.n) {
+    case 32:
+      return launch_adj_resident<32>(a, ws, ws_bytes, st);
+    case 64:
+      return launch_adj_resident<64>(a, ws, ws_bytes, st);
+    case 128:
+      return launch_adj_resident<128>(a, ws, ws_bytes, st);
+    case 256:
+      return launch_adj_streaming<256>(a, ws, ws_bytes, st);
+    case 512:
+      return launch_adj_streaming<512>(a, ws, ws_bytes, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace pty

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include "../../include/ptyrodactyl_amd.h"
+#include "adjoint.hpp"
 #include "forward.hpp"
 
 namespace {
@@ -64,6 +65,8 @@ pty::FwdParams make_params(int n, int num_slices, long long num_positions, int n
   a.prop_x = (const float2*)prop_x;
   a.patterns = nullptr;
   a.amplitude = nullptr;
+  a.ckpt = nullptr;
+  a.ckpt_every = 0;
   return a;
 }
 
@@ -98,6 +101,67 @@ int pty_forward(int n, int num_slices, long long num_positions, int num_modes, c
   a.patterns = patterns;
   a.amplitude = (float2*)amplitude;
   return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_forward");
+}
+
+int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                             long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                             const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                             float* patterns, void* checkpoints, int ckpt_every, void* workspace,
+                             size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                        origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
+  if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
+    return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
+                pty::num_checkpoints(num_slices, ckpt_every));
+  if (!patterns) return fail(PTY_ERR_INVALID, "patterns is null");
+  const size_t need = pty::forward_workspace_bytes(n, 1, num_modes, 0);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
+                need);
+  pty::FwdParams a = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k,
+                                 mode_weights, origins, shifts_px, prop_y, prop_x);
+  a.patterns = patterns;
+  a.ckpt = (float2*)checkpoints;
+  a.ckpt_every = ckpt_every;
+  return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream),
+                       "pty_forward_checkpointed");
+}
+
+size_t pty_adjoint_workspace_bytes(int n, int ckpt_every, long long num_positions, int num_modes) {
+  if (!valid_n(n) || ckpt_every < 1 || num_positions < 1 || num_modes < 1) return 0;
+  return pty::adjoint_workspace_bytes(n, ckpt_every, num_positions * num_modes);
+}
+
+int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
+                long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+                const double* shifts_px, const void* prop_y, const void* prop_x, const float* grad_patterns,
+                const void* checkpoints, int ckpt_every, void* grad_slab, void* grad_probe_k, void* workspace,
+                size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                        origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
+  if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
+    return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
+                pty::num_checkpoints(num_slices, ckpt_every));
+  if (!grad_patterns || !grad_slab) return fail(PTY_ERR_INVALID, "grad_patterns and grad_slab must be non-null");
+  const size_t need = pty::adjoint_workspace_bytes(n, ckpt_every, 1);
+  if (!workspace || workspace_bytes < need)
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one wave needs", workspace_bytes,
+                need);
+  pty::AdjParams a;
+  a.f = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights, origins,
+                    shifts_px, prop_y, prop_x);
+  a.grad_i = grad_patterns;
+  a.ckpt = (const float2*)checkpoints;
+  a.k = ckpt_every;
+  a.grad_slab = (float2*)grad_slab;
+  a.grad_probe_k = (float2*)grad_probe_k;
+  return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_adjoint");
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 // the group; every stage is written for this build (no vendor FFT library).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
 
 #define PTY_HD __host__ __device__ __forceinline__
 
@@ -30,17 +31,29 @@ PTY_HD float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 PTY_HD float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 PTY_HD float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
 
+// Complex multiply by a constant given as double-float parts (hi + lo);
+// the small terms are accumulated first so the product carries the
+// constant's full precision.  Rounding the radix-16 constants to float32
+// alone is a systematic error that accumulates coherently over the ~100
+// chained transforms of a thick specimen; this keeps it below float32 noise.
+PTY_HD float2 cmul_df(float2 z, float crh, float crl, float cih, float cil) {
+  float re = fmaf(z.x, crl, -z.y * cil);
+  re = fmaf(-z.y, cih, re);
+  re = fmaf(z.x, crh, re);
+  float im = fmaf(z.x, cil, z.y * crl);
+  im = fmaf(z.y, crh, im);
+  im = fmaf(z.x, cih, im);
+  return make_float2(re, im);
+}
+
 // Multiply by W16^E = exp(-2 pi i E / 16) (forward) or its conjugate.
 template <int E, bool INV>
 PTY_HD float2 tw16(float2 z) {
   constexpr int e = ((E % 16) + 16) % 16;
-  constexpr float C1 = 0.92387953251128675613f;  // cos(pi/8)
-  constexpr float S1 = 0.38268343236508977173f;  // sin(pi/8)
-  constexpr float R2 = 0.70710678118654752440f;  // 1/sqrt(2)
-  // (cr, ci) = exp(-2 pi i e/16)
-  constexpr float tab[16][2] = {
-      {1.f, 0.f},  {C1, -S1},  {R2, -R2},  {S1, -C1},  {0.f, -1.f}, {-S1, -C1}, {-R2, -R2}, {-C1, -S1},
-      {-1.f, 0.f}, {-C1, S1},  {-R2, R2},  {-S1, C1},  {0.f, 1.f},  {S1, C1},   {R2, R2},   {C1, S1}};
+  // cos(pi/8), sin(pi/8), 1/sqrt(2) split into float32 hi + lo
+  constexpr float C1H = 0.9238795042037964f, C1L = 2.830748968563057e-08f;
+  constexpr float S1H = 0.3826834261417389f, S1L = 6.2233507236442165e-09f;
+  constexpr float R2H = 0.7071067690849304f, R2L = 1.2101617485882343e-08f;
   if constexpr (e == 0) {
     return z;
   } else if constexpr (e == 4) {
@@ -49,10 +62,29 @@ PTY_HD float2 tw16(float2 z) {
     return make_float2(-z.x, -z.y);
   } else if constexpr (e == 12) {
     return INV ? mul_mi(z) : mul_pi(z);
+  } else if constexpr (e % 4 == 2) {
+    // W = R2 (sr + i si): z W = R2 (sr zr - si zi, sr zi + si zr)
+    constexpr float sr = (e == 2 || e == 14) ? 1.f : -1.f;
+    constexpr float si0 = (e == 2 || e == 6) ? -1.f : 1.f;
+    constexpr float si = INV ? -si0 : si0;
+    const float a = sr * z.x - si * z.y;
+    const float b = sr * z.y + si * z.x;
+    return make_float2(fmaf(a, R2H, a * R2L), fmaf(b, R2H, b * R2L));
   } else {
-    constexpr float cr = tab[e][0];
-    constexpr float ci = tab[e][1];
-    return INV ? cmul(z, make_float2(cr, -ci)) : cmul(z, make_float2(cr, ci));
+    // odd e: (cos, -sin)(2 pi e / 16) from the pi/8 pair with signs
+    constexpr int q = e / 4;           // quadrant
+    constexpr bool near = (e % 4) == 1;  // angle q*pi/2 + pi/8 (else + 3pi/8)
+    // base (c, s) for the angle within the quadrant: exp(-i theta)
+    constexpr float bch = near ? C1H : S1H, bcl = near ? C1L : S1L;
+    constexpr float bsh = near ? S1H : C1H, bsl = near ? S1L : C1L;
+    // rotate by quadrant: exp(-i (q pi/2 + theta)) = (-i)^q (c - i s)
+    float crh, crl, cih, cil;
+    if constexpr (q == 0) { crh = bch; crl = bcl; cih = -bsh; cil = -bsl; }
+    else if constexpr (q == 1) { crh = -bsh; crl = -bsl; cih = -bch; cil = -bcl; }
+    else if constexpr (q == 2) { crh = -bch; crl = -bcl; cih = bsh; cil = bsl; }
+    else { crh = bsh; crl = bsl; cih = bch; cil = bcl; }
+    if constexpr (INV) { cih = -cih; cil = -cil; }
+    return cmul_df(z, crh, crl, cih, cil);
   }
 }
 

@@ -86,6 +86,12 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
         float2 v[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rb.row[pad16(rt + T * q)];
+        if (a.ckpt && s > 0 && s % a.ckpt_every == 0) {  // checkpoint the wave entering slice s
+          const int nc = num_checkpoints(a.S, a.ckpt_every);
+          float2* ck = a.ckpt + (((size_t)p * a.M + m) * nc + (s / a.ckpt_every - 1)) * N * N + (size_t)ry * N;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) ck[rt + T * q] = v[q];
+        }
         if (s == 0) line_fft<N, true>(v, rt, rb, twt);  // finish the shifted probe: ifft along x
         int gy = g.oy + ry;
         if (gy >= a.slab_h) gy -= (int)a.slab_h;
@@ -196,6 +202,12 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   float2 v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = wrow[t + T * q];
+  if (a.ckpt && s > 0 && s % a.ckpt_every == 0) {  // checkpoint the wave entering slice s
+    const int nc = num_checkpoints(a.S, a.ckpt_every);
+    float2* ck = a.ckpt + ((size_t)w * nc + (s / a.ckpt_every - 1)) * N * N + (size_t)y * N;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) ck[t + T * q] = v[q];
+  }
   if (FLAGS & ROW_FIRST) line_fft<N, true>(v, t, rb, twt);
   int gy = g.oy + y;
   if (gy >= a.slab_h) gy -= (int)a.slab_h;

@@ -19,7 +19,12 @@ struct FwdParams {
   const float2* prop_x;      // [n] exp(-i pi lambda dz qx^2) / n
   float* patterns;           // [P][n][n] fftshifted intensities, or null
   float2* amplitude;         // [P][M][n][n] fftshifted amplitudes, or null
+  float2* ckpt;              // [P*M][C-1][n][n] wave entering slices k, 2k, ... (C = ceil(S/k)), or null
+  int ckpt_every;            // checkpoint interval k (slices)
 };
+
+// Number of stored checkpoints per wave: slices k, 2k, ... below S.
+__host__ __device__ inline int num_checkpoints(int S, int k) { return k > 0 ? (S - 1) / k : 0; }
 
 size_t forward_workspace_bytes(int n, long long num_positions, int num_modes, long long max_batch_waves);
 hipError_t launch_forward(const FwdParams& a, void* ws, size_t ws_bytes, hipStream_t st);

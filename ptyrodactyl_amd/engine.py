@@ -100,3 +100,95 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
             ws_bytes, _native.stream_handle(dev))
     _native.check(status, "pty_forward")
     return patterns, amplitude
+
+
+def _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop):
+    _require(isinstance(slab_t, torch.Tensor) and slab_t.is_cuda,
+             "slab must be a GPU tensor (the multislice path has no CPU fallback)")
+    dev = slab_t.device
+    _check_cuda(slab_t, "slab", torch.complex64, 3, dev)
+    _check_cuda(probe_k, "probe_k", torch.complex64, 3, dev)
+    m_count, n, n2 = probe_k.shape
+    _require(n == n2, "probe window must be square")
+    _require(n in SUPPORTED_N, f"window size {n} is not one of {SUPPORTED_N}")
+    s_count, hs, ws = slab_t.shape
+    _require(hs >= n and ws >= n, f"slab {hs}x{ws} is smaller than the {n}x{n} window")
+    _check_cuda(weights, "weights", torch.float32, 1, dev)
+    _require(weights.shape[0] == m_count, "weights must have shape (M,)")
+    _check_cuda(origins, "origins", torch.int32, 2, dev)
+    _check_cuda(shifts_px, "shifts_px", torch.float64, 2, dev)
+    p_count = origins.shape[0]
+    _require(p_count >= 1 and origins.shape == (p_count, 2) and shifts_px.shape == (p_count, 2),
+             "origins and shifts must both have shape (P, 2) with P >= 1")
+    _check_cuda(prop, "prop", torch.complex64, 1, dev)
+    _require(prop.shape[0] == n, "propagator factor must have n entries")
+    return dev, n, s_count, hs, ws, m_count, p_count
+
+
+def num_checkpoints(num_slices: int, ckpt_every: int) -> int:
+    return (num_slices - 1) // ckpt_every
+
+
+def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px, prop, ckpt_every: int,
+                                    max_batch_waves: Optional[int] = None):
+    """Forward that also returns the wave entering slices k, 2k, ... for the adjoint.
+
+    Returns (patterns (P, n, n) float32, checkpoints (P*M, (S-1)//k, n, n) complex64).
+    """
+    dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
+                                                                        shifts_px, prop)
+    _require(int(ckpt_every) >= 1, "ckpt_every must be >= 1")
+    nc = num_checkpoints(s_count, int(ckpt_every))
+    patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev)
+    ckpt = torch.empty((p_count * m_count, nc, n, n), dtype=torch.complex64, device=dev)
+    ws_bytes = workspace_bytes(n, p_count, m_count, max_batch_waves)
+    workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev) if ws_bytes else None
+    lib = _native.load()
+    with torch.cuda.device(dev):
+        status = lib.pty_forward_checkpointed(
+            n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
+            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(patterns), _native.ptr(ckpt) if nc else 0, int(ckpt_every), _native.ptr(workspace),
+            ws_bytes, _native.stream_handle(dev))
+    _native.check(status, "pty_forward_checkpointed")
+    return patterns, ckpt
+
+
+def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_patterns, ckpt, ckpt_every: int,
+                       grad_slab: Optional[torch.Tensor] = None, grad_probe_k: Optional[torch.Tensor] = None,
+                       want_probe: bool = True):
+    """Reverse-mode multislice: accumulate dL/dT into grad_slab and dL/dprobe_k into grad_probe_k.
+
+    Gradients are dL/dRe + i dL/dIm (torch's convention for a real loss).
+    Returns (grad_slab, grad_probe_k or None).
+    """
+    dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
+                                                                        shifts_px, prop)
+    k = int(ckpt_every)
+    _require(k >= 1, "ckpt_every must be >= 1")
+    nc = num_checkpoints(s_count, k)
+    _check_cuda(grad_patterns, "grad_patterns", torch.float32, 3, dev)
+    _require(tuple(grad_patterns.shape) == (p_count, n, n), "grad_patterns must have shape (P, n, n)")
+    if nc:
+        _check_cuda(ckpt, "checkpoints", torch.complex64, 4, dev)
+        _require(tuple(ckpt.shape) == (p_count * m_count, nc, n, n), "checkpoint buffer has the wrong shape")
+    if grad_slab is None:
+        grad_slab = torch.zeros_like(slab_t)
+    _check_cuda(grad_slab, "grad_slab", torch.complex64, 3, dev)
+    _require(grad_slab.shape == slab_t.shape, "grad_slab must match the slab")
+    if want_probe and grad_probe_k is None:
+        grad_probe_k = torch.zeros_like(probe_k)
+    if grad_probe_k is not None:
+        _check_cuda(grad_probe_k, "grad_probe_k", torch.complex64, 3, dev)
+        _require(grad_probe_k.shape == probe_k.shape, "grad_probe_k must match probe_k")
+    lib = _native.load()
+    ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, k, p_count, m_count))
+    workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        status = lib.pty_adjoint(
+            n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
+            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(grad_patterns), _native.ptr(ckpt) if nc else 0, k, _native.ptr(grad_slab),
+            _native.ptr(grad_probe_k), _native.ptr(workspace), ws_bytes, _native.stream_handle(dev))
+    _native.check(status, "pty_adjoint")
+    return grad_slab, grad_probe_k

@@ -1,0 +1,135 @@
+"""GPU parity of the HIP adjoint (gradient) kernels.
+
+Gradients are compared with the complex128 oracle's reverse mode
+(``multislice_vjp``, itself pinned by finite differences and an independent
+torch autograd twin) at relative L2 <= 1e-5, and with central finite
+differences of the oracle loss (h = 1e-3).  Accumulation uses float32
+atomics, so results are compared with a tolerance, never bitwise.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import ms_oracle as o
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _case(n, s_count, p_count, slab_hw, seed, modes=1):
+    rng = np.random.default_rng(seed)
+    slab = (rng.uniform(0.95, 1.0, (s_count,) + slab_hw) * np.exp(1j * rng.uniform(0, 0.4, (s_count,) + slab_hw)))
+    slab = slab.astype(np.complex64)
+    probes = np.stack([o.make_probe(300.0, 20.0 + 4 * m, 25.0 * (m + 1), 0.0, 0.0, (n, n), 10.0)
+                       for m in range(modes)])
+    origins = np.stack([rng.integers(0, slab_hw[0], p_count), rng.integers(0, slab_hw[1], p_count)], 1)
+    shifts = rng.uniform(-2.5, 2.5, (p_count, 2))
+    g = rng.standard_normal((p_count, n, n))
+    return slab, probes, origins.astype(np.int32), shifts, g
+
+
+def _gpu_grads(slab, probes, origins, shifts, g, n, k, weights=None, dz=1.5, kv=300.0, calib=0.1):
+    from ptyrodactyl_amd import engine, physics
+    dev = torch.device("cuda", 0)
+    m_count = probes.shape[0]
+    w = np.full(m_count, 1.0 / m_count) if weights is None else np.asarray(weights) / np.sum(weights)
+    slab_t = torch.tensor(slab, device=dev)
+    pk = torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous()
+    wt = torch.tensor(w, dtype=torch.float32, device=dev)
+    ot = torch.tensor(origins, dtype=torch.int32, device=dev)
+    st = torch.tensor(shifts, device=dev)
+    prop = physics.propagator_factors(n, dz, kv, calib, device=dev)
+    pat, ckpt = engine.multislice_forward_checkpointed(slab_t, pk, wt, ot, st, prop, k)
+    gt = torch.tensor(g, dtype=torch.float32, device=dev)
+    g_slab, g_pk = engine.multislice_adjoint(slab_t, pk, wt, ot, st, prop, gt, ckpt, k)
+    # probe_k = fft2(probe)  =>  dL/dprobe = F^H dL/dprobe_k = n^2 ifft2(.)
+    g_probe = torch.fft.ifft2(g_pk.to(torch.complex128)) * (n * n)
+    torch.cuda.synchronize()
+    return pat.cpu().numpy(), g_slab.cpu().numpy(), g_probe.cpu().numpy(), w
+
+
+def _oracle_grads(slab, probes, origins, shifts, g, w, n, dz=1.5, kv=300.0, calib=0.1):
+    prop = o.propagation_func(n, n, dz, kv, calib)
+    gs = 0
+    gps = []
+    for m in range(probes.shape[0]):
+        a, b = o.multislice_vjp(slab.astype(np.complex128), probes[m], origins, shifts, prop, w[m] * g)
+        gs = gs + a
+        gps.append(b)
+    return gs, np.stack(gps)
+
+
+@pytest.mark.parametrize("n,s_count,k,p_count,slab_hw,modes", [
+    (32, 1, 1, 3, (40, 40), 1),
+    (64, 5, 2, 4, (70, 64), 1),
+    (64, 6, 4, 3, (64, 96), 2),
+    (128, 7, 3, 3, (140, 128), 1),
+    (128, 4, 4, 2, (128, 128), 1),
+    (256, 5, 2, 3, (256, 280), 1),
+    (256, 3, 8, 2, (300, 256), 2),
+    (512, 2, 1, 2, (512, 512), 1),
+])
+def test_adjoint_matches_oracle_vjp(n, s_count, k, p_count, slab_hw, modes):
+    print(f"adjoint n={n} S={s_count} k={k} P={p_count} M={modes}", flush=True)
+    slab, probes, origins, shifts, g = _case(n, s_count, p_count, slab_hw, seed=n * 7 + s_count, modes=modes)
+    _, g_slab, g_probe, w = _gpu_grads(slab, probes, origins, shifts, g, n, k)
+    ref_slab, ref_probe = _oracle_grads(slab, probes, origins, shifts, g, w, n)
+    e_slab, e_probe = rel_l2(g_slab, ref_slab), rel_l2(g_probe, ref_probe)
+    print(f"  rel-L2 slab {e_slab:.2e} probe {e_probe:.2e}", flush=True)
+    assert e_slab < TOL and e_probe < TOL
+
+
+def test_checkpoint_interval_does_not_change_gradient():
+    n = 128
+    slab, probes, origins, shifts, g = _case(n, 8, 3, (128, 160), seed=21)
+    _, a, pa, _ = _gpu_grads(slab, probes, origins, shifts, g, n, 1)
+    _, b, pb, _ = _gpu_grads(slab, probes, origins, shifts, g, n, 8)
+    assert rel_l2(a, b) < 1e-6 and rel_l2(pa, pb) < 1e-6
+
+
+def test_autograd_function_matches_oracle():
+    from ptyrodactyl_amd import autograd, physics
+    n = 64
+    slab, probes, origins, shifts, c = _case(n, 4, 5, (80, 64), seed=33)
+    dev = torch.device("cuda", 0)
+    slab_t = torch.tensor(slab, device=dev, requires_grad=True)
+    probe = torch.tensor(probes[0], device=dev, requires_grad=True)
+    pk = torch.fft.fft2(probe).to(torch.complex64)[None].contiguous()
+    pat = autograd.multislice_patterns(slab_t, pk, torch.ones(1, device=dev), torch.tensor(origins, device=dev),
+                                       torch.tensor(shifts, device=dev),
+                                       physics.propagator_factors(n, 1.5, 300.0, 0.1, device=dev), ckpt_every=2)
+    loss = (pat * torch.tensor(c, dtype=torch.float32, device=dev)).sum()
+    loss.backward()
+    ref_slab, ref_probe = _oracle_grads(slab, probes, origins, shifts, c, [1.0], n)
+    assert rel_l2(slab_t.grad.cpu().numpy(), ref_slab) < TOL
+    assert rel_l2(probe.grad.cpu().numpy(), ref_probe[0]) < TOL
+
+
+def test_gradient_matches_central_differences():
+    """GPU gradient against central FD (h = 1e-3) of the complex128 oracle loss, real and imaginary."""
+    n = 64
+    slab, probes, origins, shifts, c = _case(n, 3, 4, (64, 64), seed=44)
+    _, g_slab, _, _ = _gpu_grads(slab, probes, origins, shifts, c, n, 2)
+    prop = o.propagation_func(n, n, 1.5, 300.0, 0.1)
+    base = slab.astype(np.complex128)
+
+    def loss(sl):
+        return float(np.sum(c * o.window_stem(sl, probes[0], origins, shifts, prop)))
+
+    rng = np.random.default_rng(45)
+    h = 1e-3
+    for _ in range(8):
+        idx = (rng.integers(0, 3), rng.integers(0, 64), rng.integers(0, 64))
+        for direction in (1.0, 1j):
+            sp, sm = base.copy(), base.copy()
+            sp[idx] += h * direction
+            sm[idx] -= h * direction
+            fd = (loss(sp) - loss(sm)) / (2 * h)
+            got = g_slab[idx].real if direction == 1.0 else g_slab[idx].imag
+            assert got == pytest.approx(fd, rel=1e-3, abs=1e-4 * np.abs(g_slab).max())

@@ -97,6 +97,44 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
                 const void* checkpoints, int ckpt_every, void* grad_slab, void* grad_probe_k, void* workspace,
                 size_t workspace_bytes, void* stream);
 
+/*
+ * Transmission T = exp(i sigma V) from a float64 potential (any shape, `count`
+ * values), written complex64; replaces transmission_func
+ * (src/ptyrodactyl/multislice/simulations.py:105-159) for a whole slab at once.
+ */
+int pty_transmission(const double* potential, double sigma, void* transmission, long long count, void* stream);
+
+#define PTY_MAX_NORMAL_PARAMS 21
+
+/* FP64 scratch for pty_normal_equations. */
+size_t pty_normal_equations_workspace_bytes(long long num_positions, int n, int num_params);
+
+/*
+ * Gauss-Newton / Levenberg-Marquardt normal equations in FP64 for Q =
+ * num_params real parameters of the probe.  amplitude is complex64
+ * [num_positions][1 + Q][n][n] as written by pty_forward with 1 + Q modes
+ * (mode 0 the probe, mode j the probe's derivative w.r.t. parameter j; the
+ * multislice is linear in the probe), data float32 [num_positions][n][n].
+ * With r = |A_0|^2 - data and J_j = 2 Re(conj(A_0) A_j):
+ *   jtj = J^T J (Q x Q), jtr = J^T r (Q), rss = sum r^2   (all FP64, device)
+ * The reference forms these implicitly through jvp/vjp
+ * (src/ptyrodactyl/jacobian/operators.py:29-185) inside
+ * levenberg_marquardt_step (src/ptyrodactyl/jacobian/solvers.py:387-521).
+ * Reduction order is fixed: results are bitwise reproducible.
+ */
+int pty_normal_equations(const void* amplitude, const float* data, long long num_positions, int n, int num_params,
+                         void* workspace, size_t workspace_bytes, double* jtj, double* jtr, double* rss,
+                         void* stream);
+
+/*
+ * Damped SPD solve in FP64 on the device: (matrix + damping I) solution = rhs
+ * (Cholesky, num_params <= 64), plus model_reduction = <x, rhs> - 1/2 <x, matrix x>,
+ * the predicted reduction of the LM gain ratio (solvers.py:480-489).
+ * status = 1 if the damped matrix was positive definite, else 0.
+ */
+int pty_damped_solve(const double* matrix, const double* rhs, double damping, int num_params, double* solution,
+                     double* model_reduction, int* status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -532,3 +532,97 @@ def jax_uniform_f64(seed: int, shape):
     one = np.array(1.0, np.float64).view(np.uint64)
     fbits = (bits >> np.uint64(12)) | one
     return (fbits.view(np.float64) - 1.0).reshape(shape)
+
+
+# ---------------------------------------------------------------------------
+# FP64 Levenberg-Marquardt over the twelve probe aberrations (the oracle run
+# the GPU mixed-precision fit is compared with).  Same update rules as
+# ``levenberg_marquardt_step`` (``jacobian/solvers.py:387-521``): step from
+# (J^T J + lambda I) s = J^T r, candidate theta - s, gain ratio
+# (|r|^2 - |r_new|^2) / (<s, g> - 1/2 <s, J^T J s> + 1e-12), lambda/3 above
+# 0.75, 2 lambda below 0.25, accept when the ratio is positive.
+# ---------------------------------------------------------------------------
+
+def aberration_basis12(n: int, calib_ang: float, voltage_kv: float):
+    """(12, n, n) d chi / d c_j on the fftfreq grid (chi is linear in the coefficients)."""
+    lam = wavelength_ang(voltage_kv)
+    q = fftfreq(n, calib_ang)
+    qy, qx = np.meshgrid(q, q, indexing="ij")
+    eye = np.eye(12)
+    return np.stack([aberration12(qy, qx, lam, eye[j]) for j in range(12)])
+
+
+def probe_k_from_coeffs(n, calib_ang, voltage_kv, aperture_mrad, coeffs):
+    """fft2 of probe_from_aberrations (computed through the FFTs, not the closed form)."""
+    return np.fft.fft2(probe_from_aberrations(n, calib_ang, voltage_kv, aperture_mrad, coeffs))
+
+
+def window_amplitudes(slab_t, probe_k_modes, origins, shifts_px, prop):
+    """(P, M, n, n) unshifted exit amplitudes F(phi_{S-1}) for every probe mode (given as fft2)."""
+    n = probe_k_modes.shape[-1]
+    out = np.empty((len(origins), probe_k_modes.shape[0], n, n), np.complex128)
+    for p in range(len(origins)):
+        ramp = ramp_1d(n, shifts_px[p][0])[:, None] * ramp_1d(n, shifts_px[p][1])[None, :]
+        wave = np.fft.ifft2(probe_k_modes * ramp[None])
+        t_win = window_of(slab_t, origins[p], n)
+        for s in range(slab_t.shape[0]):
+            wave = wave * t_win[s][None]
+            if s != slab_t.shape[0] - 1:
+                wave = np.fft.ifft2(np.fft.fft2(wave) * prop[None])
+        out[p] = np.fft.fft2(wave)
+    return out
+
+
+def lm_aberration_fit(slab_t, origins, shifts_px, data, coeffs_init, calib_ang, voltage_kv, aperture_mrad,
+                      slice_thickness, iterations=30, damping_init=1e-3, tolerance=1e-8):
+    """FP64 LM on the 12 aberration coefficients, parameters scaled to max |dchi| = 1 rad in the aperture.
+
+    data: (P, n, n) fftshifted intensities.  Returns (coefficients, residual norm history).
+    """
+    n = data.shape[-1]
+    lam = wavelength_ang(voltage_kv)
+    basis = aberration_basis12(n, calib_ang, voltage_kv)
+    q = fftfreq(n, calib_ang)
+    qy, qx = np.meshgrid(q, q, indexing="ij")
+    aperture = (np.sqrt(qx**2 + qy**2) <= (aperture_mrad / 1000.0) / lam).astype(float)
+    unit = 1.0 / np.max(np.abs(basis) * aperture[None], axis=(1, 2))
+    sbasis = basis * unit[:, None, None]
+    prop = propagation_func(n, n, slice_thickness, voltage_kv, calib_ang)
+    data_u = np.fft.ifftshift(np.asarray(data, np.float64), axes=(-2, -1))
+
+    def pk_of(theta):
+        return probe_k_from_coeffs(n, calib_ang, voltage_kv, aperture_mrad, theta * unit)
+
+    def rss_of(theta):
+        a = window_amplitudes(slab_t, pk_of(theta)[None], origins, shifts_px, prop)[:, 0]
+        return float(np.sum((np.abs(a) ** 2 - data_u) ** 2))
+
+    theta = np.asarray(coeffs_init, np.float64) / unit
+    damping = float(damping_init)
+    norm = np.sqrt(rss_of(theta))
+    hist = [norm]
+    for _ in range(iterations):
+        if norm < tolerance:
+            break
+        pk0 = pk_of(theta)
+        modes = np.concatenate([pk0[None], (-1j) * sbasis * pk0[None]], axis=0)
+        amp = window_amplitudes(slab_t, modes, origins, shifts_px, prop)
+        a0 = amp[:, 0]
+        jac = 2.0 * np.real(np.conj(a0)[:, None] * amp[:, 1:])          # (P, 12, n, n)
+        r = np.abs(a0) ** 2 - data_u
+        jmat = jac.transpose(1, 0, 2, 3).reshape(12, -1)
+        jtj = jmat @ jmat.T
+        g = jmat @ r.ravel()
+        rss = float(np.sum(r**2))
+        step = np.linalg.solve(jtj + damping * np.eye(12), g)
+        predicted = step @ g - 0.5 * step @ (jtj @ step)
+        cand = theta - step
+        rss_new = rss_of(cand)
+        gain = (rss - rss_new) / (predicted + 1e-12)
+        damping = lm_update_damping(gain, damping)
+        if gain > 0:
+            theta, norm = cand, np.sqrt(rss_new)
+        else:
+            norm = np.sqrt(rss)
+        hist.append(norm)
+    return theta * unit, hist

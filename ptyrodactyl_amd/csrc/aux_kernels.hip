@@ -1,0 +1,186 @@
+// Auxiliary gfx950 kernels around the multislice: transmission set-up,
+// FP64 normal equations for Gauss-Newton / Levenberg-Marquardt, and a small
+// FP64 damped SPD solve.
+#include <math.h>
+
+#include "aux_kernels.hpp"
+#include "common.hpp"
+
+namespace pty {
+
+// T = exp(i sigma V) from a float64 potential, stored complex64.  The phase
+// is formed and reduced in float64 (``transmission_func``,
+// src/ptyrodactyl/multislice/simulations.py:105-159).
+__global__ __launch_bounds__(256) void k_transmission(const double* pot, double sigma, float2* out, long long count) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    double s, c;
+    sincos(sigma * pot[i], &s, &c);
+    out[i] = make_float2((float)c, (float)s);
+  }
+}
+
+hipError_t launch_transmission(const double* pot, double sigma, float2* out, long long count, hipStream_t st) {
+  long long blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_transmission, dim3((unsigned)blocks), dim3(256), 0, st, pot, sigma, out, count);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Normal equations.  For every detector pixel of every position:
+//   r   = |A|^2 - data                      (A = amp[p][0])
+//   J_j = 2 Re(conj(A) dA_j)                (dA_j = amp[p][1 + j])
+// and the kernel accumulates JtJ (Q x Q), Jtr (Q) and sum r^2 in FP64.
+// Each workgroup reduces tiles of 256 pixels through LDS into per-product
+// FP64 accumulators and writes one partial; a second kernel sums the
+// partials in a fixed order (deterministic).
+// ---------------------------------------------------------------------------
+
+constexpr int NE_TILE = 256;
+
+__global__ __launch_bounds__(256) void k_normal_partials(const float2* amp, const float* data, long long positions,
+                                                         int pix, int Q, double* partials) {
+  extern __shared__ double lj[];  // [NE_TILE][Q + 1]: J_0..J_{Q-1}, r
+  const int tid = threadIdx.x;
+  const int stride = Q + 1;
+  const int nprod = Q * (Q + 1) / 2 + Q + 1;
+  // product index -> (a, b) with b == Q meaning r
+  int pa = -1, pb = -1;
+  if (tid < nprod) {
+    int t = tid, a = 0;
+    while (a <= Q && t >= (Q + 1 - a)) {
+      t -= (Q + 1 - a);
+      ++a;
+    }
+    pa = a;
+    pb = a + t;
+  }
+  double acc = 0.0;
+  const long long total = positions * (long long)pix;
+  const long long tiles = (total + NE_TILE - 1) / NE_TILE;
+  for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const long long idx = tile * NE_TILE + tid;
+    double* row = lj + tid * stride;
+    if (idx < total) {
+      const long long p = idx / pix;
+      const int q = (int)(idx % pix);
+      const float2* ap = amp + (size_t)p * (Q + 1) * pix + q;
+      const float2 a0 = ap[0];
+      for (int j = 0; j < Q; ++j) {
+        const float2 d = ap[(size_t)(j + 1) * pix];
+        row[j] = 2.0 * ((double)a0.x * (double)d.x + (double)a0.y * (double)d.y);
+      }
+      row[Q] = ((double)a0.x * a0.x + (double)a0.y * a0.y) - (double)data[idx];
+    } else {
+      for (int j = 0; j <= Q; ++j) row[j] = 0.0;
+    }
+    __syncthreads();
+    if (pa >= 0) {
+      for (int i = 0; i < NE_TILE; ++i) acc += lj[i * stride + pa] * lj[i * stride + pb];
+    }
+    __syncthreads();
+  }
+  if (tid < nprod) partials[(size_t)blockIdx.x * nprod + tid] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_normal_reduce(const double* partials, int blocks, int Q, double* jtj,
+                                                       double* jtr, double* rss) {
+  const int nprod = Q * (Q + 1) / 2 + Q + 1;
+  for (int t = threadIdx.x; t < nprod; t += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < blocks; ++b) s += partials[(size_t)b * nprod + t];
+    int tt = t, a = 0;
+    while (a <= Q && tt >= (Q + 1 - a)) {
+      tt -= (Q + 1 - a);
+      ++a;
+    }
+    const int b2 = a + tt;
+    if (a == Q) {
+      *rss = s;
+    } else if (b2 == Q) {
+      jtr[a] = s;
+    } else {
+      jtj[a * Q + b2] = s;
+      jtj[b2 * Q + a] = s;
+    }
+  }
+}
+
+int normal_partial_blocks(long long positions, int pix) {
+  const long long tiles = (positions * (long long)pix + NE_TILE - 1) / NE_TILE;
+  return (int)(tiles < 1024 ? tiles : 1024);
+}
+
+hipError_t launch_normal_equations(const float2* amp, const float* data, long long positions, int pix, int Q,
+                                   double* partials, double* jtj, double* jtr, double* rss, hipStream_t st) {
+  const int blocks = normal_partial_blocks(positions, pix);
+  const size_t lds = (size_t)NE_TILE * (Q + 1) * sizeof(double);
+  hipLaunchKernelGGL(k_normal_partials, dim3(blocks), dim3(256), lds, st, amp, data, positions, pix, Q, partials);
+  hipLaunchKernelGGL(k_normal_reduce, dim3(1), dim3(256), 0, st, partials, blocks, Q, jtj, jtr, rss);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Damped SPD solve in FP64: (A + damping I) x = b by Cholesky, plus the
+// model reduction <x, b> - 1/2 <x, A x> that the LM gain ratio needs.
+// One workgroup; Q <= 64.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void k_damped_solve(const double* A, const double* b, double damping, int Q,
+                                                    double* x, double* model_reduction, int* status) {
+  __shared__ double L[64 * 64];
+  __shared__ double y[64];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * Q; i += blockDim.x) {
+    const int r = i / Q, c = i % Q;
+    L[r * 64 + c] = A[i] + (r == c ? damping : 0.0);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int ok = 1;
+    for (int j = 0; j < Q; ++j) {
+      double d = L[j * 64 + j];
+      for (int k = 0; k < j; ++k) d -= L[j * 64 + k] * L[j * 64 + k];
+      if (!(d > 0.0)) {
+        ok = 0;
+        d = 1e-300;
+      }
+      const double ljj = sqrt(d);
+      L[j * 64 + j] = ljj;
+      for (int i = j + 1; i < Q; ++i) {
+        double s = L[i * 64 + j];
+        for (int k = 0; k < j; ++k) s -= L[i * 64 + k] * L[j * 64 + k];
+        L[i * 64 + j] = s / ljj;
+      }
+    }
+    for (int i = 0; i < Q; ++i) {  // L y = b
+      double s = b[i];
+      for (int k = 0; k < i; ++k) s -= L[i * 64 + k] * y[k];
+      y[i] = s / L[i * 64 + i];
+    }
+    for (int i = Q - 1; i >= 0; --i) {  // L^T x = y
+      double s = y[i];
+      for (int k = i + 1; k < Q; ++k) s -= L[k * 64 + i] * x[k];
+      x[i] = s / L[i * 64 + i];
+    }
+    double sb = 0.0, sas = 0.0;
+    for (int i = 0; i < Q; ++i) {
+      sb += x[i] * b[i];
+      double as = 0.0;
+      for (int k = 0; k < Q; ++k) as += A[i * Q + k] * x[k];
+      sas += x[i] * as;
+    }
+    *model_reduction = sb - 0.5 * sas;
+    *status = ok;
+  }
+}
+
+hipError_t launch_damped_solve(const double* A, const double* b, double damping, int Q, double* x,
+                               double* model_reduction, int* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_damped_solve, dim3(1), dim3(64), 0, st, A, b, damping, Q, x, model_reduction, status);
+  return hipGetLastError();
+}
+
+}  // namespace pty

@@ -8,6 +8,7 @@
 
 #include "../../include/ptyrodactyl_amd.h"
 #include "adjoint.hpp"
+#include "aux_kernels.hpp"
 #include "forward.hpp"
 
 namespace {
@@ -162,6 +163,48 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
   a.grad_slab = (float2*)grad_slab;
   a.grad_probe_k = (float2*)grad_probe_k;
   return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_adjoint");
+}
+
+int pty_transmission(const double* potential, double sigma, void* transmission, long long count, void* stream) {
+  g_err[0] = '\0';
+  if (!potential || !transmission) return fail(PTY_ERR_INVALID, "null pointer");
+  if (count < 0) return fail(PTY_ERR_INVALID, "count must be >= 0");
+  if (count == 0) return PTY_OK;
+  return device_status(pty::launch_transmission(potential, sigma, (float2*)transmission, count, (hipStream_t)stream),
+                       "pty_transmission");
+}
+
+size_t pty_normal_equations_workspace_bytes(long long num_positions, int n, int num_params) {
+  if (num_positions < 1 || n < 1 || num_params < 1 || num_params > PTY_MAX_NORMAL_PARAMS) return 0;
+  const int nprod = num_params * (num_params + 1) / 2 + num_params + 1;
+  return (size_t)pty::normal_partial_blocks(num_positions, n * n) * nprod * sizeof(double);
+}
+
+int pty_normal_equations(const void* amplitude, const float* data, long long num_positions, int n, int num_params,
+                         void* workspace, size_t workspace_bytes, double* jtj, double* jtr, double* rss,
+                         void* stream) {
+  g_err[0] = '\0';
+  if (!amplitude || !data || !jtj || !jtr || !rss) return fail(PTY_ERR_INVALID, "null pointer");
+  if (num_params < 1 || num_params > PTY_MAX_NORMAL_PARAMS)
+    return fail(PTY_ERR_INVALID, "num_params must be in [1, %d] (got %d)", PTY_MAX_NORMAL_PARAMS, num_params);
+  if (num_positions < 1 || n < 1) return fail(PTY_ERR_INVALID, "empty problem");
+  const size_t need = pty_normal_equations_workspace_bytes(num_positions, n, num_params);
+  if (!workspace || workspace_bytes < need)
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes needed", workspace_bytes, need);
+  return device_status(pty::launch_normal_equations((const float2*)amplitude, data, num_positions, n * n, num_params,
+                                                    (double*)workspace, jtj, jtr, rss, (hipStream_t)stream),
+                       "pty_normal_equations");
+}
+
+int pty_damped_solve(const double* matrix, const double* rhs, double damping, int num_params, double* solution,
+                     double* model_reduction, int* status, void* stream) {
+  g_err[0] = '\0';
+  if (!matrix || !rhs || !solution || !model_reduction || !status) return fail(PTY_ERR_INVALID, "null pointer");
+  if (num_params < 1 || num_params > 64) return fail(PTY_ERR_INVALID, "num_params must be in [1, 64]");
+  if (!(damping >= 0.0)) return fail(PTY_ERR_INVALID, "damping must be >= 0");
+  return device_status(pty::launch_damped_solve(matrix, rhs, damping, num_params, solution, model_reduction, status,
+                                                (hipStream_t)stream),
+                       "pty_damped_solve");
 }
 
 }  // extern "C"

@@ -19,7 +19,7 @@ from . import _native
 SUPPORTED_N = (32, 64, 128, 256, 512)
 # Waves (positions x modes) per streaming batch for n >= 256; the batch's
 # waves live in a device workspace that should stay cache-resident.
-DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "256"))
+DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "512"))
 
 
 def _require(cond: bool, msg: str) -> None:

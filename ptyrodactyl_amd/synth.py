@@ -130,3 +130,41 @@ def config2(scan: int = 64, step: int = 12) -> Workload:
     shifts = rng.uniform(-0.5, 0.5, size=(scan * scan, 2))
     return Workload("stem4d_256x64_p4096_slab1024", n, s_count, dz, calib, kv, 25.0, 50.0, 1e-3,
                     transmission(pot, kv), origins, shifts, probe_for(n, calib, kv, 25.0, 50.0, 1e-3))
+
+
+ABERRATIONS_TRUE = (50.0, 12.0, -8.0, 150.0, -90.0, 1.0e4, 400.0, 250.0, -1200.0, 800.0, 3000.0, -2000.0)
+
+
+@dataclass
+class AberrationWorkload:
+    n: int
+    num_slices: int
+    slice_thickness: float
+    calib: float
+    voltage_kv: float
+    aperture_mrad: float
+    slab_t: np.ndarray       # (S, H, W) complex64: the one object slice repeated S times
+    origins: np.ndarray      # (P, 2) int32
+    shifts_px: np.ndarray    # (P, 2) float64
+    coeffs_true: np.ndarray  # (12,) Angstrom
+    coeffs_init: np.ndarray  # (12,) seeded +-20 % offsets
+
+
+def config4(scan: int = 24, n: int = 128, object_px: int = 256, num_slices: int = 8) -> AberrationWorkload:
+    """Aberration fit: one 256^2 object slice (config-1 generator, 0.15 A) applied at each of 8 slices,
+    128^2 windows on a scan x scan raster, 300 kV / 20 mrad; initial coefficients offset by seeded +-20 %."""
+    calib, kv, dz = 0.15, 300.0, 1.95
+    cells = int(math.ceil(object_px * calib / LATTICE_A))
+    atoms = crystal_atoms(cells, cells, 1, seed=0)
+    pot = projected_potential(atoms, object_px, calib, 1, LATTICE_A)
+    t1 = transmission(pot, kv)[0]
+    slab = np.repeat(t1[None], num_slices, axis=0)
+    span = object_px - n
+    lin = np.round(np.linspace(0, span, scan)).astype(np.int32)
+    iy, ix = np.meshgrid(lin, lin, indexing="ij")
+    origins = np.stack([iy.ravel(), ix.ravel()], axis=1).astype(np.int32)
+    shifts = np.zeros((len(origins), 2))
+    rng = np.random.default_rng(3)
+    true = np.asarray(ABERRATIONS_TRUE, np.float64)
+    init = true * (1.0 + rng.uniform(-0.2, 0.2, size=12))
+    return AberrationWorkload(n, num_slices, dz, calib, kv, 20.0, slab, origins, shifts, true, init)

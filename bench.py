@@ -57,6 +57,23 @@ def cpu_baseline(w, sample: int, cores: int):
                       f"median of 3 after 1 warm-up; reference-algorithm CPU restatement (numpy/pocketfft)"}
 
 
+def serving_bench(count: int, qps: float):
+    """Mixed request stream (70 % forward / 30 % gradient, n in {64,128,256}, S in {8,16,32,64},
+    P in {16,256,1024,4096}, Poisson arrivals): continuous batching vs one request at a time."""
+    from ptyrodactyl_amd import serving
+    warm = serving.make_request_stream(6, qps=1e6, seed=99, scans=(16, 64))
+    serving.RequestLoop().run(warm, realtime=False)
+    serving.PlainExecutor().run(serving.make_request_stream(6, qps=1e6, seed=99, scans=(16, 64)), realtime=False)
+    plain = serving.PlainExecutor().run(serving.make_request_stream(count, qps, seed=5))
+    print(f"# serving plain {plain}", file=sys.stderr, flush=True)
+    loop = serving.RequestLoop().run(serving.make_request_stream(count, qps, seed=5))
+    print(f"# serving batched {loop}", file=sys.stderr, flush=True)
+    return {"requests": count, "qps": qps, "mix": "70% forward / 30% gradient; n {64,128,256}; S {8,16,32,64}; "
+                                                  "P {16,256,1024,4096}; Poisson arrivals",
+            "batched": loop, "plain": plain,
+            "speedup": round(loop["patterns_per_s"] / plain["patterns_per_s"], 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +82,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-waves", type=int, default=None)
+    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-qps", type=float, default=60.0)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--workload", choices=("config2", "config1"), default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
                          "config1: 128^2 x 20 slices, 256 positions (the reference's CPU-runnable case)")
@@ -140,6 +160,10 @@ def main() -> int:
                      "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
                      "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
     }
+    if not args.no_serving and args.workload == "config2":
+        del slab, out
+        torch.cuda.empty_cache()
+        result["serving"] = serving_bench(args.serving_requests, args.serving_qps)
     if not args.no_cpu_baseline:
         cores = len(os.sched_getaffinity(0))
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_sample, cores)

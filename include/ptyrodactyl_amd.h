@@ -135,6 +135,41 @@ int pty_normal_equations(const void* amplitude, const float* data, long long num
 int pty_damped_solve(const double* matrix, const double* rhs, double damping, int num_params, double* solution,
                      double* model_reduction, int* status, void* stream);
 
+/*
+ * Per-position descriptor of a batched launch that mixes scan positions of
+ * several requests with the same window n and slice count S (one probe mode
+ * each).  Offsets are element offsets into the arenas passed alongside.
+ */
+typedef struct pty_position_desc {
+  long long slab_offset;    /* complex64: the request's slab [S][slab_h][slab_w] in slab_arena */
+  long long slab_h, slab_w; /* the request's slab extent (each >= n) */
+  long long pattern_offset; /* float32: this position's [n][n] pattern (and dL/dI) in the pattern arena */
+  long long probe_offset;   /* complex64: the request's probe_k [n][n] in probe_arena */
+  long long prop_offset;    /* complex64: the request's prop_y[n] followed by prop_x[n] in prop_arena */
+  long long grad_offset;    /* complex64: the request's slab gradient in grad_slab_arena (adjoint only) */
+  long long gprobe_offset;  /* complex64: the request's probe_k gradient in grad_probe_arena (adjoint only) */
+  int32_t origin_y, origin_x;
+  double shift_y, shift_x;  /* probe shift in pixels */
+} pty_position_desc;
+
+/*
+ * pty_forward over positions of several requests in ONE launch sequence
+ * (continuous batching).  descs: DEVICE array [num_positions]; the caller
+ * validates offsets against its arenas.  checkpoints (optional, k = ckpt_every)
+ * are indexed by the call-local position as in pty_forward_checkpointed.
+ * Each position's result is bitwise identical to running its request alone.
+ */
+int pty_forward_batched(int n, int num_slices, long long num_positions, const pty_position_desc* descs,
+                        const void* slab_arena, const void* probe_arena, const void* prop_arena, float* pattern_arena,
+                        void* checkpoints, int ckpt_every, void* workspace, size_t workspace_bytes, void* stream);
+
+/* pty_adjoint for a batched launch: gradients accumulate into each request's slice of the gradient arenas. */
+int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pty_position_desc* descs,
+                        const void* slab_arena, const void* probe_arena, const void* prop_arena,
+                        const float* grad_pattern_arena, const void* checkpoints, int ckpt_every,
+                        void* grad_slab_arena, void* grad_probe_arena, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

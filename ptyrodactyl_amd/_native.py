@@ -46,7 +46,17 @@ SIGNATURES = {
     "pty_normal_equations_workspace_bytes": (c_size, [c_ll, c_int, c_int]),
     "pty_normal_equations": (c_int, [c_vp, c_vp, c_ll, c_int, c_int, c_vp, c_size, c_vp, c_vp, c_vp, c_vp]),
     "pty_damped_solve": (c_int, [c_vp, c_vp, c_dbl, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "pty_forward_batched": (c_int, [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size,
+                                    c_vp]),
+    "pty_adjoint_batched": (c_int, [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
+                                    c_size, c_vp]),
 }
+
+# numpy view of pty_position_desc (include/ptyrodactyl_amd.h); 96 bytes
+POSITION_DESC_FIELDS = [("slab_offset", "<i8"), ("slab_h", "<i8"), ("slab_w", "<i8"), ("pattern_offset", "<i8"),
+                        ("probe_offset", "<i8"), ("prop_offset", "<i8"), ("grad_offset", "<i8"),
+                        ("gprobe_offset", "<i8"), ("origin_y", "<i4"), ("origin_x", "<i4"), ("shift_y", "<f8"),
+                        ("shift_x", "<f8")]
 
 
 class NativeLibraryMissing(RuntimeError):

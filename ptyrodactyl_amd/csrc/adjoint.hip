@@ -28,7 +28,7 @@ enum : int { C_PROP = 0, C_ADJPROP = 1, C_IFFT_PROBE = 2, C_FINAL_ADJ = 3, C_PRO
 // strided distribution (thread t owns x = t + T q).
 template <int N, int PRE, int POST, class Buf>
 __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, int pw, const AdjParams& a,
-                                        const WaveGeom& g, float2* seg_row, bool seg_store, const Buf& rb,
+                                        const WaveCtx& g, float2* seg_row, bool seg_store, const Buf& rb,
                                         const TwTable& twt, const float2* px) {
   constexpr int T = N / 16;
   if constexpr (PRE == PRE_IFFT) line_fft<N, true>(v, t, rb, twt);
@@ -44,22 +44,22 @@ __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, in
   }
   if (pw != PW_NONE) {
     int gy = g.oy + y;
-    if (gy >= a.f.slab_h) gy -= (int)a.f.slab_h;
-    const long long off = (long long)s * a.f.slice_stride + (long long)gy * a.f.slab_w;
-    const float2* trow = a.f.slab + off;
+    if (gy >= g.slab_h) gy -= (int)g.slab_h;
+    const long long off = (long long)s * g.stride + (long long)gy * g.slab_w;
+    const float2* trow = g.slab + off;
     if (pw == PW_FWD) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         int x = g.ox + t + T * q;
-        if (x >= a.f.slab_w) x -= (int)a.f.slab_w;
+        if (x >= g.slab_w) x -= (int)g.slab_w;
         v[q] = cmul(v[q], trow[x]);
       }
     } else {
-      float2* grow = a.grad_slab + off;
+      float2* grow = a.grad_slab + g.grad_off + off;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         int x = g.ox + t + T * q;
-        if (x >= a.f.slab_w) x -= (int)a.f.slab_w;
+        if (x >= g.slab_w) x -= (int)g.slab_w;
         const float2 gt = cmulc(v[q], seg_row[t + T * q]);  // gphi * conj(psi)
         unsafeAtomicAdd(&grow[x].x, gt.x);
         unsafeAtomicAdd(&grow[x].y, gt.y);
@@ -80,13 +80,13 @@ __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, in
 // when the result was consumed (probe gradient) and must not be stored.
 template <int N, int OP, class Buf>
 __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long w, const AdjParams& a,
-                                        const WaveGeom& g, const Buf& cb, const TwTable& twt, const float2* py,
+                                        const WaveCtx& g, const Buf& cb, const TwTable& twt, const float2* py,
                                         const float2* ramp_y, const float2* ramp_x) {
   constexpr int T = N / 16;
   const int p = (int)(w / a.f.M), m = (int)(w % a.f.M);
   if constexpr (OP == C_IFFT_PROBE) {
     const float2 rx = ramp_x ? ramp_x[x] : shift_ramp(x, N, g.mx, g.dx);
-    const float2* pk = a.f.probe_k + (size_t)m * N * N + x;
+    const float2* pk = g.probe_k + (size_t)m * N * N + x;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int ky = t + T * q;
@@ -103,8 +103,8 @@ __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long
       line_fft<N, true>(v, t, cb, twt);
       return true;
     } else if constexpr (OP == C_FINAL_ADJ) {
-      const float two_w = 2.0f * a.f.weights[m];
-      const float* gi = a.grad_i + (size_t)p * N * N + ((x + N / 2) & (N - 1));
+      const float two_w = 2.0f * (a.f.weights ? a.f.weights[m] : 1.f);
+      const float* gi = a.grad_i + g.pat_off + ((x + N / 2) & (N - 1));
 #pragma unroll
       for (int q = 0; q < 16; ++q) v[q] = cscale(v[q], two_w * gi[(size_t)((t + T * q + N / 2) & (N - 1)) * N]);
       line_fft<N, true>(v, t, cb, twt);
@@ -112,7 +112,7 @@ __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long
     } else {  // C_PROBE_GRAD
       if (a.grad_probe_k) {
         const float2 rx = ramp_x ? ramp_x[x] : shift_ramp(x, N, g.mx, g.dx);
-        float2* gp = a.grad_probe_k + (size_t)m * N * N + x;
+        float2* gp = a.grad_probe_k + g.gprobe_off + (size_t)m * N * N + x;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
           const int ky = t + T * q;
@@ -156,7 +156,7 @@ __device__ __forceinline__ int opaque(int x) {
 }
 
 template <int N, int PRE, int POST>
-__device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams& a, const WaveGeom& g, int s, int pw,
+__device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams& a, const WaveCtx& g, int s, int pw,
                                         float2* seg, bool seg_store) {
   constexpr int T = N / 16, RS = N + N / 16, G = 64 / T;
   const int tid = opaque((int)threadIdx.x);
@@ -175,7 +175,7 @@ __device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams&
 }
 
 template <int N, int OP>
-__device__ __forceinline__ void res_col(const ResShared<N>& sh, const AdjParams& a, const WaveGeom& g, long long w) {
+__device__ __forceinline__ void res_col(const ResShared<N>& sh, const AdjParams& a, const WaveCtx& g, long long w) {
   constexpr int T = N / 16, RS = N + N / 16, G = 64 / T;
   const int tid = opaque((int)threadIdx.x);
   const int lane = tid & 63, wid = tid >> 6;
@@ -226,10 +226,6 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
   float2* ramp_x = ramp_y + N;
   const int tid = threadIdx.x;
   fill_twiddles(tw, N, tid, NT);
-  for (int i = tid; i < N; i += NT) {
-    py[i] = a.f.prop_y[i];
-    px[i] = a.f.prop_x[i];
-  }
   const ResShared<N> sh{wave, tw, py, px, ramp_y, ramp_x};
   const int S = a.f.S, k = a.k;
   const int nc = num_checkpoints(S, k), C = nc + 1;
@@ -239,9 +235,11 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
 
   for (long long w = blockIdx.x; w < waves; w += gridDim.x) {
     const int p = (int)(w / a.f.M);
-    const WaveGeom g = load_geom(a.f.origins, a.f.shifts, p, N, a.f.slab_h, a.f.slab_w);
+    const WaveCtx g = resolve_wave(a.f, p);
     __syncthreads();
     for (int i = tid; i < N; i += NT) {
+      py[i] = g.prop_y[i];
+      px[i] = g.prop_x[i];
       ramp_y[i] = shift_ramp(i, N, g.my, g.dy);
       ramp_x[i] = shift_ramp(i, N, g.mx, g.dx);
     }
@@ -313,13 +311,13 @@ __global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, Wave
   __shared__ float2 tw[N];
   __shared__ float2 px[N];
   const int tid = threadIdx.x;
-  fill_twiddles(tw, N, tid, 256);
-  for (int i = tid; i < N; i += 256) px[i] = a.f.prop_x[i];
   const int wl = blockIdx.x % nb, rblk = blockIdx.x / nb;
   const long long w = w0 + wl;
   const int p = (int)(w / a.f.M);
   const int t = tid % T, lr = tid / T, y = rblk * RPB + lr;
-  const WaveGeom g = load_geom(a.f.origins, a.f.shifts, p, N, a.f.slab_h, a.f.slab_w);
+  const WaveCtx g = resolve_wave(a.f, p);
+  fill_twiddles(tw, N, tid, 256);
+  for (int i = tid; i < N; i += 256) px[i] = g.prop_x[i];
   const TwTable twt{tw};
   const RowBufWave rb{rows + lr * RS};
   __syncthreads();
@@ -349,14 +347,14 @@ __global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, Wave
   __shared__ float2 tw[N];
   __shared__ float2 py[N];
   const int tid = threadIdx.x;
-  fill_twiddles(tw, N, tid, 256);
-  for (int i = tid; i < N; i += 256) py[i] = a.f.prop_y[i];
   const int gc = tid % NC, t = tid / NC;
   const int wl = blockIdx.x % nb, cblk = blockIdx.x / nb;
   const long long w = w0 + wl;
   const int p = (int)(w / a.f.M);
   const int x = cblk * NC + gc;
-  const WaveGeom g = load_geom(a.f.origins, a.f.shifts, p, N, a.f.slab_h, a.f.slab_w);
+  const WaveCtx g = resolve_wave(a.f, p);
+  fill_twiddles(tw, N, tid, 256);
+  for (int i = tid; i < N; i += 256) py[i] = g.prop_y[i];
   const TwTable twt{tw};
   const ColBufBlk<NC> cb{buf + gc};
   __syncthreads();

@@ -68,6 +68,7 @@ pty::FwdParams make_params(int n, int num_slices, long long num_positions, int n
   a.amplitude = nullptr;
   a.ckpt = nullptr;
   a.ckpt_every = 0;
+  a.desc = nullptr;
   return a;
 }
 
@@ -205,6 +206,74 @@ int pty_damped_solve(const double* matrix, const double* rhs, double damping, in
   return device_status(pty::launch_damped_solve(matrix, rhs, damping, num_params, solution, model_reduction, status,
                                                 (hipStream_t)stream),
                        "pty_damped_solve");
+}
+
+static_assert(sizeof(pty_position_desc) == sizeof(pty::PosDesc), "descriptor layout mismatch");
+
+static int check_batched(int n, int num_slices, long long num_positions, const void* descs, const void* slab_arena,
+                         const void* probe_arena, const void* prop_arena) {
+  if (!valid_n(n)) return fail(PTY_ERR_INVALID, "window size n=%d is not one of 32, 64, 128, 256, 512", n);
+  if (num_slices < 1) return fail(PTY_ERR_INVALID, "num_slices must be >= 1 (got %d)", num_slices);
+  if (num_positions < 1 || num_positions > (1LL << 30))
+    return fail(PTY_ERR_INVALID, "num_positions must be in [1, 2^30] (got %lld)", num_positions);
+  if (!descs || !slab_arena || !probe_arena || !prop_arena) return fail(PTY_ERR_INVALID, "null input pointer");
+  return PTY_OK;
+}
+
+static pty::FwdParams batched_params(int n, int num_slices, long long num_positions, const void* descs,
+                                     const void* slab_arena, const void* probe_arena, const void* prop_arena) {
+  pty::FwdParams a = make_params(n, num_slices, num_positions, 1, slab_arena, n, n, probe_arena, nullptr, nullptr,
+                                 nullptr, prop_arena, prop_arena);
+  a.desc = (const pty::PosDesc*)descs;
+  return a;
+}
+
+int pty_forward_batched(int n, int num_slices, long long num_positions, const pty_position_desc* descs,
+                        const void* slab_arena, const void* probe_arena, const void* prop_arena, float* pattern_arena,
+                        void* checkpoints, int ckpt_every, void* workspace, size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_batched(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
+  if (rc) return rc;
+  if (!pattern_arena) return fail(PTY_ERR_INVALID, "pattern arena is null");
+  if (checkpoints && ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1");
+  const size_t need = pty::forward_workspace_bytes(n, 1, 1, 0);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
+                need);
+  pty::FwdParams a = batched_params(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
+  a.patterns = pattern_arena;
+  if (checkpoints && pty::num_checkpoints(num_slices, ckpt_every) > 0) {
+    a.ckpt = (float2*)checkpoints;
+    a.ckpt_every = ckpt_every;
+  }
+  return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_forward_batched");
+}
+
+int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pty_position_desc* descs,
+                        const void* slab_arena, const void* probe_arena, const void* prop_arena,
+                        const float* grad_pattern_arena, const void* checkpoints, int ckpt_every,
+                        void* grad_slab_arena, void* grad_probe_arena, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  g_err[0] = '\0';
+  int rc = check_batched(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
+  if (rc) return rc;
+  if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
+  if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
+    return fail(PTY_ERR_INVALID, "checkpoint buffer is null but checkpoints are needed");
+  if (!grad_pattern_arena || !grad_slab_arena) return fail(PTY_ERR_INVALID, "gradient arenas must be non-null");
+  const size_t need = pty::adjoint_workspace_bytes(n, ckpt_every, 1);
+  if (!workspace || workspace_bytes < need)
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one wave needs", workspace_bytes,
+                need);
+  pty::AdjParams a;
+  a.f = batched_params(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
+  a.grad_i = grad_pattern_arena;
+  a.ckpt = (const float2*)checkpoints;
+  a.k = ckpt_every;
+  a.grad_slab = (float2*)grad_slab_arena;
+  a.grad_probe_k = (float2*)grad_probe_arena;
+  return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream),
+                       "pty_adjoint_batched");
 }
 
 }  // extern "C"

@@ -61,11 +61,11 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const int p = blockIdx.x;
-  const WaveGeom g = load_geom(a.origins, a.shifts, p, N, a.slab_h, a.slab_w);
+  const WaveCtx g = resolve_wave(a, p);
   fill_twiddles(tw, N, tid, NT);
   for (int i = tid; i < N; i += NT) {
-    py[i] = a.prop_y[i];
-    px[i] = a.prop_x[i];
+    py[i] = g.prop_y[i];
+    px[i] = g.prop_x[i];
     ramp_y[i] = shift_ramp(i, N, g.my, g.dy);
     ramp_x[i] = shift_ramp(i, N, g.mx, g.dx);
   }
@@ -94,12 +94,12 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
         }
         if (s == 0) line_fft<N, true>(v, rt, rb, twt);  // finish the shifted probe: ifft along x
         int gy = g.oy + ry;
-        if (gy >= a.slab_h) gy -= (int)a.slab_h;
-        const float2* trow = a.slab + (long long)s * a.slice_stride + (long long)gy * a.slab_w;
+        if (gy >= g.slab_h) gy -= (int)g.slab_h;
+        const float2* trow = g.slab + (long long)s * g.stride + (long long)gy * g.slab_w;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
           int x = g.ox + rt + T * q;
-          if (x >= a.slab_w) x -= (int)a.slab_w;
+          if (x >= g.slab_w) x -= (int)g.slab_w;
           v[q] = cmul(v[q], trow[x]);
         }
         line_fft<N, false>(v, rt, rb, twt);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
         float2 v[16];
         if (s < 0) {  // probe_k column * shift ramp, inverse FFT along y
           const float2 rx = ramp_x[cx];
-          const float2* pk = a.probe_k + (size_t)m * N * N + cx;
+          const float2* pk = g.probe_k + (size_t)m * N * N + cx;
 #pragma unroll
           for (int q = 0; q < 16; ++q) {
             const int ky = ct + T * q;
@@ -143,9 +143,9 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
           for (int q = 0; q < 16; ++q) col[(ct + T * q) * RS] = v[q];
         } else {
           const int xs = (cx + N / 2) & (N - 1);
-          const float wm = a.weights[m];
+          const float wm = (a.weights ? a.weights[m] : 1.f);
           if (a.patterns) {
-            float* out = a.patterns + (size_t)p * N * N + xs;
+            float* out = a.patterns + g.pat_off + xs;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
               float* o = out + (size_t)((ct + T * q + N / 2) & (N - 1)) * N;
@@ -184,16 +184,16 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   __shared__ float2 tw[N];
   __shared__ float2 px[N];
   const int tid = threadIdx.x;
-  fill_twiddles(tw, N, tid, 256);
-  if (!(FLAGS & ROW_LAST))
-    for (int i = tid; i < N; i += 256) px[i] = a.prop_x[i];
   const int wl = blockIdx.x % nb;  // consecutive blocks -> different waves (XCD round robin)
   const int rblk = blockIdx.x / nb;
   const int w = w0 + wl;
   const int p = w / a.M;
   const int t = tid % T, lr = tid / T;
   const int y = rblk * RPB + lr;
-  const WaveGeom g = load_geom(a.origins, a.shifts, p, N, a.slab_h, a.slab_w);
+  const WaveCtx g = resolve_wave(a, p);
+  fill_twiddles(tw, N, tid, 256);
+  if (!(FLAGS & ROW_LAST))
+    for (int i = tid; i < N; i += 256) px[i] = g.prop_x[i];
   const TwTable twt{tw};
   const RowBufWave rb{rows + lr * RS};
   __syncthreads();
@@ -210,12 +210,12 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   }
   if (FLAGS & ROW_FIRST) line_fft<N, true>(v, t, rb, twt);
   int gy = g.oy + y;
-  if (gy >= a.slab_h) gy -= (int)a.slab_h;
-  const float2* trow = a.slab + (long long)s * a.slice_stride + (long long)gy * a.slab_w;
+  if (gy >= g.slab_h) gy -= (int)g.slab_h;
+  const float2* trow = g.slab + (long long)s * g.stride + (long long)gy * g.slab_w;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     int x = g.ox + t + T * q;
-    if (x >= a.slab_w) x -= (int)a.slab_w;
+    if (x >= g.slab_w) x -= (int)g.slab_w;
     v[q] = cmul(v[q], trow[x]);
   }
   line_fft<N, false>(v, t, rb, twt);
@@ -248,23 +248,23 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
   __shared__ float2 tw[N];
   __shared__ float2 py[N];
   const int tid = threadIdx.x;
-  fill_twiddles(tw, N, tid, 256);
-  if (MODE == COL_MID)
-    for (int i = tid; i < N; i += 256) py[i] = a.prop_y[i];
   const int gc = tid % NC, t = tid / NC;
   const int unit = blockIdx.x % nb;  // wave (INIT/MID) or position (FINAL) inside the batch
   const int cblk = blockIdx.x / nb;
   const int x = cblk * NC + gc;
+  const int pos = MODE == COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M;
+  const WaveCtx g = resolve_wave(a, pos);
+  fill_twiddles(tw, N, tid, 256);
+  if (MODE == COL_MID)
+    for (int i = tid; i < N; i += 256) py[i] = g.prop_y[i];
   const TwTable twt{tw};
   const ColBufBlock<NC> cb{buf + gc};
   __syncthreads();
 
   if constexpr (MODE == COL_INIT) {
-    const int w = w0 + unit;
-    const int p = w / a.M, m = w % a.M;
-    const WaveGeom g = load_geom(a.origins, a.shifts, p, N, a.slab_h, a.slab_w);
+    const int m = (w0 + unit) % a.M;
     const float2 rx = shift_ramp(x, N, g.mx, g.dx);
-    const float2* pk = a.probe_k + (size_t)m * N * N + x;
+    const float2* pk = g.probe_k + (size_t)m * N * N + x;
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
     for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
   } else {
     const int pl = unit;  // position inside the batch
-    const int p = w0 / a.M + pl;
+    const int p = pos;
     float acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.f;
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
 #pragma unroll
       for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
       line_fft<N, false>(v, t, cb, twt);
-      const float wm = a.weights[m];
+      const float wm = (a.weights ? a.weights[m] : 1.f);
 #pragma unroll
       for (int q = 0; q < 16; ++q) acc[q] += wm * cabs2(v[q]);
       if (a.amplitude) {
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
       }
     }
     if (a.patterns) {
-      float* out = a.patterns + (size_t)p * N * N;
+      float* out = a.patterns + g.pat_off;
 #pragma unroll
       for (int q = 0; q < 16; ++q) out[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = acc[q];
     }

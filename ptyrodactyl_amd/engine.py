@@ -192,3 +192,62 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
             _native.ptr(grad_probe_k), _native.ptr(workspace), ws_bytes, _native.stream_handle(dev))
     _native.check(status, "pty_adjoint")
     return grad_slab, grad_probe_k
+
+
+# ---------------------------------------------------------------------------
+# Batched launches over positions of several requests (continuous batching)
+# ---------------------------------------------------------------------------
+
+_BASES = {}
+
+
+def arena_base(device) -> torch.Tensor:
+    """A small per-device allocation used as the common base of every arena.
+
+    Descriptor offsets are (tensor address - base address) / element size, so
+    one launch (or one captured graph) can address tensors that were
+    allocated independently by different requests.
+    """
+    key = str(device)
+    if key not in _BASES:
+        _BASES[key] = torch.zeros(64, dtype=torch.float64, device=device)
+    return _BASES[key]
+
+
+def element_offset(t: torch.Tensor, base: torch.Tensor) -> int:
+    diff = int(t.data_ptr()) - int(base.data_ptr())
+    size = t.element_size()
+    _require(diff % size == 0, "tensor is not aligned to its element size relative to the arena base")
+    return diff // size
+
+
+def forward_batched(n: int, num_slices: int, descs: torch.Tensor, base: torch.Tensor,
+                    ckpt: Optional[torch.Tensor] = None, ckpt_every: int = 0,
+                    max_batch_waves: Optional[int] = None) -> None:
+    """Launch the forward over ``descs`` (device uint8 view of pty_position_desc records)."""
+    dev = base.device
+    p_count = descs.numel() // 88
+    _require(p_count >= 1 and descs.numel() == p_count * 88, "descriptor buffer must hold whole records")
+    ws_bytes = workspace_bytes(n, p_count, 1, max_batch_waves)
+    workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev) if ws_bytes else None
+    lib = _native.load()
+    b = _native.ptr(base)
+    status = lib.pty_forward_batched(int(n), int(num_slices), p_count, _native.ptr(descs), b, b, b, b,
+                                     _native.ptr(ckpt), int(ckpt_every), _native.ptr(workspace), ws_bytes,
+                                     _native.stream_handle(dev))
+    _native.check(status, "pty_forward_batched")
+
+
+def adjoint_batched(n: int, num_slices: int, descs: torch.Tensor, base: torch.Tensor, ckpt: Optional[torch.Tensor],
+                    ckpt_every: int) -> None:
+    """Adjoint over ``descs``: dL/dI at pattern offsets, gradients into each request's buffers."""
+    dev = base.device
+    p_count = descs.numel() // 88
+    lib = _native.load()
+    ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, int(ckpt_every), p_count, 1))
+    workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
+    b = _native.ptr(base)
+    status = lib.pty_adjoint_batched(int(n), int(num_slices), p_count, _native.ptr(descs), b, b, b, b,
+                                     _native.ptr(ckpt), int(ckpt_every), b, b, _native.ptr(workspace), ws_bytes,
+                                     _native.stream_handle(dev))
+    _native.check(status, "pty_adjoint_batched")

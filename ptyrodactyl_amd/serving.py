@@ -1,0 +1,371 @@
+"""Continuous-batching request loop for mixed 4D-STEM simulation and gradient requests.
+
+A request is a forward 4D-STEM simulation or a gradient evaluation
+(loss sum (sqrt(I + eps) - sqrt(I_meas))^2 and its gradient w.r.t. the
+transmission slab and probe) with its own window n, slice count S, scan
+size P and specimen.  :class:`RequestLoop` serves a stream of such requests:
+
+* admission by HBM budget: a request is admitted when it has arrived and
+  its device footprint (inputs, outputs, checkpoints) fits the budget; its
+  inputs are then built on the device and stay resident until it completes;
+* continuous batching: every step takes chunks of positions from all
+  admitted requests of each size class (n, S), in arrival order, and runs
+  them in ONE batched launch sequence (``pty_forward_batched`` /
+  ``pty_adjoint_batched``): per-position descriptors point each wave at its
+  own request's slab, probe, propagator and outputs, so requests that
+  arrive while others run join the next step;
+* size classes run on their own HIP streams so their work overlaps;
+* per-request latency = completion - arrival (queueing included), reported
+  as p50 / p99 beside patterns/s.
+
+:class:`PlainExecutor` is the comparison: the same requests one at a time,
+in arrival order, each through the ordinary per-request engine calls.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _native, autograd, engine, physics, synth
+
+EPS_SQRT = 1e-6                                        # loss regulariser: sqrt(I + eps)
+CALIB = {32: 0.25, 64: 0.2, 128: 0.15, 256: 0.1, 512: 0.1}
+SLICE_DZ = {32: 1.95, 64: 1.95, 128: 1.95, 256: 1.0, 512: 1.0}
+VOLTAGE_KV, APERTURE_MRAD, DEFOCUS, C3 = 300.0, 25.0, 50.0, 1e-3
+CKPT_EVERY = 4
+DESC_DTYPE = np.dtype(_native.POSITION_DESC_FIELDS)
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str          # "forward" or "gradient"
+    n: int
+    slices: int
+    positions: int
+    seed: int
+    arrival: float = 0.0
+    # results
+    patterns: Optional[torch.Tensor] = None
+    loss: Optional[float] = None
+    grad_slab: Optional[torch.Tensor] = None
+    grad_probe_k: Optional[torch.Tensor] = None
+    t_admit: Optional[float] = None
+    t_done: Optional[float] = None
+    inputs: Optional["RequestInputs"] = None  # kept after completion only when the loop is asked to
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+@dataclass
+class RequestInputs:
+    slab: torch.Tensor       # (S, L, L) complex64
+    probe_k: torch.Tensor    # (n, n) complex64
+    prop: torch.Tensor       # (2n,) complex64: prop_y then prop_x
+    origins: torch.Tensor    # (P, 2) int32
+    shifts: torch.Tensor     # (P, 2) float64
+    measured: Optional[torch.Tensor] = None  # (P, n, n) float32 for gradient requests
+    origins_host: Optional[np.ndarray] = None
+    shifts_host: Optional[np.ndarray] = None
+
+
+def slab_side(n: int) -> int:
+    return 2 * n
+
+
+def request_bytes(req: Request) -> int:
+    """Device footprint of a request while resident (inputs, outputs, checkpoints, gradients)."""
+    n, s, p = req.n, req.slices, req.positions
+    side = slab_side(n)
+    b = 8 * s * side * side + 8 * n * n + 16 * n + 16 * p + 4 * p * n * n
+    if req.kind == "gradient":
+        nc = (s - 1) // CKPT_EVERY
+        b += 4 * p * n * n * 2 + 8 * s * side * side + 8 * n * n + 8 * p * nc * n * n
+    return b
+
+
+def build_inputs(req: Request, device) -> RequestInputs:
+    """Per-request specimen and scan from the request's seed, built on the device.
+
+    A perovskite-like crystal (the synth generator) fills a periodic slab of
+    side 2n; the scan is a sqrt(P) x sqrt(P) raster over the slab with seeded
+    sub-pixel remainders.  Gradient requests measure the vacuum-like aperture
+    disk with Poisson noise at 1e5 electrons per pattern.
+    """
+    n, s_count, p_count = req.n, req.slices, req.positions
+    calib, dz = CALIB[n], SLICE_DZ[n]
+    side = slab_side(n)
+    rng = np.random.default_rng(1000 + req.seed)
+    cells = int(math.ceil(side * calib / synth.LATTICE_A))
+    cells_z = int(math.ceil(s_count * dz / synth.LATTICE_A)) + 1
+    atoms = synth.crystal_atoms(cells, cells, cells_z, seed=1000 + req.seed)
+    atoms[:, :2] += rng.uniform(0, synth.LATTICE_A, size=2)
+    pot = synth.projected_potential_torch(atoms, side, calib, s_count, dz, device)
+    slab = torch.empty(pot.shape, dtype=torch.complex64, device=device)
+    lib = _native.load()
+    _native.check(lib.pty_transmission(_native.ptr(pot), physics.interaction_sigma(VOLTAGE_KV), _native.ptr(slab),
+                                       pot.numel(), _native.stream_handle(device)), "pty_transmission")
+    probe = synth.probe_for(n, calib, VOLTAGE_KV, APERTURE_MRAD, DEFOCUS, C3)
+    probe_k = torch.fft.fft2(torch.from_numpy(probe).to(device)).to(torch.complex64).contiguous()
+    pf = physics.propagator_factors(n, dz, VOLTAGE_KV, calib, device=device)
+    prop = torch.cat([pf, pf]).contiguous()
+    side_pos = int(math.ceil(math.sqrt(p_count)))
+    idx = np.arange(p_count)
+    step = side / side_pos
+    pos = np.stack([(idx // side_pos) * step, (idx % side_pos) * step], axis=1) + rng.uniform(0, 1, (p_count, 2))
+    origins = np.floor(pos).astype(np.int32) - n // 2
+    shifts = pos - np.floor(pos)
+    measured = None
+    if req.kind == "gradient":
+        lam = physics.relativistic_wavelength_ang(VOLTAGE_KV)
+        q = torch.fft.fftshift(physics.fftfreq(n, calib, device=device))
+        disk = (torch.sqrt(q[:, None] ** 2 + q[None, :] ** 2) <= (APERTURE_MRAD / 1000.0) / lam).to(torch.float64)
+        dose = 1e5
+        gen = torch.Generator(device=device)
+        gen.manual_seed(2000 + req.seed)
+        counts = torch.poisson((dose * disk / disk.sum()).expand(p_count, n, n).contiguous(), generator=gen)
+        measured = (counts * (disk.sum() / dose)).to(torch.float32).contiguous()
+    return RequestInputs(slab.contiguous(), probe_k, prop, torch.from_numpy(origins).to(device),
+                         torch.from_numpy(shifts).to(device), measured, origins, shifts)
+
+
+def loss_and_grad_intensity(patterns: torch.Tensor, measured: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    """L = sum (sqrt(I + eps) - sqrt(I_meas))^2 (FP64 reduction) and dL/dI (float32)."""
+    root = torch.sqrt(patterns + EPS_SQRT)
+    diff = root - torch.sqrt(measured)
+    loss = torch.sum(diff.to(torch.float64) ** 2)
+    return loss, (diff / root).to(torch.float32).contiguous()
+
+
+def make_request_stream(count: int, qps: float, seed: int = 0, sizes=(64, 128, 256), slices=(8, 16, 32, 64),
+                        scans=(16, 256, 1024, 4096), gradient_fraction: float = 0.3) -> List[Request]:
+    """Poisson arrivals at ``qps``; 70 % forward / 30 % gradient; sizes uniform over the given sets."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(count):
+        t += rng.exponential(1.0 / qps)
+        kind = "gradient" if rng.uniform() < gradient_fraction else "forward"
+        reqs.append(Request(i, kind, int(rng.choice(sizes)), int(rng.choice(slices)), int(rng.choice(scans)),
+                            seed=int(rng.integers(0, 2**31)), arrival=t))
+    return reqs
+
+
+# ---------------------------------------------------------------------------
+# Plain per-request path
+# ---------------------------------------------------------------------------
+
+
+class PlainExecutor:
+    """One request at a time, in arrival order, through the ordinary engine calls."""
+
+    def __init__(self, device=None):
+        self.device = device or torch.device("cuda", 0)
+
+    def run_one(self, req: Request, inp: Optional[RequestInputs] = None) -> Request:
+        dev = self.device
+        inp = inp or build_inputs(req, dev)
+        one = torch.ones(1, dtype=torch.float32, device=dev)
+        prop = inp.prop[: req.n].contiguous()
+        pk = inp.probe_k[None]
+        if req.kind == "forward":
+            req.patterns, _ = engine.multislice_forward(inp.slab, pk, one, inp.origins, inp.shifts, prop)
+        else:
+            slab = inp.slab.clone().requires_grad_(True)
+            pkg = pk.clone().requires_grad_(True)
+            pat = autograd.multislice_patterns(slab, pkg, one, inp.origins, inp.shifts, prop, CKPT_EVERY)
+            loss, _ = loss_and_grad_intensity(pat, inp.measured)
+            loss.backward()
+            req.patterns, req.loss = pat.detach(), float(loss)
+            req.grad_slab, req.grad_probe_k = slab.grad, pkg.grad[0]
+        return req
+
+    def run(self, requests: List[Request], realtime: bool = True) -> Dict:
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in reqs:
+            if realtime:
+                wait = r.arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+            r.t_admit = time.perf_counter() - t0
+            self.run_one(r)
+            torch.cuda.synchronize()
+            r.t_done = time.perf_counter() - t0
+        return summarize(reqs, time.perf_counter() - t0)
+
+
+def summarize(reqs: List[Request], wall: float) -> Dict:
+    lat = np.array([r.latency for r in reqs]) * 1000.0
+    pats = sum(r.positions for r in reqs)
+    return {"requests": len(reqs), "patterns": pats, "wall_s": round(wall, 4),
+            "patterns_per_s": round(pats / wall, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3),
+                           "mean": round(float(lat.mean()), 3)}}
+
+
+# ---------------------------------------------------------------------------
+# Continuous batching
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class _Active:
+    req: Request
+    inp: RequestInputs
+    next_pos: int = 0
+    done_pos: int = 0
+    loss: float = 0.0
+
+
+@dataclass
+class RequestLoop:
+    """Continuous-batching executor (see module docstring)."""
+
+    hbm_budget_bytes: int = 64 << 30
+    keep_inputs: bool = False
+    chunk_positions: Dict[int, int] = field(default_factory=lambda: {32: 8192, 64: 8192, 128: 4096, 256: 512,
+                                                                     512: 128})
+    device: Optional[torch.device] = None
+
+    def __post_init__(self):
+        self.device = self.device or torch.device("cuda", 0)
+        self.base = engine.arena_base(self.device)
+        self.streams: Dict[Tuple[int, int], torch.cuda.Stream] = {}
+
+    def _stream(self, key) -> torch.cuda.Stream:
+        if key not in self.streams:
+            self.streams[key] = torch.cuda.Stream(device=self.device)
+        return self.streams[key]
+
+    def _descs(self, items: List[Tuple[_Active, int, int]], grad: bool, out_tensors) -> torch.Tensor:
+        """Descriptor records for (active, start, count) chunks; out_tensors[i] receives chunk i's patterns."""
+        total = sum(c for _, _, c in items)
+        rec = np.zeros(total, dtype=DESC_DTYPE)
+        k = 0
+        for (a, start, count), out in zip(items, out_tensors):
+            inp = a.inp
+            sl = slice(k, k + count)
+            rec["slab_offset"][sl] = engine.element_offset(inp.slab, self.base)
+            rec["slab_h"][sl] = inp.slab.shape[1]
+            rec["slab_w"][sl] = inp.slab.shape[2]
+            n2 = a.req.n * a.req.n
+            rec["pattern_offset"][sl] = engine.element_offset(out, self.base) + np.arange(count) * n2
+            rec["probe_offset"][sl] = engine.element_offset(inp.probe_k, self.base)
+            rec["prop_offset"][sl] = engine.element_offset(inp.prop, self.base)
+            if grad:
+                rec["grad_offset"][sl] = engine.element_offset(a.req.grad_slab, self.base)
+                rec["gprobe_offset"][sl] = engine.element_offset(a.req.grad_probe_k, self.base)
+            o = inp.origins_host[start:start + count]
+            s = inp.shifts_host[start:start + count]
+            rec["origin_y"][sl], rec["origin_x"][sl] = o[:, 0], o[:, 1]
+            rec["shift_y"][sl], rec["shift_x"][sl] = s[:, 0], s[:, 1]
+            k += count
+        return torch.from_numpy(rec.view(np.uint8).copy()).to(self.device, non_blocking=False)
+
+    def _step_class(self, key, actives: List[_Active]) -> None:
+        """One batched step of size class key = (n, S) over chunks of the given requests."""
+        n, s_count = key
+        budget = self.chunk_positions.get(n, 512)
+        fwd_items, grad_items = [], []
+        for a in actives:
+            if budget <= 0:
+                break
+            count = min(a.req.positions - a.next_pos, budget)
+            if count <= 0:
+                continue
+            (grad_items if a.req.kind == "gradient" else fwd_items).append((a, a.next_pos, count))
+            a.next_pos += count
+            budget -= count
+        dev = self.device
+        if fwd_items:
+            outs = [a.req.patterns[start:start + count] for a, start, count in fwd_items]
+            descs = self._descs(fwd_items, False, outs)
+            engine.forward_batched(n, s_count, descs, self.base)
+            for a, _, count in fwd_items:
+                a.done_pos += count
+        if grad_items:
+            outs = [a.req.patterns[start:start + count] for a, start, count in grad_items]
+            total = sum(c for _, _, c in grad_items)
+            nc = (s_count - 1) // CKPT_EVERY
+            ckpt = torch.empty((total, nc, n, n), dtype=torch.complex64, device=dev) if nc else None
+            descs = self._descs(grad_items, True, outs)
+            engine.forward_batched(n, s_count, descs, self.base, ckpt, CKPT_EVERY)
+            gis = []
+            for a, start, count in grad_items:
+                loss, gi = loss_and_grad_intensity(a.req.patterns[start:start + count],
+                                                   a.inp.measured[start:start + count])
+                a.loss += float(loss)
+                gis.append(gi)
+            adescs = self._descs(grad_items, True, gis)
+            engine.adjoint_batched(n, s_count, adescs, self.base, ckpt, CKPT_EVERY)
+            for a, _, count in grad_items:
+                a.done_pos += count
+
+    def _admit(self, req: Request, now: float) -> _Active:
+        dev = self.device
+        inp = build_inputs(req, dev)
+        req.patterns = torch.empty((req.positions, req.n, req.n), dtype=torch.float32, device=dev)
+        if req.kind == "gradient":
+            req.grad_slab = torch.zeros_like(inp.slab)
+            req.grad_probe_k = torch.zeros_like(inp.probe_k)
+        req.t_admit = now
+        return _Active(req, inp)
+
+    def run(self, requests: List[Request], realtime: bool = True) -> Dict:
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active: List[_Active] = []
+        in_use = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        finished: List[Request] = []
+        while pending or active:
+            now = time.perf_counter() - t0
+            while pending and (not realtime or pending[0].arrival <= now):
+                need = request_bytes(pending[0])
+                if active and in_use + need > self.hbm_budget_bytes:
+                    break
+                r = pending.pop(0)
+                active.append(self._admit(r, now))
+                in_use += need
+            if not active:
+                time.sleep(max(0.0, min(0.005, pending[0].arrival - now)))
+                continue
+            classes: Dict[Tuple[int, int], List[_Active]] = {}
+            for a in active:
+                if a.next_pos < a.req.positions:
+                    classes.setdefault((a.req.n, a.req.slices), []).append(a)
+            main = torch.cuda.current_stream(self.device)
+            for key, members in classes.items():
+                st = self._stream(key)
+                st.wait_stream(main)
+                with torch.cuda.stream(st):
+                    self._step_class(key, members)
+            for key in classes:
+                main.wait_stream(self._stream(key))
+            torch.cuda.synchronize()
+            now = time.perf_counter() - t0
+            still = []
+            for a in active:
+                if a.done_pos >= a.req.positions:
+                    if a.req.kind == "gradient":
+                        a.req.loss = a.loss
+                    a.req.t_done = now
+                    if self.keep_inputs:
+                        a.req.inputs = a.inp
+                    in_use -= request_bytes(a.req)
+                    finished.append(a.req)
+                else:
+                    still.append(a)
+            active = still
+        return summarize(finished, time.perf_counter() - t0)

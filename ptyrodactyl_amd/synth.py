@@ -168,3 +168,28 @@ def config4(scan: int = 24, n: int = 128, object_px: int = 256, num_slices: int 
     true = np.asarray(ABERRATIONS_TRUE, np.float64)
     init = true * (1.0 + rng.uniform(-0.2, 0.2, size=12))
     return AberrationWorkload(n, num_slices, dz, calib, kv, 20.0, slab, origins, shifts, true, init)
+
+
+def projected_potential_torch(atoms: np.ndarray, grid: int, calib: float, num_slices: int, slice_thickness: float,
+                              device, width: float = BLOB_WIDTH, peak_per_z: float = PEAK_PER_Z):
+    """GPU version of :func:`projected_potential` (same blobs, periodic, float64)."""
+    import torch
+    a = torch.as_tensor(atoms, dtype=torch.float64, device=device)
+    sl = torch.floor(a[:, 2] / slice_thickness).to(torch.int64)
+    keep = (sl >= 0) & (sl < num_slices)
+    a, sl = a[keep], sl[keep]
+    fov = grid * calib
+    y = torch.remainder(a[:, 0], fov) / calib
+    x = torch.remainder(a[:, 1], fov) / calib
+    radius = int(math.ceil(5 * width / calib))
+    offs = torch.arange(-radius, radius + 1, device=device)
+    py = torch.floor(y).to(torch.int64)[:, None] + offs[None, :]
+    px = torch.floor(x).to(torch.int64)[:, None] + offs[None, :]
+    gy = torch.exp(-((py - y[:, None]) * calib) ** 2 / (2 * width**2))
+    gx = torch.exp(-((px - x[:, None]) * calib) ** 2 / (2 * width**2))
+    stamps = (peak_per_z * a[:, 3])[:, None, None] * gy[:, :, None] * gx[:, None, :]
+    flat = (sl[:, None, None] * grid * grid + torch.remainder(py, grid)[:, :, None] * grid
+            + torch.remainder(px, grid)[:, None, :])
+    pot = torch.zeros(num_slices * grid * grid, dtype=torch.float64, device=device)
+    pot.index_add_(0, flat.reshape(-1), stamps.reshape(-1))
+    return pot.reshape(num_slices, grid, grid)

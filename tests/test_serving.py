@@ -1,0 +1,66 @@
+"""Request loop: CPU checks of the request model and GPU checks that every
+result is the same alone and in the mix, and that admission respects the
+HBM budget."""
+
+import numpy as np
+import pytest
+import torch
+
+from ptyrodactyl_amd import serving
+
+
+def test_request_stream_mix_and_arrivals():
+    reqs = serving.make_request_stream(400, qps=50.0, seed=1)
+    frac = np.mean([r.kind == "gradient" for r in reqs])
+    assert 0.22 < frac < 0.38
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0)
+    assert 400 / arr[-1] == pytest.approx(50.0, rel=0.2)
+    assert {r.n for r in reqs} == {64, 128, 256}
+    assert {r.slices for r in reqs} == {8, 16, 32, 64}
+    assert {r.positions for r in reqs} == {16, 256, 1024, 4096}
+
+
+def test_request_bytes_grow_with_size():
+    a = serving.Request(0, "forward", 64, 8, 16, 0)
+    b = serving.Request(1, "forward", 256, 64, 4096, 0)
+    g = serving.Request(2, "gradient", 256, 64, 4096, 0)
+    assert 0 < serving.request_bytes(a) < serving.request_bytes(b) < serving.request_bytes(g)
+
+
+def _mix():
+    specs = [("forward", 64, 8, 16), ("gradient", 64, 8, 16), ("forward", 128, 16, 64), ("gradient", 128, 8, 40),
+             ("forward", 256, 8, 24), ("gradient", 256, 8, 12), ("forward", 64, 8, 300), ("forward", 128, 16, 20),
+             ("gradient", 64, 16, 33), ("forward", 256, 8, 9)]
+    return [serving.Request(i, k, n, s, p, seed=10 + i, arrival=0.001 * i) for i, (k, n, s, p) in enumerate(specs)]
+
+
+@pytest.mark.gpu
+def test_results_same_alone_and_in_mix():
+    loop = serving.RequestLoop(hbm_budget_bytes=256 << 20, chunk_positions={64: 128, 128: 48, 256: 16},
+                               keep_inputs=True)
+    mixed = _mix()
+    stats = loop.run(mixed, realtime=False)
+    assert stats["requests"] == len(mixed)
+    plain = serving.PlainExecutor()
+    for r in mixed:
+        alone = plain.run_one(serving.Request(r.rid, r.kind, r.n, r.slices, r.positions, r.seed), r.inputs)
+        print(f"request {r.rid} {r.kind} n={r.n} S={r.slices} P={r.positions}", flush=True)
+        assert torch.equal(r.patterns, alone.patterns), f"patterns differ for request {r.rid}"
+        if r.kind == "gradient":
+            assert r.loss == pytest.approx(alone.loss, rel=1e-9)
+            for got, ref in ((r.grad_slab, alone.grad_slab), (r.grad_probe_k, alone.grad_probe_k)):
+                err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+                assert err < 1e-5, f"gradient rel-L2 {err:.2e} for request {r.rid}"
+
+
+@pytest.mark.gpu
+def test_admission_respects_hbm_budget():
+    reqs = _mix()
+    budget = max(serving.request_bytes(r) for r in reqs) + 1
+    loop = serving.RequestLoop(hbm_budget_bytes=budget)
+    stats = loop.run(reqs, realtime=False)
+    assert stats["requests"] == len(reqs)
+    # requests were admitted one at a time when the budget only fits one of the largest
+    admits = sorted(r.t_admit for r in reqs)
+    assert admits[-1] > admits[0]

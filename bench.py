@@ -85,7 +85,7 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-qps", type=float, default=60.0)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--workload", choices=("config2", "config1"), default="config2",
+    ap.add_argument("--workload", choices=("config2", "config1", "serving"), default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
                          "config1: 128^2 x 20 slices, 256 positions (the reference's CPU-runnable case)")
     args = ap.parse_args()
@@ -97,6 +97,12 @@ def main() -> int:
     import torch
 
     from ptyrodactyl_amd import engine, physics, synth
+
+    if args.workload == "serving":
+        res = serving_bench(args.serving_requests, args.serving_qps)
+        print(json.dumps({"metric": "4D-STEM patterns/s, mixed request stream", "value": res["batched"]["patterns_per_s"],
+                          "unit": "patterns/s", "n_gpus": 1, "higher_is_better": True, "serving": res}), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     w = synth.config2() if args.workload == "config2" else synth.config1()

@@ -193,3 +193,51 @@ def projected_potential_torch(atoms: np.ndarray, grid: int, calib: float, num_sl
     pot = torch.zeros(num_slices * grid * grid, dtype=torch.float64, device=device)
     pot.index_add_(0, flat.reshape(-1), stamps.reshape(-1))
     return pot.reshape(num_slices, grid, grid)
+
+
+@dataclass
+class GradientWorkload:
+    n: int
+    num_slices: int
+    slice_thickness: float
+    calib: float
+    voltage_kv: float
+    object_t: np.ndarray     # (S, H, W) complex64 object to differentiate at (+-5 % amplitude noise)
+    object_true: np.ndarray  # (S, H, W) complex64 unperturbed object
+    probe: np.ndarray        # (n, n) complex128
+    origins: np.ndarray
+    shifts_px: np.ndarray
+    measured: np.ndarray     # (P, n, n) float64 fftshifted Poisson intensities (units of the simulation)
+    ckpt_every: int = 4
+
+
+def config3(scan: int = 32, n: int = 128, object_px: int = 512, num_slices: int = 16) -> GradientWorkload:
+    """Gradient workload: 16 object slices of 512^2 (config-1 generator at 0.15 A) with seeded +-5 %
+    amplitude noise, 128^2 probe at 300 kV / 20 mrad / C1 = 30 A, 32 x 32 scan with seeded +-0.2 px
+    jitter; measured intensities from the unperturbed object with Poisson noise at 1e5 e- per pattern
+    (computed by the caller-supplied forward; filled in by :func:`config3_measure`)."""
+    calib, kv, dz = 0.15, 300.0, 1.95
+    cells = int(math.ceil(object_px * calib / LATTICE_A))
+    atoms = crystal_atoms(cells, cells, int(math.ceil(num_slices * dz / LATTICE_A)) + 1, seed=0)
+    pot = projected_potential(atoms, object_px, calib, num_slices, dz)
+    t_true = transmission(pot, kv)
+    rng = np.random.default_rng(2)
+    amp = 1.0 + rng.uniform(-0.05, 0.05, size=t_true.shape)
+    t_obj = (t_true * amp).astype(np.complex64)
+    span = object_px - n
+    lin = np.linspace(0, span, scan)
+    iy, ix = np.meshgrid(lin, lin, indexing="ij")
+    pos = np.stack([iy.ravel(), ix.ravel()], axis=1) + rng.uniform(-0.2, 0.2, size=(scan * scan, 2))
+    origins = np.floor(pos).astype(np.int32)
+    shifts = pos - origins
+    probe = probe_for(n, calib, kv, 20.0, 30.0, 0.0)
+    return GradientWorkload(n, num_slices, dz, calib, kv, t_obj, t_true, probe, origins, shifts,
+                            np.zeros((0, n, n)))
+
+
+def poisson_measure(intensity: np.ndarray, dose: float = 1e5, seed: int = 2) -> np.ndarray:
+    """Poisson counts at `dose` electrons per pattern, rescaled back to simulation units."""
+    rng = np.random.default_rng(seed)
+    total = intensity.sum(axis=(-2, -1), keepdims=True)
+    counts = rng.poisson(dose * intensity / total)
+    return counts * total / dose

@@ -133,3 +133,62 @@ def test_gradient_matches_central_differences():
             fd = (loss(sp) - loss(sm)) / (2 * h)
             got = g_slab[idx].real if direction == 1.0 else g_slab[idx].imag
             assert got == pytest.approx(fd, rel=1e-3, abs=1e-4 * np.abs(g_slab).max())
+
+
+def test_config3_full_gradient():
+    """Gradient workload at full size: 16 x 512^2 object, 128^2 probe, 32 x 32 jittered scan,
+    sqrt-intensity loss against Poisson data, checkpoints every 4 slices.  GPU gradients w.r.t. all
+    slices and the probe against the oracle's reverse mode (rel-L2) and central FD (h = 1e-3)."""
+    from ptyrodactyl_amd import autograd, physics, synth
+    w = synth.config3()
+    n = w.n
+    prop = o.propagation_func(n, n, w.slice_thickness, w.voltage_kv, w.calib)
+    clean = o.window_stem_batched(w.object_true, w.probe, w.origins, w.shifts_px, prop, workers=8)
+    measured = synth.poisson_measure(clean)
+    eps = 1e-6
+    dev = torch.device("cuda", 0)
+    obj = torch.tensor(w.object_t, device=dev, requires_grad=True)
+    probe = torch.tensor(w.probe, device=dev, requires_grad=True)
+    pk = torch.fft.fft2(probe).to(torch.complex64)[None].contiguous()
+    print("config3 forward+adjoint on GPU", flush=True)
+    pat = autograd.multislice_patterns(obj, pk, torch.ones(1, device=dev), torch.tensor(w.origins, device=dev),
+                                       torch.tensor(w.shifts_px, device=dev),
+                                       physics.propagator_factors(n, w.slice_thickness, w.voltage_kv, w.calib,
+                                                                  device=dev), ckpt_every=w.ckpt_every)
+    meas_t = torch.tensor(measured, dtype=torch.float32, device=dev)
+    loss = ((torch.sqrt(pat + eps) - torch.sqrt(meas_t)) ** 2).sum()
+    loss.backward()
+    g_obj, g_probe = obj.grad.cpu().numpy(), probe.grad.cpu().numpy()
+    # oracle reverse mode with the same upstream gradient dL/dI = (sqrt(I+eps) - sqrt(M)) / sqrt(I+eps)
+    inten = o.window_stem_batched(w.object_t.astype(np.complex128), w.probe, w.origins, w.shifts_px, prop,
+                                  workers=8)
+    root = np.sqrt(inten + eps)
+    gi = (root - np.sqrt(measured)) / root
+    print("config3 oracle reverse mode", flush=True)
+    r_obj, r_probe = o.multislice_vjp(w.object_t.astype(np.complex128), w.probe, w.origins, w.shifts_px, prop, gi)
+    e_obj, e_probe = rel_l2(g_obj, r_obj), rel_l2(g_probe, r_probe)
+    print(f"config3 gradient rel-L2: slices {e_obj:.2e}  probe {e_probe:.2e}", flush=True)
+    assert e_obj < TOL and e_probe < TOL
+
+    # central differences of the complex128 oracle loss on random slab pixels (real and imaginary)
+    rng = np.random.default_rng(64)
+    h = 1e-3
+    base = w.object_t.astype(np.complex128)
+    worst = 0.0
+    for _ in range(8):
+        s, y, x = rng.integers(0, w.num_slices), rng.integers(64, 448), rng.integers(64, 448)
+        covers = np.nonzero(((y - w.origins[:, 0]) % 512 < n) & ((x - w.origins[:, 1]) % 512 < n))[0]
+
+        def loss_at(sl):
+            i = o.window_stem_batched(sl, w.probe, w.origins[covers], w.shifts_px[covers], prop, workers=8)
+            return float(np.sum((np.sqrt(i + eps) - np.sqrt(measured[covers])) ** 2))
+
+        for direction in (1.0, 1j):
+            sp, sm = base.copy(), base.copy()
+            sp[s, y, x] += h * direction
+            sm[s, y, x] -= h * direction
+            fd = (loss_at(sp) - loss_at(sm)) / (2 * h)
+            got = g_obj[s, y, x].real if direction == 1.0 else g_obj[s, y, x].imag
+            worst = max(worst, abs(got - fd) / (abs(fd) + 1e-3 * np.abs(g_obj).max()))
+    print(f"config3 FD check: worst relative deviation {worst:.2e} over 16 directional derivatives", flush=True)
+    assert worst < 1e-2

@@ -91,3 +91,21 @@ def test_aberration_lm_matches_fp64_oracle_small():
     print("residual history GPU", [f"{h:.3e}" for h in fit.history[:6]], "oracle", [f"{h:.3e}" for h in hist[:6]])
     assert np.max(err_oracle) < 1e-3
     assert np.max(err_true) < 1e-3
+
+
+def test_config4_full_against_fp64_oracle_fixture():
+    """Aberration fit workload at full size (24x24 positions, 128^2 windows, 8 slices, 256^2 object,
+    30 iterations) against the FP64 oracle run stored in tests/golden/config4_fp64_lm.npz."""
+    from pathlib import Path
+    from ptyrodactyl_amd import synth
+    ref = np.load(Path(__file__).resolve().parent / "golden" / "config4_fp64_lm.npz")
+    w = synth.config4()
+    assert np.allclose(ref["coeffs_init"], w.coeffs_init) and np.allclose(ref["coeffs_true"], w.coeffs_true)
+    data = _oracle_data(w)
+    fit = _fit(w, data, iterations=30)
+    got = fit.coefficients.cpu().numpy()
+    err = np.abs(got - ref["coefficients"]) / np.abs(w.coeffs_true)
+    print("config4 GPU vs FP64 oracle, relative per coefficient:", np.array2string(err, precision=2), flush=True)
+    print(f"config4 max relative parameter error vs FP64 oracle {err.max():.2e}; residual {fit.residual_norm:.3e}"
+          f" after {fit.iterations} iterations", flush=True)
+    assert err.max() < 1e-3

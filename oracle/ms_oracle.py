@@ -626,3 +626,50 @@ def lm_aberration_fit(slab_t, origins, shifts_px, data, coeffs_init, calib_ang, 
             norm = np.sqrt(rss)
         hist.append(norm)
     return theta * unit, hist
+
+
+# ---------------------------------------------------------------------------
+# Atom-based slice provider and stem4d_sharded (``multislice/parallelized.py``)
+# ---------------------------------------------------------------------------
+
+def slice_potential_from_atoms(atom_coords, atom_types, z_min, z_max, atom_potentials, grid_shape, calib_ang,
+                               atom_mask=None):
+    """``_compute_slice_potential`` (``parallelized.py:54-169``).
+
+    Atoms with z_min <= z < z_max are scattered to a delta grid per type at
+    (floor(y/calib) % H, floor(x/calib) % W) (coordinates are (x, y, z)) and
+    FFT-convolved with that type's kernel; the real parts are summed.
+    """
+    h, w = grid_shape
+    coords = np.asarray(atom_coords, np.float64)
+    types = np.asarray(atom_types)
+    kern = np.asarray(atom_potentials, np.float64)
+    in_slice = ((coords[:, 2] >= z_min) & (coords[:, 2] < z_max)).astype(np.float64)
+    if atom_mask is not None:
+        in_slice = in_slice * np.asarray(atom_mask, np.float64)
+    x_idx = np.floor(coords[:, 0] / calib_ang).astype(np.int64) % w
+    y_idx = np.floor(coords[:, 1] / calib_ang).astype(np.int64) % h
+    total = np.zeros((h, w))
+    for t in range(kern.shape[0]):
+        grid = np.zeros((h, w))
+        np.add.at(grid, (y_idx, x_idx), (types == t) * in_slice)
+        total += np.real(np.fft.ifft2(np.fft.fft2(grid) * np.fft.fft2(kern[t])))
+    return total
+
+
+def stem4d_from_atoms(probe_modes, weights, atom_coords, atom_types, slice_z_bounds, atom_potentials, voltage_kv,
+                      calib_ang, scan_positions_ang, atom_mask=None):
+    """``stem4d_sharded`` (``parallelized.py:348-536``) without a mesh: (P, H, W) float64 patterns."""
+    modes = np.asarray(probe_modes, np.complex128)
+    if modes.ndim == 2:
+        modes = modes[..., None]
+    h, w = modes.shape[0], modes.shape[1]
+    bounds = np.asarray(slice_z_bounds, np.float64)
+    dz = bounds[0, 1] - bounds[0, 0]
+    trans = np.stack([transmission_func(slice_potential_from_atoms(atom_coords, atom_types, b[0], b[1],
+                                                                   atom_potentials, (h, w), calib_ang, atom_mask),
+                                        voltage_kv) for b in bounds])
+    prop = propagation_func(h, w, dz, voltage_kv, calib_ang)
+    shifted = shift_beam_fourier(modes, scan_positions_ang, calib_ang)       # (P, H, W, M)
+    amp = multislice_amplitude(trans, np.moveaxis(shifted, 3, 1), prop)       # (P, M, H, W)
+    return np.einsum("m,pmhw->phw", np.asarray(weights, np.float64), np.abs(amp) ** 2)

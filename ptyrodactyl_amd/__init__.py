@@ -7,6 +7,8 @@ runs in hand-written gfx950 HIP kernels behind the C-ABI declared in
 ``include/ptyrodactyl_amd.h``.
 """
 
+from .checked import checked_cbed_image, checked_make_probe, checked_stem4d_sharded, checked_stem_4d
+from .parallelized import AtomicSliceData, create_atomic_slice_data, stem4d_sharded
 from .physics import interaction_sigma, phase_interaction_parameter, relativistic_wavelength_ang
 from .simulations import (aberration, cbed_amplitude, cbed_image, make_probe, propagation_func,
                           shift_beam_fourier, stem_4d, stem_4d_windows, transmission_func)
@@ -21,5 +23,6 @@ __all__ = [
     "relativistic_wavelength_ang", "STEM4D", "CalibratedArray", "DetectorConfig", "EnsembleAxes",
     "MicroscopeConfig", "PotentialSlices", "ProbeModes", "RuntimeCheckError", "create_calibrated_array",
     "create_detector_config", "create_ensemble_axes", "create_microscope_config", "create_potential_slices",
-    "create_probe_modes", "create_stem4d",
+    "create_probe_modes", "create_stem4d", "checked_cbed_image", "checked_make_probe", "checked_stem4d_sharded",
+    "checked_stem_4d", "AtomicSliceData", "create_atomic_slice_data", "stem4d_sharded",
 ]

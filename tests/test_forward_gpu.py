@@ -176,3 +176,35 @@ def test_config2_subset_full_size():
     err = rel_l2(got, ref)
     print(f"config2 subset rel-L2 {err:.3e}", flush=True)
     assert err < TOL
+
+
+def test_golden_stem4d_sharded_through_hip_drop_in():
+    """stem4d_sharded (atom slice provider) reproduces the reference's golden capture on the GPU."""
+    import ptyrodactyl_amd as pa
+    ref = np.load(GOLDEN)
+    pot = o.jax_uniform_f64(42, (32, 32, 2)) * 10.0
+    mic = pa.create_microscope_config(voltage_kv=100.0, aperture_mrad=20.0, probe_shape=(32, 32))
+    det = pa.create_detector_config(real_space_calib_ang=0.2, probe_calibration_pm=20.0)
+    pm = pa.create_probe_modes(pa.make_probe(mic, det)[..., None], torch.tensor([1.0]), 0.2)
+    sample = pa.create_atomic_slice_data(
+        np.array([[2.0, 2.5, 0.25], [4.5, 5.0, 2.25], [3.0, 4.0, 1.25]]), np.zeros(3, int),
+        np.array([[0.0, 2.0], [2.0, 4.0]]), pot[..., 0][None])
+    positions = torch.tensor([[12.0, 12.0], [12.0, 20.0], [20.0, 12.0], [20.0, 20.0]], dtype=torch.float64) * 0.2
+    det2 = pa.create_detector_config(real_space_calib_ang=0.2, probe_calibration_pm=20.0,
+                                     scan_positions_ang=positions)
+    s4d = pa.stem4d_sharded(pm, sample, mic, det2)
+    assert rel_l2(s4d.data.cpu().numpy(), ref["stem4d_sharded"]) < TOL
+    chk = pa.checked_stem4d_sharded(pm, sample, mic, det2)
+    assert torch.equal(chk.data, s4d.data)
+
+
+def test_checked_wrappers_are_transparent():
+    import ptyrodactyl_amd as pa
+    rng = np.random.default_rng(7)
+    n = 64
+    pot = pa.create_potential_slices(torch.tensor(rng.uniform(0, 10, (n, n, 3))), 1.0, 0.2)
+    mic = pa.create_microscope_config(voltage_kv=200.0, aperture_mrad=20.0, probe_shape=(n, n))
+    det = pa.create_detector_config(real_space_calib_ang=0.2, scan_positions_px=[[3.0, 4.0], [10.5, 60.2]])
+    beam = pa.create_probe_modes(pa.checked_make_probe(mic, det)[..., None], [1.0], 0.2)
+    assert torch.equal(pa.checked_cbed_image(pot, beam, mic).data_array, pa.cbed_image(pot, beam, mic).data_array)
+    assert torch.equal(pa.checked_stem_4d(pot, beam, mic, det).data, pa.stem_4d(pot, beam, mic, det).data)

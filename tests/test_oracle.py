@@ -277,3 +277,18 @@ def test_threefry_known_answer():
     """Random123 known-answer vector for Threefry-2x32-20 (key=0, ctr=0)."""
     x0, x1 = o.threefry2x32((0, 0), np.array([0], np.uint32), np.array([0], np.uint32))
     assert (int(x0[0]), int(x1[0])) == (0x6B200159, 0x99BA4EFE)
+
+
+def _plan01_atoms():
+    """Atom inputs of tests/test_data/plan01_ug_capture.py:71-87 in the reference."""
+    pot, probe = _plan01_inputs()
+    coords = np.array([[2.0, 2.5, 0.25], [4.5, 5.0, 2.25], [3.0, 4.0, 1.25]])
+    return pot, probe, coords, np.zeros(3, int), np.array([[0.0, 2.0], [2.0, 4.0]]), pot[..., 0][None]
+
+
+def test_golden_stem4d_sharded_matches_reference_capture():
+    ref = np.load(GOLDEN)
+    pot, probe, coords, types, bounds, kern = _plan01_atoms()
+    pos = np.array([[12.0, 12.0], [12.0, 20.0], [20.0, 12.0], [20.0, 20.0]]) * 0.2
+    got = o.stem4d_from_atoms(probe[..., None], [1.0], coords, types, bounds, kern, 100.0, 0.2, pos)
+    assert np.max(np.abs(got - ref["stem4d_sharded"])) < 1e-12

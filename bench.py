@@ -82,6 +82,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-waves", type=int, default=None)
+    ap.add_argument("--path", choices=("streaming", "persistent"), default="streaming",
+                    help="n=256 forward: per-pass launches, or the single persistent XCD-local launch")
+    ap.add_argument("--group-waves", type=int, default=2)
+    ap.add_argument("--blocks-per-cu", type=int, default=3)
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-qps", type=float, default=60.0)
     ap.add_argument("--no-serving", action="store_true")
@@ -117,9 +121,22 @@ def main() -> int:
     p_count = len(w.origins)
     out = torch.empty((p_count, w.n, w.n), dtype=torch.float32, device=dev)
 
+    persistent = args.path == "persistent" and w.n == 256
+    if persistent:
+        lib = engine._native.load()
+        state = torch.zeros(int(lib.pty_persistent_state_bytes()) // 4, dtype=torch.int32, device=dev)
+        pws = torch.empty(int(lib.pty_persistent_workspace_bytes(w.n, args.group_waves)), dtype=torch.uint8,
+                          device=dev)
+        grid = torch.cuda.get_device_properties(dev).multi_processor_count * args.blocks_per_cu
+
     def step():
-        engine.multislice_forward(slab, pk, weights, origins, shifts, prop, patterns_out=out,
-                                  max_batch_waves=args.batch_waves)
+        if persistent:
+            engine.multislice_forward_persistent(slab, pk, weights, origins, shifts, prop, patterns_out=out,
+                                                 group_waves=args.group_waves, grid=grid, state=state,
+                                                 workspace=pws, check=False)
+        else:
+            engine.multislice_forward(slab, pk, weights, origins, shifts, prop, patterns_out=out,
+                                      max_batch_waves=args.batch_waves)
 
     for _ in range(args.warmup):
         step()
@@ -134,6 +151,9 @@ def main() -> int:
         print(f"# step {i} issued", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    if persistent and int(state[0].item()) != 0:
+        print("persistent forward reported an error flag", file=sys.stderr, flush=True)
+        return 1
     dev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     ms_per_step = wall * 1000.0 / args.steps
     value = p_count * args.steps / wall
@@ -157,7 +177,8 @@ def main() -> int:
         "data": "synthetic (seeded perovskite-like crystal, generator in ptyrodactyl_amd/synth.py)",
         "config": {"workload": w.name, "window": w.n, "slices": w.num_slices, "positions": p_count,
                    "slab": list(w.slab_t.shape[1:]), "voltage_kv": w.voltage_kv, "aperture_mrad": w.aperture_mrad,
-                   "defocus_ang": w.defocus_ang, "cs_um": w.c3_ang * 1e3},
+                   "defocus_ang": w.defocus_ang, "cs_um": w.c3_ang * 1e3,
+                   "path": "persistent" if persistent else "streaming"},
         "latency_ms": {"p50": round(float(np.percentile(dev_ms, 50)), 3),
                        "p99": round(float(np.percentile(dev_ms, 99)), 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

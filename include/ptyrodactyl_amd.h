@@ -170,6 +170,27 @@ int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pt
                         void* grad_slab_arena, void* grad_probe_arena, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* Device bytes of the state block and workspace of pty_forward_persistent. */
+size_t pty_persistent_state_bytes(void);
+size_t pty_persistent_workspace_bytes(int n, int group_waves);
+
+/*
+ * Persistent variant of pty_forward for n = 256, one probe mode: one launch
+ * runs every slice; groups of `group_waves` waves stay in one XCD's L2 while
+ * that XCD's workgroups take row and column tasks from a per-XCD ticket
+ * counter.  `grid` workgroups of 256 threads (2 per CU recommended).  `state`
+ * (pty_persistent_state_bytes, reset by the call) holds at int index 0 an
+ * error flag (bit 0: a wait timed out; bit 1: an XCD left tasks undone),
+ * then the per-XCD counters.  group_waves is at most 16.
+ * Results are bitwise equal to pty_forward.  Same multislice as pty_forward
+ * (src/ptyrodactyl/multislice/simulations.py:501-603).
+ */
+int pty_forward_persistent(int n, int num_slices, long long num_positions, const void* slab, long long slab_h,
+                           long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+                           const double* shifts_px, const void* prop_y, const void* prop_x, float* patterns,
+                           int group_waves, int grid, void* workspace, size_t workspace_bytes, int* state,
+                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 #include "adjoint.hpp"
 #include "aux_kernels.hpp"
 #include "forward.hpp"
+#include "xl2.hpp"
 
 namespace {
 
@@ -206,6 +207,36 @@ int pty_damped_solve(const double* matrix, const double* rhs, double damping, in
   return device_status(pty::launch_damped_solve(matrix, rhs, damping, num_params, solution, model_reduction, status,
                                                 (hipStream_t)stream),
                        "pty_damped_solve");
+}
+
+size_t pty_persistent_state_bytes(void) { return pty::xl2_state_bytes(); }
+
+size_t pty_persistent_workspace_bytes(int n, int group_waves) {
+  if (n != 256 || group_waves < 1) return 0;
+  return pty::xl2_workspace_bytes(n, group_waves);
+}
+
+int pty_forward_persistent(int n, int num_slices, long long num_positions, const void* slab, long long slab_h,
+                           long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+                           const double* shifts_px, const void* prop_y, const void* prop_x, float* patterns,
+                           int group_waves, int grid, void* workspace, size_t workspace_bytes, int* state,
+                           void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, 1, slab, slab_h, slab_w, probe_k, mode_weights, origins,
+                        shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (n != 256) return fail(PTY_ERR_INVALID, "the persistent forward supports n = 256 (got %d)", n);
+  if (!patterns || !state) return fail(PTY_ERR_INVALID, "patterns and state must be non-null");
+  if (group_waves < 1 || group_waves > 16) return fail(PTY_ERR_INVALID, "group_waves must be in [1, 16]");
+  if (grid < 8 || grid > 65536) return fail(PTY_ERR_INVALID, "grid must be in [8, 65536]");
+  if (!workspace || workspace_bytes < pty::xl2_workspace_bytes(n, group_waves))
+    return fail(PTY_ERR_INVALID, "workspace below pty_persistent_workspace_bytes");
+  pty::FwdParams a = make_params(n, num_slices, num_positions, 1, slab, slab_h, slab_w, probe_k, mode_weights,
+                                 origins, shifts_px, prop_y, prop_x);
+  a.patterns = patterns;
+  return device_status(pty::launch_forward_xl2(a, workspace, workspace_bytes, state, group_waves, grid,
+                                               (hipStream_t)stream),
+                       "pty_forward_persistent");
 }
 
 static_assert(sizeof(pty_position_desc) == sizeof(pty::PosDesc), "descriptor layout mismatch");

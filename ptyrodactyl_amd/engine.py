@@ -102,6 +102,60 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
     return patterns, amplitude
 
 
+PERSISTENT_GROUP_WAVES = 2
+PERSISTENT_BLOCKS_PER_CU = 3
+_STATE_ERROR_INDEX = 0
+
+
+def multislice_forward_persistent(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: torch.Tensor,
+                                  origins: torch.Tensor, shifts_px: torch.Tensor, prop: torch.Tensor, *,
+                                  patterns_out: Optional[torch.Tensor] = None,
+                                  group_waves: int = PERSISTENT_GROUP_WAVES, grid: Optional[int] = None,
+                                  state: Optional[torch.Tensor] = None, workspace: Optional[torch.Tensor] = None,
+                                  check: bool = True) -> torch.Tensor:
+    """Single-launch forward (n = 256, one probe mode) that keeps wave groups in one XCD's L2.
+
+    Same inputs and bitwise the same patterns as :func:`multislice_forward`.
+    ``state`` and ``workspace`` may be passed in to reuse them across calls;
+    with ``check`` the call waits for the launch and raises if a wait timed
+    out or an XCD left work undone.
+    """
+    dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
+                                                                        shifts_px, prop)
+    _require(n == 256 and m_count == 1, "the persistent forward supports n = 256 with one probe mode")
+    g = int(group_waves)
+    lib = _native.load()
+    if patterns_out is None:
+        patterns_out = torch.empty((p_count, n, n), dtype=torch.float32, device=dev)
+    _check_cuda(patterns_out, "patterns_out", torch.float32, 3, dev)
+    _require(tuple(patterns_out.shape) == (p_count, n, n), "patterns_out has the wrong shape")
+    state_bytes = int(lib.pty_persistent_state_bytes())
+    if state is None:
+        state = torch.empty((state_bytes // 4,), dtype=torch.int32, device=dev)
+    _require(state.dtype == torch.int32 and state.numel() * 4 >= state_bytes, "state buffer too small")
+    ws_bytes = int(lib.pty_persistent_workspace_bytes(n, g))
+    _require(ws_bytes > 0, "group_waves must be >= 1")
+    if workspace is None:
+        workspace = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
+    _require(workspace.numel() * workspace.element_size() >= ws_bytes, "workspace too small")
+    if grid is None:
+        grid = torch.cuda.get_device_properties(dev).multi_processor_count * PERSISTENT_BLOCKS_PER_CU
+    with torch.cuda.device(dev):
+        status = lib.pty_forward_persistent(
+            n, s_count, p_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
+            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(patterns_out), g, int(grid), _native.ptr(workspace), ws_bytes, _native.ptr(state),
+            _native.stream_handle(dev))
+    _native.check(status, "pty_forward_persistent")
+    if check:
+        flag = int(state[_STATE_ERROR_INDEX].item())
+        if flag & 1:
+            raise RuntimeError("pty_forward_persistent: a wait timed out; the patterns are incomplete")
+        if flag & 2:
+            raise RuntimeError("pty_forward_persistent: an XCD left tasks undone (unexpected XCD numbering)")
+    return patterns_out
+
+
 def _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop):
     _require(isinstance(slab_t, torch.Tensor) and slab_t.is_cuda,
              "slab must be a GPU tensor (the multislice path has no CPU fallback)")

@@ -30,6 +30,15 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
+# HBM bytes per step of the config-2 forward (streaming path, 512-wave batches) from two rocprofv3 --pmc passes
+# (FETCH_SIZE, then WRITE_SIZE) summed over the step's launches.  FETCH_SIZE reads 256 MiB for a 512 MiB copy
+# kernel and WRITE_SIZE 512 MiB, so FETCH is scaled by 2.0 and WRITE by 1.0.  Commands and per-kernel figures:
+# profiles/forward_256x64_streaming.md.
+MEASURED_TRAFFIC = {
+    ("stem4d_256x64_p4096_slab1024", "streaming", 512): {"fetch_raw_mib": 139765.8, "fetch_scale": 2.0,
+                                                           "write_raw_mib": 263361.0, "write_scale": 1.0},
+}
+
 
 def algorithmic_bytes(n: int, s: int) -> int:
     return 8 * s * n * n + 8 * n * n + 4 * n * n
@@ -160,6 +169,11 @@ def main() -> int:
     kernel_ms = float(np.mean(dev_ms))
     bytes_step = algorithmic_bytes(w.n, w.num_slices) * p_count
     achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
+    batch = args.batch_waves or engine.DEFAULT_BATCH_WAVES
+    pmc = MEASURED_TRAFFIC.get((w.name, "persistent" if persistent else "streaming", batch))
+    traffic = None
+    if pmc is not None:
+        traffic = int((pmc["fetch_raw_mib"] * pmc["fetch_scale"] + pmc["write_raw_mib"] * pmc["write_scale"]) * 2**20)
     flops_step = algorithmic_flops(w.n, w.num_slices) * p_count
     result = {
         "metric": "4D-STEM patterns/s at 256^2 x 64 slices" if args.workload == "config2" else
@@ -182,7 +196,9 @@ def main() -> int:
         "latency_ms": {"p50": round(float(np.percentile(dev_ms, 50)), 3),
                        "p99": round(float(np.percentile(dev_ms, 99)), 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": bytes_step,
+                     "traffic_gbs": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "kernel": "multislice forward (all launches of one step)", "kernel_ms": round(kernel_ms, 3),
                      "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
                      "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},

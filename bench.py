@@ -66,6 +66,63 @@ def cpu_baseline(w, sample: int, cores: int):
                       f"median of 3 after 1 warm-up; reference-algorithm CPU restatement (numpy/pocketfft)"}
 
 
+def gradient_algorithmic_bytes(n: int, s: int, k: int) -> int:
+    """Per pattern: transmission read three times (forward, segment recompute, adjoint) and its gradient
+    written once, checkpoints written and read, probe read and probe gradient written, pattern written
+    and dL/dI read."""
+    nc = (s - 1) // k
+    return 8 * n * n * (4 * s + 2 * nc + 2) + 8 * n * n
+
+
+def gradient_bench(steps: int, warmup: int):
+    """Gradient workload (16 x 512^2 object, 128^2 probe, 32 x 32 jittered scan, sqrt-intensity loss,
+    checkpoints every 4 slices): one step = forward with checkpoints + loss gradient + adjoint over all
+    1024 positions, gradients w.r.t. every slice and the probe."""
+    import torch
+    from ptyrodactyl_amd import engine, physics, serving, synth
+    w = synth.config3()
+    dev = torch.device("cuda", 0)
+    obj = torch.from_numpy(w.object_t).to(dev)
+    pk = torch.fft.fft2(torch.from_numpy(w.probe).to(dev)).to(torch.complex64)[None].contiguous()
+    weights = torch.ones(1, dtype=torch.float32, device=dev)
+    origins = torch.from_numpy(w.origins).to(dev)
+    shifts = torch.from_numpy(w.shifts_px).to(dev)
+    prop = physics.propagator_factors(w.n, w.slice_thickness, w.voltage_kv, w.calib, device=dev)
+    # measured data: Poisson counts of the unperturbed object's patterns (GPU forward, seeded)
+    clean, _ = engine.multislice_forward(torch.from_numpy(w.object_true).to(dev), pk, weights, origins, shifts, prop)
+    measured = torch.from_numpy(synth.poisson_measure(clean.double().cpu().numpy())).to(dev, torch.float32)
+    grad_obj = torch.zeros_like(obj)
+    grad_pk = torch.zeros_like(pk)
+
+    def step():
+        grad_obj.zero_()
+        grad_pk.zero_()
+        pat, ckpt = engine.multislice_forward_checkpointed(obj, pk, weights, origins, shifts, prop, w.ckpt_every)
+        _, g = serving.loss_and_grad_intensity(pat, measured)
+        engine.multislice_adjoint(obj, pk, weights, origins, shifts, prop, g, ckpt, w.ckpt_every,
+                                  grad_slab=grad_obj, grad_probe_k=grad_pk)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * steps)]
+    for i in range(steps):
+        ev[2 * i].record()
+        step()
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    ms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)]
+    p_count = len(w.origins)
+    kernel_ms = float(np.mean(ms))
+    b = gradient_algorithmic_bytes(w.n, w.num_slices, w.ckpt_every)
+    achieved = b * p_count / (kernel_ms * 1e-3) / 1e9
+    return {"workload": "gradient_128x16_p1024_obj512_ckpt4", "patterns_per_s": round(p_count / (kernel_ms * 1e-3), 1),
+            "ms_per_step": round(kernel_ms, 3), "steps": steps,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_pattern": b,
+                         "formula": "8 n^2 (4 S + 2 floor((S-1)/k) + 2) + 8 n^2"}}
+
+
 def serving_bench(count: int, qps: float):
     """Mixed request stream (70 % forward / 30 % gradient, n in {64,128,256}, S in {8,16,32,64},
     P in {16,256,1024,4096}, Poisson arrivals): continuous batching vs one request at a time."""
@@ -98,7 +155,8 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-qps", type=float, default=60.0)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--workload", choices=("config2", "config1", "serving"), default="config2",
+    ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient"), default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
                          "config1: 128^2 x 20 slices, 256 positions (the reference's CPU-runnable case)")
     args = ap.parse_args()
@@ -115,6 +173,13 @@ def main() -> int:
         res = serving_bench(args.serving_requests, args.serving_qps)
         print(json.dumps({"metric": "4D-STEM patterns/s, mixed request stream", "value": res["batched"]["patterns_per_s"],
                           "unit": "patterns/s", "n_gpus": 1, "higher_is_better": True, "serving": res}), flush=True)
+        return 0
+
+    if args.workload == "gradient":
+        res = gradient_bench(args.steps, args.warmup)
+        print(json.dumps({"metric": "gradient patterns/s at 128^2 x 16 slices", "value": res["patterns_per_s"],
+                          "unit": "patterns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": res["ms_per_step"], "higher_is_better": True, "gradient": res}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -203,6 +268,9 @@ def main() -> int:
                      "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
                      "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
     }
+    if not args.no_gradient and args.workload == "config2":
+        result["gradient"] = gradient_bench(max(args.steps, 3), 1)
+        print(f"# gradient {result['gradient']}", file=sys.stderr, flush=True)
     if not args.no_serving and args.workload == "config2":
         del slab, out
         torch.cuda.empty_cache()

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 1
+#define PTY_ABI_VERSION 2
 
 #define PTY_OK 0
 #define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
@@ -77,8 +77,12 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
                              float* patterns, void* checkpoints, int ckpt_every, void* workspace,
                              size_t workspace_bytes, void* stream);
 
-/* Device scratch pty_adjoint needs (bounded by the number of waves kept in flight). */
-size_t pty_adjoint_workspace_bytes(int n, int ckpt_every, long long num_positions, int num_modes);
+/*
+ * Device scratch for pty_adjoint: with at least this many bytes the gradients
+ * are gathered in a fixed order (bitwise reproducible).  A smaller workspace,
+ * down to the scratch of one wave, runs with float32 atomics instead.
+ */
+size_t pty_adjoint_workspace_bytes(int n, int num_slices, int ckpt_every, long long num_positions, int num_modes);
 
 /*
  * Reverse-mode multislice: given dL/dI (float32 [num_positions][n][n], the
@@ -88,8 +92,10 @@ size_t pty_adjoint_workspace_bytes(int n, int ckpt_every, long long num_position
  * Replaces the VJP that jax.value_and_grad builds through stem_4d in the
  * reconstructors (src/ptyrodactyl/multislice/multislice_recon.py:406-434,
  * 849-885, 1400-1436, 1915-1951).  Each k-slice segment is recomputed from
- * the checkpoints of pty_forward_checkpointed.  Accumulation uses float32
- * atomics: results are reproducible to rounding, not bitwise.
+ * the checkpoints of pty_forward_checkpointed.  Each wave's contributions are
+ * stored and then summed per slab pixel (and probe element) in wave order,
+ * so the gradients are bitwise reproducible when the workspace is at least
+ * pty_adjoint_workspace_bytes; a smaller one falls back to float32 atomics.
  */
 int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
                 long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,

@@ -28,8 +28,8 @@ enum : int { C_PROP = 0, C_ADJPROP = 1, C_IFFT_PROBE = 2, C_FINAL_ADJ = 3, C_PRO
 // strided distribution (thread t owns x = t + T q).
 template <int N, int PRE, int POST, class Buf>
 __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, int pw, const AdjParams& a,
-                                        const WaveCtx& g, float2* seg_row, bool seg_store, const Buf& rb,
-                                        const TwTable& twt, const float2* px) {
+                                        const WaveCtx& g, float2* seg_row, bool seg_store, float2* gt_row,
+                                        const Buf& rb, const TwTable& twt, const float2* px) {
   constexpr int T = N / 16;
   if constexpr (PRE == PRE_IFFT) line_fft<N, true>(v, t, rb, twt);
   if constexpr (PRE == PRE_ADJPROP) {
@@ -61,8 +61,12 @@ __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, in
         int x = g.ox + t + T * q;
         if (x >= g.slab_w) x -= (int)g.slab_w;
         const float2 gt = cmulc(v[q], seg_row[t + T * q]);  // gphi * conj(psi)
-        unsafeAtomicAdd(&grow[x].x, gt.x);
-        unsafeAtomicAdd(&grow[x].y, gt.y);
+        if (gt_row) {
+          gt_row[t + T * q] = gt;  // window coordinates, gathered by k_grad_reduce
+        } else {
+          unsafeAtomicAdd(&grow[x].x, gt.x);
+          unsafeAtomicAdd(&grow[x].y, gt.y);
+        }
         v[q] = cmulc(v[q], trow[x]);  // conj(T) * gphi
       }
     }
@@ -113,13 +117,18 @@ __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long
       if (a.grad_probe_k) {
         const float2 rx = ramp_x ? ramp_x[x] : shift_ramp(x, N, g.mx, g.dx);
         float2* gp = a.grad_probe_k + g.gprobe_off + (size_t)m * N * N + x;
+        float2* gc = a.gp_contrib ? a.gp_contrib + (size_t)(w - a.w_begin) * N * N + x : nullptr;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
           const int ky = t + T * q;
           const float2 ry = ramp_y ? ramp_y[ky] : shift_ramp(ky, N, g.my, g.dy);
           const float2 val = cmulc(v[q], cmul(ry, rx));
-          unsafeAtomicAdd(&gp[(size_t)ky * N].x, val.x);
-          unsafeAtomicAdd(&gp[(size_t)ky * N].y, val.y);
+          if (gc) {
+            gc[(size_t)ky * N] = val;  // gathered by k_probe_reduce
+          } else {
+            unsafeAtomicAdd(&gp[(size_t)ky * N].x, val.x);
+            unsafeAtomicAdd(&gp[(size_t)ky * N].y, val.y);
+          }
         }
       }
       return false;
@@ -157,7 +166,7 @@ __device__ __forceinline__ int opaque(int x) {
 
 template <int N, int PRE, int POST>
 __device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams& a, const WaveCtx& g, int s, int pw,
-                                        float2* seg, bool seg_store) {
+                                        float2* seg, bool seg_store, float2* gt_wave = nullptr) {
   constexpr int T = N / 16, RS = N + N / 16, G = 64 / T;
   const int tid = opaque((int)threadIdx.x);
   const int lane = tid & 63, wid = tid >> 6;
@@ -167,7 +176,9 @@ __device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams&
   float2 v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = rb.row[pad16(rt + T * q)];
-  row_ops<N, PRE, POST>(v, rt, ry, s, pw, a, g, seg ? seg + (size_t)ry * N : nullptr, seg_store, rb, twt, sh.px);
+  float2* gt_row = gt_wave ? gt_wave + ((size_t)s * N + ry) * N : nullptr;
+  row_ops<N, PRE, POST>(v, rt, ry, s, pw, a, g, seg ? seg + (size_t)ry * N : nullptr, seg_store, gt_row, rb, twt,
+                        sh.px);
   wave_sync();
 #pragma unroll
   for (int q = 0; q < 16; ++q) rb.row[pad16(rt + T * q)] = v[q];
@@ -231,11 +242,10 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
   const int nc = num_checkpoints(S, k), C = nc + 1;
   float2* seg = scratch + (size_t)blockIdx.x * (k + 1) * N * N;
   float2* park = seg + (size_t)k * N * N;
-  const long long waves = a.f.P * a.f.M;
-
-  for (long long w = blockIdx.x; w < waves; w += gridDim.x) {
+  for (long long w = a.w_begin + blockIdx.x; w < a.w_end; w += gridDim.x) {
     const int p = (int)(w / a.f.M);
     const WaveCtx g = resolve_wave(a.f, p);
+    float2* gtw = a.gt_contrib ? a.gt_contrib + (size_t)(w - a.w_begin) * a.contrib_slices * N * N : nullptr;
     __syncthreads();
     for (int i = tid; i < N; i += NT) {
       py[i] = g.prop_y[i];
@@ -272,14 +282,14 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
       if (c == C - 1) {  // exit plane: gphi_{S-1} = F^H (2 w g A)
         res_row<N, PRE_NONE, POST_FFT>(sh, a, g, S - 1, PW_FWD, nullptr, false);
         res_col<N, C_FINAL_ADJ>(sh, a, g, w);
-        res_row<N, PRE_IFFT, POST_NONE>(sh, a, g, S - 1, PW_ADJ, seg + (size_t)(S - 1 - s0) * N * N, false);
+        res_row<N, PRE_IFFT, POST_NONE>(sh, a, g, S - 1, PW_ADJ, seg + (size_t)(S - 1 - s0) * N * N, false, gtw);
         s = S - 2;
       } else {
         res_copy<N>(sh, park, true);
       }
       for (; s >= s0; --s) {
         res_col<N, C_ADJPROP>(sh, a, g, w);
-        res_row<N, PRE_ADJPROP, POST_NONE>(sh, a, g, s, PW_ADJ, seg + (size_t)(s - s0) * N * N, false);
+        res_row<N, PRE_ADJPROP, POST_NONE>(sh, a, g, s, PW_ADJ, seg + (size_t)(s - s0) * N * N, false, gtw);
       }
     }
     res_row<N, PRE_NONE, POST_FFT>(sh, a, g, 0, PW_NONE, nullptr, false);
@@ -326,7 +336,8 @@ __global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, Wave
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = srow[t + T * q];
   float2* seg_row = segb.base ? segb.at(wl) + ((size_t)seg_idx * N + y) * N : nullptr;
-  row_ops<N, PRE, POST>(v, t, y, s, pw, a, g, seg_row, seg_store != 0, rb, twt, px);
+  float2* gt_row = a.gt_contrib ? a.gt_contrib + ((size_t)wl * N + y) * N : nullptr;  // one slice per launch
+  row_ops<N, PRE, POST>(v, t, y, s, pw, a, g, seg_row, seg_store != 0, gt_row, rb, twt, px);
   float2* drow = dst.at(wl) + (size_t)y * N;
 #pragma unroll
   for (int q = 0; q < 16; ++q) drow[t + T * q] = v[q];
@@ -372,6 +383,137 @@ __global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, Wave
 }
 
 // ---------------------------------------------------------------------------
+// Deterministic gathers of the per-wave contributions
+// ---------------------------------------------------------------------------
+
+constexpr int RED_TILE = 32;
+
+// Offset of index i in the periodic frame that starts at o (0 <= i, o < m).
+__device__ __forceinline__ int frame_offset(int i, int o, int m) {
+  const int d = i - o;
+  return d < 0 ? d + m : d;
+}
+
+// One workgroup per 32 x 32 tile of slab slice s_first + blockIdx.z.  Waves w0 .. w0 + cnt - 1 are
+// visited in order; every pixel adds the contribution of each window that covers it, so the sum
+// has a fixed order and the gradient is bitwise reproducible.  Contributions of wave w0 + i for
+// buffer slice blockIdx.z sit at gt_contrib[(i * contrib_slices + blockIdx.z) * n * n].
+template <int N>
+__global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, int cnt, int s_first) {
+  __shared__ int3 hits[256];  // (wave index in the chunk, oy, ox) of the windows meeting this tile
+  __shared__ int wave_hits[4];
+  const int sb = blockIdx.z;
+  const int H = (int)a.f.slab_h, W = (int)a.f.slab_w;
+  const int Y0 = blockIdx.y * RED_TILE, X0 = blockIdx.x * RED_TILE;
+  const int ylen = min(RED_TILE, H - Y0), xlen = min(RED_TILE, W - X0);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tx = tid % RED_TILE, ty = tid / RED_TILE;  // 8 rows per sweep, 4 sweeps
+  const int X = X0 + tx;
+  float2 acc[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = make_float2(0.f, 0.f);
+  for (int base = 0; base < cnt; base += 256) {
+    // 1. each thread tests one window against the tile; the hits are compacted in wave order
+    const int i = base + tid;
+    int oy = 0, ox = 0;
+    bool hit = false;
+    if (i < cnt) {
+      const long long p = (w0 + i) / a.f.M;
+      oy = wrap_index(a.f.origins[2 * p], H);
+      ox = wrap_index(a.f.origins[2 * p + 1], W);
+      // the tile meets the periodic window [o, o + N) if its first row / column lies inside it or the
+      // tile runs past the window's start
+      const int ey = frame_offset(Y0, oy, H), ex = frame_offset(X0, ox, W);
+      hit = (ey < N || ey + ylen > H) && (ex < N || ex + xlen > W);
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();  // the previous chunk's list has been consumed
+    if (lane == 0) wave_hits[wid] = __popcll(mask);
+    __syncthreads();
+    int offset = 0, total = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      offset += v < wid ? wave_hits[v] : 0;
+      total += wave_hits[v];
+    }
+    if (hit) hits[offset + before] = make_int3(i, oy, ox);
+    __syncthreads();
+    // 2. ascending wave order over the hits: a fixed summation order
+    if (tx < xlen) {
+#pragma unroll 4
+      for (int h = 0; h < total; ++h) {
+        const int3 e = hits[h];
+        const int dx = frame_offset(X, e.z, W);
+        // unconditional loads from clamped (in-window) addresses keep several in flight; pixels
+        // outside the window add nothing
+        const float2* cw = a.gt_contrib + ((size_t)e.x * a.contrib_slices + sb) * N * N + min(dx, N - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int yl = ty + 8 * r;
+          const int dy = frame_offset(Y0 + yl, e.y, H);
+          const float2 val = cw[min(dy, N - 1) * N];
+          if (yl < ylen && dy < N && dx < N) acc[r] = cadd(acc[r], val);
+        }
+      }
+    }
+  }
+  if (tx >= xlen) return;
+  float2* gs = a.grad_slab + (size_t)(s_first + sb) * H * W;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int yl = ty + 8 * r;
+    if (yl < ylen) {
+      const size_t o = (size_t)(Y0 + yl) * W + X;
+      gs[o] = cadd(gs[o], acc[r]);
+    }
+  }
+}
+
+// Probe gradient: element e of mode m sums gp_contrib[i][e] over the waves w0 + i of mode m.  Eight
+// interleaved partial sums keep eight loads in flight; they are combined in a fixed order.
+template <int N>
+__global__ __launch_bounds__(256) void k_probe_reduce(AdjParams a, long long w0, int cnt) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int M = a.f.M;
+  if (idx >= (long long)M * N * N) return;
+  const int m = (int)(idx / (N * N));
+  const int e = (int)(idx % (N * N));
+  const int first = (int)(((m - w0 % M) % M + M) % M);  // first i with (w0 + i) % M == m
+  const float2* src = a.gp_contrib + e;
+  const size_t stride = (size_t)M * N * N;
+  float2 part[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) part[u] = make_float2(0.f, 0.f);
+  const int count = first < cnt ? (cnt - first + M - 1) / M : 0;  // waves of mode m in the chunk
+  const float2* s0 = src + (size_t)first * N * N;
+  int j = 0;
+  for (; j + 8 <= count; j += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) part[u] = cadd(part[u], s0[(size_t)(j + u) * stride]);
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (j + u < count) part[u] = cadd(part[u], s0[(size_t)(j + u) * stride]);
+  float2 acc = part[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) acc = cadd(acc, part[u]);
+  a.grad_probe_k[idx] = cadd(a.grad_probe_k[idx], acc);
+}
+
+template <int N>
+static void launch_reduces(const AdjParams& a, long long w0, int cnt, int s_first, int s_count, bool probe,
+                           hipStream_t st) {
+  const dim3 grid((unsigned)((a.f.slab_w + RED_TILE - 1) / RED_TILE), (unsigned)((a.f.slab_h + RED_TILE - 1) / RED_TILE),
+                  (unsigned)s_count);
+  if (s_count > 0) hipLaunchKernelGGL(k_grad_reduce<N>, grid, dim3(256), 0, st, a, w0, cnt, s_first);
+  if (probe) {
+    const long long total = (long long)a.f.M * N * N;
+    hipLaunchKernelGGL(k_probe_reduce<N>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a, w0, cnt);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 
@@ -388,22 +530,40 @@ static long long resident_slots(int n) {
   return (long long)device_cus() * per_cu;
 }
 
-static long long streaming_batch(int n) { return n >= 512 ? 16 : 64; }
+// 512 waves of 256^2 (or 128 of 512^2) per streaming batch
+static long long streaming_batch(int n) { return n >= 512 ? 128 : 512; }
 
-size_t adjoint_workspace_bytes(int n, int k, long long waves) {
+// Budget for the contribution buffer of the resident path: 4 GiB of the 288 GB of HBM.
+constexpr size_t CONTRIB_BUDGET = (size_t)4 << 30;
+
+static long long balanced_chunk(long long waves, long long max_chunk) {
+  const long long chunks = (waves + max_chunk - 1) / max_chunk;
+  return (waves + chunks - 1) / chunks;
+}
+
+size_t adjoint_min_workspace_bytes(int n, int k) {
+  const size_t wave_bytes = (size_t)n * n * sizeof(float2);
+  return (size_t)(n <= 128 ? k + 1 : k + 2) * wave_bytes;
+}
+
+size_t adjoint_workspace_bytes(int n, int S, int k, long long waves) {
   const size_t wave_bytes = (size_t)n * n * sizeof(float2);
   if (n <= 128) {
     long long slots = resident_slots(n);
     if (slots > waves) slots = waves;
-    return (size_t)slots * (k + 1) * wave_bytes;
+    const size_t per_wave = (size_t)(S + 1) * wave_bytes;  // slab contributions of every slice + probe
+    long long chunk = (long long)(CONTRIB_BUDGET / per_wave);
+    if (chunk < slots) chunk = slots;
+    if (chunk > waves) chunk = waves;
+    return (size_t)slots * (k + 1) * wave_bytes + (size_t)chunk * per_wave;
   }
   long long nb = streaming_batch(n);
   if (nb > waves) nb = waves;
-  return (size_t)nb * (k + 2) * wave_bytes;  // seg[k], X, G per wave
+  return (size_t)nb * (k + 4) * wave_bytes;  // seg[k], X, G, slice and probe contributions per wave
 }
 
 template <int N>
-static hipError_t launch_adj_resident(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st) {
+static hipError_t launch_adj_resident(const AdjParams& a0, void* ws, size_t ws_bytes, hipStream_t st) {
   const size_t lds = resident_adj_lds_bytes<N>();
   static bool attr_set[64] = {false};
   int dev = 0;
@@ -414,23 +574,54 @@ static hipError_t launch_adj_resident(const AdjParams& a, void* ws, size_t ws_by
     if (e != hipSuccess) return e;
     attr_set[dev] = true;
   }
-  const long long waves = a.f.P * a.f.M;
-  const size_t per_slot = (size_t)(a.k + 1) * N * N * sizeof(float2);
-  long long slots = (long long)(ws_bytes / per_slot);
+  const long long waves = a0.f.P * a0.f.M;
+  const size_t NN = (size_t)N * N;
+  const size_t per_slot = (size_t)(a0.k + 1) * NN * sizeof(float2);
+  const size_t per_wave = (size_t)(a0.f.S + 1) * NN * sizeof(float2);
+  long long slots = resident_slots(N);
   if (slots > waves) slots = waves;
-  if (slots > resident_slots(N)) slots = resident_slots(N);
+  AdjParams a = a0;
+  a.gt_contrib = nullptr;
+  a.gp_contrib = nullptr;
+  a.contrib_slices = a0.f.S;
+  long long chunk = waves;
+  bool gathered = false;
+  if (!a0.f.desc && ws_bytes >= slots * per_slot + (size_t)slots * per_wave) {
+    gathered = true;
+    chunk = balanced_chunk(waves, (long long)((ws_bytes - slots * per_slot) / per_wave));
+  } else {
+    const long long fit = (long long)(ws_bytes / per_slot);
+    if (slots > fit) slots = fit;
+  }
   if (slots < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_adjoint_resident<N>, dim3((unsigned)slots), dim3(N * N / 16), lds, st, a, (float2*)ws);
-  return hipGetLastError();
+  float2* scratch = (float2*)ws;
+  float2* contrib = scratch + (size_t)slots * (a0.k + 1) * NN;
+  for (long long c0 = 0; c0 < waves; c0 += chunk) {
+    const long long cnt = waves - c0 < chunk ? waves - c0 : chunk;
+    a.w_begin = c0;
+    a.w_end = c0 + cnt;
+    if (gathered) {
+      a.gt_contrib = contrib;
+      a.gp_contrib = a0.grad_probe_k ? contrib + (size_t)chunk * a0.f.S * NN : nullptr;
+    }
+    const long long grid = slots < cnt ? slots : cnt;
+    hipLaunchKernelGGL(k_adjoint_resident<N>, dim3((unsigned)grid), dim3(N * N / 16), lds, st, a, scratch);
+    if (gathered) launch_reduces<N>(a, c0, (int)cnt, 0, a0.f.S, a0.grad_probe_k != nullptr, st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <int N>
-static hipError_t launch_adj_streaming(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st) {
+static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_bytes, hipStream_t st) {
   constexpr int T = N / 16, RPB = 256 / T, NC = 256 / T;
   const long long NN = (long long)N * N;
-  const int S = a.f.S, k = a.k, nc = num_checkpoints(S, k), C = nc + 1;
-  const long long waves = a.f.P * a.f.M;
-  long long nbmax = (long long)(ws_bytes / ((size_t)(k + 2) * NN * sizeof(float2)));
+  const int S = a0.f.S, k = a0.k, nc = num_checkpoints(S, k), C = nc + 1;
+  const long long waves = a0.f.P * a0.f.M;
+  // gathered (deterministic) when there are no descriptors and the workspace holds the contributions
+  const bool gathered = !a0.f.desc && ws_bytes >= (size_t)(k + 4) * NN * sizeof(float2);
+  long long nbmax = (long long)(ws_bytes / ((size_t)(k + (gathered ? 4 : 2)) * NN * sizeof(float2)));
   if (nbmax > waves) nbmax = waves;
   if (nbmax < 1) return hipErrorInvalidValue;
   if (nbmax >= 8) nbmax -= nbmax % 8;
@@ -441,6 +632,12 @@ static hipError_t launch_adj_streaming(const AdjParams& a, void* ws, size_t ws_b
     const WaveBuf X{base + (size_t)nb * k * NN, NN};
     const WaveBuf Gb{base + (size_t)nb * (k + 1) * NN, NN};
     const WaveBuf none{nullptr, 0};
+    AdjParams a = a0;
+    a.w_begin = w0;
+    a.w_end = w0 + nb;
+    a.contrib_slices = 1;
+    a.gt_contrib = gathered ? base + (size_t)nb * (k + 2) * NN : nullptr;
+    a.gp_contrib = gathered && a0.grad_probe_k ? base + (size_t)nb * (k + 3) * NN : nullptr;
     const dim3 rg(nb * (N / RPB)), cg(nb * (N / NC)), blk(256);
     for (int c = C - 1; c >= 0; --c) {
       const int s0 = c * k, s1 = min(S, s0 + k);
@@ -476,17 +673,20 @@ static hipError_t launch_adj_streaming(const AdjParams& a, void* ws, size_t ws_b
         hipLaunchKernelGGL((k_adj_cols<N, C_FINAL_ADJ>), cg, blk, 0, st, a, X, Gb, w0, nb);
         hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, S - 1,
                            (int)PW_ADJ, S - 1 - s0, 0);
+        if (gathered) launch_reduces<N>(a, w0, nb, S - 1, 1, false, st);
         s = S - 2;
       }
       for (; s >= s0; --s) {
         hipLaunchKernelGGL((k_adj_cols<N, C_ADJPROP>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
         hipLaunchKernelGGL((k_adj_rows<N, PRE_ADJPROP, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, s,
                            (int)PW_ADJ, s - s0, 0);
+        if (gathered) launch_reduces<N>(a, w0, nb, s, 1, false, st);
       }
     }
     hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, Gb, Gb, none, w0, nb, 0,
                        (int)PW_NONE, 0, 0);
     hipLaunchKernelGGL((k_adj_cols<N, C_PROBE_GRAD>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
+    if (gathered && a0.grad_probe_k) launch_reduces<N>(a, w0, nb, 0, 0, true, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

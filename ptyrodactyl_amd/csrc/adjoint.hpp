@@ -12,12 +12,22 @@ struct AdjParams {
   const float* grad_i;      // [P][n][n] dL/dI in the fftshifted pattern layout
   const float2* ckpt;       // [P*M][nc][n][n] checkpoints written by the forward (nc = (S-1)/k)
   int k;                    // checkpoint interval used by the forward
-  float2* grad_slab;        // [S][Hs][Ws] += dL/dRe + i dL/dIm of the transmission (atomic)
-  float2* grad_probe_k;     // [M][n][n]   += gradient w.r.t. probe_k (atomic), or null
+  float2* grad_slab;        // [S][Hs][Ws] += dL/dRe + i dL/dIm of the transmission
+  float2* grad_probe_k;     // [M][n][n]   += gradient w.r.t. probe_k, or null
+  // Set by the launchers.  When gt_contrib is non-null the per-wave contributions to the slab
+  // gradient are stored (window coordinates, [wave - w_begin][contrib_slices][n][n]) and gathered
+  // in a fixed order by k_grad_reduce; otherwise they are added with float32 atomics.  gp_contrib
+  // plays the same role for the probe gradient ([wave - w_begin][n][n]).
+  float2* gt_contrib;
+  float2* gp_contrib;
+  int contrib_slices;
+  long long w_begin, w_end;  // waves handled by one launch of the resident kernel
 };
 
-// Device scratch the adjoint needs for `slots` waves in flight.
-size_t adjoint_workspace_bytes(int n, int k, long long waves);
+// Device scratch for the deterministic (gathered) adjoint of `waves` waves with S slices; a
+// workspace down to adjoint_min_workspace_bytes runs with atomics instead.
+size_t adjoint_workspace_bytes(int n, int S, int k, long long waves);
+size_t adjoint_min_workspace_bytes(int n, int k);
 hipError_t launch_adjoint(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st);
 
 }  // namespace pty

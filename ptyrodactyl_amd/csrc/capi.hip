@@ -133,9 +133,9 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
                        "pty_forward_checkpointed");
 }
 
-size_t pty_adjoint_workspace_bytes(int n, int ckpt_every, long long num_positions, int num_modes) {
-  if (!valid_n(n) || ckpt_every < 1 || num_positions < 1 || num_modes < 1) return 0;
-  return pty::adjoint_workspace_bytes(n, ckpt_every, num_positions * num_modes);
+size_t pty_adjoint_workspace_bytes(int n, int num_slices, int ckpt_every, long long num_positions, int num_modes) {
+  if (!valid_n(n) || num_slices < 1 || ckpt_every < 1 || num_positions < 1 || num_modes < 1) return 0;
+  return pty::adjoint_workspace_bytes(n, num_slices, ckpt_every, num_positions * num_modes);
 }
 
 int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
@@ -152,11 +152,11 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
     return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
                 pty::num_checkpoints(num_slices, ckpt_every));
   if (!grad_patterns || !grad_slab) return fail(PTY_ERR_INVALID, "grad_patterns and grad_slab must be non-null");
-  const size_t need = pty::adjoint_workspace_bytes(n, ckpt_every, 1);
+  const size_t need = pty::adjoint_min_workspace_bytes(n, ckpt_every);
   if (!workspace || workspace_bytes < need)
     return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one wave needs", workspace_bytes,
                 need);
-  pty::AdjParams a;
+  pty::AdjParams a{};
   a.f = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights, origins,
                     shifts_px, prop_y, prop_x);
   a.grad_i = grad_patterns;
@@ -292,11 +292,11 @@ int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pt
   if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
     return fail(PTY_ERR_INVALID, "checkpoint buffer is null but checkpoints are needed");
   if (!grad_pattern_arena || !grad_slab_arena) return fail(PTY_ERR_INVALID, "gradient arenas must be non-null");
-  const size_t need = pty::adjoint_workspace_bytes(n, ckpt_every, 1);
+  const size_t need = pty::adjoint_min_workspace_bytes(n, ckpt_every);
   if (!workspace || workspace_bytes < need)
     return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one wave needs", workspace_bytes,
                 need);
-  pty::AdjParams a;
+  pty::AdjParams a{};
   a.f = batched_params(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
   a.grad_i = grad_pattern_arena;
   a.ckpt = (const float2*)checkpoints;

@@ -236,7 +236,7 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
         _check_cuda(grad_probe_k, "grad_probe_k", torch.complex64, 3, dev)
         _require(grad_probe_k.shape == probe_k.shape, "grad_probe_k must match probe_k")
     lib = _native.load()
-    ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, k, p_count, m_count))
+    ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, s_count, k, p_count, m_count))
     workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
         status = lib.pty_adjoint(
@@ -298,7 +298,7 @@ def adjoint_batched(n: int, num_slices: int, descs: torch.Tensor, base: torch.Te
     dev = base.device
     p_count = descs.numel() // 88
     lib = _native.load()
-    ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, int(ckpt_every), p_count, 1))
+    ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, int(num_slices), int(ckpt_every), p_count, 1))
     workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
     b = _native.ptr(base)
     status = lib.pty_adjoint_batched(int(n), int(num_slices), p_count, _native.ptr(descs), b, b, b, b,

@@ -3,8 +3,10 @@
 Gradients are compared with the complex128 oracle's reverse mode
 (``multislice_vjp``, itself pinned by finite differences and an independent
 torch autograd twin) at relative L2 <= 1e-5, and with central finite
-differences of the oracle loss (h = 1e-3).  Accumulation uses float32
-atomics, so results are compared with a tolerance, never bitwise.
+differences of the oracle loss (h = 1e-3).  With the full workspace the
+slab and probe gradients are gathered in a fixed order and must be bitwise
+reproducible; the atomic fallback (small workspace) is compared with a
+tolerance.
 """
 
 import numpy as np
@@ -203,3 +205,59 @@ def test_config3_full_gradient():
             worst = max(worst, abs(got - fd) / (abs(fd) + 1e-3 * np.abs(g_obj).max()))
     print(f"config3 FD check: worst relative deviation {worst:.2e} over 16 directional derivatives", flush=True)
     assert worst < 1e-2
+
+
+def _adjoint_through_abi(slab, probes, origins, shifts, g, n, k, workspace_bytes=None):
+    """pty_forward_checkpointed + pty_adjoint through the C-ABI; workspace_bytes=None takes the full size."""
+    from ptyrodactyl_amd import _native, engine, physics
+    dev = torch.device("cuda", 0)
+    slab_t = torch.tensor(slab, device=dev)
+    pk = torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous()
+    m_count = probes.shape[0]
+    wt = torch.full((m_count,), 1.0 / m_count, dtype=torch.float32, device=dev)
+    ot = torch.tensor(origins, dtype=torch.int32, device=dev)
+    st = torch.tensor(shifts, device=dev)
+    prop = physics.propagator_factors(n, 1.5, 300.0, 0.1, device=dev)
+    _, ckpt = engine.multislice_forward_checkpointed(slab_t, pk, wt, ot, st, prop, k)
+    gt = torch.tensor(g, dtype=torch.float32, device=dev)
+    lib = _native.load()
+    s_count, p_count = slab.shape[0], origins.shape[0]
+    full = int(lib.pty_adjoint_workspace_bytes(n, s_count, k, p_count, m_count))
+    nbytes = full if workspace_bytes is None else int(workspace_bytes)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    g_slab = torch.zeros_like(slab_t)
+    g_pk = torch.zeros_like(pk)
+    nc = (s_count - 1) // k
+    status = lib.pty_adjoint(n, s_count, p_count, m_count, _native.ptr(slab_t), slab.shape[1], slab.shape[2],
+                             _native.ptr(pk), _native.ptr(wt), _native.ptr(ot), _native.ptr(st), _native.ptr(prop),
+                             _native.ptr(prop), _native.ptr(gt), _native.ptr(ckpt) if nc else 0, k,
+                             _native.ptr(g_slab), _native.ptr(g_pk), _native.ptr(ws), nbytes,
+                             _native.stream_handle(dev))
+    _native.check(status, "pty_adjoint")
+    torch.cuda.synchronize()
+    return g_slab, g_pk, full
+
+
+@pytest.mark.parametrize("n,s_count,p_count,slab_hw,modes", [(128, 6, 37, (300, 280), 1), (64, 5, 300, (200, 260), 2),
+                                                          (256, 5, 21, (300, 300), 1)])
+def test_gathered_gradients_are_bitwise_reproducible(n, s_count, p_count, slab_hw, modes):
+    print(f"gathered adjoint n={n} S={s_count} P={p_count} M={modes}", flush=True)
+    slab, probes, origins, shifts, g = _case(n, s_count, p_count, slab_hw, seed=n + p_count, modes=modes)
+    a_slab, a_pk, _ = _adjoint_through_abi(slab, probes, origins, shifts, g, n, 2)
+    b_slab, b_pk, _ = _adjoint_through_abi(slab, probes, origins, shifts, g, n, 2)
+    assert torch.equal(a_slab, b_slab) and torch.equal(a_pk, b_pk)
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_atomic_fallback_matches_gathered(n):
+    """A workspace below the gathered size runs the atomic path; both sum the same contributions."""
+    from ptyrodactyl_amd import _native
+    slab, probes, origins, shifts, g = _case(n, 4, 19, (300, 290), seed=3 * n)
+    full_slab, full_pk, full = _adjoint_through_abi(slab, probes, origins, shifts, g, n, 2)
+    k = 2
+    small = (k + 3) * n * n * 8  # scratch for one wave in flight, no room for the contributions
+    assert small < full
+    at_slab, at_pk, _ = _adjoint_through_abi(slab, probes, origins, shifts, g, n, k, workspace_bytes=small)
+    assert rel_l2(at_slab.cpu().numpy(), full_slab.cpu().numpy()) < 1e-6
+    assert rel_l2(at_pk.cpu().numpy(), full_pk.cpu().numpy()) < 1e-6
+    assert _native.load().pty_abi_version() == 2

@@ -336,7 +336,8 @@ __global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, Wave
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = srow[t + T * q];
   float2* seg_row = segb.base ? segb.at(wl) + ((size_t)seg_idx * N + y) * N : nullptr;
-  float2* gt_row = a.gt_contrib ? a.gt_contrib + ((size_t)wl * N + y) * N : nullptr;  // one slice per launch
+  // contributions of the k slices of the current segment: [wave][slice - segment start][n][n]
+  float2* gt_row = a.gt_contrib ? a.gt_contrib + (((size_t)wl * a.contrib_slices + seg_idx) * N + y) * N : nullptr;
   row_ops<N, PRE, POST>(v, t, y, s, pw, a, g, seg_row, seg_store != 0, gt_row, rb, twt, px);
   float2* drow = dst.at(wl) + (size_t)y * N;
 #pragma unroll
@@ -398,20 +399,21 @@ __device__ __forceinline__ int frame_offset(int i, int o, int m) {
 // visited in order; every pixel adds the contribution of each window that covers it, so the sum
 // has a fixed order and the gradient is bitwise reproducible.  Contributions of wave w0 + i for
 // buffer slice blockIdx.z sit at gt_contrib[(i * contrib_slices + blockIdx.z) * n * n].
-template <int N>
+template <int N, int TH>
 __global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, int cnt, int s_first) {
-  __shared__ int3 hits[256];  // (wave index in the chunk, oy, ox) of the windows meeting this tile
+  constexpr int RPT = TH / 8;  // rows per thread: 8 rows per sweep of the 256 threads
+  __shared__ int3 hits[256];   // (wave index in the chunk, oy, ox) of the windows meeting this tile
   __shared__ int wave_hits[4];
   const int sb = blockIdx.z;
   const int H = (int)a.f.slab_h, W = (int)a.f.slab_w;
-  const int Y0 = blockIdx.y * RED_TILE, X0 = blockIdx.x * RED_TILE;
-  const int ylen = min(RED_TILE, H - Y0), xlen = min(RED_TILE, W - X0);
+  const int Y0 = blockIdx.y * TH, X0 = blockIdx.x * RED_TILE;
+  const int ylen = min(TH, H - Y0), xlen = min(RED_TILE, W - X0);
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int tx = tid % RED_TILE, ty = tid / RED_TILE;  // 8 rows per sweep, 4 sweeps
+  const int tx = tid % RED_TILE, ty = tid / RED_TILE;
   const int X = X0 + tx;
-  float2 acc[4];
+  float2 acc[RPT];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) acc[r] = make_float2(0.f, 0.f);
+  for (int r = 0; r < RPT; ++r) acc[r] = make_float2(0.f, 0.f);
   for (int base = 0; base < cnt; base += 256) {
     // 1. each thread tests one window against the tile; the hits are compacted in wave order
     const int i = base + tid;
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, 
         // outside the window add nothing
         const float2* cw = a.gt_contrib + ((size_t)e.x * a.contrib_slices + sb) * N * N + min(dx, N - 1);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < RPT; ++r) {
           const int yl = ty + 8 * r;
           const int dy = frame_offset(Y0 + yl, e.y, H);
           const float2 val = cw[min(dy, N - 1) * N];
@@ -461,7 +463,7 @@ __global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, 
   if (tx >= xlen) return;
   float2* gs = a.grad_slab + (size_t)(s_first + sb) * H * W;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < RPT; ++r) {
     const int yl = ty + 8 * r;
     if (yl < ylen) {
       const size_t o = (size_t)(Y0 + yl) * W + X;
@@ -504,9 +506,17 @@ __global__ __launch_bounds__(256) void k_probe_reduce(AdjParams a, long long w0,
 template <int N>
 static void launch_reduces(const AdjParams& a, long long w0, int cnt, int s_first, int s_count, bool probe,
                            hipStream_t st) {
-  const dim3 grid((unsigned)((a.f.slab_w + RED_TILE - 1) / RED_TILE), (unsigned)((a.f.slab_h + RED_TILE - 1) / RED_TILE),
-                  (unsigned)s_count);
-  if (s_count > 0) hipLaunchKernelGGL(k_grad_reduce<N>, grid, dim3(256), 0, st, a, w0, cnt, s_first);
+  if (s_count > 0) {
+    const unsigned gx = (unsigned)((a.f.slab_w + RED_TILE - 1) / RED_TILE);
+    const unsigned tiles32 = gx * (unsigned)((a.f.slab_h + 31) / 32) * (unsigned)s_count;
+    if (tiles32 >= 2048) {  // enough 32 x 32 tiles to fill the GPU
+      const dim3 grid(gx, (unsigned)((a.f.slab_h + 31) / 32), (unsigned)s_count);
+      hipLaunchKernelGGL((k_grad_reduce<N, 32>), grid, dim3(256), 0, st, a, w0, cnt, s_first);
+    } else {  // small slabs: 32 x 8 tiles, four times the workgroups
+      const dim3 grid(gx, (unsigned)((a.f.slab_h + 7) / 8), (unsigned)s_count);
+      hipLaunchKernelGGL((k_grad_reduce<N, 8>), grid, dim3(256), 0, st, a, w0, cnt, s_first);
+    }
+  }
   if (probe) {
     const long long total = (long long)a.f.M * N * N;
     hipLaunchKernelGGL(k_probe_reduce<N>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a, w0, cnt);
@@ -559,7 +569,7 @@ size_t adjoint_workspace_bytes(int n, int S, int k, long long waves) {
   }
   long long nb = streaming_batch(n);
   if (nb > waves) nb = waves;
-  return (size_t)nb * (k + 4) * wave_bytes;  // seg[k], X, G, slice and probe contributions per wave
+  return (size_t)nb * (2 * k + 3) * wave_bytes;  // seg[k], X, G, k slices of contributions, probe
 }
 
 template <int N>
@@ -620,8 +630,9 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
   const int S = a0.f.S, k = a0.k, nc = num_checkpoints(S, k), C = nc + 1;
   const long long waves = a0.f.P * a0.f.M;
   // gathered (deterministic) when there are no descriptors and the workspace holds the contributions
-  const bool gathered = !a0.f.desc && ws_bytes >= (size_t)(k + 4) * NN * sizeof(float2);
-  long long nbmax = (long long)(ws_bytes / ((size_t)(k + (gathered ? 4 : 2)) * NN * sizeof(float2)));
+  // of one k-slice segment and the probe: k + 1 more waves per wave in flight
+  const bool gathered = !a0.f.desc && ws_bytes >= (size_t)(2 * k + 3) * NN * sizeof(float2);
+  long long nbmax = (long long)(ws_bytes / ((size_t)(gathered ? 2 * k + 3 : k + 2) * NN * sizeof(float2)));
   if (nbmax > waves) nbmax = waves;
   if (nbmax < 1) return hipErrorInvalidValue;
   if (nbmax >= 8) nbmax -= nbmax % 8;
@@ -635,9 +646,9 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
     AdjParams a = a0;
     a.w_begin = w0;
     a.w_end = w0 + nb;
-    a.contrib_slices = 1;
+    a.contrib_slices = k;
     a.gt_contrib = gathered ? base + (size_t)nb * (k + 2) * NN : nullptr;
-    a.gp_contrib = gathered && a0.grad_probe_k ? base + (size_t)nb * (k + 3) * NN : nullptr;
+    a.gp_contrib = gathered && a0.grad_probe_k ? base + (size_t)nb * (2 * k + 2) * NN : nullptr;
     const dim3 rg(nb * (N / RPB)), cg(nb * (N / NC)), blk(256);
     for (int c = C - 1; c >= 0; --c) {
       const int s0 = c * k, s1 = min(S, s0 + k);
@@ -673,15 +684,14 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
         hipLaunchKernelGGL((k_adj_cols<N, C_FINAL_ADJ>), cg, blk, 0, st, a, X, Gb, w0, nb);
         hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, S - 1,
                            (int)PW_ADJ, S - 1 - s0, 0);
-        if (gathered) launch_reduces<N>(a, w0, nb, S - 1, 1, false, st);
         s = S - 2;
       }
       for (; s >= s0; --s) {
         hipLaunchKernelGGL((k_adj_cols<N, C_ADJPROP>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
         hipLaunchKernelGGL((k_adj_rows<N, PRE_ADJPROP, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, s,
                            (int)PW_ADJ, s - s0, 0);
-        if (gathered) launch_reduces<N>(a, w0, nb, s, 1, false, st);
       }
+      if (gathered) launch_reduces<N>(a, w0, nb, s0, s1 - s0, false, st);  // the segment's slices
     }
     hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, Gb, Gb, none, w0, nb, 0,
                        (int)PW_NONE, 0, 0);

@@ -125,19 +125,23 @@ def gradient_bench(steps: int, warmup: int):
 
 def serving_bench(count: int, qps: float):
     """Mixed request stream (70 % forward / 30 % gradient, n in {64,128,256}, S in {8,16,32,64},
-    P in {16,256,1024,4096}, Poisson arrivals): continuous batching vs one request at a time."""
+    P in {16,256,1024,4096}, Poisson arrivals): continuous batching vs one request at a time.
+    Throughput is taken at saturation (all requests arrive within a few ms); latency is reported both
+    there and at the paced rate `qps`."""
     from ptyrodactyl_amd import serving
     warm = serving.make_request_stream(6, qps=1e6, seed=99, scans=(16, 64))
     serving.RequestLoop().run(warm, realtime=False)
     serving.PlainExecutor().run(serving.make_request_stream(6, qps=1e6, seed=99, scans=(16, 64)), realtime=False)
-    plain = serving.PlainExecutor().run(serving.make_request_stream(count, qps, seed=5))
-    print(f"# serving plain {plain}", file=sys.stderr, flush=True)
-    loop = serving.RequestLoop().run(serving.make_request_stream(count, qps, seed=5))
-    print(f"# serving batched {loop}", file=sys.stderr, flush=True)
-    return {"requests": count, "qps": qps, "mix": "70% forward / 30% gradient; n {64,128,256}; S {8,16,32,64}; "
-                                                  "P {16,256,1024,4096}; Poisson arrivals",
-            "batched": loop, "plain": plain,
-            "speedup": round(loop["patterns_per_s"] / plain["patterns_per_s"], 3)}
+    out = {"requests": count,
+           "mix": "70% forward / 30% gradient; n {64,128,256}; S {8,16,32,64}; P {16,256,1024,4096}; Poisson arrivals"}
+    for label, rate in (("saturated", 1e4), (f"paced_{qps:g}qps", qps)):
+        plain = serving.PlainExecutor().run(serving.make_request_stream(count, rate, seed=5))
+        loop = serving.RequestLoop().run(serving.make_request_stream(count, rate, seed=5))
+        print(f"# serving {label} plain {plain}", file=sys.stderr, flush=True)
+        print(f"# serving {label} batched {loop}", file=sys.stderr, flush=True)
+        out[label] = {"qps": rate, "batched": loop, "plain": plain,
+                      "speedup": round(loop["patterns_per_s"] / plain["patterns_per_s"], 3)}
+    return out
 
 
 def main() -> int:
@@ -171,7 +175,8 @@ def main() -> int:
 
     if args.workload == "serving":
         res = serving_bench(args.serving_requests, args.serving_qps)
-        print(json.dumps({"metric": "4D-STEM patterns/s, mixed request stream", "value": res["batched"]["patterns_per_s"],
+        print(json.dumps({"metric": "4D-STEM patterns/s, mixed request stream",
+                          "value": res["saturated"]["batched"]["patterns_per_s"],
                           "unit": "patterns/s", "n_gpus": 1, "higher_is_better": True, "serving": res}), flush=True)
         return 0
 

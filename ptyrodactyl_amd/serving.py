@@ -225,7 +225,8 @@ class _Active:
     inp: RequestInputs
     next_pos: int = 0
     done_pos: int = 0
-    loss: float = 0.0
+    loss: Optional[torch.Tensor] = None  # FP64 device accumulator, read once at completion
+    ready: Optional[torch.cuda.Event] = None  # inputs built (on the builder stream)
 
 
 @dataclass
@@ -234,6 +235,7 @@ class RequestLoop:
 
     hbm_budget_bytes: int = 64 << 30
     keep_inputs: bool = False
+    admit_per_step: int = 4   # requests built per step while the GPU runs (bounds the host work between launches)
     chunk_positions: Dict[int, int] = field(default_factory=lambda: {32: 8192, 64: 8192, 128: 4096, 256: 512,
                                                                      512: 128})
     device: Optional[torch.device] = None
@@ -271,7 +273,9 @@ class RequestLoop:
             rec["origin_y"][sl], rec["origin_x"][sl] = o[:, 0], o[:, 1]
             rec["shift_y"][sl], rec["shift_x"][sl] = s[:, 0], s[:, 1]
             k += count
-        return torch.from_numpy(rec.view(np.uint8).copy()).to(self.device, non_blocking=False)
+        # pinned staging + asynchronous copy: the upload is ordered on the class stream without
+        # making the host wait for that stream's earlier launches
+        return torch.from_numpy(rec.view(np.uint8)).pin_memory().to(self.device, non_blocking=True)
 
     def _step_class(self, key, actives: List[_Active]) -> None:
         """One batched step of size class key = (n, S) over chunks of the given requests."""
@@ -305,22 +309,51 @@ class RequestLoop:
             for a, start, count in grad_items:
                 loss, gi = loss_and_grad_intensity(a.req.patterns[start:start + count],
                                                    a.inp.measured[start:start + count])
-                a.loss += float(loss)
+                a.loss = loss if a.loss is None else a.loss + loss
                 gis.append(gi)
-            adescs = self._descs(grad_items, True, gis)
-            engine.adjoint_batched(n, s_count, adescs, self.base, ckpt, CKPT_EVERY)
+            # adjoint per request chunk (each a full-GPU launch sequence): one slab per call lets the
+            # gradients be gathered in a fixed order instead of accumulated with atomics
+            one = torch.ones(1, dtype=torch.float32, device=dev)
+            k0 = 0
+            for (a, start, count), gi in zip(grad_items, gis):
+                inp = a.inp
+                engine.multislice_adjoint(inp.slab, inp.probe_k[None], one, inp.origins[start:start + count],
+                                          inp.shifts[start:start + count], inp.prop[:n].contiguous(), gi,
+                                          ckpt[k0:k0 + count] if nc else None, CKPT_EVERY,
+                                          grad_slab=a.req.grad_slab, grad_probe_k=a.req.grad_probe_k[None])
+                k0 += count
             for a, _, count in grad_items:
                 a.done_pos += count
 
     def _admit(self, req: Request, now: float) -> _Active:
+        """Build the request's inputs and outputs on the builder stream; the class streams wait for
+        the recorded event before the request's first step."""
         dev = self.device
-        inp = build_inputs(req, dev)
-        req.patterns = torch.empty((req.positions, req.n, req.n), dtype=torch.float32, device=dev)
-        if req.kind == "gradient":
-            req.grad_slab = torch.zeros_like(inp.slab)
-            req.grad_probe_k = torch.zeros_like(inp.probe_k)
+        builder = self._stream("build")
+        with torch.cuda.stream(builder):
+            inp = build_inputs(req, dev)
+            req.patterns = torch.empty((req.positions, req.n, req.n), dtype=torch.float32, device=dev)
+            if req.kind == "gradient":
+                req.grad_slab = torch.zeros_like(inp.slab)
+                req.grad_probe_k = torch.zeros_like(inp.probe_k)
+            ready = torch.cuda.Event()
+            ready.record(builder)
         req.t_admit = now
-        return _Active(req, inp)
+        return _Active(req, inp, ready=ready)
+
+    def _admit_arrivals(self, pending: List[Request], active: List[_Active], in_use: int, t0: float,
+                        realtime: bool, limit: int) -> int:
+        now = time.perf_counter() - t0
+        admitted = 0
+        while pending and admitted < limit and (not realtime or pending[0].arrival <= now):
+            need = request_bytes(pending[0])
+            if active and in_use + need > self.hbm_budget_bytes:
+                break
+            r = pending.pop(0)
+            active.append(self._admit(r, now))
+            in_use += need
+            admitted += 1
+        return in_use
 
     def run(self, requests: List[Request], realtime: bool = True) -> Dict:
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -330,17 +363,12 @@ class RequestLoop:
         t0 = time.perf_counter()
         finished: List[Request] = []
         while pending or active:
-            now = time.perf_counter() - t0
-            while pending and (not realtime or pending[0].arrival <= now):
-                need = request_bytes(pending[0])
-                if active and in_use + need > self.hbm_budget_bytes:
-                    break
-                r = pending.pop(0)
-                active.append(self._admit(r, now))
-                in_use += need
             if not active:
-                time.sleep(max(0.0, min(0.005, pending[0].arrival - now)))
-                continue
+                in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, 1)
+                if not active:
+                    now = time.perf_counter() - t0
+                    time.sleep(max(0.0, min(0.005, pending[0].arrival - now)))
+                    continue
             classes: Dict[Tuple[int, int], List[_Active]] = {}
             for a in active:
                 if a.next_pos < a.req.positions:
@@ -349,17 +377,22 @@ class RequestLoop:
             for key, members in classes.items():
                 st = self._stream(key)
                 st.wait_stream(main)
+                for a in members:
+                    if a.ready is not None:
+                        st.wait_event(a.ready)
                 with torch.cuda.stream(st):
                     self._step_class(key, members)
             for key in classes:
                 main.wait_stream(self._stream(key))
+            # requests that arrived meanwhile are built while the GPU runs this step; they join the next one
+            in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, self.admit_per_step)
             torch.cuda.synchronize()
             now = time.perf_counter() - t0
             still = []
             for a in active:
                 if a.done_pos >= a.req.positions:
                     if a.req.kind == "gradient":
-                        a.req.loss = a.loss
+                        a.req.loss = float(a.loss)
                     a.req.t_done = now
                     if self.keep_inputs:
                         a.req.inputs = a.inp

@@ -64,3 +64,33 @@ def test_admission_respects_hbm_budget():
     # requests were admitted one at a time when the budget only fits one of the largest
     admits = sorted(r.t_admit for r in reqs)
     assert admits[-1] > admits[0]
+
+
+@pytest.mark.gpu
+def test_adjoint_batched_matches_per_request():
+    """pty_adjoint_batched (descriptors addressing two requests' slabs, atomic accumulation) agrees with the
+    per-request gathered adjoint."""
+    from ptyrodactyl_amd import engine
+    n, s_count, k = 64, 5, 4
+    loop = serving.RequestLoop()
+    reqs = [serving.Request(0, "gradient", n, s_count, 24, seed=3), serving.Request(1, "gradient", n, s_count, 40, seed=4)]
+    acts = [loop._admit(r, 0.0) for r in reqs]
+    torch.cuda.synchronize()
+    items = [(a, 0, a.req.positions) for a in acts]
+    total = sum(a.req.positions for a in acts)
+    nc = (s_count - 1) // k
+    ckpt = torch.empty((total, nc, n, n), dtype=torch.complex64, device="cuda")
+    engine.forward_batched(n, s_count, loop._descs(items, True, [a.req.patterns for a in acts]), loop.base, ckpt, k)
+    gis = [serving.loss_and_grad_intensity(a.req.patterns, a.inp.measured)[1] for a in acts]
+    engine.adjoint_batched(n, s_count, loop._descs(items, True, gis), loop.base, ckpt, k)
+    one = torch.ones(1, dtype=torch.float32, device="cuda")
+    k0 = 0
+    for a, gi in zip(acts, gis):
+        inp, p = a.inp, a.req.positions
+        ref_slab, ref_pk = engine.multislice_adjoint(inp.slab, inp.probe_k[None], one, inp.origins, inp.shifts,
+                                                     inp.prop[:n].contiguous(), gi, ckpt[k0:k0 + p], k)
+        k0 += p
+        for got, ref in ((a.req.grad_slab, ref_slab), (a.req.grad_probe_k, ref_pk[0])):
+            err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+            print(f"request {a.req.rid}: batched vs per-request adjoint rel-L2 {err:.2e}", flush=True)
+            assert err < 1e-5

@@ -441,22 +441,31 @@ __global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, 
     }
     if (hit) hits[offset + before] = make_int3(i, oy, ox);
     __syncthreads();
-    // 2. ascending wave order over the hits: a fixed summation order
+    // 2. ascending wave order over the hits: a fixed summation order.  Four hits at a time: all their
+    // loads are issued first (clamped to in-window addresses, always in bounds) and the values are
+    // then added through selects, so the loads stay unconditional and overlap.
     if (tx < xlen) {
-#pragma unroll 4
-      for (int h = 0; h < total; ++h) {
-        const int3 e = hits[h];
-        const int dx = frame_offset(X, e.z, W);
-        // unconditional loads from clamped (in-window) addresses keep several in flight; pixels
-        // outside the window add nothing
-        const float2* cw = a.gt_contrib + ((size_t)e.x * a.contrib_slices + sb) * N * N + min(dx, N - 1);
+      for (int h0 = 0; h0 < total; h0 += 4) {
+        float2 val[4][RPT];
+        bool use[4][RPT];
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-          const int yl = ty + 8 * r;
-          const int dy = frame_offset(Y0 + yl, e.y, H);
-          const float2 val = cw[min(dy, N - 1) * N];
-          if (yl < ylen && dy < N && dx < N) acc[r] = cadd(acc[r], val);
+        for (int u = 0; u < 4; ++u) {
+          const int3 e = hits[min(h0 + u, total - 1)];
+          const int dx = frame_offset(X, e.z, W);
+          const float2* cw = a.gt_contrib + ((size_t)e.x * a.contrib_slices + sb) * N * N + min(dx, N - 1);
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) {
+            const int yl = ty + 8 * r;
+            const int dy = frame_offset(Y0 + yl, e.y, H);
+            val[u][r] = cw[min(dy, N - 1) * N];
+            use[u][r] = h0 + u < total && yl < ylen && dy < N && dx < N;
+          }
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int r = 0; r < RPT; ++r)
+            acc[r] = cadd(acc[r], use[u][r] ? val[u][r] : make_float2(0.f, 0.f));
       }
     }
   }

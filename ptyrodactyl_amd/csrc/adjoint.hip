@@ -47,27 +47,40 @@ __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, in
     if (gy >= g.slab_h) gy -= (int)g.slab_h;
     const long long off = (long long)s * g.stride + (long long)gy * g.slab_w;
     const float2* trow = g.slab + off;
-    if (pw == PW_FWD) {
+    // groups of GP points: each group issues all of its loads before using any of them, so the loads
+    // overlap instead of waiting one by one under register pressure
+    constexpr int GP = N <= 128 ? 4 : 8;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
+    for (int grp = 0; grp < 16 / GP; ++grp) {
+      float2 tv[GP], sv[GP];
+#pragma unroll
+      for (int u = 0; u < GP; ++u) {
+        const int q = GP * grp + u;
         int x = g.ox + t + T * q;
         if (x >= g.slab_w) x -= (int)g.slab_w;
-        v[q] = cmul(v[q], trow[x]);
+        tv[u] = trow[x];
+        if (pw == PW_ADJ) sv[u] = seg_row[t + T * q];
       }
-    } else {
-      float2* grow = a.grad_slab + g.grad_off + off;
+      asm volatile("" ::: "memory");
+      if (pw == PW_FWD) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        int x = g.ox + t + T * q;
-        if (x >= g.slab_w) x -= (int)g.slab_w;
-        const float2 gt = cmulc(v[q], seg_row[t + T * q]);  // gphi * conj(psi)
-        if (gt_row) {
-          gt_row[t + T * q] = gt;  // window coordinates, gathered by k_grad_reduce
-        } else {
-          unsafeAtomicAdd(&grow[x].x, gt.x);
-          unsafeAtomicAdd(&grow[x].y, gt.y);
+        for (int u = 0; u < GP; ++u) v[GP * grp + u] = cmul(v[GP * grp + u], tv[u]);
+      } else {
+        float2* grow = a.grad_slab + g.grad_off + off;
+#pragma unroll
+        for (int u = 0; u < GP; ++u) {
+          const int q = GP * grp + u;
+          const float2 gt = cmulc(v[q], sv[u]);  // gphi * conj(psi)
+          if (gt_row) {
+            gt_row[t + T * q] = gt;  // window coordinates, gathered by k_grad_reduce
+          } else {
+            int x = g.ox + t + T * q;
+            if (x >= g.slab_w) x -= (int)g.slab_w;
+            unsafeAtomicAdd(&grow[x].x, gt.x);
+            unsafeAtomicAdd(&grow[x].y, gt.y);
+          }
+          v[q] = cmulc(v[q], tv[u]);  // conj(T) * gphi
         }
-        v[q] = cmulc(v[q], trow[x]);  // conj(T) * gphi
       }
     }
   }
@@ -215,12 +228,19 @@ __device__ __forceinline__ void res_copy(const ResShared<N>& sh, float2* gmem, b
   const int rt = lane % T, ry = wid * G + lane / T;
   float2* row = sh.wave + ry * RS;
   float2* grow = gmem + (size_t)ry * N;
+  float2 v[16];
+  if (to_lds) {
+    // all sixteen loads are issued before the first LDS write, so the copy costs one memory latency
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    if (to_lds)
-      row[pad16(rt + T * q)] = grow[rt + T * q];
-    else
-      grow[rt + T * q] = row[pad16(rt + T * q)];
+    for (int q = 0; q < 16; ++q) v[q] = grow[rt + T * q];
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int q = 0; q < 16; ++q) row[pad16(rt + T * q)] = v[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = row[pad16(rt + T * q)];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) grow[rt + T * q] = v[q];
   }
   __syncthreads();
 }

@@ -176,6 +176,15 @@ int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pt
                         void* grad_slab_arena, void* grad_probe_arena, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/*
+ * Virtual annular detector over stored patterns: out[p] = sum_i mask[i] * patterns[p][i] in FP64
+ * (fixed summation order, bitwise reproducible).  mask is 0/1 per pixel of the fftshifted pattern;
+ * the caller builds it from the collection angles exactly as annular_detector does
+ * (src/ptyrodactyl/multislice/simulations.py:1540-1651).
+ */
+int pty_annular_integrate(const float* patterns, long long num_patterns, long long pixels_per_pattern,
+                          const unsigned char* mask, double* out, void* stream);
+
 /* Device bytes of the state block and workspace of pty_forward_persistent. */
 size_t pty_persistent_state_bytes(void);
 size_t pty_persistent_workspace_bytes(int n, int group_waves);

@@ -265,6 +265,25 @@ def stem_4d(pot_slices, slice_thickness, pot_calib, beam_modes, beam_weights,
 # Window-crop semantics (a slab larger than the probe window).
 # ---------------------------------------------------------------------------
 
+def annular_detector(data, fourier_space_calib: float, voltage_kv: float, inner_mrad: float, outer_mrad: float,
+                     scan_shape):
+    """Virtual annular detector (``multislice/simulations.py:1540-1651``): per pattern, the sum of
+    the pixels whose radius sqrt(qy^2 + qx^2) * fourier_space_calib, with q measured from
+    (H//2, W//2), lies in [inner, outer] (inclusive) in inverse Angstrom (mrad / 1000 / lambda);
+    the sums are reshaped to the scan raster."""
+    data = np.asarray(data, np.float64)
+    lam = wavelength_ang(voltage_kv)
+    inner_k = (inner_mrad / 1000.0) / lam
+    outer_k = (outer_mrad / 1000.0) / lam
+    _, hh, ww = data.shape
+    qy = np.arange(hh) - hh // 2
+    qx = np.arange(ww) - ww // 2
+    qya, qxa = np.meshgrid(qy, qx, indexing="ij")
+    radius = np.sqrt(qya**2 + qxa**2) * fourier_space_calib
+    mask = (radius >= inner_k) & (radius <= outer_k)
+    return (data * mask).sum(axis=(1, 2)).reshape(scan_shape)
+
+
 def ramp_1d(n: int, shift_px: float):
     """exp(-2 pi i k shift/n) with k = fftfreq(n)*n (Nyquist negative)."""
     k = np.fft.fftfreq(n) * n

@@ -10,7 +10,7 @@ runs in hand-written gfx950 HIP kernels behind the C-ABI declared in
 from .checked import checked_cbed_image, checked_make_probe, checked_stem4d_sharded, checked_stem_4d
 from .parallelized import AtomicSliceData, create_atomic_slice_data, stem4d_sharded
 from .physics import interaction_sigma, phase_interaction_parameter, relativistic_wavelength_ang
-from .simulations import (aberration, cbed_amplitude, cbed_image, make_probe, propagation_func,
+from .simulations import (aberration, annular_detector, cbed_amplitude, cbed_image, make_probe, propagation_func,
                           shift_beam_fourier, stem_4d, stem_4d_windows, transmission_func)
 from .types import (STEM4D, CalibratedArray, DetectorConfig, EnsembleAxes, MicroscopeConfig, PotentialSlices,
                     ProbeModes, RuntimeCheckError, create_calibrated_array, create_detector_config,
@@ -18,7 +18,7 @@ from .types import (STEM4D, CalibratedArray, DetectorConfig, EnsembleAxes, Micro
                     create_stem4d)
 
 __all__ = [
-    "aberration", "cbed_amplitude", "cbed_image", "make_probe", "propagation_func", "shift_beam_fourier",
+    "aberration", "annular_detector", "cbed_amplitude", "cbed_image", "make_probe", "propagation_func", "shift_beam_fourier",
     "stem_4d", "stem_4d_windows", "transmission_func", "interaction_sigma", "phase_interaction_parameter",
     "relativistic_wavelength_ang", "STEM4D", "CalibratedArray", "DetectorConfig", "EnsembleAxes",
     "MicroscopeConfig", "PotentialSlices", "ProbeModes", "RuntimeCheckError", "create_calibrated_array",

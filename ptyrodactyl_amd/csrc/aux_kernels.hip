@@ -183,4 +183,36 @@ hipError_t launch_damped_solve(const double* A, const double* b, double damping,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Virtual annular detector: masked sum of each stored pattern
+// ---------------------------------------------------------------------------
+
+// One workgroup per pattern.  Each thread sums its pixels (stride 256) in FP64 with the 0/1 mask,
+// then a fixed-order tree in LDS combines the threads: the result is bitwise reproducible.
+__global__ __launch_bounds__(256) void k_annular(const float* patterns, long long pixels, const unsigned char* mask,
+                                                 double* out) {
+  __shared__ double red[256];
+  const float* pat = patterns + (long long)blockIdx.x * pixels;
+  double acc = 0.0;
+  for (long long i = threadIdx.x; i < pixels; i += 256) {
+    const float v = pat[i];  // unconditional load; the mask selects
+    acc += mask[i] ? (double)v : 0.0;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+hipError_t launch_annular(const float* patterns, long long count, long long pixels, const unsigned char* mask,
+                          double* out, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  if (count > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_annular, dim3((unsigned)count), dim3(256), 0, st, patterns, pixels, mask, out);
+  return hipGetLastError();
+}
+
 }  // namespace pty

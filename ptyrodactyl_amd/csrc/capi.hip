@@ -209,6 +209,18 @@ int pty_damped_solve(const double* matrix, const double* rhs, double damping, in
                        "pty_damped_solve");
 }
 
+int pty_annular_integrate(const float* patterns, long long num_patterns, long long pixels_per_pattern,
+                          const unsigned char* mask, double* out, void* stream) {
+  g_err[0] = '\0';
+  if (num_patterns < 0 || pixels_per_pattern < 1)
+    return fail(PTY_ERR_INVALID, "num_patterns must be >= 0 and pixels_per_pattern >= 1");
+  if (num_patterns > 0x7fffffffLL) return fail(PTY_ERR_INVALID, "at most 2^31 - 1 patterns per call");
+  if (num_patterns > 0 && (!patterns || !mask || !out)) return fail(PTY_ERR_INVALID, "null pointer");
+  return device_status(pty::launch_annular(patterns, num_patterns, pixels_per_pattern, mask, out,
+                                           (hipStream_t)stream),
+                       "pty_annular_integrate");
+}
+
 size_t pty_persistent_state_bytes(void) { return pty::xl2_state_bytes(); }
 
 size_t pty_persistent_workspace_bytes(int n, int group_waves) {

@@ -215,3 +215,52 @@ def stem_4d_windows(slab_t: torch.Tensor, probe: torch.Tensor, origins: torch.Te
     patterns, _ = engine.multislice_forward(slab_t.contiguous(), pk, w, origins.to(dev, torch.int32).contiguous(),
                                             shifts_px.to(dev, torch.float64).contiguous(), prop)
     return patterns
+
+
+def annular_mask(shape, fourier_space_calib: float, voltage_kv: float, inner_mrad: float, outer_mrad: float,
+                 device=None) -> torch.Tensor:
+    """0/1 (uint8) mask of the fftshifted pattern pixels inside the annulus, computed in float64 with the
+    reference's operation order (``simulations.py:1596-1618``): radius = sqrt(qy^2 + qx^2) * calib with q
+    measured from (H//2, W//2), inner_mrad / 1000 / lambda <= radius <= outer_mrad / 1000 / lambda."""
+    hh, ww = int(shape[0]), int(shape[1])
+    lam = physics.relativistic_wavelength_ang(voltage_kv)
+    inner_k = (float(inner_mrad) / 1000.0) / lam
+    outer_k = (float(outer_mrad) / 1000.0) / lam
+    qy = torch.arange(hh, dtype=torch.float64, device=device) - hh // 2
+    qx = torch.arange(ww, dtype=torch.float64, device=device) - ww // 2
+    qya, qxa = torch.meshgrid(qy, qx, indexing="ij")
+    radius = torch.sqrt(qya**2 + qxa**2) * float(fourier_space_calib)
+    return ((radius >= inner_k) & (radius <= outer_k)).to(torch.uint8).contiguous()
+
+
+def annular_detector(stem4d_data: STEM4D, detector: DetectorConfig) -> CalibratedArray:
+    """Integrate 4D-STEM data with a virtual annular detector (``simulations.py:1540-1651``).
+
+    Every pattern is summed over the pixels between ``detector.collection_inner_mrad`` and
+    ``collection_outer_mrad`` (inclusive, converted with the wavelength at ``stem4d_data.voltage_kv``)
+    by the HIP reduction ``pty_annular_integrate`` (FP64 accumulation, fixed order), and the sums are
+    reshaped to ``detector.scan_shape``.  Returns a real-space ``CalibratedArray`` (float64) with the
+    4D-STEM's real-space calibration on both axes.  Raises ``ValueError`` without a scan shape.
+    """
+    if detector.scan_shape is None:
+        raise ValueError("detector.scan_shape is required")
+    from . import _native
+    data = stem4d_data.data
+    if not (isinstance(data, torch.Tensor) and data.is_cuda):
+        raise ValueError("stem4d_data.data must be a GPU tensor (the detector integration has no CPU fallback)")
+    p_count, hh, ww = data.shape
+    ny, nx = (int(v) for v in detector.scan_shape)
+    if ny * nx != p_count:
+        raise ValueError(f"scan_shape {detector.scan_shape} does not hold {p_count} patterns")
+    dev = data.device
+    mask = annular_mask((hh, ww), stem4d_data.fourier_space_calib, stem4d_data.voltage_kv,
+                        detector.collection_inner_mrad, detector.collection_outer_mrad, device=dev)
+    pats = data.to(torch.float32).contiguous()
+    out = torch.empty(p_count, dtype=torch.float64, device=dev)
+    lib = _native.load()
+    with torch.cuda.device(dev):
+        status = lib.pty_annular_integrate(_native.ptr(pats), p_count, hh * ww, _native.ptr(mask), _native.ptr(out),
+                                           _native.stream_handle(dev))
+    _native.check(status, "pty_annular_integrate")
+    return create_calibrated_array(out.reshape(ny, nx), stem4d_data.real_space_calib, stem4d_data.real_space_calib,
+                                   True)

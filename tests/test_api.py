@@ -26,6 +26,7 @@ def test_public_signatures_keep_reference_names_and_order():
         "stem4d_sharded": ["probe_modes", "sample", "microscope", "detector", "mesh"],
         "checked_stem_4d": ["pot_slice", "beam", "microscope", "detector"],
         "checked_cbed_image": ["pot_slices", "beam", "microscope"],
+        "annular_detector": ["stem4d_data", "detector"],
     }
     for name, params in expect.items():
         sig = inspect.signature(getattr(pa, name))
@@ -115,3 +116,49 @@ def test_atomic_slice_data_validation():
         pa.create_atomic_slice_data(np.zeros((3, 3)), [0, 1, 2], [[0, 1]], np.zeros((2, 4, 4)))
     with pytest.raises(pa.RuntimeCheckError, match="thicknesses must be positive"):
         pa.create_atomic_slice_data(np.zeros((1, 3)), [0], [[1.0, 1.0]], np.zeros((1, 4, 4)))
+
+
+# The reference's golden vector (tests/test_ptyrodactyl/test_multislice/test_simulations.py:84-122): 4x4 patterns
+# arange(96), fourier calib 0.25 1/A, 80 kV, collection 0 - 1000 mrad, raster (2, 3).
+ANNULAR_EXPECTED = np.array([[120.0, 376.0, 632.0], [888.0, 1144.0, 1400.0]])
+
+
+def test_annular_oracle_golden_and_ring():
+    data = np.arange(6 * 4 * 4, dtype=np.float64).reshape(6, 4, 4)
+    assert np.array_equal(o.annular_detector(data, 0.25, 80.0, 0.0, 1000.0, (2, 3)), ANNULAR_EXPECTED)
+    # calib = 1 / (1000 lambda) makes the radius in pixels equal the angle in mrad: the 0.5 - 2.5 mrad ring
+    # holds the lattice points with r^2 in {1, 2, 4, 5}: 4 + 4 + 4 + 8 = 20 pixels
+    lam = o.wavelength_ang(200.0)
+    assert o.annular_detector(np.ones((1, 8, 8)), 1 / (1000 * lam), 200.0, 0.5, 2.5, (1, 1))[0, 0] == 20.0
+    # the centre pixel is (H//2, W//2): a 0 - 0.5 mrad disk selects exactly that pixel
+    d = np.zeros((1, 7, 6))
+    d[0, 3, 3] = 5.0
+    d[0, 3, 2] = 7.0
+    assert o.annular_detector(d, 1 / (1000 * lam), 200.0, 0.0, 0.5, (1, 1))[0, 0] == 5.0
+
+
+def test_annular_mask_matches_oracle_mask():
+    """The device-independent mask builder reproduces the reference's float64 mask bit for bit."""
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        h, w = int(rng.integers(4, 40)), int(rng.integers(4, 40))
+        calib, kv = float(rng.uniform(0.01, 0.5)), float(rng.choice([80.0, 200.0, 300.0]))
+        inner = float(rng.uniform(0, 30))
+        outer = inner + float(rng.uniform(0, 60))
+        got = pa.simulations.annular_mask((h, w), calib, kv, inner, outer).numpy().astype(bool)
+        eye = np.zeros((h * w, h, w))
+        eye[np.arange(h * w), np.arange(h * w) // w, np.arange(h * w) % w] = 1.0
+        ref = o.annular_detector(eye, calib, kv, inner, outer, (h, w)).astype(bool)
+        assert np.array_equal(got, ref)
+
+
+def test_annular_detector_host_errors():
+    stem = pa.create_stem4d(torch.zeros(6, 4, 4, dtype=torch.float32), 0.5, 0.25, torch.zeros(6, 2), 80.0)
+    det = pa.create_detector_config(real_space_calib_ang=0.5, collection_inner_mrad=0.0,
+                                    collection_outer_mrad=1000.0)
+    with pytest.raises(ValueError, match="scan_shape is required"):
+        pa.annular_detector(stem, det)
+    det = pa.create_detector_config(real_space_calib_ang=0.5, collection_inner_mrad=0.0,
+                                    collection_outer_mrad=1000.0, scan_shape=(2, 3))
+    with pytest.raises(ValueError, match="GPU tensor"):
+        pa.annular_detector(stem, det)

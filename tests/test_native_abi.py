@@ -46,3 +46,18 @@ def test_invalid_arguments_rejected_without_gpu():
         _native.check(rc, "pty_forward")
     assert lib.pty_forward_workspace_bytes(256, 10, 1, 4) == 4 * 256 * 256 * 8
     assert lib.pty_forward_workspace_bytes(128, 10, 1, 0) == 0
+
+
+def test_annular_and_adjoint_queries_validate_on_host():
+    from ptyrodactyl_amd import _native
+    if not _native.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    lib = _native.load()
+    assert lib.pty_annular_integrate(None, 4, 0, None, None, None) == _native.PTY_ERR_INVALID
+    assert lib.pty_annular_integrate(None, 0, 16, None, None, None) == _native.PTY_OK
+    assert lib.pty_annular_integrate(None, 2, 16, None, None, None) == _native.PTY_ERR_INVALID
+    assert b"null pointer" in lib.pty_last_error()
+    # the gathered adjoint needs room for the contributions of every slice
+    assert lib.pty_adjoint_workspace_bytes(128, 16, 4, 1024, 1) > lib.pty_adjoint_workspace_bytes(128, 4, 4, 1024, 1)
+    assert lib.pty_adjoint_workspace_bytes(100, 16, 4, 1024, 1) == 0
+    assert lib.pty_persistent_workspace_bytes(128, 2) == 0

@@ -66,6 +66,27 @@ def cpu_baseline(w, sample: int, cores: int):
                       f"median of 3 after 1 warm-up; reference-algorithm CPU restatement (numpy/pocketfft)"}
 
 
+def detectors_bench(w, slab, pk, weights, origins, shifts, prop, steps: int):
+    """The same forward with three annular virtual detectors (BF 0-25, ABF 12.5-25, ADF 50-95 mrad) fused into
+    the epilogue and no pattern stack written."""
+    import torch
+    from ptyrodactyl_amd import engine, simulations
+    angles = [(0.0, 25.0), (12.5, 25.0), (50.0, 95.0)]
+    masks = torch.stack([simulations.annular_mask((w.n, w.n), 1.0 / (w.n * w.calib), w.voltage_kv, lo, hi,
+                                                  device=slab.device) for lo, hi in angles]).contiguous()
+    engine.multislice_forward_detectors(slab, pk, weights, origins, shifts, prop, masks)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        engine.multislice_forward_detectors(slab, pk, weights, origins, shifts, prop, masks)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"detectors_mrad": angles, "patterns_written": False, "ms_per_step": round(ms, 3),
+            "patterns_per_s": round(len(w.origins) / (ms * 1e-3), 1)}
+
+
 def gradient_algorithmic_bytes(n: int, s: int, k: int) -> int:
     """Per pattern: transmission read three times (forward, segment recompute, adjoint) and its gradient
     written once, checkpoints written and read, probe read and probe gradient written, pattern written
@@ -273,6 +294,9 @@ def main() -> int:
                      "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
                      "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
     }
+    if args.workload == "config2" and not persistent:
+        result["virtual_detectors"] = detectors_bench(w, slab, pk, weights, origins, shifts, prop, args.steps)
+        print(f"# detectors {result['virtual_detectors']}", file=sys.stderr, flush=True)
     if not args.no_gradient and args.workload == "config2":
         result["gradient"] = gradient_bench(max(args.steps, 3), 1)
         print(f"# gradient {result['gradient']}", file=sys.stderr, flush=True)

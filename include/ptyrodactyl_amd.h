@@ -77,6 +77,22 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
                              float* patterns, void* checkpoints, int ckpt_every, void* workspace,
                              size_t workspace_bytes, void* stream);
 
+#define PTY_MAX_DETECTORS 4
+
+/*
+ * pty_forward with virtual detectors fused into the |psi|^2 epilogue: for each of num_detectors
+ * 0/1 masks ([num_detectors][n][n], fftshifted pattern layout, built from the collection angles as
+ * annular_detector does, src/ptyrodactyl/multislice/simulations.py:1540-1651) detector_out
+ * [num_positions][num_detectors] receives the sum of the pattern intensity over the mask (FP64,
+ * fixed order).  patterns may be NULL: the pattern stack is then never written.  Workspace as for
+ * pty_forward.
+ */
+int pty_forward_detectors(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                          long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                          const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                          float* patterns, int num_detectors, const unsigned char* masks, double* detector_out,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
 /*
  * Device scratch for pty_adjoint: with at least this many bytes the gradients
  * are gathered in a fixed order (bitwise reproducible).  A smaller workspace,

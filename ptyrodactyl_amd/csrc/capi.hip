@@ -70,6 +70,9 @@ pty::FwdParams make_params(int n, int num_slices, long long num_positions, int n
   a.ckpt = nullptr;
   a.ckpt_every = 0;
   a.desc = nullptr;
+  a.det_masks = nullptr;
+  a.det_out = nullptr;
+  a.det_count = 0;
   return a;
 }
 
@@ -104,6 +107,32 @@ int pty_forward(int n, int num_slices, long long num_positions, int num_modes, c
   a.patterns = patterns;
   a.amplitude = (float2*)amplitude;
   return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_forward");
+}
+
+int pty_forward_detectors(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                          long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                          const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                          float* patterns, int num_detectors, const unsigned char* masks, double* detector_out,
+                          void* workspace, size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                        origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (num_detectors < 1 || num_detectors > PTY_MAX_DETECTORS)
+    return fail(PTY_ERR_INVALID, "num_detectors must be in [1, %d] (got %d)", PTY_MAX_DETECTORS, num_detectors);
+  if (!masks || !detector_out) return fail(PTY_ERR_INVALID, "masks and detector_out must be non-null");
+  const size_t need = pty::forward_workspace_bytes(n, 1, num_modes, 0);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
+                need);
+  pty::FwdParams a = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k,
+                                 mode_weights, origins, shifts_px, prop_y, prop_x);
+  a.patterns = patterns;
+  a.det_masks = masks;
+  a.det_out = detector_out;
+  a.det_count = num_detectors;
+  return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream),
+                       "pty_forward_detectors");
 }
 
 int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
@@ -251,6 +280,7 @@ int pty_forward_persistent(int n, int num_slices, long long num_positions, const
                        "pty_forward_persistent");
 }
 
+static_assert(PTY_MAX_DETECTORS == pty::MAX_DETECTORS, "detector limit");
 static_assert(sizeof(pty_position_desc) == sizeof(pty::PosDesc), "descriptor layout mismatch");
 
 static int check_batched(int n, int num_slices, long long num_positions, const void* descs, const void* slab_arena,

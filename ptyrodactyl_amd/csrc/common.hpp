@@ -34,4 +34,21 @@ struct RowBufWave {
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
+// Deterministic FP64 sum over a workgroup of NT threads (NT a multiple of 64): a fixed xor
+// butterfly inside each wavefront, then thread 0 adds the wavefront sums in order.  `scratch`
+// holds NT / 64 doubles of LDS; every thread of the group must call it.  The total is returned
+// on thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum_fp64(double v, double* scratch) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int tid = threadIdx.x;
+  __syncthreads();  // scratch may still be read from a previous call
+  if ((tid & 63) == 0) scratch[tid >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (tid == 0)
+    for (int w = 0; w < NT / 64; ++w) s += scratch[w];
+  return s;
+}
+
 }  // namespace pty

@@ -43,7 +43,7 @@ struct ColBufResident {
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
-template <int N>
+template <int N, bool DET>
 __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
   constexpr int T = N / 16;       // threads per line
   constexpr int NT = N * N / 16;  // threads per workgroup
@@ -56,6 +56,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
   float2* px = py + N;
   float2* ramp_y = px + N;
   float2* ramp_x = ramp_y + N;
+  double* red = (double*)(ramp_x + N);  // NT / 64 doubles for the detector sums
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -157,6 +158,22 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
 #pragma unroll
             for (int q = 0; q < 16; ++q) amp[(size_t)((ct + T * q + N / 2) & (N - 1)) * N] = v[q];
           }
+          // virtual detectors: the masked sum is linear in the intensity, so each mode's w |A|^2 is
+          // summed and added to the position's signal in mode order
+          for (int d = 0; DET && d < a.det_count; ++d) {
+            const unsigned char* mk = a.det_masks + (size_t)d * N * N + xs;
+            double part = 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+              const float val = wm * cabs2(v[q]);
+              part += mk[(size_t)((ct + T * q + N / 2) & (N - 1)) * N] ? (double)val : 0.0;
+            }
+            const double total = block_sum_fp64<NT>(part, red);
+            if (tid == 0) {
+              double* o = a.det_out + (size_t)p * a.det_count + d;
+              *o = (m == 0 ? 0.0 : *o) + total;
+            }
+          }
         }
         __syncthreads();
       }
@@ -166,7 +183,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
 
 template <int N>
 static size_t resident_lds_bytes() {
-  return sizeof(float2) * ((size_t)N * (N + N / 16) + 5 * N);
+  return sizeof(float2) * ((size_t)N * (N + N / 16) + 5 * N) + sizeof(double) * (N * N / 16 / 64);
 }
 
 // ---------------------------------------------------------------------------
@@ -237,7 +254,7 @@ struct ColBufBlock {
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
-enum : int { COL_INIT = 0, COL_MID = 1, COL_FINAL = 2 };
+enum : int { COL_INIT = 0, COL_MID = 1, COL_FINAL = 2, COL_FINAL_DET = 3 };
 
 template <int N, int MODE>
 __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0, int nb) {
@@ -247,12 +264,13 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
   __shared__ float2 buf[PS * NC];
   __shared__ float2 tw[N];
   __shared__ float2 py[N];
+  __shared__ double red[4];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
   const int unit = blockIdx.x % nb;  // wave (INIT/MID) or position (FINAL) inside the batch
   const int cblk = blockIdx.x / nb;
   const int x = cblk * NC + gc;
-  const int pos = MODE == COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M;
+  const int pos = MODE >= COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M;
   const WaveCtx g = resolve_wave(a, pos);
   fill_twiddles(tw, N, tid, 256);
   if (MODE == COL_MID)
@@ -313,27 +331,60 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
 #pragma unroll
       for (int q = 0; q < 16; ++q) out[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = acc[q];
     }
+    // virtual detectors: this tile's masked sums go to row 0, columns cblk * NC + d of the position's
+    // first wave in the workspace (only this workgroup reads those columns; its reads are complete
+    // after the reduction's barrier) and k_det_finalize adds the tiles in order
+    for (int d = 0; MODE == COL_FINAL_DET && d < a.det_count; ++d) {
+      const unsigned char* mk = a.det_masks + (size_t)d * N * N + xs;
+      double part = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) part += mk[(size_t)((t + T * q + N / 2) & (N - 1)) * N] ? (double)acc[q] : 0.0;
+      const double total = block_sum_fp64<256>(part, red);
+      if (tid == 0) {
+        double* slot = (double*)(waves + (size_t)pl * a.M * N * N + cblk * NC + d);
+        *slot = total;
+      }
+    }
   }
+}
+
+// Sum of the N / NC tile partials of every (position, detector) of a streaming batch, in tile order.
+template <int N>
+__global__ __launch_bounds__(256) void k_det_finalize(FwdParams a, const float2* waves, long long p0, int np) {
+  constexpr int NC = 256 / (N / 16);
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= np * a.det_count) return;
+  const int pl = idx / a.det_count, d = idx % a.det_count;
+  const double* row = (const double*)(waves + (size_t)pl * a.M * N * N);
+  double s = 0.0;
+  for (int cb = 0; cb < N / NC; ++cb) s += row[cb * NC + d];
+  a.det_out[(p0 + pl) * a.det_count + d] = s;
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 
-template <int N>
-static hipError_t launch_resident(const FwdParams& a, hipStream_t st) {
+template <int N, bool DET>
+static hipError_t launch_resident_variant(const FwdParams& a, hipStream_t st) {
   const size_t lds = resident_lds_bytes<N>();
   static bool attr_set[64] = {false};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   if (dev >= 0 && dev < 64 && !attr_set[dev]) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_forward_resident<N>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_forward_resident<N, DET>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     attr_set[dev] = true;
   }
-  hipLaunchKernelGGL(k_forward_resident<N>, dim3(a.P), dim3(N * N / 16), lds, st, a);
+  hipLaunchKernelGGL((k_forward_resident<N, DET>), dim3(a.P), dim3(N * N / 16), lds, st, a);
   return hipGetLastError();
+}
+
+// The detector epilogue is a separate instantiation so the plain forward keeps its register budget.
+template <int N>
+static hipError_t launch_resident(const FwdParams& a, hipStream_t st) {
+  return a.det_count > 0 ? launch_resident_variant<N, true>(a, st) : launch_resident_variant<N, false>(a, st);
 }
 
 template <int N>
@@ -365,7 +416,12 @@ static hipError_t launch_streaming(const FwdParams& a, float2* ws, size_t ws_byt
       if (!last)
         hipLaunchKernelGGL((k_cols<N, COL_MID>), dim3(nb * (N / NC)), dim3(256), 0, st, a, ws, w0, nb);
     }
-    hipLaunchKernelGGL((k_cols<N, COL_FINAL>), dim3(np * (N / NC)), dim3(256), 0, st, a, ws, w0, np);
+    if (a.det_count > 0) {
+      hipLaunchKernelGGL((k_cols<N, COL_FINAL_DET>), dim3(np * (N / NC)), dim3(256), 0, st, a, ws, w0, np);
+      hipLaunchKernelGGL(k_det_finalize<N>, dim3((np * a.det_count + 255) / 256), dim3(256), 0, st, a, ws, p0, np);
+    } else {
+      hipLaunchKernelGGL((k_cols<N, COL_FINAL>), dim3(np * (N / NC)), dim3(256), 0, st, a, ws, w0, np);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -38,7 +38,12 @@ struct FwdParams {
   float2* ckpt;              // [P*M][C-1][n][n] wave entering slices k, 2k, ... (C = ceil(S/k)), or null
   int ckpt_every;            // checkpoint interval k (slices)
   const PosDesc* desc;       // [P] per-position descriptors (batched launches, M == 1), or null
+  const unsigned char* det_masks;  // [D][n][n] 0/1 virtual-detector masks (fftshifted layout), or null
+  double* det_out;                 // [P][D] sum over each mask of the pattern intensity, or null
+  int det_count;                   // D, at most MAX_DETECTORS
 };
+
+constexpr int MAX_DETECTORS = 4;
 
 // Number of stored checkpoints per wave: slices k, 2k, ... below S.
 __host__ __device__ inline int num_checkpoints(int S, int k) { return k > 0 ? (S - 1) / k : 0; }

@@ -102,6 +102,41 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
     return patterns, amplitude
 
 
+MAX_DETECTORS = 4
+
+
+def multislice_forward_detectors(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: torch.Tensor,
+                                 origins: torch.Tensor, shifts_px: torch.Tensor, prop: torch.Tensor,
+                                 masks: torch.Tensor, *, want_patterns: bool = False,
+                                 max_batch_waves: Optional[int] = None
+                                 ) -> Tuple[Optional[torch.Tensor], torch.Tensor]:
+    """Forward with virtual detectors fused into the |psi|^2 epilogue.
+
+    masks: (D, n, n) uint8 0/1 in the fftshifted pattern layout, D <= 4.  Returns (patterns or None,
+    signals (P, D) float64) where signals[p, d] is the sum of pattern p over mask d.  Without
+    ``want_patterns`` the pattern stack is never written.
+    """
+    dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
+                                                                        shifts_px, prop)
+    _check_cuda(masks, "masks", torch.uint8, 3, dev)
+    d_count = masks.shape[0]
+    _require(1 <= d_count <= MAX_DETECTORS and tuple(masks.shape[1:]) == (n, n),
+             f"masks must have shape (D, {n}, {n}) with 1 <= D <= {MAX_DETECTORS}")
+    patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev) if want_patterns else None
+    signals = torch.empty((p_count, d_count), dtype=torch.float64, device=dev)
+    ws_bytes = workspace_bytes(n, p_count, m_count, max_batch_waves)
+    workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev) if ws_bytes else None
+    lib = _native.load()
+    with torch.cuda.device(dev):
+        status = lib.pty_forward_detectors(
+            n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
+            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(patterns), d_count, _native.ptr(masks), _native.ptr(signals), _native.ptr(workspace),
+            ws_bytes, _native.stream_handle(dev))
+    _native.check(status, "pty_forward_detectors")
+    return patterns, signals
+
+
 PERSISTENT_GROUP_WAVES = 2
 PERSISTENT_BLOCKS_PER_CU = 3
 _STATE_ERROR_INDEX = 0

@@ -264,3 +264,26 @@ def annular_detector(stem4d_data: STEM4D, detector: DetectorConfig) -> Calibrate
     _native.check(status, "pty_annular_integrate")
     return create_calibrated_array(out.reshape(ny, nx), stem4d_data.real_space_calib, stem4d_data.real_space_calib,
                                    True)
+
+
+def stem_4d_windows_detectors(slab_t: torch.Tensor, probe: torch.Tensor, origins: torch.Tensor,
+                              shifts_px: torch.Tensor, slice_thickness: float, voltage_kv: float, calib_ang: float,
+                              collection_mrad, weights: Optional[torch.Tensor] = None, keep_patterns: bool = False):
+    """Windowed 4D-STEM (as :func:`stem_4d_windows`) with annular virtual detectors fused into the kernel
+    epilogue.  collection_mrad: up to four (inner, outer) angle pairs; detector d integrates every
+    pattern over the annulus as :func:`annular_detector` does (Fourier calibration 1 / (n calib_ang)).
+    Returns (signals (P, D) float64, patterns (P, n, n) float32 or None); without ``keep_patterns``
+    the pattern stack is never written."""
+    dev = slab_t.device
+    pr = probe if probe.ndim == 3 else probe[None]
+    pk = torch.fft.fft2(pr.to(device=dev, dtype=torch.complex128)).to(torch.complex64).contiguous()
+    m_count, n = pk.shape[0], pk.shape[-1]
+    w = torch.full((m_count,), 1.0 / m_count, dtype=torch.float32, device=dev) if weights is None else \
+        (weights / weights.sum()).to(device=dev, dtype=torch.float32).contiguous()
+    prop = physics.propagator_factors(n, slice_thickness, voltage_kv, calib_ang, device=dev)
+    masks = torch.stack([annular_mask((n, n), 1.0 / (n * calib_ang), voltage_kv, lo, hi, device=dev)
+                         for lo, hi in collection_mrad]).contiguous()
+    patterns, signals = engine.multislice_forward_detectors(
+        slab_t.contiguous(), pk, w, origins.to(dev, torch.int32).contiguous(),
+        shifts_px.to(dev, torch.float64).contiguous(), prop, masks, want_patterns=keep_patterns)
+    return signals, patterns

@@ -60,3 +60,36 @@ def test_vacuum_bright_field_counts_the_aperture():
                                    collection_outer_mrad=4 * ap, scan_shape=(2, 2))
     np.testing.assert_allclose(pa.annular_detector(stem, bf).data_array.cpu().numpy(), aperture_pixels, rtol=1e-5)
     assert float(pa.annular_detector(stem, df).data_array.abs().max()) < 1e-4
+
+
+@pytest.mark.parametrize("n,s_count,p_count,modes", [(64, 3, 11, 1), (128, 4, 6, 2), (256, 3, 10, 1), (256, 2, 5, 2)])
+def test_fused_detectors_match_integrating_the_patterns(n, s_count, p_count, modes):
+    """Detector signals from the kernel epilogue equal annular integration of the stored patterns, with and
+    without writing the patterns, and are bitwise reproducible."""
+    from ptyrodactyl_amd import engine, physics, simulations
+    print(f"fused detectors n={n} S={s_count} P={p_count} M={modes}", flush=True)
+    rng = np.random.default_rng(n + p_count)
+    hw = (n + 40, n + 24)
+    slab = np.exp(1j * rng.uniform(0, 0.4, (s_count,) + hw)).astype(np.complex64)
+    probes = np.stack([o.make_probe(300.0, 20.0 + 3 * m, 30.0 * m, 0.0, 0.0, (n, n), 10.0) for m in range(modes)])
+    dev = torch.device("cuda", 0)
+    args = (torch.tensor(slab, device=dev),
+            torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous(),
+            torch.full((modes,), 1.0 / modes, dtype=torch.float32, device=dev),
+            torch.tensor(np.stack([rng.integers(0, hw[0], p_count), rng.integers(0, hw[1], p_count)], 1),
+                         dtype=torch.int32, device=dev),
+            torch.tensor(rng.uniform(-2, 2, (p_count, 2)), device=dev),
+            physics.propagator_factors(n, 1.5, 300.0, 0.1, device=dev))
+    angles = [(0.0, 10.0), (10.0, 25.0), (30.0, 80.0)]
+    masks = torch.stack([simulations.annular_mask((n, n), 1.0 / (n * 0.1), 300.0, lo, hi, device=dev)
+                         for lo, hi in angles]).contiguous()
+    pats, sig = engine.multislice_forward_detectors(*args, masks, want_patterns=True)
+    _, sig_only = engine.multislice_forward_detectors(*args, masks)
+    _, sig_again = engine.multislice_forward_detectors(*args, masks)
+    plain, _ = engine.multislice_forward(*args)
+    torch.cuda.synchronize()
+    assert torch.equal(pats, plain)
+    assert torch.equal(sig_only, sig_again)
+    ref = (pats.double()[:, None] * masks.double()[None]).sum(dim=(2, 3))
+    np.testing.assert_allclose(sig.cpu().numpy(), ref.cpu().numpy(), rtol=1e-6)
+    np.testing.assert_allclose(sig_only.cpu().numpy(), ref.cpu().numpy(), rtol=1e-6)

@@ -154,11 +154,13 @@ __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long
 // over waves so that the global scratch is bounded by the grid size.
 // ---------------------------------------------------------------------------
 
+// The resident wave keeps row y at LDS row pad16(y) (one spare row per 16): column-exchange accesses
+// spread over the banks and every address is a per-thread base plus a compile-time offset.
 struct ColBufRes {
   float2* col;
   int rs;
-  __device__ __forceinline__ float2 ld(int e) const { return col[(e ^ ((e >> 4) & 3)) * rs]; }
-  __device__ __forceinline__ void st(int e, float2 v) const { col[(e ^ ((e >> 4) & 3)) * rs] = v; }
+  __device__ __forceinline__ float2 ld(int e) const { return col[pad16(e) * rs]; }
+  __device__ __forceinline__ void st(int e, float2 v) const { col[pad16(e) * rs] = v; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
@@ -184,7 +186,7 @@ __device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams&
   const int tid = opaque((int)threadIdx.x);
   const int lane = tid & 63, wid = tid >> 6;
   const int rt = lane % T, ry = wid * G + lane / T;
-  const RowBufWave rb{sh.wave + ry * RS};
+  const RowBufWave rb{sh.wave + pad16(ry) * RS};
   const TwTable twt{sh.tw};
   float2 v[16];
 #pragma unroll
@@ -210,12 +212,12 @@ __device__ __forceinline__ void res_col(const ResShared<N>& sh, const AdjParams&
   float2 v[16];
   if constexpr (OP != C_IFFT_PROBE) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = col[(ct + T * q) * RS];
+    for (int q = 0; q < 16; ++q) v[q] = col[pad16(ct + T * q) * RS];
   }
   if (col_ops<N, OP>(v, ct, cx, w, a, g, cb, twt, sh.py, sh.ramp_y, sh.ramp_x)) {
     wave_sync();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) col[(ct + T * q) * RS] = v[q];
+    for (int q = 0; q < 16; ++q) col[pad16(ct + T * q) * RS] = v[q];
   }
   __syncthreads();
 }
@@ -226,7 +228,7 @@ __device__ __forceinline__ void res_copy(const ResShared<N>& sh, float2* gmem, b
   const int tid = opaque((int)threadIdx.x);
   const int lane = tid & 63, wid = tid >> 6;
   const int rt = lane % T, ry = wid * G + lane / T;
-  float2* row = sh.wave + ry * RS;
+  float2* row = sh.wave + pad16(ry) * RS;
   float2* grow = gmem + (size_t)ry * N;
   float2 v[16];
   if (to_lds) {
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
   constexpr int NT = N * N / 16, RS = N + N / 16;
   extern __shared__ float2 lds[];
   float2* wave = lds;
-  float2* tw = wave + N * RS;
+  float2* tw = wave + (N + N / 16) * RS;
   float2* py = tw + N;
   float2* px = py + N;
   float2* ramp_y = px + N;
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
 
 template <int N>
 static size_t resident_adj_lds_bytes() {
-  return sizeof(float2) * ((size_t)N * (N + N / 16) + 5 * N);
+  return sizeof(float2) * ((size_t)(N + N / 16) * (N + N / 16) + 5 * N);
 }
 
 // ---------------------------------------------------------------------------

@@ -33,13 +33,15 @@ namespace pty {
 // LDS-resident kernel (n = 64, 128)
 // ---------------------------------------------------------------------------
 
-// Column x of the resident wave used as FFT exchange scratch; element e sits
-// in row e ^ ((e >> 4) & 3) so the strided exchange spreads over banks.
+// The resident wave keeps row y at LDS row pad16(y): one spare row after every 16 spreads the
+// column exchange's stride-16 accesses over the banks, and every row/column address is a
+// per-thread base plus a compile-time offset (no per-element index arithmetic).
+// Column x of the resident wave used as FFT exchange scratch.
 struct ColBufResident {
   float2* col;
   int rs;
-  __device__ __forceinline__ float2 ld(int e) const { return col[(e ^ ((e >> 4) & 3)) * rs]; }
-  __device__ __forceinline__ void st(int e, float2 v) const { col[(e ^ ((e >> 4) & 3)) * rs] = v; }
+  __device__ __forceinline__ float2 ld(int e) const { return col[pad16(e) * rs]; }
+  __device__ __forceinline__ void st(int e, float2 v) const { col[pad16(e) * rs] = v; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
@@ -51,7 +53,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
   constexpr int G = 64 / T;       // lines per wavefront
   extern __shared__ float2 lds[];
   float2* wave = lds;
-  float2* tw = wave + N * RS;
+  float2* tw = wave + (N + N / 16) * RS;
   float2* py = tw + N;
   float2* px = py + N;
   float2* ramp_y = px + N;
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
       const int lane_ = tid_ & 63, wid_ = tid_ >> 6;
       if (s >= 0) {  // row phase of slice s
         const int rt = lane_ % T, ry = wid_ * G + lane_ / T;
-        const RowBufWave rb{wave + ry * RS};
+        const RowBufWave rb{wave + pad16(ry) * RS};
         float2 v[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rb.row[pad16(rt + T * q)];
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
           line_fft<N, true>(v, ct, cb, twt);
         } else {
 #pragma unroll
-          for (int q = 0; q < 16; ++q) v[q] = col[(ct + T * q) * RS];
+          for (int q = 0; q < 16; ++q) v[q] = col[pad16(ct + T * q) * RS];
           line_fft<N, false>(v, ct, cb, twt);
           if (s < a.S - 1) {
 #pragma unroll
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
         if (s < a.S - 1) {
           wave_sync();
 #pragma unroll
-          for (int q = 0; q < 16; ++q) col[(ct + T * q) * RS] = v[q];
+          for (int q = 0; q < 16; ++q) col[pad16(ct + T * q) * RS] = v[q];
         } else {
           const int xs = (cx + N / 2) & (N - 1);
           const float wm = (a.weights ? a.weights[m] : 1.f);
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
 
 template <int N>
 static size_t resident_lds_bytes() {
-  return sizeof(float2) * ((size_t)N * (N + N / 16) + 5 * N) + sizeof(double) * (N * N / 16 / 64);
+  return sizeof(float2) * ((size_t)(N + N / 16) * (N + N / 16) + 5 * N) + sizeof(double) * (N * N / 16 / 64);
 }
 
 // ---------------------------------------------------------------------------

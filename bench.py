@@ -87,6 +87,29 @@ def detectors_bench(w, slab, pk, weights, origins, shifts, prop, steps: int):
             "patterns_per_s": round(len(w.origins) / (ms * 1e-3), 1)}
 
 
+def captured_small_bench(w, slab, pk, weights, origins, shifts, prop, positions: int = 16, reps: int = 20):
+    """Latency of a small request (the first `positions` positions of the workload): the eager launch sequence
+    against its HIP-graph replay (engine.CapturedForward)."""
+    import torch
+    from ptyrodactyl_amd import engine
+    o, s = origins[:positions].contiguous(), shifts[:positions].contiguous()
+    cap = engine.CapturedForward(slab, pk, weights, o, s, prop)
+    out = torch.empty((positions, w.n, w.n), dtype=torch.float32, device=slab.device)
+    res = {}
+    for name, fn in (("eager", lambda: engine.multislice_forward(slab, pk, weights, o, s, prop, patterns_out=out)),
+                     ("graph", cap.replay)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        res[f"{name}_ms"] = round((time.perf_counter() - t0) * 1000.0 / reps, 3)
+    res["positions"] = positions
+    res["launches_per_call"] = 2 * w.num_slices + 2 if w.n >= 256 else 1
+    return res
+
+
 def gradient_algorithmic_bytes(n: int, s: int, k: int) -> int:
     """Per pattern: transmission read three times (forward, segment recompute, adjoint) and its gradient
     written once, checkpoints written and read, probe read and probe gradient written, pattern written
@@ -296,6 +319,8 @@ def main() -> int:
     }
     if args.workload == "config2" and not persistent:
         result["virtual_detectors"] = detectors_bench(w, slab, pk, weights, origins, shifts, prop, args.steps)
+        result["small_request_graph"] = captured_small_bench(w, slab, pk, weights, origins, shifts, prop)
+        print(f"# small request {result['small_request_graph']}", file=sys.stderr, flush=True)
         print(f"# detectors {result['virtual_detectors']}", file=sys.stderr, flush=True)
     if not args.no_gradient and args.workload == "config2":
         result["gradient"] = gradient_bench(max(args.steps, 3), 1)

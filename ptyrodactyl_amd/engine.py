@@ -102,6 +102,35 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
     return patterns, amplitude
 
 
+class CapturedForward:
+    """A forward over fixed operands captured once in a HIP graph and replayed.
+
+    The launch sequence of :func:`multislice_forward` (one launch per position
+    group for n <= 128, 2S + 2 launches per batch for n >= 256) is recorded on
+    a capture stream after one eager warm-up call, together with its
+    workspace; :meth:`replay` re-runs it on the current stream with whatever
+    the operand tensors hold at that time (update them in place between
+    replays, e.g. the object in a reconstruction loop).  The captured
+    sequence contains kernels only (no memset or copy nodes).
+    """
+
+    def __init__(self, slab_t, probe_k, weights, origins, shifts_px, prop, max_batch_waves: Optional[int] = None):
+        _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop)
+        self.operands = (slab_t, probe_k, weights, origins, shifts_px, prop)
+        p_count, n = origins.shape[0], probe_k.shape[-1]
+        self.patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=slab_t.device)
+        self.max_batch_waves = max_batch_waves
+        multislice_forward(*self.operands, patterns_out=self.patterns, max_batch_waves=max_batch_waves)  # warm-up
+        torch.cuda.synchronize(slab_t.device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            multislice_forward(*self.operands, patterns_out=self.patterns, max_batch_waves=max_batch_waves)
+
+    def replay(self) -> torch.Tensor:
+        self.graph.replay()
+        return self.patterns
+
+
 MAX_DETECTORS = 4
 
 

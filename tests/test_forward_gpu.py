@@ -209,3 +209,26 @@ def test_checked_wrappers_are_transparent():
     beam = pa.create_probe_modes(pa.checked_make_probe(mic, det)[..., None], [1.0], 0.2)
     assert torch.equal(pa.checked_cbed_image(pot, beam, mic).data_array, pa.cbed_image(pot, beam, mic).data_array)
     assert torch.equal(pa.checked_stem_4d(pot, beam, mic, det).data, pa.stem_4d(pot, beam, mic, det).data)
+
+
+@pytest.mark.parametrize("n,s_count,p_count", [(128, 4, 9), (256, 3, 6)])
+def test_captured_forward_replays_match_eager(n, s_count, p_count):
+    """Graph replays, interleaved with eager work and synchronisations, each equal the eager forward
+    bitwise, also after the object is updated in place."""
+    from ptyrodactyl_amd import engine, physics
+    slab, probes, origins, shifts, _ = _random_case(n, s_count, p_count, (n + 40, n + 30), seed=21)
+    dev = torch.device("cuda", 0)
+    slab_t = torch.tensor(slab, device=dev)
+    args = (slab_t, torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous(),
+            torch.ones(1, dtype=torch.float32, device=dev), torch.tensor(origins, device=dev),
+            torch.tensor(shifts, device=dev), physics.propagator_factors(n, 1.5, 300.0, 0.1, device=dev))
+    cap = engine.CapturedForward(*args)
+    for it in range(4):
+        if it == 2:
+            slab_t.mul_(torch.exp(1j * torch.full_like(slab_t.real, 0.05)).to(slab_t.dtype))  # in-place object update
+        eager, _ = engine.multislice_forward(*args)
+        torch.cuda.synchronize()
+        got = cap.replay().clone()
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device=dev).sum().item()  # unrelated eager work between replays
+        assert torch.equal(got, eager), f"replay {it} differs"

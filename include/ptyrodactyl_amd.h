@@ -120,6 +120,19 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
                 size_t workspace_bytes, void* stream);
 
 /*
+ * pty_adjoint seeded by the gradient of the complex amplitudes instead of the intensities:
+ * grad_amplitude is complex64 [num_positions][num_modes][n][n] in the fftshifted layout of
+ * pty_forward's amplitude output, in the dL/dRe + i dL/dIm convention.  The VJP of
+ * cbed_amplitude (src/ptyrodactyl/multislice/simulations.py:606-679) under jax.grad.  Same
+ * workspace and accumulation rules as pty_adjoint.
+ */
+int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                          long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                          const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                          const void* grad_amplitude, const void* checkpoints, int ckpt_every, void* grad_slab,
+                          void* grad_probe_k, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
  * Transmission T = exp(i sigma V) from a float64 potential (any shape, `count`
  * values), written complex64; replaces transmission_func
  * (src/ptyrodactyl/multislice/simulations.py:105-159) for a whole slab at once.

@@ -42,6 +42,8 @@ SIGNATURES = {
     "pty_adjoint_workspace_bytes": (c_size, [c_int, c_int, c_int, c_ll, c_int]),
     "pty_adjoint": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "pty_adjoint_amplitude": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                      c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_size, c_vp]),
     "pty_transmission": (c_int, [c_vp, c_dbl, c_vp, c_ll, c_vp]),
     "pty_normal_equations_workspace_bytes": (c_size, [c_ll, c_int, c_int]),
     "pty_normal_equations": (c_int, [c_vp, c_vp, c_ll, c_int, c_int, c_vp, c_size, c_vp, c_vp, c_vp, c_vp]),

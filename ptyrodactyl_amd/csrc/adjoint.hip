@@ -120,10 +120,16 @@ __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long
       line_fft<N, true>(v, t, cb, twt);
       return true;
     } else if constexpr (OP == C_FINAL_ADJ) {
-      const float two_w = 2.0f * (a.f.weights ? a.f.weights[m] : 1.f);
-      const float* gi = a.grad_i + g.pat_off + ((x + N / 2) & (N - 1));
+      if (a.grad_amp) {  // amplitude seed: gPhi = dL/dA as given
+        const float2* ga = a.grad_amp + ((size_t)p * a.f.M + m) * N * N + ((x + N / 2) & (N - 1));
 #pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = cscale(v[q], two_w * gi[(size_t)((t + T * q + N / 2) & (N - 1)) * N]);
+        for (int q = 0; q < 16; ++q) v[q] = ga[(size_t)((t + T * q + N / 2) & (N - 1)) * N];
+      } else {  // intensity seed: gPhi = 2 w g A
+        const float two_w = 2.0f * (a.f.weights ? a.f.weights[m] : 1.f);
+        const float* gi = a.grad_i + g.pat_off + ((x + N / 2) & (N - 1));
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = cscale(v[q], two_w * gi[(size_t)((t + T * q + N / 2) & (N - 1)) * N]);
+      }
       line_fft<N, true>(v, t, cb, twt);
       return true;
     } else {  // C_PROBE_GRAD

@@ -9,7 +9,8 @@ namespace pty {
 
 struct AdjParams {
   FwdParams f;              // forward operands (geometry, slab, probe_k, weights, propagators)
-  const float* grad_i;      // [P][n][n] dL/dI in the fftshifted pattern layout
+  const float* grad_i;      // [P][n][n] dL/dI in the fftshifted pattern layout, or null with grad_amp
+  const float2* grad_amp;   // [P][M][n][n] dL/dA of the fftshifted amplitudes (replaces 2 w g A), or null
   const float2* ckpt;       // [P*M][nc][n][n] checkpoints written by the forward (nc = (S-1)/k)
   int k;                    // checkpoint interval used by the forward
   float2* grad_slab;        // [S][Hs][Ws] += dL/dRe + i dL/dIm of the transmission

@@ -196,6 +196,37 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
   return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_adjoint");
 }
 
+int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                          long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                          const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                          const void* grad_amplitude, const void* checkpoints, int ckpt_every, void* grad_slab,
+                          void* grad_probe_k, void* workspace, size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                        origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
+  if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
+    return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
+                pty::num_checkpoints(num_slices, ckpt_every));
+  if (!grad_amplitude || !grad_slab) return fail(PTY_ERR_INVALID, "grad_amplitude and grad_slab must be non-null");
+  const size_t need = pty::adjoint_min_workspace_bytes(n, ckpt_every);
+  if (!workspace || workspace_bytes < need)
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one wave needs", workspace_bytes,
+                need);
+  pty::AdjParams a{};
+  a.f = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights, origins,
+                    shifts_px, prop_y, prop_x);
+  a.grad_i = nullptr;
+  a.grad_amp = (const float2*)grad_amplitude;
+  a.ckpt = (const float2*)checkpoints;
+  a.k = ckpt_every;
+  a.grad_slab = (float2*)grad_slab;
+  a.grad_probe_k = (float2*)grad_probe_k;
+  return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream),
+                       "pty_adjoint_amplitude");
+}
+
 int pty_transmission(const double* potential, double sigma, void* transmission, long long count, void* stream) {
   g_err[0] = '\0';
   if (!potential || !transmission) return fail(PTY_ERR_INVALID, "null pointer");

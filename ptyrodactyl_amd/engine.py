@@ -274,10 +274,12 @@ def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px
 
 def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_patterns, ckpt, ckpt_every: int,
                        grad_slab: Optional[torch.Tensor] = None, grad_probe_k: Optional[torch.Tensor] = None,
-                       want_probe: bool = True):
+                       want_probe: bool = True, grad_amplitude: Optional[torch.Tensor] = None):
     """Reverse-mode multislice: accumulate dL/dT into grad_slab and dL/dprobe_k into grad_probe_k.
 
-    Gradients are dL/dRe + i dL/dIm (torch's convention for a real loss).
+    The seed is dL/dI (``grad_patterns``, (P, n, n) float32), or, when ``grad_amplitude`` is given,
+    dL/dA of the fftshifted per-mode amplitudes ((P, M, n, n) complex64; ``grad_patterns`` is then
+    ignored).  Gradients are dL/dRe + i dL/dIm (torch's convention for a real loss).
     Returns (grad_slab, grad_probe_k or None).
     """
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
@@ -285,8 +287,12 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
     k = int(ckpt_every)
     _require(k >= 1, "ckpt_every must be >= 1")
     nc = num_checkpoints(s_count, k)
-    _check_cuda(grad_patterns, "grad_patterns", torch.float32, 3, dev)
-    _require(tuple(grad_patterns.shape) == (p_count, n, n), "grad_patterns must have shape (P, n, n)")
+    if grad_amplitude is not None:
+        _check_cuda(grad_amplitude, "grad_amplitude", torch.complex64, 4, dev)
+        _require(tuple(grad_amplitude.shape) == (p_count, m_count, n, n), "grad_amplitude must have shape (P, M, n, n)")
+    else:
+        _check_cuda(grad_patterns, "grad_patterns", torch.float32, 3, dev)
+        _require(tuple(grad_patterns.shape) == (p_count, n, n), "grad_patterns must have shape (P, n, n)")
     if nc:
         _check_cuda(ckpt, "checkpoints", torch.complex64, 4, dev)
         _require(tuple(ckpt.shape) == (p_count * m_count, nc, n, n), "checkpoint buffer has the wrong shape")
@@ -303,12 +309,14 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
     ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, s_count, k, p_count, m_count))
     workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
-        status = lib.pty_adjoint(
+        fn = lib.pty_adjoint_amplitude if grad_amplitude is not None else lib.pty_adjoint
+        seed = grad_amplitude if grad_amplitude is not None else grad_patterns
+        status = fn(
             n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
             _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
-            _native.ptr(grad_patterns), _native.ptr(ckpt) if nc else 0, k, _native.ptr(grad_slab),
+            _native.ptr(seed), _native.ptr(ckpt) if nc else 0, k, _native.ptr(grad_slab),
             _native.ptr(grad_probe_k), _native.ptr(workspace), ws_bytes, _native.stream_handle(dev))
-    _native.check(status, "pty_adjoint")
+    _native.check(status, "pty_adjoint_amplitude" if grad_amplitude is not None else "pty_adjoint")
     return grad_slab, grad_probe_k
 
 

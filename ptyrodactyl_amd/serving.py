@@ -184,7 +184,7 @@ class PlainExecutor:
             pat = autograd.multislice_patterns(slab, pkg, one, inp.origins, inp.shifts, prop, CKPT_EVERY)
             loss, _ = loss_and_grad_intensity(pat, inp.measured)
             loss.backward()
-            req.patterns, req.loss = pat.detach(), float(loss)
+            req.patterns, req.loss = pat.detach(), float(loss.detach())
             req.grad_slab, req.grad_probe_k = slab.grad, pkg.grad[0]
         return req
 

@@ -153,7 +153,15 @@ def _run_positions(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Mi
     p_count = shifts_px.shape[0]
     origins = torch.zeros((p_count, 2), dtype=torch.int32, device=device)
     prop = physics.propagator_factors(n, pot_slices.slice_thickness, microscope.voltage_kv, calib, device=device)
-    return engine.multislice_forward(slab, pk, weights, origins, shifts_px.to(device).contiguous(), prop,
+    shifts = shifts_px.to(device).contiguous()
+    if torch.is_grad_enabled() and (slab.requires_grad or pk.requires_grad or weights.requires_grad):
+        # differentiable path: the potential and probe modes reach the kernels through torch (exp(i sigma V),
+        # fft2) and the HIP adjoint supplies the gradient w.r.t. the transmission and probe_k
+        from . import autograd
+        if want_amplitude:
+            return (None, autograd.multislice_amplitudes(slab, pk, origins, shifts, prop)), calib, n
+        return (autograd.multislice_patterns(slab, pk, weights, origins, shifts, prop), None), calib, n
+    return engine.multislice_forward(slab, pk, weights, origins, shifts, prop,
                                      want_patterns=True, want_amplitude=want_amplitude), calib, n
 
 

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 2
+#define PTY_ABI_VERSION 3
 
 #define PTY_OK 0
 #define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
@@ -112,12 +112,18 @@ size_t pty_adjoint_workspace_bytes(int n, int num_slices, int ckpt_every, long l
  * stored and then summed per slab pixel (and probe element) in wave order,
  * so the gradients are bitwise reproducible when the workspace is at least
  * pty_adjoint_workspace_bytes; a smaller one falls back to float32 atomics.
+ * grad_shifts (float64 [num_positions][2] as (y, x), may be NULL) ACCUMULATES
+ * dL/d(shifts_px), the gradient w.r.t. the probe positions that the
+ * position-correcting reconstructors optimise
+ * (src/ptyrodactyl/multislice/multislice_recon.py:849-885, 1915-1951); it is
+ * summed in FP64 in a fixed order from the stored probe contributions and
+ * therefore needs the full pty_adjoint_workspace_bytes.
  */
 int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
                 long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
                 const double* shifts_px, const void* prop_y, const void* prop_x, const float* grad_patterns,
-                const void* checkpoints, int ckpt_every, void* grad_slab, void* grad_probe_k, void* workspace,
-                size_t workspace_bytes, void* stream);
+                const void* checkpoints, int ckpt_every, void* grad_slab, void* grad_probe_k, double* grad_shifts,
+                void* workspace, size_t workspace_bytes, void* stream);
 
 /*
  * pty_adjoint seeded by the gradient of the complex amplitudes instead of the intensities:
@@ -130,7 +136,8 @@ int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int nu
                           long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
                           const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
                           const void* grad_amplitude, const void* checkpoints, int ckpt_every, void* grad_slab,
-                          void* grad_probe_k, void* workspace, size_t workspace_bytes, void* stream);
+                          void* grad_probe_k, double* grad_shifts, void* workspace, size_t workspace_bytes,
+                          void* stream);
 
 /*
  * Transmission T = exp(i sigma V) from a float64 potential (any shape, `count`

@@ -23,7 +23,10 @@ Parity pins (see ``tests/test_oracle.py``):
   on tiny grids, translation equivariance).
 * ``multislice_vjp`` (the gradient): central finite differences in both
   real and imaginary directions, an independent torch autograd twin, and
-  the adjoint dot-product identity.
+  the adjoint dot-product identity; its shift gradient against central
+  differences of ``window_stem`` in the shifts and against the closed form
+  for a vacuum slab (intensity is shift-invariant, so the gradient is 0
+  there while the probe gradient is not).
 * ``window_stem``: equals ``stem_4d`` when the slab is one window tiled
   periodically; equals ``stem_4d`` on an explicitly cropped window.
 * ``aberration12`` / ``probe_from_aberrations``: reduces to the reference
@@ -367,15 +370,19 @@ def window_stem_batched(slab_t, probe, origins, shifts_px, prop, workers: int = 
 # Gradient (reverse mode) of the windowed forward, complex128.
 # ---------------------------------------------------------------------------
 
-def multislice_vjp(slab_t, probe, origins, shifts_px, prop, grad_intensity):
-    """Vector-Jacobian product of ``window_stem`` w.r.t. the slab and probe.
+def multislice_vjp(slab_t, probe, origins, shifts_px, prop, grad_intensity, want_shifts: bool = False):
+    """Vector-Jacobian product of ``window_stem`` w.r.t. the slab, probe and shifts.
 
     Convention: for a real loss L, returns dL/dRe + i dL/dIm for every complex
     input (the torch convention; ``jax.grad`` returns the conjugate).  The
     reference differentiates the same scan with ``jax.value_and_grad``
-    (``multislice/multislice_recon.py:1915-1951``).
+    (``multislice/multislice_recon.py:1915-1951``), including the scan
+    positions in its position-correcting reconstructors
+    (``multislice_recon.py:849-885``).
 
-    grad_intensity: (P, N, N) real dL/dI.  Returns (grad_slab, grad_probe).
+    grad_intensity: (P, N, N) real dL/dI.  Returns (grad_slab, grad_probe),
+    plus grad_shifts (P, 2) = dL/d(shift_y, shift_x) in pixels when
+    ``want_shifts``.
     """
     n = probe.shape[-1]
     s_count = slab_t.shape[0]
@@ -383,6 +390,8 @@ def multislice_vjp(slab_t, probe, origins, shifts_px, prop, grad_intensity):
     grad_slab = np.zeros(slab_t.shape, np.complex128)
     probe_k = np.fft.fft2(np.asarray(probe, np.complex128))
     grad_probe_k = np.zeros((n, n), np.complex128)
+    grad_shifts = np.zeros((len(origins), 2))
+    kk = np.fft.fftfreq(n) * n  # Nyquist negative, as in ramp_1d
     for p in range(len(origins)):
         ramp = ramp_1d(n, shifts_px[p][0])[:, None] * ramp_1d(n, shifts_px[p][1])[None, :]
         psi = [np.fft.ifft2(probe_k * ramp)]
@@ -402,9 +411,16 @@ def multislice_vjp(slab_t, probe, origins, shifts_px, prop, grad_intensity):
                 g_phi = np.fft.ifft2(np.fft.fft2(g_psi) * np.conj(prop))
             np.add.at(grad_slab[s], (rows[:, None], cols[None, :]), np.conj(psi[s]) * g_phi)
             g_psi = np.conj(t_win[s]) * g_phi
-        # psi0 = ifft2(probe_k * ramp), probe_k = fft2(probe)
-        grad_probe_k += np.conj(ramp) * np.fft.fft2(g_psi) / (n * n)
+        # psi0 = ifft2(u), u = probe_k * ramp, probe_k = fft2(probe)
+        g_u = np.fft.fft2(g_psi) / (n * n)
+        grad_probe_k += np.conj(ramp) * g_u
+        # d ramp / d shift_y = ramp * (-2 pi i k_y / n); dL/ds = Re sum conj(g_u) du/ds
+        du = probe_k * ramp * (-2j * np.pi / n)
+        grad_shifts[p, 0] = np.real(np.sum(np.conj(g_u) * du * kk[:, None]))
+        grad_shifts[p, 1] = np.real(np.sum(np.conj(g_u) * du * kk[None, :]))
     grad_probe = np.fft.ifft2(grad_probe_k) * (n * n)
+    if want_shifts:
+        return grad_slab, grad_probe, grad_shifts
     return grad_slab, grad_probe
 
 

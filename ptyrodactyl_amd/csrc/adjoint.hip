@@ -132,10 +132,10 @@ __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long
       }
       line_fft<N, true>(v, t, cb, twt);
       return true;
-    } else {  // C_PROBE_GRAD
-      if (a.grad_probe_k) {
+    } else {  // C_PROBE_GRAD (its per-wave contributions also feed the position gradient)
+      if (a.grad_probe_k || a.gp_contrib) {
         const float2 rx = ramp_x ? ramp_x[x] : shift_ramp(x, N, g.mx, g.dx);
-        float2* gp = a.grad_probe_k + g.gprobe_off + (size_t)m * N * N + x;
+        float2* gp = a.grad_probe_k ? a.grad_probe_k + g.gprobe_off + (size_t)m * N * N + x : nullptr;
         float2* gc = a.gp_contrib ? a.gp_contrib + (size_t)(w - a.w_begin) * N * N + x : nullptr;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
@@ -540,6 +540,39 @@ __global__ __launch_bounds__(256) void k_probe_reduce(AdjParams a, long long w0,
   a.grad_probe_k[idx] = cadd(a.grad_probe_k[idx], acc);
 }
 
+// Position gradient.  With u = probe_k R, R(k) = exp(-2 pi i k.s / n) / n^2 and the stored per-wave
+// gp = conj(R) F(g psi_0), dL/ds_y = Re sum_k conj(gp) probe_k (-2 pi i k_y / n)
+//                                  = (2 pi / n) sum_k k_y Im(conj(gp) probe_k),
+// k in fftfreq order (Nyquist negative); likewise for x.  One workgroup per position of the chunk,
+// summing its modes' waves in the chunk in a fixed order.
+template <int N>
+__global__ __launch_bounds__(256) void k_pos_grad(AdjParams a, long long w0, int cnt) {
+  __shared__ double red[4];
+  const int M = a.f.M;
+  const long long p = w0 / M + blockIdx.x;
+  double sy = 0.0, sx = 0.0;
+  for (int m = 0; m < M; ++m) {
+    const long long w = p * M + m;
+    if (w < w0 || w >= w0 + cnt) continue;  // uniform over the workgroup
+    const float2* gp = a.gp_contrib + (size_t)(w - w0) * N * N;
+    const float2* pk = a.f.probe_k + (size_t)m * N * N;
+    for (int e = threadIdx.x; e < N * N; e += 256) {
+      const int ky = e / N, kx = e % N;
+      const float2 v = cmulc(pk[e], gp[e]);  // probe_k conj(gp)
+      const double im = (double)v.y;
+      sy += (double)(ky < N / 2 ? ky : ky - N) * im;
+      sx += (double)(kx < N / 2 ? kx : kx - N) * im;
+    }
+  }
+  const double ty = block_sum_fp64<256>(sy, red);
+  const double tx = block_sum_fp64<256>(sx, red);
+  if (threadIdx.x == 0) {
+    const double scale = 2.0 * 3.14159265358979323846 / N;
+    a.grad_shifts[2 * p] += scale * ty;
+    a.grad_shifts[2 * p + 1] += scale * tx;
+  }
+}
+
 template <int N>
 static void launch_reduces(const AdjParams& a, long long w0, int cnt, int s_first, int s_count, bool probe,
                            hipStream_t st) {
@@ -554,9 +587,13 @@ static void launch_reduces(const AdjParams& a, long long w0, int cnt, int s_firs
       hipLaunchKernelGGL((k_grad_reduce<N, 8>), grid, dim3(256), 0, st, a, w0, cnt, s_first);
     }
   }
-  if (probe) {
+  if (probe && a.grad_probe_k) {
     const long long total = (long long)a.f.M * N * N;
     hipLaunchKernelGGL(k_probe_reduce<N>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a, w0, cnt);
+  }
+  if (probe && a.grad_shifts) {
+    const long long positions = (w0 + cnt - 1) / a.f.M - w0 / a.f.M + 1;
+    hipLaunchKernelGGL(k_pos_grad<N>, dim3((unsigned)positions), dim3(256), 0, st, a, w0, cnt);
   }
 }
 
@@ -640,7 +677,7 @@ static hipError_t launch_adj_resident(const AdjParams& a0, void* ws, size_t ws_b
     const long long fit = (long long)(ws_bytes / per_slot);
     if (slots > fit) slots = fit;
   }
-  if (slots < 1) return hipErrorInvalidValue;
+  if (slots < 1 || (a0.grad_shifts && !gathered)) return hipErrorInvalidValue;
   float2* scratch = (float2*)ws;
   float2* contrib = scratch + (size_t)slots * (a0.k + 1) * NN;
   for (long long c0 = 0; c0 < waves; c0 += chunk) {
@@ -649,11 +686,11 @@ static hipError_t launch_adj_resident(const AdjParams& a0, void* ws, size_t ws_b
     a.w_end = c0 + cnt;
     if (gathered) {
       a.gt_contrib = contrib;
-      a.gp_contrib = a0.grad_probe_k ? contrib + (size_t)chunk * a0.f.S * NN : nullptr;
+      a.gp_contrib = a0.grad_probe_k || a0.grad_shifts ? contrib + (size_t)chunk * a0.f.S * NN : nullptr;
     }
     const long long grid = slots < cnt ? slots : cnt;
     hipLaunchKernelGGL(k_adjoint_resident<N>, dim3((unsigned)grid), dim3(N * N / 16), lds, st, a, scratch);
-    if (gathered) launch_reduces<N>(a, c0, (int)cnt, 0, a0.f.S, a0.grad_probe_k != nullptr, st);
+    if (gathered) launch_reduces<N>(a, c0, (int)cnt, 0, a0.f.S, a0.grad_probe_k || a0.grad_shifts, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -671,7 +708,7 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
   const bool gathered = !a0.f.desc && ws_bytes >= (size_t)(2 * k + 3) * NN * sizeof(float2);
   long long nbmax = (long long)(ws_bytes / ((size_t)(gathered ? 2 * k + 3 : k + 2) * NN * sizeof(float2)));
   if (nbmax > waves) nbmax = waves;
-  if (nbmax < 1) return hipErrorInvalidValue;
+  if (nbmax < 1 || (a0.grad_shifts && !gathered)) return hipErrorInvalidValue;
   if (nbmax >= 8) nbmax -= nbmax % 8;
   float2* base = (float2*)ws;
   for (long long w0 = 0; w0 < waves; w0 += nbmax) {
@@ -685,7 +722,7 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
     a.w_end = w0 + nb;
     a.contrib_slices = k;
     a.gt_contrib = gathered ? base + (size_t)nb * (k + 2) * NN : nullptr;
-    a.gp_contrib = gathered && a0.grad_probe_k ? base + (size_t)nb * (2 * k + 2) * NN : nullptr;
+    a.gp_contrib = gathered && (a0.grad_probe_k || a0.grad_shifts) ? base + (size_t)nb * (2 * k + 2) * NN : nullptr;
     const dim3 rg(nb * (N / RPB)), cg(nb * (N / NC)), blk(256);
     for (int c = C - 1; c >= 0; --c) {
       const int s0 = c * k, s1 = min(S, s0 + k);
@@ -733,7 +770,7 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
     hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, Gb, Gb, none, w0, nb, 0,
                        (int)PW_NONE, 0, 0);
     hipLaunchKernelGGL((k_adj_cols<N, C_PROBE_GRAD>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
-    if (gathered && a0.grad_probe_k) launch_reduces<N>(a, w0, nb, 0, 0, true, st);
+    if (gathered && (a0.grad_probe_k || a0.grad_shifts)) launch_reduces<N>(a, w0, nb, 0, 0, true, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -15,6 +15,7 @@ struct AdjParams {
   int k;                    // checkpoint interval used by the forward
   float2* grad_slab;        // [S][Hs][Ws] += dL/dRe + i dL/dIm of the transmission
   float2* grad_probe_k;     // [M][n][n]   += gradient w.r.t. probe_k, or null
+  double* grad_shifts;      // [P][2] += dL/d(probe shift in pixels), or null (gathered mode only)
   // Set by the launchers.  When gt_contrib is non-null the per-wave contributions to the slab
   // gradient are stored (window coordinates, [wave - w_begin][contrib_slices][n][n]) and gathered
   // in a fixed order by k_grad_reduce; otherwise they are added with float32 atomics.  gp_contrib

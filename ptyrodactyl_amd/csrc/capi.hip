@@ -170,8 +170,8 @@ size_t pty_adjoint_workspace_bytes(int n, int num_slices, int ckpt_every, long l
 int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
                 long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
                 const double* shifts_px, const void* prop_y, const void* prop_x, const float* grad_patterns,
-                const void* checkpoints, int ckpt_every, void* grad_slab, void* grad_probe_k, void* workspace,
-                size_t workspace_bytes, void* stream) {
+                const void* checkpoints, int ckpt_every, void* grad_slab, void* grad_probe_k, double* grad_shifts,
+                void* workspace, size_t workspace_bytes, void* stream) {
   g_err[0] = '\0';
   int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
                         origins, shifts_px, prop_y, prop_x);
@@ -185,6 +185,12 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
   if (!workspace || workspace_bytes < need)
     return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one wave needs", workspace_bytes,
                 need);
+  if (grad_shifts) {
+    const size_t full = pty::adjoint_workspace_bytes(n, num_slices, ckpt_every, num_positions * num_modes);
+    if (workspace_bytes < full)
+      return fail(PTY_ERR_INVALID, "position gradients need the gathered workspace of %zu bytes (got %zu)", full,
+                  workspace_bytes);
+  }
   pty::AdjParams a{};
   a.f = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights, origins,
                     shifts_px, prop_y, prop_x);
@@ -193,6 +199,7 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
   a.k = ckpt_every;
   a.grad_slab = (float2*)grad_slab;
   a.grad_probe_k = (float2*)grad_probe_k;
+  a.grad_shifts = grad_shifts;
   return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_adjoint");
 }
 
@@ -200,7 +207,8 @@ int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int nu
                           long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
                           const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
                           const void* grad_amplitude, const void* checkpoints, int ckpt_every, void* grad_slab,
-                          void* grad_probe_k, void* workspace, size_t workspace_bytes, void* stream) {
+                          void* grad_probe_k, double* grad_shifts, void* workspace, size_t workspace_bytes,
+                          void* stream) {
   g_err[0] = '\0';
   int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
                         origins, shifts_px, prop_y, prop_x);
@@ -214,6 +222,12 @@ int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int nu
   if (!workspace || workspace_bytes < need)
     return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one wave needs", workspace_bytes,
                 need);
+  if (grad_shifts) {
+    const size_t full = pty::adjoint_workspace_bytes(n, num_slices, ckpt_every, num_positions * num_modes);
+    if (workspace_bytes < full)
+      return fail(PTY_ERR_INVALID, "position gradients need the gathered workspace of %zu bytes (got %zu)", full,
+                  workspace_bytes);
+  }
   pty::AdjParams a{};
   a.f = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights, origins,
                     shifts_px, prop_y, prop_x);
@@ -223,6 +237,7 @@ int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int nu
   a.k = ckpt_every;
   a.grad_slab = (float2*)grad_slab;
   a.grad_probe_k = (float2*)grad_probe_k;
+  a.grad_shifts = grad_shifts;
   return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream),
                        "pty_adjoint_amplitude");
 }

@@ -274,12 +274,14 @@ def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px
 
 def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_patterns, ckpt, ckpt_every: int,
                        grad_slab: Optional[torch.Tensor] = None, grad_probe_k: Optional[torch.Tensor] = None,
-                       want_probe: bool = True, grad_amplitude: Optional[torch.Tensor] = None):
+                       want_probe: bool = True, grad_amplitude: Optional[torch.Tensor] = None,
+                       grad_shifts: Optional[torch.Tensor] = None):
     """Reverse-mode multislice: accumulate dL/dT into grad_slab and dL/dprobe_k into grad_probe_k.
 
     The seed is dL/dI (``grad_patterns``, (P, n, n) float32), or, when ``grad_amplitude`` is given,
     dL/dA of the fftshifted per-mode amplitudes ((P, M, n, n) complex64; ``grad_patterns`` is then
     ignored).  Gradients are dL/dRe + i dL/dIm (torch's convention for a real loss).
+    ``grad_shifts`` ((P, 2) float64, may be None) accumulates dL/d(shifts_px).
     Returns (grad_slab, grad_probe_k or None).
     """
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
@@ -305,6 +307,9 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
     if grad_probe_k is not None:
         _check_cuda(grad_probe_k, "grad_probe_k", torch.complex64, 3, dev)
         _require(grad_probe_k.shape == probe_k.shape, "grad_probe_k must match probe_k")
+    if grad_shifts is not None:
+        _check_cuda(grad_shifts, "grad_shifts", torch.float64, 2, dev)
+        _require(tuple(grad_shifts.shape) == (p_count, 2), "grad_shifts must have shape (P, 2)")
     lib = _native.load()
     ws_bytes = int(lib.pty_adjoint_workspace_bytes(n, s_count, k, p_count, m_count))
     workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
@@ -315,7 +320,8 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
             n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
             _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
             _native.ptr(seed), _native.ptr(ckpt) if nc else 0, k, _native.ptr(grad_slab),
-            _native.ptr(grad_probe_k), _native.ptr(workspace), ws_bytes, _native.stream_handle(dev))
+            _native.ptr(grad_probe_k), _native.ptr(grad_shifts), _native.ptr(workspace), ws_bytes,
+            _native.stream_handle(dev))
     _native.check(status, "pty_adjoint_amplitude" if grad_amplitude is not None else "pty_adjoint")
     return grad_slab, grad_probe_k
 

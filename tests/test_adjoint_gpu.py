@@ -231,7 +231,7 @@ def _adjoint_through_abi(slab, probes, origins, shifts, g, n, k, workspace_bytes
     status = lib.pty_adjoint(n, s_count, p_count, m_count, _native.ptr(slab_t), slab.shape[1], slab.shape[2],
                              _native.ptr(pk), _native.ptr(wt), _native.ptr(ot), _native.ptr(st), _native.ptr(prop),
                              _native.ptr(prop), _native.ptr(gt), _native.ptr(ckpt) if nc else 0, k,
-                             _native.ptr(g_slab), _native.ptr(g_pk), _native.ptr(ws), nbytes,
+                             _native.ptr(g_slab), _native.ptr(g_pk), 0, _native.ptr(ws), nbytes,
                              _native.stream_handle(dev))
     _native.check(status, "pty_adjoint")
     torch.cuda.synchronize()
@@ -260,4 +260,4 @@ def test_atomic_fallback_matches_gathered(n):
     at_slab, at_pk, _ = _adjoint_through_abi(slab, probes, origins, shifts, g, n, k, workspace_bytes=small)
     assert rel_l2(at_slab.cpu().numpy(), full_slab.cpu().numpy()) < 1e-6
     assert rel_l2(at_pk.cpu().numpy(), full_pk.cpu().numpy()) < 1e-6
-    assert _native.load().pty_abi_version() == 2
+    assert _native.load().pty_abi_version() == _native.ABI_VERSION

@@ -236,6 +236,58 @@ def test_vjp_matches_torch_autograd_twin():
     assert np.allclose(tp.grad.numpy(), g_probe, atol=1e-9)
 
 
+def test_shift_vjp_matches_central_differences():
+    slab, probe, origins, shifts, prop, c = _small_problem(seed=11)
+    _, _, g_shift = o.multislice_vjp(slab, probe, origins, shifts, prop, c, want_shifts=True)
+
+    def loss(sh):
+        return float(np.sum(c * o.window_stem(slab, probe, origins, sh, prop)))
+
+    h = 1e-6
+    for p in range(len(origins)):
+        for ax in range(2):
+            sp, sm = shifts.copy(), shifts.copy()
+            sp[p, ax] += h
+            sm[p, ax] -= h
+            fd = (loss(sp) - loss(sm)) / (2 * h)
+            assert fd == pytest.approx(g_shift[p, ax], rel=1e-5, abs=1e-7)
+
+
+def test_shift_vjp_matches_torch_autograd_twin():
+    torch = pytest.importorskip("torch")
+    slab, probe, origins, shifts, prop, c = _small_problem(seed=12)
+    _, _, g_shift = o.multislice_vjp(slab, probe, origins, shifts, prop, c, want_shifts=True)
+    n = probe.shape[0]
+    tsh = torch.tensor(shifts, requires_grad=True)
+    tprop = torch.tensor(prop)
+    k = torch.tensor(np.fft.fftfreq(n) * n)
+    pk = torch.fft.fft2(torch.tensor(probe))
+    total = 0
+    for p in range(len(origins)):
+        ry = torch.exp(-2j * np.pi * k * tsh[p, 0] / n)
+        rx = torch.exp(-2j * np.pi * k * tsh[p, 1] / n)
+        wave = torch.fft.ifft2(pk * ry[:, None] * rx[None, :])
+        win = torch.tensor(o.window_of(slab, origins[p], n))
+        for s in range(slab.shape[0]):
+            wave = wave * win[s]
+            if s != slab.shape[0] - 1:
+                wave = torch.fft.ifft2(torch.fft.fft2(wave) * tprop)
+        inten = torch.abs(torch.fft.fftshift(torch.fft.fft2(wave))) ** 2
+        total = total + (torch.tensor(c[p]) * inten).sum()
+    total.backward()
+    assert np.allclose(tsh.grad.numpy(), g_shift, rtol=1e-9, atol=1e-9)
+
+
+def test_shift_vjp_vanishes_in_vacuum():
+    """With T = 1 every pattern is |F(P psi0)|^2 and the shift is a Fourier phase: the intensity does not depend
+    on it, so the shift gradient is zero while the probe gradient is not."""
+    slab, probe, origins, shifts, prop, c = _small_problem(seed=13)
+    vac = np.ones_like(slab)
+    _, g_probe, g_shift = o.multislice_vjp(vac, probe, origins, shifts, prop, c, want_shifts=True)
+    assert np.abs(g_probe).max() > 1e-3
+    assert np.abs(g_shift).max() < 1e-9 * np.abs(g_probe).max() * probe.size
+
+
 def test_jvp_vjp_adjoint_identity():
     slab, probe, origins, shifts, prop, c = _small_problem(seed=9)
     rng = np.random.default_rng(10)

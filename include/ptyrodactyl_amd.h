@@ -221,6 +221,25 @@ int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pt
 int pty_annular_integrate(const float* patterns, long long num_patterns, long long pixels_per_pattern,
                           const unsigned char* mask, double* out, void* stream);
 
+/*
+ * One in-place step of the Optax update rules the reconstructors use
+ * (optax 0.2.3 as configured in src/ptyrodactyl/multislice/multislice_recon.py:70-83,
+ * applied by _apply_optimizer_step :166-210 and the position updates :914-1000):
+ *   PTY_OPTAX_ADAM     adam(lr): b1 0.9, b2 0.999, eps 1e-8
+ *   PTY_OPTAX_ADAGRAD  adagrad(lr, initial_accumulator_value=0, eps=1e-8)
+ *   PTY_OPTAX_RMSPROP  rmsprop(lr, decay=0.9, eps=1e-8, initial_scale=0)
+ * params, grads and mu hold `count` elements of float32 (fp64 = 0) or float64 (fp64 = 1),
+ * interleaved (re, im) when is_complex; nu holds `count` real second moments (|g|^2 for complex
+ * elements, as Optax's abs_sq); mu is used by adam only.  grads is torch's dL/dRe + i dL/dIm,
+ * which equals the conj(jax.grad) the reference passes to Optax.  step is the 1-based count
+ * after this update (adam's bias correction).  Moments start at zero (optimizer init).
+ */
+#define PTY_OPTAX_ADAM 0
+#define PTY_OPTAX_ADAGRAD 1
+#define PTY_OPTAX_RMSPROP 2
+int pty_optimizer_update(int rule, int fp64, int is_complex, long long count, void* params, const void* grads,
+                         void* mu, void* nu, long long step, double learning_rate, void* stream);
+
 /* Device bytes of the state block and workspace of pty_forward_persistent. */
 size_t pty_persistent_state_bytes(void);
 size_t pty_persistent_workspace_bytes(int n, int group_waves);

@@ -2,6 +2,7 @@
 // stream; no framework types cross this line.  Every entry point returns a
 // status (PTY_OK, PTY_ERR_INVALID, PTY_ERR_DEVICE) and records a message
 // readable with pty_last_error() on the calling thread.
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -10,6 +11,7 @@
 #include "adjoint.hpp"
 #include "aux_kernels.hpp"
 #include "forward.hpp"
+#include "optim.hpp"
 #include "xl2.hpp"
 
 namespace {
@@ -294,6 +296,32 @@ int pty_annular_integrate(const float* patterns, long long num_patterns, long lo
   return device_status(pty::launch_annular(patterns, num_patterns, pixels_per_pattern, mask, out,
                                            (hipStream_t)stream),
                        "pty_annular_integrate");
+}
+
+int pty_optimizer_update(int rule, int fp64, int is_complex, long long count, void* params, const void* grads,
+                         void* mu, void* nu, long long step, double learning_rate, void* stream) {
+  g_err[0] = '\0';
+  if (rule < PTY_OPTAX_ADAM || rule > PTY_OPTAX_RMSPROP) return fail(PTY_ERR_INVALID, "unknown update rule %d", rule);
+  if (count < 0) return fail(PTY_ERR_INVALID, "count must be >= 0");
+  if (count > 0 && (!params || !grads || !nu)) return fail(PTY_ERR_INVALID, "null pointer");
+  if (count > 0 && rule == PTY_OPTAX_ADAM && !mu) return fail(PTY_ERR_INVALID, "adam needs the first-moment buffer");
+  if (rule == PTY_OPTAX_ADAM && step < 1) return fail(PTY_ERR_INVALID, "adam step count must be >= 1");
+  pty::OptaxStep s{};
+  s.neg_lr = -learning_rate;
+  s.eps = 1e-8;
+  if (rule == PTY_OPTAX_ADAM) {
+    s.b1 = 0.9;
+    s.b2 = 0.999;
+    s.bc1 = 1.0 - pow(0.9, (double)step);
+    s.bc2 = 1.0 - pow(0.999, (double)step);
+  } else if (rule == PTY_OPTAX_RMSPROP) {
+    s.b1 = 0.9;  // decay
+  }
+  s.one_minus_b1 = 1.0 - s.b1;
+  s.one_minus_b2 = 1.0 - s.b2;
+  return device_status(pty::launch_optax(rule, fp64 != 0, is_complex != 0, count, params, grads, mu, nu, s,
+                                         (hipStream_t)stream),
+                       "pty_optimizer_update");
 }
 
 size_t pty_persistent_state_bytes(void) { return pty::xl2_state_bytes(); }

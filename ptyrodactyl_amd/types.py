@@ -17,6 +17,7 @@ as in the reference:
 from __future__ import annotations
 
 import os
+from enum import Enum
 from dataclasses import dataclass, field
 from typing import Optional, Tuple
 
@@ -91,6 +92,15 @@ def _optional_positions(pos, name: str) -> Optional[torch.Tensor]:
         raise ValueError(f"{name} must have shape (P, 2)")
     error_if(~torch.isfinite(t).all(), f"{name} contain non-finite values")
     return t
+
+
+class LossType(str, Enum):
+    """Static loss selection of the reconstructors (``types/custom_types.py:50-70``): mean absolute,
+    mean squared or root-mean-squared error.  ``LossType("bogus")`` raises ``ValueError``."""
+
+    MAE = "mae"
+    MSE = "mse"
+    RMSE = "rmse"
 
 
 @dataclass(frozen=True)

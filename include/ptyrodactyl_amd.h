@@ -197,13 +197,18 @@ typedef struct pty_position_desc {
 /*
  * pty_forward over positions of several requests in ONE launch sequence
  * (continuous batching).  descs: DEVICE array [num_positions]; the caller
- * validates offsets against its arenas.  checkpoints (optional, k = ckpt_every)
+ * validates offsets against its arenas.  amplitudes (optional, complex64
+ * [num_positions][n][n], fftshifted) and checkpoints (optional, k = ckpt_every)
  * are indexed by the call-local position as in pty_forward_checkpointed.
  * Each position's result is bitwise identical to running its request alone.
+ * The ensemble-axis path of cbed_image / stem_4d
+ * (src/ptyrodactyl/multislice/simulations.py:769-887, 1130-1233) runs one
+ * descriptor per (position, ensemble sample) through this entry point.
  */
 int pty_forward_batched(int n, int num_slices, long long num_positions, const pty_position_desc* descs,
                         const void* slab_arena, const void* probe_arena, const void* prop_arena, float* pattern_arena,
-                        void* checkpoints, int ckpt_every, void* workspace, size_t workspace_bytes, void* stream);
+                        void* amplitudes, void* checkpoints, int ckpt_every, void* workspace, size_t workspace_bytes,
+                        void* stream);
 
 /* pty_adjoint for a batched launch: gradients accumulate into each request's slice of the gradient arenas. */
 int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pty_position_desc* descs,

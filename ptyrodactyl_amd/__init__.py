@@ -8,12 +8,18 @@ runs in hand-written gfx950 HIP kernels behind the C-ABI declared in
 """
 
 from .checked import checked_cbed_image, checked_make_probe, checked_stem4d_sharded, checked_stem_4d
+from .distributions import (TRIVIAL, TRIVIAL_DISTRIBUTION, AxisUpdate, Distribution, ReductionMode,
+                            combine_axis_updates, create_axis_update, create_distribution,
+                            create_trivial_distribution)
+from .producers import coherence_to_distribution, position_jitter_to_distribution
+from .reduce import apply_distribution, apply_distributions
 from .parallelized import AtomicSliceData, create_atomic_slice_data, stem4d_sharded
 from .physics import interaction_sigma, phase_interaction_parameter, relativistic_wavelength_ang
 from .recon import (OPTIMIZERS, multi_slice_multi_modal, single_slice_multi_modal, single_slice_poscorrected,
                     single_slice_ptychography)
-from .simulations import (aberration, annular_detector, cbed_amplitude, cbed_image, make_probe, propagation_func,
-                          shift_beam_fourier, stem_4d, stem_4d_windows, transmission_func)
+from .simulations import (aberration, annular_detector, bind_cbed_axes, cbed_amplitude, cbed_image, make_probe,
+                          probe_modes_to_distribution, propagation_func, shift_beam_fourier, stem_4d, stem_4d_windows,
+                          transmission_func)
 from .types import (STEM4D, CalibratedArray, DetectorConfig, EnsembleAxes, LossType, MicroscopeConfig, PotentialSlices,
                     ProbeModes, RuntimeCheckError, create_calibrated_array, create_detector_config,
                     create_ensemble_axes, create_microscope_config, create_potential_slices, create_probe_modes,
@@ -28,4 +34,8 @@ __all__ = [
     "create_probe_modes", "create_stem4d", "checked_cbed_image", "checked_make_probe", "checked_stem4d_sharded",
     "checked_stem_4d", "AtomicSliceData", "create_atomic_slice_data", "stem4d_sharded", "LossType", "OPTIMIZERS",
     "single_slice_ptychography", "single_slice_poscorrected", "single_slice_multi_modal", "multi_slice_multi_modal",
+    "TRIVIAL", "TRIVIAL_DISTRIBUTION", "AxisUpdate", "Distribution", "ReductionMode", "combine_axis_updates",
+    "create_axis_update", "create_distribution", "create_trivial_distribution", "coherence_to_distribution",
+    "position_jitter_to_distribution", "apply_distribution", "apply_distributions", "bind_cbed_axes",
+    "probe_modes_to_distribution",
 ]

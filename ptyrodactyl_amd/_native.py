@@ -48,7 +48,7 @@ SIGNATURES = {
     "pty_normal_equations_workspace_bytes": (c_size, [c_ll, c_int, c_int]),
     "pty_normal_equations": (c_int, [c_vp, c_vp, c_ll, c_int, c_int, c_vp, c_size, c_vp, c_vp, c_vp, c_vp]),
     "pty_damped_solve": (c_int, [c_vp, c_vp, c_dbl, c_int, c_vp, c_vp, c_vp, c_vp]),
-    "pty_forward_batched": (c_int, [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size,
+    "pty_forward_batched": (c_int, [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size,
                                     c_vp]),
     "pty_adjoint_batched": (c_int, [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
                                     c_size, c_vp]),

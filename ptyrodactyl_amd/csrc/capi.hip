@@ -377,7 +377,8 @@ static pty::FwdParams batched_params(int n, int num_slices, long long num_positi
 
 int pty_forward_batched(int n, int num_slices, long long num_positions, const pty_position_desc* descs,
                         const void* slab_arena, const void* probe_arena, const void* prop_arena, float* pattern_arena,
-                        void* checkpoints, int ckpt_every, void* workspace, size_t workspace_bytes, void* stream) {
+                        void* amplitudes, void* checkpoints, int ckpt_every, void* workspace, size_t workspace_bytes,
+                        void* stream) {
   g_err[0] = '\0';
   int rc = check_batched(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
   if (rc) return rc;
@@ -389,6 +390,7 @@ int pty_forward_batched(int n, int num_slices, long long num_positions, const pt
                 need);
   pty::FwdParams a = batched_params(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
   a.patterns = pattern_arena;
+  a.amplitude = (float2*)amplitudes;
   if (checkpoints && pty::num_checkpoints(num_slices, ckpt_every) > 0) {
     a.ckpt = (float2*)checkpoints;
     a.ckpt_every = ckpt_every;

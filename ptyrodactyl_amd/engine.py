@@ -355,18 +355,24 @@ def element_offset(t: torch.Tensor, base: torch.Tensor) -> int:
 
 def forward_batched(n: int, num_slices: int, descs: torch.Tensor, base: torch.Tensor,
                     ckpt: Optional[torch.Tensor] = None, ckpt_every: int = 0,
-                    max_batch_waves: Optional[int] = None) -> None:
-    """Launch the forward over ``descs`` (device uint8 view of pty_position_desc records)."""
+                    max_batch_waves: Optional[int] = None, amplitudes: Optional[torch.Tensor] = None) -> None:
+    """Launch the forward over ``descs`` (device uint8 view of pty_position_desc records).
+
+    ``amplitudes`` (optional, (P, n, n) complex64, call-local position order) receives the fftshifted exit-plane
+    amplitudes."""
     dev = base.device
     p_count = descs.numel() // 88
     _require(p_count >= 1 and descs.numel() == p_count * 88, "descriptor buffer must hold whole records")
     ws_bytes = workspace_bytes(n, p_count, 1, max_batch_waves)
     workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev) if ws_bytes else None
     lib = _native.load()
+    if amplitudes is not None:
+        _check_cuda(amplitudes, "amplitudes", torch.complex64, 3, dev)
+        _require(tuple(amplitudes.shape) == (p_count, n, n), "amplitudes must have shape (P, n, n)")
     b = _native.ptr(base)
     status = lib.pty_forward_batched(int(n), int(num_slices), p_count, _native.ptr(descs), b, b, b, b,
-                                     _native.ptr(ckpt), int(ckpt_every), _native.ptr(workspace), ws_bytes,
-                                     _native.stream_handle(dev))
+                                     _native.ptr(amplitudes), _native.ptr(ckpt), int(ckpt_every),
+                                     _native.ptr(workspace), ws_bytes, _native.stream_handle(dev))
     _native.check(status, "pty_forward_batched")
 
 

@@ -17,9 +17,18 @@ from typing import Optional
 
 import torch
 
-from . import engine, physics
+from dataclasses import replace as _replace
+
+import numpy as np
+
+from . import _native, engine, physics
+from .distributions import AxisUpdate, Distribution, ReductionMode, combine_axis_updates, create_axis_update, \
+    create_distribution
+from .producers import AXIS_COHERENCE, AXIS_POSITION_JITTER, AXIS_PROBE_MODES, COHERENCE_DIM, POSITION_DIM, \
+    PROBE_MODE_DIM
+from .reduce import apply_distributions, cartesian_samples
 from .types import (STEM4D, CalibratedArray, DetectorConfig, MicroscopeConfig, PotentialSlices, ProbeModes,
-                    create_calibrated_array, create_stem4d)
+                    create_calibrated_array, create_probe_modes, create_stem4d)
 
 
 def _device_for(*tensors) -> torch.device:
@@ -168,8 +177,8 @@ def _run_positions(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Mi
 
 
 def cbed_amplitude(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig) -> torch.Tensor:
-    """Complex detector amplitude per mode, (H, W, M) complex64 (``simulations.py:606-679``)."""
-    _check_no_ensemble(microscope)
+    """Complex detector amplitude per mode, (H, W, M) complex64 (``simulations.py:606-679``).  Like the
+    reference it uses the beam's modes as given and ignores ``microscope.ensemble``."""
     dev = _device_for(pot_slices.slices, beam.modes)
     shifts = torch.zeros((1, 2), dtype=torch.float64, device=dev)
     (_, amp), _, _ = _run_positions(pot_slices, beam, microscope, shifts, True, dev)
@@ -177,9 +186,25 @@ def cbed_amplitude(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Mi
 
 
 def cbed_image(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig) -> CalibratedArray:
-    """CBED intensity sum_m w_m |A_m|^2 with Fourier calibration (``simulations.py:1130-1233``)."""
-    _check_no_ensemble(microscope)
+    """CBED intensity sum_m w_m |A_m|^2 with Fourier calibration (``simulations.py:1130-1233``).
+
+    With ensemble axes on ``microscope.ensemble`` the pattern is the reduction of every sample's amplitude
+    (``bind_cbed_axes`` + ``apply_distributions``, float64 result): all samples run in one batched HIP
+    launch, or sample by sample through the differentiable :func:`bind_cbed_axes` when an input requires
+    grad."""
     dev = _device_for(pot_slices.slices, beam.modes)
+    if not microscope.ensemble.is_empty():
+        calib = min(float(pot_slices.calib), float(beam.calib))
+        n = beam.modes.shape[0]
+        if torch.is_grad_enabled() and (pot_slices.slices.requires_grad or beam.modes.requires_grad):
+            from .types import create_detector_config
+            axes = _ensemble_axes_for_cbed(microscope.ensemble, beam)
+            det = create_detector_config(real_space_calib_ang=calib, probe_calibration_pm=calib * 100.0)
+            pattern = apply_distributions(axes, bind_cbed_axes(pot_slices, beam, microscope, det, axes))
+        else:
+            pattern = _ensemble_patterns(pot_slices, beam, microscope, None, calib, dev)[0]
+        inv = 1.0 / (n * calib)
+        return create_calibrated_array(pattern, inv, inv, False)
     shifts = torch.zeros((1, 2), dtype=torch.float64, device=dev)
     (patterns, _), calib, n = _run_positions(pot_slices, beam, microscope, shifts, False, dev)
     inv = 1.0 / (n * calib)
@@ -196,13 +221,265 @@ def stem_4d(pot_slice: PotentialSlices, beam: ProbeModes, microscope: Microscope
     """
     if detector.scan_positions_px is None:
         raise ValueError("detector.scan_positions_px is required")
-    _check_no_ensemble(microscope)
     dev = _device_for(pot_slice.slices, beam.modes, detector.scan_positions_px)
     positions = detector.scan_positions_px.to(device=dev, dtype=torch.float64)
+    if not microscope.ensemble.is_empty():
+        if torch.is_grad_enabled() and (pot_slice.slices.requires_grad or beam.modes.requires_grad
+                                        or positions.requires_grad):
+            raise NotImplementedError("gradients through stem_4d with ensemble axes are not supported; "
+                                      "differentiate cbed_image per position instead")
+        n = beam.modes.shape[0]
+        calib = min(float(pot_slice.calib), float(beam.calib))
+        patterns = _ensemble_patterns(pot_slice, beam, microscope, positions, detector.real_space_calib_ang, dev)
+        return create_stem4d(patterns, detector.real_space_calib_ang, 1.0 / (n * calib),
+                             positions * detector.real_space_calib_ang, microscope.voltage_kv)
     shifts = physics.shift_to_pixels(positions, detector.real_space_calib_ang)
     (patterns, _), calib, n = _run_positions(pot_slice, beam, microscope, shifts, False, dev)
     return create_stem4d(patterns, detector.real_space_calib_ang, 1.0 / (n * calib),
                          positions * detector.real_space_calib_ang, microscope.voltage_kv)
+
+
+# ---------------------------------------------------------------------------
+# Ensemble axes: probe modes, position jitter, partial coherence
+# (simulations.py:682-945, reduce.py:92-201)
+# ---------------------------------------------------------------------------
+
+def probe_modes_to_distribution(probe: ProbeModes) -> Distribution:
+    """Incoherent axis over the mode indices with the probe's weights (``simulations.py:682-711``)."""
+    m = probe.modes.shape[2] if probe.modes.ndim == 3 else 1
+    return create_distribution(torch.arange(m, dtype=torch.float64)[:, None], probe.weights.detach().cpu(),
+                               ReductionMode.INCOHERENT, AXIS_PROBE_MODES)
+
+
+def _ensemble_axes_for_cbed(ensemble, beam: ProbeModes):
+    """Axes in the reference's order: modes (explicit, or the probe's when M > 1), jitter, coherence; the probe
+    modes alone when nothing else is set (``simulations.py:726-745``)."""
+    axes = []
+    m = beam.modes.shape[2] if beam.modes.ndim == 3 else 1
+    if ensemble.probe_modes is not None:
+        axes.append(ensemble.probe_modes)
+    elif m > 1:
+        axes.append(probe_modes_to_distribution(beam))
+    if ensemble.position_jitter is not None:
+        axes.append(ensemble.position_jitter)
+    if ensemble.coherence is not None:
+        axes.append(ensemble.coherence)
+    if not axes:
+        axes.append(probe_modes_to_distribution(beam))
+    return tuple(axes)
+
+
+_AXIS_DIMS = {AXIS_PROBE_MODES: PROBE_MODE_DIM, AXIS_POSITION_JITTER: POSITION_DIM, AXIS_COHERENCE: COHERENCE_DIM}
+
+
+def _resolve_column_maps(axes, column_maps):
+    if len(column_maps) == 0:
+        out = []
+        for axis in axes:
+            if axis.axis_id is None:
+                raise ValueError("distribution axis_id is required")
+            out.append(axis.axis_id)
+        return tuple(out)
+    if len(column_maps) == len(axes):
+        return tuple(column_maps)
+    raise ValueError("column_maps must be empty or match axes length")
+
+
+def _expected_axis_dim(axis_map: str) -> int:
+    if axis_map not in _AXIS_DIMS:
+        raise ValueError(f"unknown axis_id {axis_map!r}")
+    return _AXIS_DIMS[axis_map]
+
+
+def _validate_axis_maps(axes, axis_maps, probe_modes: ProbeModes) -> None:
+    for i, (axis, axis_map) in enumerate(zip(axes, axis_maps)):
+        expected = _expected_axis_dim(axis_map)
+        if axis.samples.shape[1] != expected:
+            raise ValueError(f"{axis_map!r} axis must have sample width {expected}")
+        if axis_map == AXIS_PROBE_MODES and i != 0:
+            raise ValueError("probe_modes axis must be first")
+    m = probe_modes.modes.shape[2] if probe_modes.modes.ndim == 3 else 1
+    if AXIS_PROBE_MODES not in axis_maps and m != 1:
+        raise ValueError("multimode probes require a first probe_modes axis")
+
+
+def _axis_update_from_sample(sample: torch.Tensor, axes, axis_maps):
+    """Split one concatenated sample row into the combined update and the mode index
+    (``simulations.py:948-990``)."""
+    cursor = 0
+    updates = []
+    mode_idx = 0
+    for axis, axis_map in zip(axes, axis_maps):
+        dim = axis.samples.shape[1]
+        cols = sample[cursor:cursor + dim]
+        if axis_map == AXIS_PROBE_MODES:
+            mode_idx = int(cols[0])
+            updates.append(create_axis_update())
+        elif axis_map == AXIS_POSITION_JITTER:
+            updates.append(create_axis_update(position_delta_ang=cols))
+        elif axis_map == AXIS_COHERENCE:
+            updates.append(create_axis_update(energy_delta_ev=cols[0], tilt_delta_mrad=cols[1:3]))
+        else:
+            raise ValueError(f"unknown axis_id {axis_map!r}")
+        cursor += dim
+    return combine_axis_updates(tuple(updates)), mode_idx
+
+
+def _tilt_ramp(n_y: int, n_x: int, tilt_delta_mrad, voltage_kv: float, calib_ang: float, device) -> torch.Tensor:
+    """exp(i 2 pi / lambda (theta_x x + theta_y y)) on centred coordinates (i - (n-1)/2) calib, float64."""
+    y = (torch.arange(n_y, dtype=torch.float64, device=device) - (n_y - 1) / 2.0) * calib_ang
+    x = (torch.arange(n_x, dtype=torch.float64, device=device) - (n_x - 1) / 2.0) * calib_ang
+    yy, xx = torch.meshgrid(y, x, indexing="ij")
+    t = tilt_delta_mrad.to(device=device, dtype=torch.float64)
+    phase = (2.0 * math.pi / physics.relativistic_wavelength_ang(voltage_kv)) * (t[0] / 1000.0 * xx
+                                                                                 + t[1] / 1000.0 * yy)
+    return torch.polar(torch.ones_like(phase), phase)
+
+
+def _apply_tilt_phase_ramp(modes: torch.Tensor, tilt_delta_mrad, voltage_kv, calib_ang) -> torch.Tensor:
+    """Beam tilt as a real-space phase ramp on every mode (``simulations.py:890-945``)."""
+    ramp = _tilt_ramp(modes.shape[0], modes.shape[1], torch.as_tensor(tilt_delta_mrad, dtype=torch.float64),
+                      float(voltage_kv), float(calib_ang), modes.device)
+    return modes * ramp.to(modes.dtype)[..., None]
+
+
+def bind_cbed_axes(pot_slices: PotentialSlices, probe_modes: ProbeModes, microscope: MicroscopeConfig,
+                   detector: DetectorConfig, axes, column_maps=()):
+    """Callable sample -> (H, W) complex amplitude of one ensemble member (``simulations.py:769-887``).
+
+    Per sample: select the mode, shift it by the jitter (Angstrom, Fourier ramp at the detector
+    calibration), tilt it (real-space ramp at the sample's energy), and run ``cbed_amplitude`` at the
+    sample's voltage.  Differentiable w.r.t. the potential, the modes and the jitter/tilt samples; the
+    energy enters as a number.  Each call is one HIP launch sequence; ``cbed_image`` and ``stem_4d``
+    evaluate whole ensembles in one batched launch instead."""
+    axis_maps = _resolve_column_maps(axes, column_maps)
+    _validate_axis_maps(axes, axis_maps, probe_modes)
+    expected_dim = sum(a.samples.shape[1] for a in axes)
+    calib_ang = float(detector.real_space_calib_ang)
+
+    def bound_amplitude_fn(sample):
+        if sample.shape[0] != expected_dim:
+            raise ValueError("sample width does not match bound axes")
+        update, mode_idx = _axis_update_from_sample(sample, axes, axis_maps)
+        kv = float(microscope.voltage_kv) + float(update.energy_delta_ev) / 1000.0
+        modes = probe_modes.modes if probe_modes.modes.ndim == 3 else probe_modes.modes[..., None]
+        selected = modes[..., mode_idx:mode_idx + 1]
+        shifted = shift_beam_fourier(selected, update.position_delta_ang.to(selected.device)[None, :], calib_ang)[0]
+        tilted = _apply_tilt_phase_ramp(shifted, update.tilt_delta_mrad, kv, calib_ang)
+        probe = create_probe_modes(tilted, torch.ones(1, dtype=torch.float64), probe_modes.calib)
+        return cbed_amplitude(pot_slices, probe, _replace(microscope, voltage_kv=kv))[..., 0]
+
+    return bound_amplitude_fn
+
+
+_ENSEMBLE_CHUNK_BYTES = 4 << 30
+_DESC_DTYPE = np.dtype(_native.POSITION_DESC_FIELDS)
+
+
+def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig,
+                       positions: Optional[torch.Tensor], det_calib: float, dev) -> torch.Tensor:
+    """(P, n, n) float64 ensemble-reduced patterns; P = 1 without positions (``cbed_image``).
+
+    Every (position, ensemble sample) pair becomes one wave of a batched HIP launch: its incident wave is
+    built in complex128 in the reference's order (position shift at ``det_calib``, mode selection, jitter
+    shift and tilt at the cbed calibration), its transmission and propagator are those of the sample's
+    energy.  Incoherent axes are summed from the kernel's intensities, coherent axes from its amplitudes."""
+    n = _square_n(beam.modes.shape[0], beam.modes.shape[1])
+    if tuple(pot_slices.slices.shape[:2]) != (n, n):
+        raise ValueError("beam.modes spatial shape must match pot_slices.slices")
+    calib = min(float(pot_slices.calib), float(beam.calib))
+    axes = _ensemble_axes_for_cbed(microscope.ensemble, beam)
+    maps = _resolve_column_maps(axes, ())
+    _validate_axis_maps(axes, maps, beam)
+    counts = [a.samples.shape[0] for a in axes]
+    cart = cartesian_samples(axes)
+    k_count = cart.shape[0]
+    mode_idx, dpos, energy, tilt = [], [], [], []
+    for row in cart:
+        upd, m = _axis_update_from_sample(row, axes, maps)
+        mode_idx.append(m)
+        dpos.append(upd.position_delta_ang)
+        energy.append(float(upd.energy_delta_ev))
+        tilt.append(upd.tilt_delta_mrad)
+    dpos = torch.stack(dpos).to(dev)
+    tilt = torch.stack(tilt).to(dev)
+    energies = sorted(set(energy))
+    group = [energies.index(e) for e in energy]
+    coherent = any(a.reduction is ReductionMode.COHERENT for a in axes)
+
+    v = pot_slices.slices if pot_slices.slices.ndim == 3 else pot_slices.slices[..., None]
+    v = v.detach().to(device=dev, dtype=torch.float64)
+    s_count = v.shape[2]
+    slabs = torch.empty((len(energies), s_count, n, n), dtype=torch.complex64, device=dev)
+    props = torch.empty((len(energies), 2 * n), dtype=torch.complex64, device=dev)
+    kvs = [float(microscope.voltage_kv) + e / 1000.0 for e in energies]
+    for g, kv in enumerate(kvs):
+        t = torch.polar(torch.ones_like(v), v * physics.interaction_sigma(kv))
+        slabs[g] = t.permute(2, 0, 1).to(torch.complex64)
+        f = physics.propagator_factors(n, pot_slices.slice_thickness, kv, calib, device=dev)
+        props[g, :n] = f
+        props[g, n:] = f
+    q = physics.fftfreq(n, calib, device=dev)
+    jitter = None
+    if bool((dpos != 0).any()):
+        jitter = torch.polar(torch.ones((k_count, n, n), dtype=torch.float64, device=dev),
+                             -2.0 * math.pi * (q[None, :, None] * dpos[:, 0, None, None]
+                                               + q[None, None, :] * dpos[:, 1, None, None]))
+    tilts = None
+    if bool((tilt != 0).any()):
+        tilts = torch.stack([_tilt_ramp(n, n, tilt[k], kvs[group[k]], calib, dev) for k in range(k_count)])
+    weights = None
+    if not coherent:
+        grids = torch.meshgrid(*[torch.arange(c) for c in counts], indexing="ij")
+        weights = torch.ones(k_count, dtype=torch.float64)
+        for a, gi in zip(axes, grids):
+            weights = weights * a.weights.detach().cpu()[gi.reshape(-1)]
+        weights = weights.to(dev)
+
+    modes = beam.modes if beam.modes.ndim == 3 else beam.modes[..., None]
+    modes = modes.detach().to(device=dev, dtype=torch.complex128)
+    p_count = 1 if positions is None else positions.shape[0]
+    chunk = max(1, min(p_count, _ENSEMBLE_CHUNK_BYTES // (k_count * n * n * 80)))
+    out = torch.empty((p_count, n, n), dtype=torch.float64, device=dev)
+    base = engine.arena_base(dev)
+    sel_idx = torch.as_tensor(mode_idx, device=dev)
+    for p0 in range(0, p_count, chunk):
+        pc = min(chunk, p_count - p0)
+        if positions is None:
+            shifted = modes[None]
+        else:
+            shifted = shift_beam_fourier(modes, positions[p0:p0 + pc].detach(), det_calib)
+        waves = shifted.index_select(3, sel_idx).permute(0, 3, 1, 2)  # (pc, K, n, n)
+        if jitter is not None:
+            waves = torch.fft.ifft2(torch.fft.fft2(waves) * jitter[None])
+        if tilts is not None:
+            waves = waves * tilts[None]
+        probe_k = torch.fft.fft2(waves).to(torch.complex64).reshape(pc * k_count, n, n).contiguous()
+        del waves
+        patterns = torch.empty((pc * k_count, n, n), dtype=torch.float32, device=dev)
+        amps = torch.empty((pc * k_count, n, n), dtype=torch.complex64, device=dev) if coherent else None
+        rec = np.zeros(pc * k_count, dtype=_DESC_DTYPE)
+        wave_group = np.tile(np.asarray(group, dtype=np.int64), pc)
+        rec["slab_offset"] = engine.element_offset(slabs, base) + wave_group * (s_count * n * n)
+        rec["slab_h"] = n
+        rec["slab_w"] = n
+        rec["pattern_offset"] = engine.element_offset(patterns, base) + np.arange(pc * k_count) * n * n
+        rec["probe_offset"] = engine.element_offset(probe_k, base) + np.arange(pc * k_count) * n * n
+        rec["prop_offset"] = engine.element_offset(props, base) + wave_group * (2 * n)
+        descs = torch.from_numpy(rec.view(np.uint8)).to(dev)
+        engine.forward_batched(n, s_count, descs, base, amplitudes=amps)
+        if coherent:
+            a = amps.to(torch.complex128).reshape(pc, *counts, n, n)
+            out[p0:p0 + pc] = torch.stack([_reduce_one(axes, a[i]) for i in range(pc)])
+        else:
+            out[p0:p0 + pc] = torch.einsum("k,pkhw->phw", weights,
+                                           patterns.to(torch.float64).reshape(pc, k_count, n, n))
+    return out
+
+
+def _reduce_one(axes, amplitudes: torch.Tensor) -> torch.Tensor:
+    from .reduce import reduce_amplitudes
+    return reduce_amplitudes(axes, amplitudes)
 
 
 def stem_4d_windows(slab_t: torch.Tensor, probe: torch.Tensor, origins: torch.Tensor, shifts_px: torch.Tensor,

@@ -105,18 +105,16 @@ class LossType(str, Enum):
 
 @dataclass(frozen=True)
 class EnsembleAxes:
-    """Optional ensemble distributions (``electron_types.py:74-101``).
-
-    The multislice path here supports the empty carrier only; populated
-    axes are rejected by the simulators.
-    """
+    """Optional ensemble distributions (``electron_types.py:74-101``): probe modes (sample width 1, mode
+    index), position jitter (width 2, Angstrom) and partial coherence (width 3: energy eV, tilt mrad x, y).
+    Any axis set routes ``cbed_image`` / ``stem_4d`` through the ensemble reduction."""
 
     probe_modes: object = None
     position_jitter: object = None
     coherence: object = None
 
     def is_empty(self) -> bool:
-        return self.position_jitter is None and self.coherence is None
+        return self.probe_modes is None and self.position_jitter is None and self.coherence is None
 
 
 @dataclass(frozen=True)
@@ -184,8 +182,24 @@ class STEM4D:
     voltage_kv: float
 
 
+def _validate_distribution_axis(axis, expected_axis_id: str, expected_dim: int) -> None:
+    if axis is None:
+        return
+    if axis.samples.ndim != 2:
+        raise ValueError(f"{expected_axis_id}.samples must be 2D")
+    if axis.samples.shape[1] != expected_dim:
+        raise ValueError(f"{expected_axis_id}.samples must have width {expected_dim}")
+    if axis.weights.ndim != 1 or tuple(axis.weights.shape) != (axis.samples.shape[0],):
+        raise ValueError(f"{expected_axis_id}.weights must have shape (N,)")
+    if axis.axis_id not in (None, expected_axis_id):
+        raise ValueError(f"{expected_axis_id}.axis_id must be {expected_axis_id!r}")
+
+
 def create_ensemble_axes(probe_modes=None, position_jitter=None, coherence=None) -> EnsembleAxes:
-    """``electron_types.py:488-527``."""
+    """``electron_types.py:488-527``: each axis is a ``Distribution`` of the stated sample width or None."""
+    _validate_distribution_axis(probe_modes, "probe_modes", 1)
+    _validate_distribution_axis(position_jitter, "position_jitter", 2)
+    _validate_distribution_axis(coherence, "coherence", 3)
     return EnsembleAxes(probe_modes, position_jitter, coherence)
 
 

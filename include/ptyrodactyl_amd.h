@@ -245,6 +245,13 @@ int pty_annular_integrate(const float* patterns, long long num_patterns, long lo
 int pty_optimizer_update(int rule, int fp64, int is_complex, long long count, void* params, const void* grads,
                          void* mu, void* nu, long long step, double learning_rate, void* stream);
 
+/*
+ * count standard normal float64 variates equal to jax.random.normal(key, (count,), float64) for the
+ * raw key (key0, key1) (partitionable Threefry-2x32 counters, 64-bit uniforms, sqrt(2) erfinv):
+ * the random mode basis of decompose_beam_to_modes (src/ptyrodactyl/multislice/simulations.py:1446-1536).
+ */
+int pty_random_normal(unsigned key0, unsigned key1, long long count, double* out, void* stream);
+
 /* Device bytes of the state block and workspace of pty_forward_persistent. */
 size_t pty_persistent_state_bytes(void);
 size_t pty_persistent_workspace_bytes(int n, int group_waves);

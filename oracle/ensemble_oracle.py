@@ -158,3 +158,18 @@ def stem_4d(pot, thickness, pot_calib, modes, weights, beam_calib, voltage_kv, p
     shifted = o.shift_beam_fourier(modes, positions, det_calib)
     return np.stack([cbed_image(pot, thickness, pot_calib, shifted[p], weights, beam_calib, voltage_kv, jitter, coh)
                      for p in range(shifted.shape[0])])
+
+
+def decompose_beam_to_modes(beam, num_modes: int, seed: int, first_mode_weight: float = 0.6):
+    """``simulations.py:1446-1536`` with key = PRNGKey(seed): (modes (H, W, M), weights (M,))."""
+    from . import recon_oracle as ro
+    k1, k2 = ro.jax_split(ro.jax_key(seed), 2)
+    hh, ww = beam.shape
+    m = int(num_modes)
+    mat = ro.jax_normal_f64(k1, (hh * ww, m)) + 1j * ro.jax_normal_f64(k2, (hh * ww, m))
+    q, _ = np.linalg.qr(mat, mode="reduced")
+    weights = np.zeros(m)
+    weights[0] = first_mode_weight
+    if m > 1:
+        weights[1:] = (1.0 - first_mode_weight) / (m - 1)
+    return (q * np.abs(np.asarray(beam).reshape(-1, 1))).reshape(hh, ww, m), weights

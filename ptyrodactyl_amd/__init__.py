@@ -17,7 +17,8 @@ from .parallelized import AtomicSliceData, create_atomic_slice_data, stem4d_shar
 from .physics import interaction_sigma, phase_interaction_parameter, relativistic_wavelength_ang
 from .recon import (OPTIMIZERS, multi_slice_multi_modal, single_slice_multi_modal, single_slice_poscorrected,
                     single_slice_ptychography)
-from .simulations import (aberration, annular_detector, bind_cbed_axes, cbed_amplitude, cbed_image, make_probe,
+from .simulations import (aberration, annular_detector, bind_cbed_axes, cbed_amplitude, cbed_image,
+                          decompose_beam_to_modes, make_probe,
                           probe_modes_to_distribution, propagation_func, shift_beam_fourier, stem_4d, stem_4d_windows,
                           transmission_func)
 from .types import (STEM4D, CalibratedArray, DetectorConfig, EnsembleAxes, LossType, MicroscopeConfig, PotentialSlices,
@@ -37,5 +38,5 @@ __all__ = [
     "TRIVIAL", "TRIVIAL_DISTRIBUTION", "AxisUpdate", "Distribution", "ReductionMode", "combine_axis_updates",
     "create_axis_update", "create_distribution", "create_trivial_distribution", "coherence_to_distribution",
     "position_jitter_to_distribution", "apply_distribution", "apply_distributions", "bind_cbed_axes",
-    "probe_modes_to_distribution",
+    "probe_modes_to_distribution", "decompose_beam_to_modes",
 ]

@@ -215,4 +215,52 @@ hipError_t launch_annular(const float* patterns, long long count, long long pixe
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Counter-based normal variates with JAX's key layout (the random modes of
+// decompose_beam_to_modes, simulations.py:1446-1536).  Threefry-2x32 with 20
+// rounds (Salmon et al., SC'11, Random123) keyed by (k0, k1); element i of the
+// flat C-order output uses the counter (i >> 32, i & 0xffffffff), the 64-bit
+// word out0 << 32 | out1, the uniform bitcast((bits >> 12) | bits(1.0)) - 1
+// scaled to [lo, 1) with lo = nextafter(-1, 0), and sqrt(2) erfinv(u).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned rotl32(unsigned x, int r) { return (x << r) | (x >> (32 - r)); }
+
+__device__ __forceinline__ void threefry2x32(unsigned k0, unsigned k1, unsigned& x0, unsigned& x1) {
+  const unsigned ks[3] = {k0, k1, k0 ^ k1 ^ 0x1BD11BDAu};
+  const int rot[2][4] = {{13, 15, 26, 6}, {17, 29, 16, 24}};
+  x0 += ks[0];
+  x1 += ks[1];
+#pragma unroll
+  for (int block = 0; block < 5; ++block) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      x0 += x1;
+      x1 = rotl32(x1, rot[block & 1][j]);
+      x1 ^= x0;
+    }
+    x0 += ks[(block + 1) % 3];
+    x1 += ks[(block + 2) % 3] + (unsigned)(block + 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_random_normal(unsigned k0, unsigned k1, long long count, double* out) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  unsigned x0 = (unsigned)((unsigned long long)i >> 32), x1 = (unsigned)((unsigned long long)i & 0xffffffffull);
+  threefry2x32(k0, k1, x0, x1);
+  const unsigned long long bits = ((unsigned long long)x0 << 32) | (unsigned long long)x1;
+  const double f = __longlong_as_double((long long)((bits >> 12) | 0x3FF0000000000000ull)) - 1.0;
+  const double lo = -1.0 + 0x1.0p-53;  // nextafter(-1, 0)
+  double u = f * (1.0 - lo) + lo;
+  u = u > lo ? u : lo;
+  out[i] = 1.4142135623730951 * erfinv(u);
+}
+
+hipError_t launch_random_normal(unsigned k0, unsigned k1, long long count, double* out, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_random_normal, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, k0, k1, count, out);
+  return hipGetLastError();
+}
+
 }  // namespace pty

@@ -11,6 +11,7 @@ hipError_t launch_normal_equations(const float2* amp, const float* data, long lo
                                    double* partials, double* jtj, double* jtr, double* rss, hipStream_t st);
 hipError_t launch_damped_solve(const double* A, const double* b, double damping, int Q, double* x,
                                double* model_reduction, int* status, hipStream_t st);
+hipError_t launch_random_normal(unsigned k0, unsigned k1, long long count, double* out, hipStream_t st);
 hipError_t launch_annular(const float* patterns, long long count, long long pixels, const unsigned char* mask,
                           double* out, hipStream_t st);
 

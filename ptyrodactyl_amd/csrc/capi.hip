@@ -324,6 +324,13 @@ int pty_optimizer_update(int rule, int fp64, int is_complex, long long count, vo
                        "pty_optimizer_update");
 }
 
+int pty_random_normal(unsigned key0, unsigned key1, long long count, double* out, void* stream) {
+  g_err[0] = '\0';
+  if (count < 0 || count > (1LL << 40)) return fail(PTY_ERR_INVALID, "count must be in [0, 2^40]");
+  if (count > 0 && !out) return fail(PTY_ERR_INVALID, "null pointer");
+  return device_status(pty::launch_random_normal(key0, key1, count, out, (hipStream_t)stream), "pty_random_normal");
+}
+
 size_t pty_persistent_state_bytes(void) { return pty::xl2_state_bytes(); }
 
 size_t pty_persistent_workspace_bytes(int n, int group_waves) {

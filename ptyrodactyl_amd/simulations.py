@@ -372,6 +372,29 @@ def bind_cbed_axes(pot_slices: PotentialSlices, probe_modes: ProbeModes, microsc
     return bound_amplitude_fn
 
 
+def decompose_beam_to_modes(beam: CalibratedArray, num_modes: int, key, first_mode_weight: float = 0.6) -> ProbeModes:
+    """Split a beam into ``num_modes`` modes (``simulations.py:1446-1536``).
+
+    Orthonormal random columns (reduced QR of a complex normal (H W, M) matrix drawn from ``key`` as
+    ``jax.random`` would: two split keys for the real and imaginary parts, generated by the HIP Threefry
+    kernel) scaled by the beam's amplitude |beam|; weights ``first_mode_weight`` then equal shares of the
+    rest.  ``key`` is a raw ``uint32[2]`` key (``ptyrodactyl_amd.random.PRNGKey``) or an integer seed."""
+    from . import random as _random
+    hh, ww = beam.data_array.shape
+    tp, m = hh * ww, int(num_modes)
+    k1, k2 = _random.split(key, 2)
+    dev = _device_for(beam.data_array)
+    matrix = torch.complex(_random.normal(k1, (tp, m), dev), _random.normal(k2, (tp, m), dev))
+    qq, _ = torch.linalg.qr(matrix, mode="reduced")
+    amplitude = torch.abs(beam.data_array.reshape(-1).to(device=dev, dtype=torch.complex128))
+    weights = torch.zeros(m, dtype=torch.float64)
+    weights[0] = float(first_mode_weight)
+    if m > 1:
+        weights[1:] = (1.0 - float(first_mode_weight)) / (m - 1)
+    modes = (qq * amplitude[:, None]).reshape(hh, ww, m)
+    return create_probe_modes(modes, weights, beam.calib_y)
+
+
 _ENSEMBLE_CHUNK_BYTES = 4 << 30
 _DESC_DTYPE = np.dtype(_native.POSITION_DESC_FIELDS)
 

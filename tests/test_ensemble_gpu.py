@@ -142,3 +142,31 @@ def test_ensemble_cbed_image_gradient_wrt_potential():
         worst = max(worst, abs(gv[idx] - fd) / (abs(fd) + 1e-3 * np.abs(gv).max()))
     print(f"ensemble dL/dV vs oracle FD: worst {worst:.2e}", flush=True)
     assert worst < 1e-2
+
+
+def test_random_normal_matches_jax_restatement():
+    from ptyrodactyl_amd import random as prandom
+    from oracle import recon_oracle as ro
+    for seed, shape in ((0, (16, 3)), (11, (4, 4)), (2**33 + 7, (1000,))):
+        for key_p, key_o in zip(prandom.split(prandom.PRNGKey(seed), 2), ro.jax_split(ro.jax_key(seed), 2)):
+            got = prandom.normal(key_p, shape, DEV).cpu().numpy()
+            want = ro.jax_normal_f64(key_o, shape)
+            np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-15)
+
+
+def test_decompose_beam_to_modes_pinned_value():
+    """test_simulations.py:140-189: PRNGKey(0), 3 modes of arange(16) + 1j, first weight 0.6."""
+    import ptyrodactyl_amd as pa
+    from ptyrodactyl_amd import random as prandom
+    beam = pa.create_calibrated_array(torch.arange(16, dtype=torch.float64, device=DEV).reshape(4, 4) + 1j,
+                                      0.5, 0.5, True)
+    first = pa.decompose_beam_to_modes(beam, 3, prandom.PRNGKey(0), first_mode_weight=0.6)
+    second = pa.decompose_beam_to_modes(beam, 3, prandom.PRNGKey(0), first_mode_weight=0.6)
+    assert torch.equal(first.modes, second.modes)
+    value = complex(first.modes[0, 0, 0])
+    print(f"decompose_beam_to_modes modes[0,0,0] = {value!r}", flush=True)
+    assert abs(value - (-0.34562459928563927 + 0.25753846176370626j)) < 1e-13
+    np.testing.assert_array_equal(first.weights.cpu().numpy(), [0.6, 0.2, 0.2])
+    assert first.calib == 0.5
+    with pytest.raises(TypeError):
+        pa.decompose_beam_to_modes(beam, 3)

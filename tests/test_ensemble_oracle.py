@@ -168,3 +168,22 @@ def test_jitter_is_weighted_sum_over_shifted_probes():
         shifted = o.shift_beam_fourier(modes[..., :1], [[dy, dx]], 0.5)[0]
         want = want + w * o.cbed_image(pot, 1.0, 0.5, shifted, [1.0], 0.5, 80.0)[0]
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_decompose_beam_to_modes_pinned_value():
+    """test_simulations.py:140-189: PRNGKey(0), 3 modes of arange(16) + 1j."""
+    beam = np.arange(16, dtype=np.float64).reshape(4, 4) + 1j
+    modes, weights = eo.decompose_beam_to_modes(beam, 3, 0, 0.6)
+    assert abs(modes[0, 0, 0] - (-0.34562459928563927 + 0.25753846176370626j)) < 1e-15
+    np.testing.assert_array_equal(weights, [0.6, 0.2, 0.2])
+    q = modes.reshape(16, 3) / np.abs(beam.reshape(-1, 1))
+    np.testing.assert_allclose(q.conj().T @ q, np.eye(3), atol=1e-14)
+
+
+def test_product_split_matches_oracle_split():
+    from ptyrodactyl_amd import random as prandom
+    from oracle import recon_oracle as ro
+    for seed in (0, 11, 2**40 + 5):
+        got = prandom.split(prandom.PRNGKey(seed), 5)
+        want = ro.jax_split(ro.jax_key(seed), 5)
+        assert all(np.array_equal(g, np.array(w, dtype=np.uint32)) for g, w in zip(got, want))

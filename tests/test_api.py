@@ -27,6 +27,15 @@ def test_public_signatures_keep_reference_names_and_order():
         "checked_stem_4d": ["pot_slice", "beam", "microscope", "detector"],
         "checked_cbed_image": ["pot_slices", "beam", "microscope"],
         "annular_detector": ["stem4d_data", "detector"],
+        "bind_cbed_axes": ["pot_slices", "probe_modes", "microscope", "detector", "axes", "column_maps"],
+        "apply_distribution": ["distribution", "bound_amplitude_fn"],
+        "apply_distributions": ["distributions", "bound_amplitude_fn"],
+        "probe_modes_to_distribution": ["probe"],
+        "coherence_to_distribution": ["energy_spread_ev", "angular_divergence_mrad", "n_quad"],
+        "position_jitter_to_distribution": ["sigma_ang", "n_quad"],
+        "decompose_beam_to_modes": ["beam", "num_modes", "key", "first_mode_weight"],
+        "create_distribution": ["samples", "weights", "reduction", "axis_id"],
+        "create_ensemble_axes": ["probe_modes", "position_jitter", "coherence"],
     }
     for name, params in expect.items():
         sig = inspect.signature(getattr(pa, name))

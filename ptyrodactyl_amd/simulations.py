@@ -196,7 +196,8 @@ def cbed_image(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Micros
     if not microscope.ensemble.is_empty():
         calib = min(float(pot_slices.calib), float(beam.calib))
         n = beam.modes.shape[0]
-        if torch.is_grad_enabled() and (pot_slices.slices.requires_grad or beam.modes.requires_grad):
+        if torch.is_grad_enabled() and (pot_slices.slices.requires_grad or beam.modes.requires_grad
+                                        or _ensemble_requires_grad(microscope.ensemble)):
             from .types import create_detector_config
             axes = _ensemble_axes_for_cbed(microscope.ensemble, beam)
             det = create_detector_config(real_space_calib_ang=calib, probe_calibration_pm=calib * 100.0)
@@ -225,7 +226,7 @@ def stem_4d(pot_slice: PotentialSlices, beam: ProbeModes, microscope: Microscope
     positions = detector.scan_positions_px.to(device=dev, dtype=torch.float64)
     if not microscope.ensemble.is_empty():
         if torch.is_grad_enabled() and (pot_slice.slices.requires_grad or beam.modes.requires_grad
-                                        or positions.requires_grad):
+                                        or positions.requires_grad or _ensemble_requires_grad(microscope.ensemble)):
             raise NotImplementedError("gradients through stem_4d with ensemble axes are not supported; "
                                       "differentiate cbed_image per position instead")
         n = beam.modes.shape[0]
@@ -249,6 +250,11 @@ def probe_modes_to_distribution(probe: ProbeModes) -> Distribution:
     m = probe.modes.shape[2] if probe.modes.ndim == 3 else 1
     return create_distribution(torch.arange(m, dtype=torch.float64)[:, None], probe.weights.detach().cpu(),
                                ReductionMode.INCOHERENT, AXIS_PROBE_MODES)
+
+
+def _ensemble_requires_grad(ensemble) -> bool:
+    axes = (ensemble.probe_modes, ensemble.position_jitter, ensemble.coherence)
+    return any(a is not None and (a.samples.requires_grad or a.weights.requires_grad) for a in axes)
 
 
 def _ensemble_axes_for_cbed(ensemble, beam: ProbeModes):

@@ -67,6 +67,12 @@ def _apply_optimizer_step(params: torch.Tensor, grads: torch.Tensor, optimizer: 
     optimizer.update_(params, grads, optimizer_state)
 
 
+def _owned(t: torch.Tensor, dev) -> torch.Tensor:
+    """A contiguous copy on ``dev``: the update rules write in place and the caller's arrays stay unchanged,
+    as the reference's functional loop leaves them."""
+    return t.detach().to(device=dev, copy=True).contiguous()
+
+
 def _learning_rates(learning_rate) -> Tuple[float, float]:
     """A scalar applies to parameters and positions; a length-2 array is (parameters, positions)."""
     lr = torch.as_tensor(learning_rate, dtype=torch.float64)
@@ -156,11 +162,10 @@ def single_slice_ptychography(experimental_data: STEM4D, initial_potential: Cali
     loss_mode = LossType(loss_type)
     optimizer = get_optimizer(optimizer_name, learning_rate)
     loss_mode, dev, data, calib_ang, microscope, positions = _setup(experimental_data, loss_mode)
-    pot = _promote_to_complex(initial_potential.data_array).to(dev).contiguous()
-    beam = _promote_to_complex(initial_beam.data_array).to(dev)
+    pot = _owned(_promote_to_complex(initial_potential.data_array), dev)
+    beam = _owned(_promote_to_complex(initial_beam.data_array), dev)
     if not initial_beam.real_space:
-        beam = torch.fft.ifft2(beam)
-    beam = beam.contiguous()
+        beam = torch.fft.ifft2(beam).contiguous()
     forward = _single_mode_forward(slice_thickness, calib_ang, microscope, data.dtype, fixed_positions=positions)
     params, snaps, _ = _optimise(lambda pot_slice, beam: forward(pot_slice, beam), {"pot_slice": pot, "beam": beam},
                                  {"pot_slice": optimizer, "beam": optimizer}, data, loss_mode, save_every,
@@ -185,8 +190,8 @@ def single_slice_poscorrected(experimental_data: STEM4D, initial_potential: Cali
     param_lr, pos_lr = _learning_rates(learning_rate)
     param_opt, pos_opt = get_optimizer(optimizer_name, param_lr), get_optimizer(optimizer_name, pos_lr)
     loss_mode, dev, data, calib_ang, microscope, positions = _setup(experimental_data, loss_mode)
-    pot = _promote_to_complex(initial_potential.data_array).to(dev).contiguous()
-    beam = _promote_to_complex(initial_beam.data_array).to(dev).contiguous()
+    pot = _owned(_promote_to_complex(initial_potential.data_array), dev)
+    beam = _owned(_promote_to_complex(initial_beam.data_array), dev)
     forward = _single_mode_forward(slice_thickness, calib_ang, microscope, data.dtype)
     params, snaps, _ = _optimise(forward, {"pot_slice": pot, "beam": beam, "pos_list": positions.clone()},
                                  {"pot_slice": param_opt, "beam": param_opt, "pos_list": pos_opt}, data, loss_mode,
@@ -212,8 +217,8 @@ def single_slice_multi_modal(experimental_data: STEM4D, initial_pot_slice: torch
     param_lr, pos_lr = _learning_rates(learning_rate)
     param_opt, pos_opt = get_optimizer(optimizer_name, param_lr), get_optimizer(optimizer_name, pos_lr)
     loss_mode, dev, data, calib_ang, microscope, positions = _setup(experimental_data, loss_mode)
-    pot = _promote_to_complex(initial_pot_slice).to(dev).contiguous()
-    modes = _promote_to_complex(initial_beam.modes).to(dev).contiguous()
+    pot = _owned(_promote_to_complex(initial_pot_slice), dev)
+    modes = _owned(_promote_to_complex(initial_beam.modes), dev)
     weights, beam_calib = initial_beam.weights, initial_beam.calib
 
     def forward(pot_slice, modes, pos_list):
@@ -240,8 +245,8 @@ def multi_slice_multi_modal(experimental_data: STEM4D, initial_pot_slice: torch.
     param_opt = get_optimizer(optimizer_name, learning_rate)
     pos_opt = get_optimizer(optimizer_name, pos_learning_rate)
     loss_mode, dev, data, calib_ang, microscope, positions = _setup(experimental_data, loss_mode)
-    pot = _promote_to_complex(initial_pot_slice).to(dev).contiguous()
-    beam = _promote_to_complex(initial_beam).to(dev).contiguous()
+    pot = _owned(_promote_to_complex(initial_pot_slice), dev)
+    beam = _owned(_promote_to_complex(initial_beam), dev)
     forward = _single_mode_forward(slice_thickness, calib_ang, microscope, data.dtype)
     params, snaps, _ = _optimise(forward, {"pot_slice": pot, "beam": beam, "pos_list": positions.clone()},
                                  {"pot_slice": param_opt, "beam": param_opt, "pos_list": pos_opt}, data, loss_mode,

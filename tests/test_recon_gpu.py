@@ -211,3 +211,23 @@ def test_real_reciprocal_beam_is_promoted_and_inverse_transformed():
     assert out[1].data_array.dtype == torch.complex128
     ref = _oracle(pb, 0.001, 0.001, "mse", "adam", False, 1, 1, modes=np.fft.ifft2(beam_k.numpy())[..., None])
     assert _rel_change_error(out[1].data_array.cpu(), np.fft.ifft2(beam_k.numpy()), ref[1][..., 0]) < TOL_RECON
+
+
+def test_reconstructors_leave_their_inputs_unchanged():
+    """The update rules run in place on the reconstructors' own copies: GPU inputs already in the parameter dtype
+    are not modified (the reference's loop is functional)."""
+    import ptyrodactyl_amd as pa
+    pb = _problem(seed=6, modes=2)
+    exp = _experimental(pb)
+    pot_t = torch.tensor(pb["pot0"], device=DEV)
+    beam_t = torch.tensor(pb["beam0"][..., 0], device=DEV)
+    modes_t = torch.tensor(pb["beam0"], device=DEV)
+    snap = [t.clone() for t in (pot_t, beam_t, modes_t, exp.scan_positions)]
+    pa.single_slice_poscorrected(exp, pa.create_calibrated_array(pot_t, 0.5, 0.5, True),
+                                 pa.create_calibrated_array(beam_t, 0.5, 0.5, True), pb["dz"], save_every=1,
+                                 num_iterations=2)
+    pa.single_slice_multi_modal(exp, pot_t, pa.ProbeModes(modes_t, torch.tensor(pb["weights"]), 0.5), pb["dz"],
+                                save_every=1, num_iterations=2)
+    pa.multi_slice_multi_modal(exp, pot_t, beam_t, pb["dz"], save_every=1, num_iterations=2)
+    for before, after in zip(snap, (pot_t, beam_t, modes_t, exp.scan_positions)):
+        assert torch.equal(before, after)

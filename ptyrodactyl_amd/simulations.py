@@ -225,13 +225,21 @@ def stem_4d(pot_slice: PotentialSlices, beam: ProbeModes, microscope: Microscope
     dev = _device_for(pot_slice.slices, beam.modes, detector.scan_positions_px)
     positions = detector.scan_positions_px.to(device=dev, dtype=torch.float64)
     if not microscope.ensemble.is_empty():
-        if torch.is_grad_enabled() and (pot_slice.slices.requires_grad or beam.modes.requires_grad
-                                        or positions.requires_grad or _ensemble_requires_grad(microscope.ensemble)):
-            raise NotImplementedError("gradients through stem_4d with ensemble axes are not supported; "
-                                      "differentiate cbed_image per position instead")
         n = beam.modes.shape[0]
         calib = min(float(pot_slice.calib), float(beam.calib))
-        patterns = _ensemble_patterns(pot_slice, beam, microscope, positions, detector.real_space_calib_ang, dev)
+        if torch.is_grad_enabled() and (pot_slice.slices.requires_grad or beam.modes.requires_grad
+                                        or positions.requires_grad or _ensemble_requires_grad(microscope.ensemble)):
+            # differentiable: per position, the shifted modes through the ensemble cbed_image (its per-sample
+            # binder), as the reference maps cbed_image over the positions (simulations.py:1395-1427)
+            modes = beam.modes if beam.modes.ndim == 3 else beam.modes[..., None]
+            patterns = torch.stack([
+                cbed_image(pot_slice, ProbeModes(shift_beam_fourier(modes, positions[i:i + 1],
+                                                                    detector.real_space_calib_ang)[0],
+                                                 beam.weights, beam.calib), microscope).data_array
+                for i in range(positions.shape[0])])
+        else:
+            patterns = _ensemble_patterns(pot_slice, beam, microscope, positions, detector.real_space_calib_ang,
+                                          dev)
         return create_stem4d(patterns, detector.real_space_calib_ang, 1.0 / (n * calib),
                              positions * detector.real_space_calib_ang, microscope.voltage_kv)
     shifts = physics.shift_to_pixels(positions, detector.real_space_calib_ang)

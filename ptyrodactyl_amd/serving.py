@@ -16,7 +16,15 @@ size P and specimen.  :class:`RequestLoop` serves a stream of such requests:
   arrive while others run join the next step;
 * size classes run on their own HIP streams so their work overlaps;
 * per-request latency = completion - arrival (queueing included), reported
-  as p50 / p99 beside patterns/s.
+  as p50 / p99 beside patterns/s;
+* failure isolation: a request that cannot be built, or whose work raises
+  when launched, is marked ``failed`` with the error and its memory is
+  released while the other requests continue (a failing batched step is
+  retried request by request, so only the offending request fails); a
+  request larger than the whole budget is ``rejected`` at admission;
+  ``cancel(rid)`` drops a pending or running request at the next step;
+* per-step metrics (time, active and pending requests, HBM in use) are kept
+  in ``metrics`` and can be streamed to a callback as JSON-ready dicts.
 
 :class:`PlainExecutor` is the comparison: the same requests one at a time,
 in arrival order, each through the ordinary per-request engine calls.
@@ -27,7 +35,7 @@ from __future__ import annotations
 import math
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Tuple
+from typing import Callable, Dict, List, Optional, Tuple
 
 import numpy as np
 import torch
@@ -59,6 +67,8 @@ class Request:
     t_admit: Optional[float] = None
     t_done: Optional[float] = None
     inputs: Optional["RequestInputs"] = None  # kept after completion only when the loop is asked to
+    status: str = "pending"   # pending, running, done, failed, rejected, cancelled
+    error: Optional[str] = None
 
     @property
     def latency(self) -> float:
@@ -204,14 +214,18 @@ class PlainExecutor:
         return summarize(reqs, time.perf_counter() - t0)
 
 
-def summarize(reqs: List[Request], wall: float) -> Dict:
+def summarize(reqs: List[Request], wall: float, others: Optional[List[Request]] = None) -> Dict:
+    """Throughput and latency of the completed requests; counts of the others by status."""
     lat = np.array([r.latency for r in reqs]) * 1000.0
     pats = sum(r.positions for r in reqs)
-    return {"requests": len(reqs), "patterns": pats, "wall_s": round(wall, 4),
-            "patterns_per_s": round(pats / wall, 2),
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                           "p99": round(float(np.percentile(lat, 99)), 3),
-                           "mean": round(float(lat.mean()), 3)}}
+    out = {"requests": len(reqs), "patterns": pats, "wall_s": round(wall, 4),
+           "patterns_per_s": round(pats / wall, 2) if wall > 0 else 0.0,
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+                          "p99": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+                          "mean": round(float(lat.mean()), 3) if len(lat) else None}}
+    for r in others or []:
+        out[r.status] = out.get(r.status, 0) + 1
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -227,6 +241,7 @@ class _Active:
     done_pos: int = 0
     loss: Optional[torch.Tensor] = None  # FP64 device accumulator, read once at completion
     ready: Optional[torch.cuda.Event] = None  # inputs built (on the builder stream)
+    pending_loss: Optional[torch.Tensor] = None  # this step's loss, added once the step succeeded
 
 
 @dataclass
@@ -239,11 +254,18 @@ class RequestLoop:
     chunk_positions: Dict[int, int] = field(default_factory=lambda: {32: 8192, 64: 8192, 128: 4096, 256: 512,
                                                                      512: 128})
     device: Optional[torch.device] = None
+    metrics_sink: Optional[Callable[[Dict], None]] = None
 
     def __post_init__(self):
         self.device = self.device or torch.device("cuda", 0)
         self.base = engine.arena_base(self.device)
         self.streams: Dict[Tuple[int, int], torch.cuda.Stream] = {}
+        self.metrics: List[Dict] = []
+        self._cancelled: set = set()
+
+    def cancel(self, rid: int) -> None:
+        """Drop request ``rid``: not admitted if pending, released at the next step if running."""
+        self._cancelled.add(rid)
 
     def _stream(self, key) -> torch.cuda.Stream:
         if key not in self.streams:
@@ -277,27 +299,64 @@ class RequestLoop:
         # making the host wait for that stream's earlier launches
         return torch.from_numpy(rec.view(np.uint8)).pin_memory().to(self.device, non_blocking=True)
 
-    def _step_class(self, key, actives: List[_Active]) -> None:
-        """One batched step of size class key = (n, S) over chunks of the given requests."""
-        n, s_count = key
+    def _step_class(self, key, actives: List[_Active]) -> List[Tuple[_Active, str]]:
+        """One batched step of size class key = (n, S) over chunks of the given requests.  Returns the
+        requests whose work raised (with the message); if the batch raises, every chunk is retried alone so
+        that only the offending requests fail."""
+        n = key[0]
         budget = self.chunk_positions.get(n, 512)
-        fwd_items, grad_items = [], []
+        items = []
         for a in actives:
             if budget <= 0:
                 break
             count = min(a.req.positions - a.next_pos, budget)
             if count <= 0:
                 continue
-            (grad_items if a.req.kind == "gradient" else fwd_items).append((a, a.next_pos, count))
-            a.next_pos += count
+            items.append((a, a.next_pos, count))
             budget -= count
+        completed: list = []
+        try:
+            self._step_items(key, items, completed)
+        except (ValueError, RuntimeError, KeyError):
+            # chunks that finished before the error keep their results; the others are retried one by one
+            self._advance(completed)
+            done_ids = {id(it) for it in completed}
+            failed = []
+            for item in items:
+                if id(item) in done_ids:
+                    continue
+                alone: list = []
+                try:
+                    self._step_items(key, [item], alone)
+                except (ValueError, RuntimeError, KeyError) as e:
+                    failed.append((item[0], f"{type(e).__name__}: {e}"))
+                    continue
+                self._advance(alone)
+            return failed
+        self._advance(completed)
+        return []
+
+    @staticmethod
+    def _advance(items) -> None:
+        for a, _, count in items:
+            a.next_pos += count
+            a.done_pos += count
+            if a.pending_loss is not None:
+                a.loss = a.pending_loss if a.loss is None else a.loss + a.pending_loss
+                a.pending_loss = None
+
+    def _step_items(self, key, items, completed: list) -> None:
+        """Launch the forward (and for gradient requests the loss, gradient and adjoint) of the chunks;
+        each chunk is appended to ``completed`` once all of its work is enqueued."""
+        n, s_count = key
+        fwd_items = [it for it in items if it[0].req.kind != "gradient"]
+        grad_items = [it for it in items if it[0].req.kind == "gradient"]
         dev = self.device
         if fwd_items:
             outs = [a.req.patterns[start:start + count] for a, start, count in fwd_items]
             descs = self._descs(fwd_items, False, outs)
             engine.forward_batched(n, s_count, descs, self.base)
-            for a, _, count in fwd_items:
-                a.done_pos += count
+            completed.extend(fwd_items)
         if grad_items:
             outs = [a.req.patterns[start:start + count] for a, start, count in grad_items]
             total = sum(c for _, _, c in grad_items)
@@ -309,21 +368,21 @@ class RequestLoop:
             for a, start, count in grad_items:
                 loss, gi = loss_and_grad_intensity(a.req.patterns[start:start + count],
                                                    a.inp.measured[start:start + count])
-                a.loss = loss if a.loss is None else a.loss + loss
+                a.pending_loss = loss
                 gis.append(gi)
             # adjoint per request chunk (each a full-GPU launch sequence): one slab per call lets the
             # gradients be gathered in a fixed order instead of accumulated with atomics
             one = torch.ones(1, dtype=torch.float32, device=dev)
             k0 = 0
-            for (a, start, count), gi in zip(grad_items, gis):
+            for item, gi in zip(grad_items, gis):
+                a, start, count = item
                 inp = a.inp
                 engine.multislice_adjoint(inp.slab, inp.probe_k[None], one, inp.origins[start:start + count],
                                           inp.shifts[start:start + count], inp.prop[:n].contiguous(), gi,
                                           ckpt[k0:k0 + count] if nc else None, CKPT_EVERY,
                                           grad_slab=a.req.grad_slab, grad_probe_k=a.req.grad_probe_k[None])
+                completed.append(item)
                 k0 += count
-            for a, _, count in grad_items:
-                a.done_pos += count
 
     def _admit(self, req: Request, now: float) -> _Active:
         """Build the request's inputs and outputs on the builder stream; the class streams wait for
@@ -342,18 +401,39 @@ class RequestLoop:
         return _Active(req, inp, ready=ready)
 
     def _admit_arrivals(self, pending: List[Request], active: List[_Active], in_use: int, t0: float,
-                        realtime: bool, limit: int) -> int:
+                        realtime: bool, limit: int, closed: List[Request]) -> int:
         now = time.perf_counter() - t0
         admitted = 0
         while pending and admitted < limit and (not realtime or pending[0].arrival <= now):
-            need = request_bytes(pending[0])
+            r = pending[0]
+            if r.rid in self._cancelled:
+                pending.pop(0)
+                self._close(r, "cancelled", None, now, closed)
+                continue
+            need = request_bytes(r)
+            if need > self.hbm_budget_bytes:
+                pending.pop(0)
+                self._close(r, "rejected", f"needs {need} bytes, above the HBM budget of {self.hbm_budget_bytes}",
+                            now, closed)
+                continue
             if active and in_use + need > self.hbm_budget_bytes:
                 break
-            r = pending.pop(0)
-            active.append(self._admit(r, now))
+            pending.pop(0)
+            try:
+                active.append(self._admit(r, now))
+            except (ValueError, RuntimeError, KeyError) as e:
+                self._close(r, "failed", f"{type(e).__name__}: {e}", now, closed)
+                continue
+            r.status = "running"
             in_use += need
             admitted += 1
         return in_use
+
+    @staticmethod
+    def _close(r: Request, status: str, error: Optional[str], now: float, closed: List[Request]) -> None:
+        r.status, r.error, r.t_done = status, error, now
+        r.patterns = r.grad_slab = r.grad_probe_k = None
+        closed.append(r)
 
     def run(self, requests: List[Request], realtime: bool = True) -> Dict:
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -362,18 +442,21 @@ class RequestLoop:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         finished: List[Request] = []
+        closed: List[Request] = []
         while pending or active:
             if not active:
-                in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, 1)
+                in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, 1, closed)
                 if not active:
-                    now = time.perf_counter() - t0
-                    time.sleep(max(0.0, min(0.005, pending[0].arrival - now)))
+                    if pending:
+                        now = time.perf_counter() - t0
+                        time.sleep(max(0.0, min(0.005, pending[0].arrival - now)))
                     continue
             classes: Dict[Tuple[int, int], List[_Active]] = {}
             for a in active:
                 if a.next_pos < a.req.positions:
                     classes.setdefault((a.req.n, a.req.slices), []).append(a)
             main = torch.cuda.current_stream(self.device)
+            failures: Dict[int, str] = {}
             for key, members in classes.items():
                 st = self._stream(key)
                 st.wait_stream(main)
@@ -381,24 +464,37 @@ class RequestLoop:
                     if a.ready is not None:
                         st.wait_event(a.ready)
                 with torch.cuda.stream(st):
-                    self._step_class(key, members)
+                    for a, message in self._step_class(key, members):
+                        failures[a.req.rid] = message
             for key in classes:
                 main.wait_stream(self._stream(key))
             # requests that arrived meanwhile are built while the GPU runs this step; they join the next one
-            in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, self.admit_per_step)
+            in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, self.admit_per_step, closed)
             torch.cuda.synchronize()
             now = time.perf_counter() - t0
             still = []
             for a in active:
-                if a.done_pos >= a.req.positions:
+                if a.req.rid in failures or a.req.rid in self._cancelled:
+                    in_use -= request_bytes(a.req)
+                    if a.req.rid in failures:
+                        self._close(a.req, "failed", failures[a.req.rid], now, closed)
+                    else:
+                        self._close(a.req, "cancelled", None, now, closed)
+                elif a.done_pos >= a.req.positions:
                     if a.req.kind == "gradient":
                         a.req.loss = float(a.loss)
                     a.req.t_done = now
                     if self.keep_inputs:
                         a.req.inputs = a.inp
+                    a.req.status = "done"
                     in_use -= request_bytes(a.req)
                     finished.append(a.req)
                 else:
                     still.append(a)
             active = still
-        return summarize(finished, time.perf_counter() - t0)
+            m = {"t_s": round(now, 6), "active": len(active), "pending": len(pending), "hbm_in_use": in_use,
+                 "done": len(finished), "closed": len(closed)}
+            self.metrics.append(m)
+            if self.metrics_sink is not None:
+                self.metrics_sink(m)
+        return summarize(finished, time.perf_counter() - t0, closed)

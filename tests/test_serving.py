@@ -94,3 +94,68 @@ def test_adjoint_batched_matches_per_request():
             err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
             print(f"request {a.req.rid}: batched vs per-request adjoint rel-L2 {err:.2e}", flush=True)
             assert err < 1e-5
+
+
+def _assert_same_as_alone(r):
+    alone = serving.PlainExecutor().run_one(serving.Request(r.rid, r.kind, r.n, r.slices, r.positions, r.seed),
+                                             r.inputs)
+    assert torch.equal(r.patterns, alone.patterns), f"patterns differ for request {r.rid}"
+    if r.kind == "gradient":
+        assert r.loss == pytest.approx(alone.loss, rel=1e-9)
+        err = float(torch.linalg.norm(r.grad_slab - alone.grad_slab) / torch.linalg.norm(alone.grad_slab))
+        assert err < 1e-5, f"gradient rel-L2 {err:.2e} for request {r.rid}"
+
+
+@pytest.mark.gpu
+def test_unbuildable_request_fails_alone():
+    """A request the loop cannot build (unsupported window) fails with its error; the rest complete."""
+    reqs = _mix()[:5]
+    bad = serving.Request(99, "forward", 48, 8, 16, seed=1, arrival=0.0015)
+    loop = serving.RequestLoop(hbm_budget_bytes=256 << 20, chunk_positions={64: 128, 128: 48, 256: 16},
+                               keep_inputs=True)
+    stats = loop.run(reqs + [bad], realtime=False)
+    assert stats["requests"] == 5 and stats.get("failed") == 1
+    assert bad.status == "failed" and "48" in bad.error
+    for r in reqs:
+        assert r.status == "done"
+        _assert_same_as_alone(r)
+
+
+class _FaultyLoop(serving.RequestLoop):
+    """Raises from the host whenever a launch would include request 3 (a forced launch error)."""
+
+    def _step_items(self, key, items, completed):
+        if any(a.req.rid == 3 for a, _, _ in items):
+            # the other chunks of the batch are enqueued first, as a real mid-batch failure would leave them
+            super()._step_items(key, [it for it in items if it[0].req.rid != 3], completed)
+            raise ValueError("forced launch error for request 3")
+        super()._step_items(key, items, completed)
+
+
+@pytest.mark.gpu
+def test_failing_step_is_isolated_without_double_counting():
+    reqs = _mix()
+    loop = _FaultyLoop(hbm_budget_bytes=256 << 20, chunk_positions={64: 128, 128: 48, 256: 16}, keep_inputs=True)
+    stats = loop.run(reqs, realtime=False)
+    assert stats.get("failed") == 1 and stats["requests"] == len(reqs) - 1
+    assert reqs[3].status == "failed" and "forced launch error" in reqs[3].error
+    for r in reqs:
+        if r.rid != 3:
+            assert r.status == "done"
+            _assert_same_as_alone(r)
+
+
+@pytest.mark.gpu
+def test_oversize_rejected_cancelled_dropped_and_metrics_recorded():
+    reqs = _mix()[:4]
+    big = serving.Request(50, "gradient", 256, 64, 4096, seed=3, arrival=0.0)
+    doomed = serving.Request(51, "forward", 64, 8, 16, seed=4, arrival=0.002)
+    seen = []
+    loop = serving.RequestLoop(hbm_budget_bytes=max(serving.request_bytes(r) for r in reqs) * 4,
+                               metrics_sink=seen.append)
+    loop.cancel(51)
+    stats = loop.run(reqs + [big, doomed], realtime=False)
+    assert big.status == "rejected" and "budget" in big.error
+    assert doomed.status == "cancelled"
+    assert stats["requests"] == 4 and stats.get("rejected") == 1 and stats.get("cancelled") == 1
+    assert seen and seen == loop.metrics and seen[-1]["active"] == 0 and seen[-1]["hbm_in_use"] == 0

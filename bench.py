@@ -188,6 +188,35 @@ def serving_bench(count: int, qps: float):
     return out
 
 
+def membw_bench(sizes_mib=(256, 512, 1024, 4096), reps: int = 20):
+    """Achievable HBM bandwidth on this box: torch device copies (read + write) and reductions (read) of
+    float32 buffers, timed with events; the practical ceiling the streaming passes are compared with."""
+    import torch
+    dev = torch.device("cuda", 0)
+    out = []
+    for mib in sizes_mib:
+        n = mib * (1 << 20) // 4
+        a = torch.ones(n, device=dev)
+        b = torch.empty_like(a)
+        s = torch.empty((), device=dev)
+        row = {"mib": mib}
+        for label, fn, moved in (("copy", lambda: b.copy_(a), 2 * n * 4), ("read", lambda: s.copy_(a.sum()), n * 4)):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            row[f"{label}_gbs"] = round(moved / (ms * 1e-3) / 1e9, 1)
+        out.append(row)
+        del a, b
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +233,7 @@ def main() -> int:
     ap.add_argument("--serving-qps", type=float, default=60.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
-    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient"), default="config2",
+    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw"), default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
                          "config1: 128^2 x 20 slices, 256 positions (the reference's CPU-runnable case)")
     args = ap.parse_args()
@@ -222,6 +251,13 @@ def main() -> int:
         print(json.dumps({"metric": "4D-STEM patterns/s, mixed request stream",
                           "value": res["saturated"]["batched"]["patterns_per_s"],
                           "unit": "patterns/s", "n_gpus": 1, "higher_is_better": True, "serving": res}), flush=True)
+        return 0
+
+    if args.workload == "membw":
+        res = membw_bench()
+        print(json.dumps({"metric": "achievable HBM bandwidth (torch copy / reduction)",
+                          "value": max(r["copy_gbs"] for r in res), "unit": "GB/s", "n_gpus": 1,
+                          "higher_is_better": True, "membw": res}), flush=True)
         return 0
 
     if args.workload == "gradient":

@@ -110,6 +110,15 @@ class LMState:
     iteration: int
 
 
+@dataclass
+class LanczosState:
+    v_prev: torch.Tensor
+    v_curr: torch.Tensor
+    alpha: torch.Tensor
+    beta: torch.Tensor
+    iteration: int
+
+
 # ---------------------------------------------------------------------------
 # Operators (``operators.py:29-185``)
 # ---------------------------------------------------------------------------
@@ -169,6 +178,35 @@ def conjugate_gradient(linear_operator: Callable, rhs, x0, max_iterations: int =
         p_new = tree_add(r_new, tree_scale(beta, state.p))
         state = CGState(x=x_new, r=r_new, p=p_new, r_dot_r=rr_new, iteration=state.iteration + 1)
     return state.x, state.iteration
+
+
+def lanczos_tridiagonal(linear_operator: Callable, initial_vector: torch.Tensor, num_iterations: int):
+    """k-step Lanczos tridiagonalisation of a symmetric operator on real vectors (``solvers.py:669-777``).
+
+    Same three-term recurrence as the reference, without reorthogonalisation:
+    w = A v_i, alpha_i = <w, v_i>, w -= alpha_i v_i + beta_{i-1} v_{i-1}, beta_i = |w|,
+    v_{i+1} = w / (beta_i + 1e-12), from v_0 = initial_vector / |initial_vector|.  Returns
+    (alpha (k,), beta (k-1,)): the diagonal and off-diagonal of the tridiagonal matrix, whose eigenvalues
+    approximate the extreme eigenvalues of the operator (used on J^T J for conditioning and nullspace
+    estimates).
+    """
+    k = int(num_iterations)
+    v0 = initial_vector / torch.linalg.norm(initial_vector)
+    state = LanczosState(torch.zeros_like(v0), v0, torch.zeros(k, dtype=v0.dtype, device=v0.device),
+                         torch.zeros(k, dtype=v0.dtype, device=v0.device), 0)
+    for _ in range(k):
+        w = linear_operator(state.v_curr)
+        alpha_i = torch.dot(w, state.v_curr)
+        beta_prev = state.beta[state.iteration - 1] if state.iteration > 0 else torch.zeros((), dtype=v0.dtype,
+                                                                                             device=v0.device)
+        w_orth = w - alpha_i * state.v_curr - beta_prev * state.v_prev
+        beta_i = torch.linalg.norm(w_orth)
+        alpha = state.alpha.clone()
+        beta = state.beta.clone()
+        alpha[state.iteration] = alpha_i
+        beta[state.iteration] = beta_i
+        state = LanczosState(state.v_curr, w_orth / (beta_i + 1e-12), alpha, beta, state.iteration + 1)
+    return state.alpha, state.beta[:-1]
 
 
 def _residual_sum_squares(forward_fn, params, data) -> torch.Tensor:

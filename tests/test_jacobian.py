@@ -91,3 +91,28 @@ def test_probe_k_closed_form_matches_fft_of_probe():
     pk = jac.probe_k_from_chi(jac.aperture_mask(n, calib, kv, mrad), chi).numpy()
     ref = o.probe_k_from_coeffs(n, calib, kv, mrad, coeffs)
     assert np.allclose(pk, ref, atol=1e-10)
+
+
+def test_lanczos_diagonal_operator_closed_forms():
+    """test_solvers.py:229-269: diag(1, 2, 4) from ones(3): alpha_0 = 7/3, beta_0 = sqrt(14)/3 and the 3-step
+    tridiagonal matrix has the operator's spectrum."""
+    diagonal = torch.tensor([1.0, 2.0, 4.0], dtype=torch.float64)
+    alpha, beta = jac.lanczos_tridiagonal(lambda v: diagonal * v, torch.ones(3, dtype=torch.float64), 3)
+    assert alpha.shape == (3,) and beta.shape == (2,)
+    assert float(alpha[0]) == pytest.approx(7.0 / 3.0, rel=1e-14)
+    assert float(beta[0]) == pytest.approx(14.0 ** 0.5 / 3.0, rel=1e-14)
+    t = torch.diag(alpha) + torch.diag(beta, 1) + torch.diag(beta, -1)
+    assert torch.allclose(torch.linalg.eigvalsh(t), diagonal, atol=1e-10)
+
+
+def test_lanczos_extreme_eigenvalues_of_jtj():
+    """On a J^T J operator of a random 40x12 Jacobian the 12-step tridiagonal matrix reproduces its spectrum."""
+    g = torch.Generator().manual_seed(3)
+    jm = torch.randn(40, 12, generator=g, dtype=torch.float64)
+    alpha, beta = jac.lanczos_tridiagonal(lambda v: jm.T @ (jm @ v), torch.randn(12, generator=g,
+                                                                                   dtype=torch.float64), 12)
+    t = torch.diag(alpha) + torch.diag(beta, 1) + torch.diag(beta, -1)
+    want = torch.linalg.eigvalsh(jm.T @ jm)
+    got = torch.linalg.eigvalsh(t)
+    assert float(got[-1]) == pytest.approx(float(want[-1]), rel=1e-8)
+    assert float(got[0]) == pytest.approx(float(want[0]), rel=1e-6)

@@ -13,7 +13,8 @@ from .distributions import (TRIVIAL, TRIVIAL_DISTRIBUTION, AxisUpdate, Distribut
                             create_trivial_distribution)
 from .producers import coherence_to_distribution, position_jitter_to_distribution
 from .reduce import apply_distribution, apply_distributions
-from .parallelized import AtomicSliceData, create_atomic_slice_data, stem4d_sharded
+from .parallelized import (AtomicSliceData, cbed_amplitude_from_atoms, cbed_image_from_atoms, create_atomic_slice_data,
+                           stem4d_sharded)
 from .physics import interaction_sigma, phase_interaction_parameter, relativistic_wavelength_ang
 from .recon import (OPTIMIZERS, multi_slice_multi_modal, single_slice_multi_modal, single_slice_poscorrected,
                     single_slice_ptychography)
@@ -38,5 +39,5 @@ __all__ = [
     "TRIVIAL", "TRIVIAL_DISTRIBUTION", "AxisUpdate", "Distribution", "ReductionMode", "combine_axis_updates",
     "create_axis_update", "create_distribution", "create_trivial_distribution", "coherence_to_distribution",
     "position_jitter_to_distribution", "apply_distribution", "apply_distributions", "bind_cbed_axes",
-    "probe_modes_to_distribution", "decompose_beam_to_modes",
+    "probe_modes_to_distribution", "decompose_beam_to_modes", "cbed_amplitude_from_atoms", "cbed_image_from_atoms",
 ]

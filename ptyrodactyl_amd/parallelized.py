@@ -89,6 +89,37 @@ def _transmission_slab(pot: torch.Tensor, voltage_kv: float) -> torch.Tensor:
     return torch.polar(torch.ones_like(pot), pot * sigma).to(torch.complex64).contiguous()
 
 
+def cbed_amplitude_from_atoms(beam: torch.Tensor, atom_coords, atom_types, slice_z_bounds, atom_potentials,
+                              voltage_kv, calib_ang, atom_mask=None) -> torch.Tensor:
+    """Per-mode exit amplitudes (H, W, M) complex64 of an unshifted beam through slices built from atoms
+    (``parallelized.py:172-259``): the slice thickness is that of the first slice's z bounds, the propagator and
+    the atom grid use ``calib_ang``."""
+    sample = create_atomic_slice_data(atom_coords, atom_types, slice_z_bounds, atom_potentials, atom_mask)
+    modes = beam if beam.ndim == 3 else beam[..., None]
+    dev = _device_for(modes, sample.atom_coords)
+    n = _square_n(modes.shape[0], modes.shape[1])
+    calib = float(calib_ang)
+    slab = _transmission_slab(slice_potentials(sample, (n, n), calib, dev), float(voltage_kv))
+    dz = float(sample.slice_z_bounds[0, 1] - sample.slice_z_bounds[0, 0])
+    prop = physics.propagator_factors(n, dz, float(voltage_kv), calib, device=dev)
+    pk = _probe_k(modes, dev)
+    ones = torch.ones(pk.shape[0], dtype=torch.float32, device=dev)
+    zeros = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+    _, amp = engine.multislice_forward(slab, pk, ones, zeros, torch.zeros((1, 2), dtype=torch.float64, device=dev),
+                                       prop, want_patterns=False, want_amplitude=True)
+    return amp[0].permute(1, 2, 0)
+
+
+def cbed_image_from_atoms(beam: torch.Tensor, mode_distribution, atom_coords, atom_types, slice_z_bounds,
+                          atom_potentials, voltage_kv, calib_ang, atom_mask=None) -> torch.Tensor:
+    """CBED intensity (H, W) of :func:`cbed_amplitude_from_atoms` reduced over ``mode_distribution``
+    (sample = mode index; ``parallelized.py:262-345``), float64."""
+    from .reduce import apply_distribution
+    amps = cbed_amplitude_from_atoms(beam, atom_coords, atom_types, slice_z_bounds, atom_potentials, voltage_kv,
+                                     calib_ang, atom_mask).to(torch.complex128)
+    return apply_distribution(mode_distribution, lambda sample: amps[..., int(sample[0])])
+
+
 def stem4d_sharded(probe_modes: ProbeModes, sample: AtomicSliceData, microscope: MicroscopeConfig,
                    detector: DetectorConfig, mesh=None) -> STEM4D:
     """4D-STEM over ``detector.scan_positions_ang`` with slices built from atoms (``parallelized.py:348-536``)."""

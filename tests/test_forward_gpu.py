@@ -232,3 +232,35 @@ def test_captured_forward_replays_match_eager(n, s_count, p_count):
         torch.cuda.synchronize()
         _ = torch.randn(1000, device=dev).sum().item()  # unrelated eager work between replays
         assert torch.equal(got, eager), f"replay {it} differs"
+
+
+def test_cbed_from_atoms_matches_oracle_and_sharded_at_origin():
+    """cbed_amplitude_from_atoms / cbed_image_from_atoms (parallelized.py:172-345) on the golden capture's atom
+    sample: amplitudes vs the oracle's atom provider + multislice, and the image equals stem4d_sharded at a
+    zero scan position."""
+    import ptyrodactyl_amd as pa
+    pot = o.jax_uniform_f64(42, (32, 32, 2)) * 10.0
+    coords = np.array([[2.0, 2.5, 0.25], [4.5, 5.0, 2.25], [3.0, 4.0, 1.25]])
+    types = np.zeros(3, int)
+    bounds = np.array([[0.0, 2.0], [2.0, 4.0]])
+    kern = pot[..., 0][None]
+    mic = pa.create_microscope_config(voltage_kv=100.0, aperture_mrad=20.0, probe_shape=(32, 32))
+    det = pa.create_detector_config(real_space_calib_ang=0.2, probe_calibration_pm=20.0)
+    probe = pa.make_probe(mic, det)
+    modes = torch.stack([probe, torch.roll(probe, 3, dims=1)], dim=2).to("cuda")
+    amp = pa.cbed_amplitude_from_atoms(modes, coords, types, bounds, kern, 100.0, 0.2)
+    trans = np.stack([o.transmission_func(o.slice_potential_from_atoms(coords, types, b[0], b[1], kern, (32, 32), 0.2),
+                                          100.0) for b in bounds])
+    prop = o.propagation_func(32, 32, 2.0, 100.0, 0.2)
+    want = np.moveaxis(o.multislice_amplitude(trans, np.moveaxis(modes.cpu().numpy(), 2, 0), prop), 0, 2)
+    assert rel_l2(amp.cpu().numpy(), want) < TOL
+    dist = pa.create_distribution(torch.tensor([[0.0], [1.0]]), torch.tensor([0.25, 0.75]), "incoherent",
+                                  "probe_modes")
+    img = pa.cbed_image_from_atoms(modes, dist, coords, types, bounds, kern, 100.0, 0.2)
+    want_img = 0.25 * np.abs(want[..., 0]) ** 2 + 0.75 * np.abs(want[..., 1]) ** 2
+    assert rel_l2(img.cpu().numpy(), want_img) < TOL
+    pm = pa.create_probe_modes(modes, torch.tensor([0.25, 0.75]), 0.2)
+    det0 = pa.create_detector_config(real_space_calib_ang=0.2, probe_calibration_pm=20.0,
+                                     scan_positions_ang=torch.zeros((1, 2), dtype=torch.float64))
+    s4d = pa.stem4d_sharded(pm, pa.create_atomic_slice_data(coords, types, bounds, kern), mic, det0)
+    assert rel_l2(s4d.data[0].double().cpu().numpy(), img.cpu().numpy()) < 1e-6

@@ -726,44 +726,45 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
     const dim3 rg(nb * (N / RPB)), cg(nb * (N / NC)), blk(256);
     for (int c = C - 1; c >= 0; --c) {
       const int s0 = c * k, s1 = min(S, s0 + k);
-      WaveBuf first = X;
-      if (c == 0)
-        hipLaunchKernelGGL((k_adj_cols<N, C_IFFT_PROBE>), cg, blk, 0, st, a, none, X, w0, nb);
-      else
-        first = WaveBuf{const_cast<float2*>(a.ckpt) + ((size_t)w0 * nc + (c - 1)) * NN, (long long)nc * NN};
+      // psi_s of the segment: slot s - s0 of seg, written by the column pass that produces it; psi_{s0} of a
+      // segment with c > 0 is read from its checkpoint in place, and psi_0 is stored by the first row pass
+      // (the probe's inverse transform ends there)
+      const WaveBuf ck = c > 0 ? WaveBuf{const_cast<float2*>(a.ckpt) + ((size_t)w0 * nc + (c - 1)) * NN,
+                                         (long long)nc * NN}
+                               : none;
+      auto slot = [&](int s) { return WaveBuf{seg.base + (size_t)(s - s0) * NN, seg.stride}; };
+      auto psi = [&](int s) { return (s == s0 && c > 0) ? ck : slot(s); };
+      if (c == 0) hipLaunchKernelGGL((k_adj_cols<N, C_IFFT_PROBE>), cg, blk, 0, st, a, none, X, w0, nb);
       for (int s = s0; s < s1; ++s) {
         const bool more = s < s1 - 1;
-        const WaveBuf src = s == s0 ? first : X;
         if (s == 0) {
           if (more)
-            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_PROP>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
-                               (int)PW_FWD, s - s0, 1);
+            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_PROP>), rg, blk, 0, st, a, X, X, seg, w0, nb, s,
+                               (int)PW_FWD, 0, 1);
           else
-            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
-                               (int)PW_NONE, s - s0, 1);
-        } else {
-          if (more)
-            hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_PROP>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
-                               (int)PW_FWD, s - s0, 1);
-          else
-            hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_NONE>), rg, blk, 0, st, a, src, X, seg, w0, nb, s,
-                               (int)PW_NONE, s - s0, 1);
+            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, X, X, seg, w0, nb, s,
+                               (int)PW_NONE, 0, 1);
+        } else if (more) {
+          hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_PROP>), rg, blk, 0, st, a, psi(s), X, none, w0, nb, s,
+                             (int)PW_FWD, s - s0, 0);
         }
-        if (more) hipLaunchKernelGGL((k_adj_cols<N, C_PROP>), cg, blk, 0, st, a, X, X, w0, nb);
+        if (more) hipLaunchKernelGGL((k_adj_cols<N, C_PROP>), cg, blk, 0, st, a, X, slot(s + 1), w0, nb);
       }
       int s = s1 - 1;
       if (c == C - 1) {
-        hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, X, X, none, w0, nb, S - 1,
-                           (int)PW_FWD, 0, 0);
+        hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, psi(S - 1), X, none, w0, nb,
+                           S - 1, (int)PW_FWD, 0, 0);
         hipLaunchKernelGGL((k_adj_cols<N, C_FINAL_ADJ>), cg, blk, 0, st, a, X, Gb, w0, nb);
-        hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, S - 1,
+        hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, Gb, Gb,
+                           WaveBuf{psi(S - 1).base - (size_t)(S - 1 - s0) * NN, psi(S - 1).stride}, w0, nb, S - 1,
                            (int)PW_ADJ, S - 1 - s0, 0);
         s = S - 2;
       }
       for (; s >= s0; --s) {
         hipLaunchKernelGGL((k_adj_cols<N, C_ADJPROP>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
-        hipLaunchKernelGGL((k_adj_rows<N, PRE_ADJPROP, POST_NONE>), rg, blk, 0, st, a, Gb, Gb, seg, w0, nb, s,
-                           (int)PW_ADJ, s - s0, 0);
+        hipLaunchKernelGGL((k_adj_rows<N, PRE_ADJPROP, POST_NONE>), rg, blk, 0, st, a, Gb, Gb,
+                           WaveBuf{psi(s).base - (size_t)(s - s0) * NN, psi(s).stride}, w0, nb, s, (int)PW_ADJ,
+                           s - s0, 0);
       }
       if (gathered) launch_reduces<N>(a, w0, nb, s0, s1 - s0, false, st);  // the segment's slices
     }

@@ -136,5 +136,12 @@ def stem4d_sharded(probe_modes: ProbeModes, sample: AtomicSliceData, microscope:
     origins = torch.zeros((pos.shape[0], 2), dtype=torch.int32, device=dev)
     prop = physics.propagator_factors(n, dz, microscope.voltage_kv, calib, device=dev)
     weights = probe_modes.weights.to(device=dev, dtype=torch.float32).contiguous()
-    patterns, _ = engine.multislice_forward(slab, _probe_k(probe_modes.modes, dev), weights, origins, shifts, prop)
+    pk = _probe_k(probe_modes.modes, dev)
+    if torch.is_grad_enabled() and (slab.requires_grad or pk.requires_grad or weights.requires_grad
+                                    or shifts.requires_grad):
+        # differentiable w.r.t. the modes, their weights, the atom kernels and the positions (HIP adjoint)
+        from . import autograd
+        patterns = autograd.multislice_patterns(slab, pk, weights, origins, shifts, prop)
+    else:
+        patterns, _ = engine.multislice_forward(slab, pk, weights, origins, shifts, prop)
     return create_stem4d(patterns, calib, 1.0 / (n * calib), pos, microscope.voltage_kv)

@@ -64,7 +64,11 @@ def aberration(fourier_coord, lambda_angstrom, defocus=0.0, c3=0.0, c5=0.0) -> t
                                                                                       dtype=torch.float64)
     lam = float(lambda_angstrom)
     p = lam * q.to(torch.float64)
-    chi = (float(defocus) * p**2) / 2 + (float(c3) * 1e7 * p**4) / 4 + (float(c5) * 1e7 * p**6) / 6
+
+    def coeff(v):  # tensors (which may require grad) are kept; numbers become floats
+        return v.to(device=p.device, dtype=torch.float64) if isinstance(v, torch.Tensor) else float(v)
+
+    chi = (coeff(defocus) * p**2) / 2 + (coeff(c3) * 1e7 * p**4) / 4 + (coeff(c5) * 1e7 * p**6) / 6
     return (2 * math.pi * chi) / lam
 
 

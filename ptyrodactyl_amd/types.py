@@ -54,6 +54,13 @@ def _scalar(value, name: str) -> float:
     return float(t)
 
 
+def _keep_grad(original, checked: float):
+    """The original 0-d tensor when it requires grad (validated through ``checked``), else the float."""
+    if isinstance(original, torch.Tensor) and original.requires_grad:
+        return original.to(torch.float64)
+    return checked
+
+
 def _checked_finite(value: float, name: str) -> float:
     error_if(not torch.isfinite(torch.tensor(value)), f"{name} must be finite")
     return value
@@ -208,9 +215,10 @@ def create_microscope_config(voltage_kv, aperture_mrad, defocus_ang=0.0, c3_ang=
     """``electron_types.py:529-614``."""
     voltage = _checked_positive(_scalar(voltage_kv, "voltage_kv"), "voltage_kv")
     aperture = _checked_positive(_scalar(aperture_mrad, "aperture_mrad"), "aperture_mrad")
-    defocus = _checked_finite(_scalar(defocus_ang, "defocus_ang"), "defocus_ang")
-    c3 = _checked_finite(_scalar(c3_ang, "c3_ang"), "c3_ang")
-    c5 = _checked_finite(_scalar(c5_ang, "c5_ang"), "c5_ang")
+    # the probe-forming aberrations stay tensors when they require grad: make_probe differentiates them
+    defocus = _keep_grad(defocus_ang, _checked_finite(_scalar(defocus_ang, "defocus_ang"), "defocus_ang"))
+    c3 = _keep_grad(c3_ang, _checked_finite(_scalar(c3_ang, "c3_ang"), "c3_ang"))
+    c5 = _keep_grad(c5_ang, _checked_finite(_scalar(c5_ang, "c5_ang"), "c5_ang"))
     shape = _optional_shape(probe_shape, "probe_shape")
     return MicroscopeConfig(voltage, aperture, defocus, c3, c5,
                             create_ensemble_axes() if ensemble is None else ensemble, shape)

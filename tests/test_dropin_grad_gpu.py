@@ -151,3 +151,35 @@ def test_stem4d_gradient_wrt_scan_positions_matches_oracle_fd():
     err = float(np.linalg.norm(g_pos - fd) / np.linalg.norm(fd))
     print(f"stem_4d dL/dpositions vs oracle FD: rel-L2 {err:.2e}", flush=True)
     assert err < 1e-3
+
+
+def test_cbed_image_gradient_wrt_probe_aberrations():
+    """Defocus and C3 given as tensors reach the probe through make_probe's aberration phase, so cbed_image is
+    differentiable w.r.t. them (the reference differentiates the carrier scalars, test_simulations.py:319-374);
+    compared with central differences of the oracle."""
+    import ptyrodactyl_amd as pa
+    n = 32
+    rng = np.random.default_rng(21)
+    pot = rng.uniform(0, 25, (n, n, 2))
+    g = rng.standard_normal((n, n))
+    dev = torch.device("cuda", 0)
+    defocus = torch.tensor(30.0, dtype=torch.float64, requires_grad=True)
+    c3 = torch.tensor(2e-3, dtype=torch.float64, requires_grad=True)
+    mic = pa.create_microscope_config(voltage_kv=200.0, aperture_mrad=25.0, defocus_ang=defocus, c3_ang=c3,
+                                      probe_shape=(n, n))
+    det = pa.create_detector_config(real_space_calib_ang=0.25)
+    probe = pa.make_probe(mic, det)
+    beam = pa.create_probe_modes(probe[..., None].to(dev), torch.tensor([1.0]), 0.25)
+    img = pa.cbed_image(pa.create_potential_slices(torch.tensor(pot, device=dev), 1.5, 0.25), beam, mic)
+    g_def, g_c3 = torch.autograd.grad(torch.sum(img.data_array.double() * torch.tensor(g, device=dev)), (defocus, c3))
+
+    def oracle_loss(df, cs):
+        pr = o.make_probe(200.0, 25.0, df, cs, 0.0, (n, n), 25.0)
+        return float(np.sum(g * o.cbed_image(pot, 1.5, 0.25, pr[..., None], [1.0], 0.25, 200.0)[0]))
+
+    fd_def = (oracle_loss(30.01, 2e-3) - oracle_loss(29.99, 2e-3)) / 0.02
+    fd_c3 = (oracle_loss(30.0, 2e-3 + 1e-6) - oracle_loss(30.0, 2e-3 - 1e-6)) / 2e-6
+    print(f"d/ddefocus GPU {float(g_def):.6e} FD {fd_def:.6e}; d/dC3 GPU {float(g_c3):.6e} FD {fd_c3:.6e}",
+          flush=True)
+    assert abs(float(g_def) - fd_def) <= 1e-3 * abs(fd_def) + 1e-9
+    assert abs(float(g_c3) - fd_c3) <= 1e-3 * abs(fd_c3) + 1e-9

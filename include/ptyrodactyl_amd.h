@@ -252,6 +252,45 @@ int pty_optimizer_update(int rule, int fp64, int is_complex, long long count, vo
  */
 int pty_random_normal(unsigned key0, unsigned key1, long long count, double* out, void* stream);
 
+/*
+ * Independent-atom potentials, FP64.  A coefficient row is 12 doubles:
+ * PTY_PARAM_LOBATO    a1, b1, ..., a5, b5 (last two unused) — Lobato & Van Dyck (2014);
+ * PTY_PARAM_KIRKLAND  a1, b1, a2, b2, a3, b3, c1, d1, c2, d2, c3, d3 — Kirkland (legacy 0.5292 * 14.4 scale).
+ * Bessel K0/K1 are evaluated as the reference evaluates them (Abramowitz & Stegun 9.8.5-9.8.8).
+ */
+#define PTY_PARAM_LOBATO 0
+#define PTY_PARAM_KIRKLAND 1
+
+/*
+ * Projected potential (V A) of one element at `count` radii r (A):
+ * lobato_projected_potential / kirkland_projected_potential
+ * (src/ptyrodactyl/multislice/form_factors.py:323-439).
+ */
+int pty_projected_potential(int parameterization, const double* row, long long count, const double* r, double* out,
+                            void* stream);
+
+/*
+ * Potentials of num_types elements (rows [T][12]) on a height x width grid of pixel_size A, each pixel the mean of
+ * supersampling^2 samples of the supersampled grid whose origin is sample (height*ss/2, width*ss/2); element t is
+ * centred at centers[t] = (x, y) A (NULL: the origin).  out [T][height][width].  single_atom_potential
+ * (src/ptyrodactyl/multislice/atom_potentials.py:588-681) for several elements in one launch.
+ */
+int pty_atom_potential_grid(int parameterization, int num_types, const double* rows, double pixel_size, int height,
+                            int width, int supersampling, const double* centers, double* out, void* stream);
+
+/*
+ * Fourier-space potential slices: for every slice s,
+ *   out_k[s] = sum_t kernel_k[t] * sum_{a in [segments[s*T+t], segments[s*T+t+1])} exp(2 pi i (kx sx_a + ky sy_a))
+ * with (sx_a, sy_a) = shifts[a] in pixels and kx, ky numpy's fftfreq(width), fftfreq(height).  Atoms are sorted by
+ * slice, then element; segments has num_slices * num_types + 1 offsets into the num_atoms shifts (values are
+ * clamped to [0, num_atoms]).  kernel_k [T][H][W] and out_k [S][H][W] are complex128.  Re ifft2(out_k[s]) is the
+ * reference's slice: the per-atom Fourier shift of each element's centred potential in _process_all_slices
+ * (src/ptyrodactyl/multislice/atom_potentials.py:826-993), summed by linearity.
+ */
+int pty_potential_slices_fourier(int num_slices, int num_types, int height, int width, long long num_atoms,
+                                 const double* shifts, const long long* segments, const void* kernel_k, void* out_k,
+                                 void* stream);
+
 /* Device bytes of the state block and workspace of pty_forward_persistent. */
 size_t pty_persistent_state_bytes(void);
 size_t pty_persistent_workspace_bytes(int n, int group_waves);

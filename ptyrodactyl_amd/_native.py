@@ -56,6 +56,9 @@ SIGNATURES = {
     "pty_forward_detectors": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                       c_vp, c_int, c_vp, c_vp, c_vp, c_size, c_vp]),
     "pty_random_normal": (c_int, [ctypes.c_uint, ctypes.c_uint, c_ll, c_vp, c_vp]),
+    "pty_projected_potential": (c_int, [c_int, c_vp, c_ll, c_vp, c_vp, c_vp]),
+    "pty_atom_potential_grid": (c_int, [c_int, c_int, c_vp, c_dbl, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "pty_potential_slices_fourier": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pty_optimizer_update": (c_int, [c_int, c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_ll, c_dbl, c_vp]),
     "pty_persistent_state_bytes": (c_size, []),
     "pty_persistent_workspace_bytes": (c_size, [c_int, c_int]),

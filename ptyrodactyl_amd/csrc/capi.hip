@@ -12,6 +12,7 @@
 #include "aux_kernels.hpp"
 #include "forward.hpp"
 #include "optim.hpp"
+#include "potential.hpp"
 #include "xl2.hpp"
 
 namespace {
@@ -329,6 +330,50 @@ int pty_random_normal(unsigned key0, unsigned key1, long long count, double* out
   if (count < 0 || count > (1LL << 40)) return fail(PTY_ERR_INVALID, "count must be in [0, 2^40]");
   if (count > 0 && !out) return fail(PTY_ERR_INVALID, "null pointer");
   return device_status(pty::launch_random_normal(key0, key1, count, out, (hipStream_t)stream), "pty_random_normal");
+}
+
+int pty_projected_potential(int parameterization, const double* row, long long count, const double* r, double* out,
+                            void* stream) {
+  g_err[0] = '\0';
+  if (parameterization != PTY_PARAM_LOBATO && parameterization != PTY_PARAM_KIRKLAND)
+    return fail(PTY_ERR_INVALID, "parameterization must be PTY_PARAM_LOBATO or PTY_PARAM_KIRKLAND");
+  if (count < 0 || count > (1LL << 40)) return fail(PTY_ERR_INVALID, "count must be in [0, 2^40]");
+  if (!row || (count > 0 && (!r || !out))) return fail(PTY_ERR_INVALID, "null pointer");
+  return device_status(pty::launch_projected_radii(parameterization, row, count, r, out, (hipStream_t)stream),
+                       "pty_projected_potential");
+}
+
+int pty_atom_potential_grid(int parameterization, int num_types, const double* rows, double pixel_size, int height,
+                            int width, int supersampling, const double* centers, double* out, void* stream) {
+  g_err[0] = '\0';
+  if (parameterization != PTY_PARAM_LOBATO && parameterization != PTY_PARAM_KIRKLAND)
+    return fail(PTY_ERR_INVALID, "parameterization must be PTY_PARAM_LOBATO or PTY_PARAM_KIRKLAND");
+  if (num_types < 0 || num_types > 65535) return fail(PTY_ERR_INVALID, "num_types must be in [0, 65535]");
+  if (height < 1 || width < 1 || height > 32768 || width > 32768)
+    return fail(PTY_ERR_INVALID, "height and width must be in [1, 32768]");
+  if (supersampling < 1 || supersampling > 64) return fail(PTY_ERR_INVALID, "supersampling must be in [1, 64]");
+  if (!(pixel_size > 0.0) || !isfinite(pixel_size)) return fail(PTY_ERR_INVALID, "pixel_size must be positive");
+  if (num_types > 0 && (!rows || !out)) return fail(PTY_ERR_INVALID, "null pointer");
+  return device_status(pty::launch_atom_grid(parameterization, num_types, rows, pixel_size, height, width,
+                                             supersampling, centers, out, (hipStream_t)stream),
+                       "pty_atom_potential_grid");
+}
+
+int pty_potential_slices_fourier(int num_slices, int num_types, int height, int width, long long num_atoms,
+                                 const double* shifts, const long long* segments, const void* kernel_k, void* out_k,
+                                 void* stream) {
+  g_err[0] = '\0';
+  if (num_slices < 0 || num_slices > 65535) return fail(PTY_ERR_INVALID, "num_slices must be in [0, 65535]");
+  if (num_types < 0 || num_types > 4096) return fail(PTY_ERR_INVALID, "num_types must be in [0, 4096]");
+  if (height < 1 || width < 1 || height > 32768 || width > 32768)
+    return fail(PTY_ERR_INVALID, "height and width must be in [1, 32768]");
+  if (num_atoms < 0) return fail(PTY_ERR_INVALID, "num_atoms must be non-negative");
+  if (num_slices > 0 && (!segments || !out_k || (num_types > 0 && !kernel_k) || (num_atoms > 0 && !shifts)))
+    return fail(PTY_ERR_INVALID, "null pointer");
+  return device_status(pty::launch_slice_structure(num_slices, num_types, height, width, num_atoms, shifts,
+                                                   segments, (const double2*)kernel_k, (double2*)out_k,
+                                                   (hipStream_t)stream),
+                       "pty_potential_slices_fourier");
 }
 
 size_t pty_persistent_state_bytes(void) { return pty::xl2_state_bytes(); }

@@ -1,0 +1,277 @@
+// Independent-atom potential builders on gfx950, all in FP64:
+//
+//  * k_projected_radii — the projected potential of one element at given
+//    radii (src/ptyrodactyl/multislice/form_factors.py:323-439);
+//  * k_atom_grid — every element's potential on the (height, width) grid,
+//    evaluated on the supersampled grid and block-averaged in the same
+//    thread (single_atom_potential, atom_potentials.py:588-681);
+//  * k_slice_structure — the Fourier-space slices
+//      F_s(k) = sum_t Khat_t(k) sum_{atoms a of element t in slice s}
+//               exp(2 pi i (kx sx_a + ky sy_a)),
+//    which is the reference's per-atom Fourier shift
+//    (_process_all_slices, atom_potentials.py:826-993) summed by linearity,
+//    so one inverse FFT per slice replaces one FFT pair per atom.
+//
+// K0 and K1 follow the reference's evaluation: the polynomial
+// approximations of Abramowitz & Stegun 9.8.5-9.8.8 (small branch x <= 2
+// with I0, I1 by their power series; large branch x > 2).
+#include <math.h>
+
+#include "common.hpp"
+#include "potential.hpp"
+
+namespace pty {
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kMottBethe = 47.87801;           // V A^2 (Lobato normalisation)
+constexpr double kKirklandA0Ek = 0.5292 * 14.4;   // legacy Kirkland normalisation
+
+__device__ __forceinline__ double bessel_i0_series(double x) {
+  const double q = 0.25 * x * x;
+  double term = 1.0, total = 1.0;
+#pragma unroll
+  for (int k = 1; k < 26; ++k) {
+    term *= q / (double)(k * k);
+    total += term;
+  }
+  return total;
+}
+
+__device__ __forceinline__ double bessel_i1_series(double x) {
+  const double q = 0.25 * x * x;
+  double term = 0.5 * x, total = term;
+#pragma unroll
+  for (int k = 1; k < 26; ++k) {
+    term *= q / (double)(k * (k + 1));
+    total += term;
+  }
+  return total;
+}
+
+// A&S 9.8.5 / 9.8.6.
+__device__ double bessel_k0(double x) {
+  if (x <= 2.0) {
+    const double q = 0.25 * x * x;
+    const double poly = -0.57721566 + q * (0.42278420 + q * (0.23069756 + q * (0.03488590 + q * (0.00262698 +
+                        q * (0.00010750 + q * 0.00000740)))));
+    return -log(0.5 * x) * bessel_i0_series(x) + poly;
+  }
+  const double y = 2.0 / x;
+  const double poly = 1.25331414 + y * (-0.07832358 + y * (0.02189568 + y * (-0.01062446 + y * (0.00587872 +
+                      y * (-0.00251540 + y * 0.00053208)))));
+  return exp(-x) / sqrt(x) * poly;
+}
+
+// A&S 9.8.7 / 9.8.8.
+__device__ double bessel_k1(double x) {
+  if (x <= 2.0) {
+    const double q = 0.25 * x * x;
+    const double poly = 1.0 + q * (0.15443144 + q * (-0.67278579 + q * (-0.18156897 + q * (-0.01919402 +
+                        q * (-0.00110404 + q * -0.00004686)))));
+    return log(0.5 * x) * bessel_i1_series(x) + poly / x;
+  }
+  const double y = 2.0 / x;
+  const double poly = 1.25331414 + y * (0.23498619 + y * (-0.03655620 + y * (0.01504268 + y * (-0.00780353 +
+                      y * (0.00325614 + y * -0.00068245)))));
+  return exp(-x) / sqrt(x) * poly;
+}
+
+// Projected potential (V A) at radius r of one element's coefficient row.
+__device__ __forceinline__ double projected_potential(int kind, const double* row, double r) {
+  r = fmax(r, 1e-10);
+  if (kind == PARAM_LOBATO) {
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const double a = row[2 * i], b = row[2 * i + 1];
+      const double sb = sqrt(b);
+      const double arg = 2.0 * kPi * r / sb;
+      sum += a / b * bessel_k0(arg) + a * kPi * r / (b * sb) * bessel_k1(arg);
+    }
+    return 2.0 * kPi * kMottBethe * sum;
+  }
+  double lor[3], gau[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    lor[i] = row[2 * i] * bessel_k0(2.0 * kPi * r * sqrt(row[2 * i + 1]));
+    const double c = row[6 + 2 * i], d = row[7 + 2 * i];
+    gau[i] = c / d * exp(-(kPi * kPi) * r * r / d);
+  }
+  return 4.0 * kPi * kPi * kKirklandA0Ek * ((lor[0] + lor[1]) + lor[2]) +
+         2.0 * kPi * kPi * kKirklandA0Ek * ((gau[0] + gau[1]) + gau[2]);
+}
+
+__global__ __launch_bounds__(256) void k_projected_radii(int kind, const double* __restrict__ row, long long count,
+                                                         const double* __restrict__ r, double* __restrict__ out) {
+  double p[PARAM_STRIDE];
+#pragma unroll
+  for (int i = 0; i < PARAM_STRIDE; ++i) p[i] = row[i];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = projected_potential(kind, p, r[i]);
+}
+
+// out[t][i][j] = mean over the supersampled block of pixel (i, j) of the
+// potential of element t centred at centers[t] (x, y in A; origin at the
+// supersampled index (height*ss/2, width*ss/2)).
+__global__ __launch_bounds__(256) void k_atom_grid(int kind, const double* __restrict__ rows, double pixel_size,
+                                                   int height, int width, int ss, const double* __restrict__ centers,
+                                                   double* __restrict__ out) {
+  const int t = blockIdx.y;
+  double p[PARAM_STRIDE];
+#pragma unroll
+  for (int i = 0; i < PARAM_STRIDE; ++i) p[i] = rows[(long long)t * PARAM_STRIDE + i];
+  const double cx = centers ? centers[2 * t] : 0.0;
+  const double cy = centers ? centers[2 * t + 1] : 0.0;
+  const double step = pixel_size / ss;
+  const int gh2 = (height * ss) / 2, gw2 = (width * ss) / 2;
+  const long long pix = (long long)height * width;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < pix;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / width), j = (int)(e % width);
+    double acc = 0.0;
+    for (int a = 0; a < ss; ++a) {
+      const double dy = (double)(i * ss + a - gh2) * step - cy;
+      for (int b = 0; b < ss; ++b) {
+        const double dx = (double)(j * ss + b - gw2) * step - cx;
+        acc += projected_potential(kind, p, sqrt(dx * dx + dy * dy + 1e-10));
+      }
+    }
+    out[(long long)t * pix + e] = acc / (double)(ss * ss);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Structure-factor slices.  A workgroup owns a 64 x 64 tile of one slice's
+// Fourier grid; thread (tx, ty) holds column tx and rows ty + 4 j (j < 16).
+// For each element the slice's atoms stream through LDS in chunks of 32:
+// every thread writes 16 of the chunk's 32 x (64 + 64) phase factors
+// u_a(ky) = exp(2 pi i ky sy_a), v_a(kx) = exp(2 pi i kx sx_a), then each
+// thread accumulates sum_a u_a(ky_j) v_a(kx) for its 16 pixels (u is a
+// wave-uniform broadcast read).  The element's sum is weighted by Khat_t
+// and added into the slice value.  Frequencies follow numpy's fftfreq.
+// ---------------------------------------------------------------------------
+
+constexpr int SF_TILE = 64;
+constexpr int SF_ROWS_PER_THREAD = 16;
+constexpr int SF_CHUNK = 32;
+
+__device__ __forceinline__ double2 phase_factor(int g, int n, double shift) {
+  const int k = (g < (n + 1) / 2) ? g : g - n;
+  double t = (double)k * shift / (double)n;
+  t -= rint(t);
+  double s, c;
+  sincospi(2.0 * t, &s, &c);
+  return make_double2(c, s);
+}
+
+__global__ __launch_bounds__(256) void k_slice_structure(int num_types, int height, int width, long long num_atoms,
+                                                         const double* __restrict__ shifts,
+                                                         const long long* __restrict__ segments,
+                                                         const double2* __restrict__ kernel_k,
+                                                         double2* __restrict__ out_k) {
+  __shared__ double2 u_lds[SF_CHUNK][SF_TILE];
+  __shared__ double2 v_lds[SF_CHUNK][SF_TILE];
+  const int s = blockIdx.z;
+  const int x0 = blockIdx.x * SF_TILE, y0 = blockIdx.y * SF_TILE;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int col = x0 + tx;
+  const long long plane = (long long)height * width;
+  double2 f[SF_ROWS_PER_THREAD];
+#pragma unroll
+  for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) f[j] = make_double2(0.0, 0.0);
+
+  for (int t = 0; t < num_types; ++t) {
+    long long a0 = segments[(long long)s * num_types + t];
+    long long a1 = segments[(long long)s * num_types + t + 1];
+    a0 = a0 < 0 ? 0 : (a0 > num_atoms ? num_atoms : a0);
+    a1 = a1 < 0 ? 0 : (a1 > num_atoms ? num_atoms : a1);
+    if (a1 <= a0) continue;
+    double2 acc[SF_ROWS_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) acc[j] = make_double2(0.0, 0.0);
+    for (long long c0 = a0; c0 < a1; c0 += SF_CHUNK) {
+      const int n = (int)((a1 - c0) < SF_CHUNK ? (a1 - c0) : SF_CHUNK);
+      __syncthreads();
+      for (int e = threadIdx.x; e < n * 2 * SF_TILE; e += blockDim.x) {
+        const int a = e / (2 * SF_TILE), q = e % (2 * SF_TILE);
+        const double* sh = shifts + 2 * (c0 + a);
+        if (q < SF_TILE)
+          u_lds[a][q] = phase_factor(y0 + q, height, sh[1]);
+        else
+          v_lds[a][q - SF_TILE] = phase_factor(x0 + q - SF_TILE, width, sh[0]);
+      }
+      __syncthreads();
+      for (int a = 0; a < n; ++a) {
+        const double2 v = v_lds[a][tx];
+#pragma unroll
+        for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) {
+          const double2 u = u_lds[a][ty + 4 * j];
+          acc[j].x = fma(u.x, v.x, fma(-u.y, v.y, acc[j].x));
+          acc[j].y = fma(u.x, v.y, fma(u.y, v.x, acc[j].y));
+        }
+      }
+    }
+    if (col < width) {
+      const double2* kt = kernel_k + (long long)t * plane;
+#pragma unroll
+      for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) {
+        const int row = y0 + ty + 4 * j;
+        if (row < height) {
+          const double2 k = kt[(long long)row * width + col];
+          f[j].x = fma(k.x, acc[j].x, fma(-k.y, acc[j].y, f[j].x));
+          f[j].y = fma(k.x, acc[j].y, fma(k.y, acc[j].x, f[j].y));
+        }
+      }
+    }
+  }
+  if (col < width) {
+#pragma unroll
+    for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) {
+      const int row = y0 + ty + 4 * j;
+      if (row < height) out_k[(long long)s * plane + (long long)row * width + col] = f[j];
+    }
+  }
+}
+
+long long grid_blocks(long long work) {
+  long long b = (work + 255) / 256;
+  if (b > 65536) b = 65536;
+  return b < 1 ? 1 : b;
+}
+
+}  // namespace
+
+hipError_t launch_projected_radii(int kind, const double* row, long long count, const double* r, double* out,
+                                  hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_projected_radii, dim3((unsigned)grid_blocks(count)), dim3(256), 0, st, kind, row, count, r,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_atom_grid(int kind, int num_types, const double* rows, double pixel_size, int height, int width,
+                            int supersampling, const double* centers, double* out, hipStream_t st) {
+  const long long pix = (long long)height * width;
+  if (num_types == 0 || pix == 0) return hipSuccess;
+  long long bx = grid_blocks(pix);
+  if (bx > 4096) bx = 4096;
+  hipLaunchKernelGGL(k_atom_grid, dim3((unsigned)bx, (unsigned)num_types), dim3(256), 0, st, kind, rows,
+                     pixel_size, height, width, supersampling, centers, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_slice_structure(int num_slices, int num_types, int height, int width, long long num_atoms,
+                                  const double* shifts, const long long* segments, const double2* kernel_k,
+                                  double2* out_k, hipStream_t st) {
+  if (num_slices == 0 || height == 0 || width == 0) return hipSuccess;
+  const dim3 grid((unsigned)((width + SF_TILE - 1) / SF_TILE), (unsigned)((height + SF_TILE - 1) / SF_TILE),
+                  (unsigned)num_slices);
+  hipLaunchKernelGGL(k_slice_structure, grid, dim3(256), 0, st, num_types, height, width, num_atoms, shifts,
+                     segments, kernel_k, out_k);
+  return hipGetLastError();
+}
+
+}  // namespace pty

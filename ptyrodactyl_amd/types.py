@@ -291,6 +291,51 @@ def create_potential_slices(slices, slice_thickness, calib) -> PotentialSlices:
     return PotentialSlices(slices_t, thick, calib_f)
 
 
+@dataclass(frozen=True)
+class CrystalData:
+    """``types/crystal_types.py:84-121``: positions (N, 3) A, atomic numbers (N,), lattice rows a, b, c (3, 3) A."""
+
+    positions: torch.Tensor
+    atomic_numbers: torch.Tensor
+    lattice: Optional[torch.Tensor]
+    stress: Optional[torch.Tensor] = None
+    energy: Optional[float] = None
+    properties: Optional[list] = None
+    comment: Optional[str] = None
+
+
+def create_crystal_data(positions, atomic_numbers, lattice=None, stress=None, energy=None, properties=None,
+                        comment=None) -> CrystalData:
+    """``types/crystal_types.py:244-400`` (same structural and value checks; lattice defaults to the identity)."""
+    pos = _as_tensor(positions, torch.float64)
+    nums = _as_tensor(atomic_numbers, torch.int32)
+    lat = torch.eye(3, dtype=torch.float64) if lattice is None else _as_tensor(lattice, torch.float64)
+    st = None if stress is None else _as_tensor(stress, torch.float64)
+    en = None if energy is None else torch.as_tensor(energy, dtype=torch.float64)
+    if pos.ndim != 2:
+        raise ValueError("positions must be 2D")
+    if pos.shape[1] != 3:
+        raise ValueError("positions must have shape (N, 3)")
+    if nums.ndim != 1:
+        raise ValueError("atomic_numbers must be 1D")
+    if nums.shape[0] != pos.shape[0]:
+        raise ValueError("atomic_numbers must have shape (N,)")
+    if tuple(lat.shape) != (3, 3):
+        raise ValueError("lattice must have shape (3, 3)")
+    if st is not None and tuple(st.shape) != (3, 3):
+        raise ValueError("stress must have shape (3, 3)")
+    if en is not None and en.ndim != 0:
+        raise ValueError("energy must be a scalar")
+    error_if(~torch.isfinite(pos).all(), "positions contain non-finite values")
+    error_if((nums < 0).any(), "atomic_numbers must be non-negative")
+    error_if(~torch.isfinite(lat).all(), "lattice contains non-finite values")
+    if st is not None:
+        error_if(~torch.isfinite(st).all(), "stress contains non-finite values")
+    if en is not None:
+        error_if(~torch.isfinite(en), "energy must be finite")
+    return CrystalData(pos, nums, lat, st, None if en is None else float(en), properties, comment)
+
+
 def create_stem4d(data, real_space_calib, fourier_space_calib, scan_positions, voltage_kv) -> STEM4D:
     """``electron_types.py:1401-1504``."""
     data_t = data if isinstance(data, torch.Tensor) else torch.as_tensor(data, dtype=torch.float64)

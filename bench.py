@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 
 # HBM bytes per step of the config-2 forward (streaming path, 512-wave batches) from two rocprofv3 --pmc passes
 # (FETCH_SIZE, then WRITE_SIZE) summed over the step's launches.  FETCH_SIZE reads 256 MiB for a 512 MiB copy
@@ -188,6 +189,74 @@ def serving_bench(count: int, qps: float):
     return out
 
 
+def potential_bench(steps: int = 3):
+    """Projected-potential slices of the headline slab (1024^2 at 0.1 A, 64 slices of 1 A, perovskite-like
+    crystal from ``synth.crystal_atoms``, Lobato form, supersampling 4) built on the GPU.  The coefficient rows
+    for Sr, Ti and O are synthetic (positive amplitudes and scales): the kernels' cost does not depend on the
+    values.  Times the whole builder and, with HIP events on its stream, the structure-factor kernel alone; the
+    kernel's algorithmic work is 8 FP64 flops (4 FMA) per (atom, Fourier pixel) of its slice."""
+    import math
+    import torch
+    from ptyrodactyl_amd import _native, atom_potentials as ap, form_factors as ff, synth
+    from ptyrodactyl_amd.types import create_crystal_data
+    grid, calib, s_count = 1024, 0.1, 64
+    cells = int(math.ceil(grid * calib / synth.LATTICE_A))
+    atoms = synth.crystal_atoms(cells, cells, int(math.ceil(s_count / synth.LATTICE_A)) + 1, seed=0)
+    saved = dict(ff._TABLES)
+    ff.set_form_factor_table("lobato", {z: [z / 50.0, 0.5, z / 60.0, 1.2, z / 80.0, 3.0, z / 100.0, 8.0, z / 120.0,
+                                            20.0] for z in (8, 22, 38)})
+    try:
+        dev = torch.device("cuda", 0)
+        crystal = create_crystal_data(atoms[:, [1, 0, 2]], atoms[:, 3].astype(np.int64))
+        crystal = type(crystal)(crystal.positions.to(dev), crystal.atomic_numbers.to(dev), crystal.lattice)
+        shape = (grid, grid, s_count)
+
+        def build():
+            return ap.crystal_potential_slices(crystal, calib, 1.0, padding=0.0, supersampling=4, grid_shape=shape)
+
+        build()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = build()
+        torch.cuda.synchronize()
+        total_ms = (time.perf_counter() - t0) * 1e3 / steps
+        plan = ap.plan_slices(crystal, calib, 1.0, (1, 1, 1), 0.0, shape, dev)
+        rows = np.stack([ff.coefficient_row(int(z), "lobato") for z in plan.elements.tolist()])
+        kern_k = torch.fft.fft2(ap._atom_grids(rows, "lobato", calib, grid, grid, 4, None, dev)).contiguous()
+        fk = torch.empty((s_count, grid, grid), dtype=torch.complex128, device=dev)
+        lib = _native.load()
+
+        def kernel():
+            st = lib.pty_potential_slices_fourier(s_count, rows.shape[0], grid, grid, plan.shifts.shape[0],
+                                                  _native.ptr(plan.shifts), _native.ptr(plan.segments),
+                                                  _native.ptr(kern_k), _native.ptr(fk), _native.stream_handle(dev))
+            _native.check(st, "pty_potential_slices_fourier")
+
+        kernel()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            kernel()
+        e1.record()
+        torch.cuda.synchronize()
+        k_ms = e0.elapsed_time(e1) / steps
+        n_atoms = int(plan.shifts.shape[0])
+        flops = 8.0 * n_atoms * grid * grid
+        tf = flops / (k_ms * 1e-3) / 1e12
+        return {"workload": "crystal_potential_slices_1024x1024x64_lobato_ss4", "atoms": n_atoms,
+                "elements": plan.elements.tolist(), "coefficients": "synthetic (timing only)",
+                "builder_ms": round(total_ms, 2), "slices_per_s": round(s_count / (total_ms * 1e-3), 1),
+                "structure_kernel": {"kernel": "k_slice_structure", "ms": round(k_ms, 3), "bound": "fp64",
+                                     "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                                     "formula": "8 flops x atoms x H x W (each atom over its slice's grid)"},
+                "output_checksum": float(out.slices.sum())}
+    finally:
+        ff._TABLES.update(saved)
+
+
 def membw_bench(sizes_mib=(256, 512, 1024, 4096), reps: int = 20):
     """Achievable HBM bandwidth on this box: torch device copies (read + write) and reductions (read) of
     float32 buffers, timed with events; the practical ceiling the streaming passes are compared with."""
@@ -233,7 +302,8 @@ def main() -> int:
     ap.add_argument("--serving-qps", type=float, default=60.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
-    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw"), default="config2",
+    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw", "potential"),
+                    default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
                          "config1: 128^2 x 20 slices, 256 positions (the reference's CPU-runnable case)")
     args = ap.parse_args()
@@ -253,6 +323,12 @@ def main() -> int:
                           "unit": "patterns/s", "n_gpus": 1, "higher_is_better": True, "serving": res}), flush=True)
         return 0
 
+    if args.workload == "potential":
+        res = potential_bench(max(args.steps, 1))
+        print(json.dumps({"metric": "potential slices/s, 1024^2 x 64 slab", "value": res["slices_per_s"],
+                          "unit": "slices/s", "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
+                          "data": "synthetic", "potential_builder": res}), flush=True)
+        return 0
     if args.workload == "membw":
         res = membw_bench()
         print(json.dumps({"metric": "achievable HBM bandwidth (torch copy / reduction)",
@@ -361,6 +437,9 @@ def main() -> int:
     if not args.no_gradient and args.workload == "config2":
         result["gradient"] = gradient_bench(max(args.steps, 3), 1)
         print(f"# gradient {result['gradient']}", file=sys.stderr, flush=True)
+    if args.workload == "config2" and not persistent:
+        result["potential_builder"] = potential_bench(max(min(args.steps, 3), 1))
+        print(f"# potential builder {result['potential_builder']}", file=sys.stderr, flush=True)
     if not args.no_serving and args.workload == "config2":
         del slab, out
         torch.cuda.empty_cache()

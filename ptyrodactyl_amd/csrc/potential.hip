@@ -144,19 +144,28 @@ __global__ __launch_bounds__(256) void k_atom_grid(int kind, const double* __res
 }
 
 // ---------------------------------------------------------------------------
-// Structure-factor slices.  A workgroup owns a 64 x 64 tile of one slice's
-// Fourier grid; thread (tx, ty) holds column tx and rows ty + 4 j (j < 16).
-// For each element the slice's atoms stream through LDS in chunks of 32:
-// every thread writes 16 of the chunk's 32 x (64 + 64) phase factors
-// u_a(ky) = exp(2 pi i ky sy_a), v_a(kx) = exp(2 pi i kx sx_a), then each
-// thread accumulates sum_a u_a(ky_j) v_a(kx) for its 16 pixels (u is a
-// wave-uniform broadcast read).  The element's sum is weighted by Khat_t
-// and added into the slice value.  Frequencies follow numpy's fftfreq.
+// Structure-factor slices.  A workgroup owns a 64 x 128 tile of one slice's
+// Fourier grid; thread (tr, tc) (tc = lane % 16, tr = 4 wave + lane / 16)
+// holds the 4 x 8 pixels rows tr + 16 i, columns tc + 16 j.  For each
+// element the slice's atoms stream through LDS in chunks of 16: the phase
+// factors u_a(ky) = exp(2 pi i ky sy_a) (64 rows) and v_a(kx) =
+// exp(2 pi i kx sx_a) (128 columns) are generated in runs of 16 consecutive
+// frequencies, each run from one sincospi and a multiplicative step
+// exp(2 pi i s / n) (a fresh sincospi where fftfreq wraps to negative
+// frequencies); then every thread accumulates sum_a u_a(ky_i) v_a(kx_j) for
+// its 32 pixels, 12 LDS reads per 128 FMAs.  The element's sum is weighted
+// by Khat_t and written (first element) or added (later elements) into the
+// slice in HBM by the thread that owns the pixels.  Frequencies follow
+// numpy's fftfreq.
 // ---------------------------------------------------------------------------
 
-constexpr int SF_TILE = 64;
-constexpr int SF_ROWS_PER_THREAD = 16;
-constexpr int SF_CHUNK = 32;
+constexpr int SF_ROWS = 64;
+constexpr int SF_COLS = 128;
+constexpr int SF_RPT = 4;
+constexpr int SF_CPT = 8;
+constexpr int SF_CHUNK = 16;
+constexpr int SF_RUN = 16;
+constexpr int SF_RUNS_PER_ATOM = (SF_ROWS + SF_COLS) / SF_RUN;  // 4 row runs + 8 column runs
 
 __device__ __forceinline__ double2 phase_factor(int g, int n, double shift) {
   const int k = (g < (n + 1) / 2) ? g : g - n;
@@ -167,21 +176,24 @@ __device__ __forceinline__ double2 phase_factor(int g, int n, double shift) {
   return make_double2(c, s);
 }
 
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
 __global__ __launch_bounds__(256) void k_slice_structure(int num_types, int height, int width, long long num_atoms,
                                                          const double* __restrict__ shifts,
                                                          const long long* __restrict__ segments,
                                                          const double2* __restrict__ kernel_k,
                                                          double2* __restrict__ out_k) {
-  __shared__ double2 u_lds[SF_CHUNK][SF_TILE];
-  __shared__ double2 v_lds[SF_CHUNK][SF_TILE];
+  __shared__ double2 u_lds[SF_CHUNK][SF_ROWS];
+  __shared__ double2 v_lds[SF_CHUNK][SF_COLS];
   const int s = blockIdx.z;
-  const int x0 = blockIdx.x * SF_TILE, y0 = blockIdx.y * SF_TILE;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int col = x0 + tx;
+  const int x0 = blockIdx.x * SF_COLS, y0 = blockIdx.y * SF_ROWS;
+  const int lane = threadIdx.x & 63;
+  const int tc = lane & 15, tr = (threadIdx.x >> 6) * 4 + (lane >> 4);
   const long long plane = (long long)height * width;
-  double2 f[SF_ROWS_PER_THREAD];
-#pragma unroll
-  for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) f[j] = make_double2(0.0, 0.0);
+  double2* out_s = out_k + (long long)s * plane;
+  bool written = false;
 
   for (int t = 0; t < num_types; ++t) {
     long long a0 = segments[(long long)s * num_types + t];
@@ -189,49 +201,84 @@ __global__ __launch_bounds__(256) void k_slice_structure(int num_types, int heig
     a0 = a0 < 0 ? 0 : (a0 > num_atoms ? num_atoms : a0);
     a1 = a1 < 0 ? 0 : (a1 > num_atoms ? num_atoms : a1);
     if (a1 <= a0) continue;
-    double2 acc[SF_ROWS_PER_THREAD];
+    double2 acc[SF_RPT][SF_CPT];
 #pragma unroll
-    for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) acc[j] = make_double2(0.0, 0.0);
+    for (int i = 0; i < SF_RPT; ++i)
+#pragma unroll
+      for (int j = 0; j < SF_CPT; ++j) acc[i][j] = make_double2(0.0, 0.0);
     for (long long c0 = a0; c0 < a1; c0 += SF_CHUNK) {
       const int n = (int)((a1 - c0) < SF_CHUNK ? (a1 - c0) : SF_CHUNK);
       __syncthreads();
-      for (int e = threadIdx.x; e < n * 2 * SF_TILE; e += blockDim.x) {
-        const int a = e / (2 * SF_TILE), q = e % (2 * SF_TILE);
+      if ((int)threadIdx.x < n * SF_RUNS_PER_ATOM) {
+        const int a = threadIdx.x / SF_RUNS_PER_ATOM, run = threadIdx.x % SF_RUNS_PER_ATOM;
         const double* sh = shifts + 2 * (c0 + a);
-        if (q < SF_TILE)
-          u_lds[a][q] = phase_factor(y0 + q, height, sh[1]);
-        else
-          v_lds[a][q - SF_TILE] = phase_factor(x0 + q - SF_TILE, width, sh[0]);
+        const bool is_row = run < SF_ROWS / SF_RUN;
+        const int dim = is_row ? height : width;
+        const double shift = is_row ? sh[1] : sh[0];
+        const int q0 = is_row ? run * SF_RUN : (run - SF_ROWS / SF_RUN) * SF_RUN;
+        const int g0 = (is_row ? y0 : x0) + q0;
+        double2* dst = is_row ? &u_lds[a][q0] : &v_lds[a][q0];
+        double ts = shift / (double)dim;
+        ts -= rint(ts);
+        double ss, cs;
+        sincospi(2.0 * ts, &ss, &cs);
+        const double2 step = make_double2(cs, ss);
+        const int wrap = (dim + 1) / 2;
+        double2 val = phase_factor(g0, dim, shift);
+        dst[0] = val;
+        for (int q = 1; q < SF_RUN; ++q) {
+          val = (g0 + q == wrap) ? phase_factor(g0 + q, dim, shift) : cmul(val, step);
+          dst[q] = val;
+        }
       }
       __syncthreads();
+#pragma unroll 2
       for (int a = 0; a < n; ++a) {
-        const double2 v = v_lds[a][tx];
+        double2 u[SF_RPT], v[SF_CPT];
 #pragma unroll
-        for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) {
-          const double2 u = u_lds[a][ty + 4 * j];
-          acc[j].x = fma(u.x, v.x, fma(-u.y, v.y, acc[j].x));
-          acc[j].y = fma(u.x, v.y, fma(u.y, v.x, acc[j].y));
-        }
+        for (int i = 0; i < SF_RPT; ++i) u[i] = u_lds[a][tr + 16 * i];
+#pragma unroll
+        for (int j = 0; j < SF_CPT; ++j) v[j] = v_lds[a][tc + 16 * j];
+#pragma unroll
+        for (int i = 0; i < SF_RPT; ++i)
+#pragma unroll
+          for (int j = 0; j < SF_CPT; ++j) {
+            acc[i][j].x = fma(u[i].x, v[j].x, fma(-u[i].y, v[j].y, acc[i][j].x));
+            acc[i][j].y = fma(u[i].x, v[j].y, fma(u[i].y, v[j].x, acc[i][j].y));
+          }
       }
     }
-    if (col < width) {
-      const double2* kt = kernel_k + (long long)t * plane;
+    const double2* kt = kernel_k + (long long)t * plane;
 #pragma unroll
-      for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) {
-        const int row = y0 + ty + 4 * j;
-        if (row < height) {
-          const double2 k = kt[(long long)row * width + col];
-          f[j].x = fma(k.x, acc[j].x, fma(-k.y, acc[j].y, f[j].x));
-          f[j].y = fma(k.x, acc[j].y, fma(k.y, acc[j].x, f[j].y));
+    for (int i = 0; i < SF_RPT; ++i) {
+      const int row = y0 + tr + 16 * i;
+      if (row >= height) continue;
+#pragma unroll
+      for (int j = 0; j < SF_CPT; ++j) {
+        const int col = x0 + tc + 16 * j;
+        if (col >= width) continue;
+        const long long e = (long long)row * width + col;
+        double2 f = cmul(kt[e], acc[i][j]);
+        if (written) {
+          const double2 o = out_s[e];
+          f.x += o.x;
+          f.y += o.y;
         }
+        out_s[e] = f;
       }
     }
+    written = true;
   }
-  if (col < width) {
+  if (!written) {
 #pragma unroll
-    for (int j = 0; j < SF_ROWS_PER_THREAD; ++j) {
-      const int row = y0 + ty + 4 * j;
-      if (row < height) out_k[(long long)s * plane + (long long)row * width + col] = f[j];
+    for (int i = 0; i < SF_RPT; ++i) {
+      const int row = y0 + tr + 16 * i;
+      if (row >= height) continue;
+#pragma unroll
+      for (int j = 0; j < SF_CPT; ++j) {
+        const int col = x0 + tc + 16 * j;
+        if (col < width) out_s[(long long)row * width + col] = make_double2(0.0, 0.0);
+      }
     }
   }
 }
@@ -267,7 +314,7 @@ hipError_t launch_slice_structure(int num_slices, int num_types, int height, int
                                   const double* shifts, const long long* segments, const double2* kernel_k,
                                   double2* out_k, hipStream_t st) {
   if (num_slices == 0 || height == 0 || width == 0) return hipSuccess;
-  const dim3 grid((unsigned)((width + SF_TILE - 1) / SF_TILE), (unsigned)((height + SF_TILE - 1) / SF_TILE),
+  const dim3 grid((unsigned)((width + SF_COLS - 1) / SF_COLS), (unsigned)((height + SF_ROWS - 1) / SF_ROWS),
                   (unsigned)num_slices);
   hipLaunchKernelGGL(k_slice_structure, grid, dim3(256), 0, st, num_types, height, width, num_atoms, shifts,
                      segments, kernel_k, out_k);

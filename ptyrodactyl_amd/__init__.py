@@ -28,6 +28,7 @@ from .simulations import (aberration, annular_detector, bind_cbed_axes, cbed_amp
                           decompose_beam_to_modes, make_probe,
                           probe_modes_to_distribution, propagation_func, shift_beam_fourier, stem_4d, stem_4d_windows,
                           transmission_func)
+from .workflows import crystal2stem4d, crystal2stem4d_tiled
 from .types import (STEM4D, CalibratedArray, CrystalData, DetectorConfig, EnsembleAxes, LossType, MicroscopeConfig, PotentialSlices,
                     ProbeModes, RuntimeCheckError, create_calibrated_array, create_detector_config,
                     create_crystal_data, create_ensemble_axes, create_microscope_config, create_potential_slices, create_probe_modes,
@@ -50,5 +51,6 @@ __all__ = [
     "KirklandParameters", "LobatoParameters", "atomic_form_factor", "create_kirkland_parameters",
     "create_lobato_parameters", "kirkland_form_factor", "kirkland_potentials", "kirkland_projected_potential",
     "load_form_factor_tables", "lobato_bandlimited_peak", "lobato_form_factor", "lobato_potentials",
-    "lobato_projected_potential", "projected_atom_potential", "set_form_factor_table",
+    "lobato_projected_potential", "projected_atom_potential", "set_form_factor_table", "crystal2stem4d",
+    "crystal2stem4d_tiled",
 ]

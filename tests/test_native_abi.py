@@ -61,3 +61,23 @@ def test_annular_and_adjoint_queries_validate_on_host():
     assert lib.pty_adjoint_workspace_bytes(128, 16, 4, 1024, 1) > lib.pty_adjoint_workspace_bytes(128, 4, 4, 1024, 1)
     assert lib.pty_adjoint_workspace_bytes(100, 16, 4, 1024, 1) == 0
     assert lib.pty_persistent_workspace_bytes(128, 2) == 0
+
+
+def test_potential_builders_validate_on_host():
+    from ptyrodactyl_amd import _native
+    if not _native.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    lib = _native.load()
+    assert lib.pty_projected_potential(5, None, 0, None, None, None) == _native.PTY_ERR_INVALID
+    assert b"parameterization" in lib.pty_last_error()
+    assert lib.pty_projected_potential(0, None, 4, None, None, None) == _native.PTY_ERR_INVALID
+    assert lib.pty_atom_potential_grid(0, 1, None, 0.1, 0, 8, 4, None, None, None) == _native.PTY_ERR_INVALID
+    assert b"height and width" in lib.pty_last_error()
+    assert lib.pty_atom_potential_grid(1, 1, None, 0.1, 8, 8, 0, None, None, None) == _native.PTY_ERR_INVALID
+    assert b"supersampling" in lib.pty_last_error()
+    assert lib.pty_atom_potential_grid(1, 1, None, -0.1, 8, 8, 2, None, None, None) == _native.PTY_ERR_INVALID
+    assert b"pixel_size" in lib.pty_last_error()
+    assert lib.pty_potential_slices_fourier(1, 1, 8, 8, -1, None, None, None, None, None) == _native.PTY_ERR_INVALID
+    assert b"num_atoms" in lib.pty_last_error()
+    assert lib.pty_potential_slices_fourier(2, 1, 8, 8, 3, None, None, None, None, None) == _native.PTY_ERR_INVALID
+    assert b"null pointer" in lib.pty_last_error()

@@ -34,6 +34,24 @@ struct RowBufWave {
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
+// The same row buffer for lines whose N / 16 threads span several wavefronts
+// (N >= 2048): every exchange needs a workgroup barrier.
+struct RowBufBlock {
+  float2* row;
+  __device__ __forceinline__ float2 ld(int e) const { return row[pad16(e)]; }
+  __device__ __forceinline__ void st(int e, float2 v) const { row[pad16(e)] = v; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+};
+
+template <bool ONE_WAVE>
+struct RowBufFor {
+  using type = RowBufWave;
+};
+template <>
+struct RowBufFor<false> {
+  using type = RowBufBlock;
+};
+
 // Deterministic FP64 sum over a workgroup of NT threads (NT a multiple of 64): a fixed xor
 // butterfly inside each wavefront, then thread 0 adds the wavefront sums in order.  `scratch`
 // holds NT / 64 doubles of LDS; every thread of the group must call it.  The total is returned

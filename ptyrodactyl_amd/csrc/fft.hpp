@@ -198,6 +198,21 @@ struct Plan<512> {
   static constexpr int NP = 3;
   static constexpr int R[3] = {16, 16, 2};
 };
+template <>
+struct Plan<1024> {
+  static constexpr int NP = 3;
+  static constexpr int R[3] = {16, 16, 4};
+};
+template <>
+struct Plan<2048> {
+  static constexpr int NP = 3;
+  static constexpr int R[3] = {16, 16, 8};
+};
+template <>
+struct Plan<4096> {
+  static constexpr int NP = 3;
+  static constexpr int R[3] = {16, 16, 16};
+};
 
 // One Stockham pass.  NS = product of the radices of earlier passes.
 // Buf provides ld(e), st(e, v) and sync() for the group's scratch; TW(m)

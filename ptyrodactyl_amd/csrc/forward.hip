@@ -189,7 +189,7 @@ static size_t resident_lds_bytes() {
 }
 
 // ---------------------------------------------------------------------------
-// Streaming row / column passes (n = 256, 512)
+// Streaming row / column passes (n = 256 ... 4096)
 // ---------------------------------------------------------------------------
 
 enum : int { ROW_FIRST = 1, ROW_LAST = 2 };
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   if (!(FLAGS & ROW_LAST))
     for (int i = tid; i < N; i += 256) px[i] = g.prop_x[i];
   const TwTable twt{tw};
-  const RowBufWave rb{rows + lr * RS};
+  const typename RowBufFor<(T <= 64)>::type rb{rows + lr * RS};
   __syncthreads();
 
   float2* wrow = waves + ((size_t)wl * N + y) * N;
@@ -393,6 +393,8 @@ template <int N>
 static hipError_t launch_streaming(const FwdParams& a, float2* ws, size_t ws_bytes, hipStream_t st) {
   constexpr int T = N / 16;
   constexpr int RPB = 256 / T, NC = 256 / T;
+  // a detector's tile partial goes to one of the workgroup's own NC columns
+  if (a.det_count > NC) return hipErrorInvalidValue;
   const size_t wave_bytes = (size_t)N * N * sizeof(float2);
   const long long max_waves = (long long)(ws_bytes / wave_bytes);
   long long pb = max_waves / a.M;
@@ -452,6 +454,12 @@ hipError_t launch_forward(const FwdParams& a, void* ws, size_t ws_bytes, hipStre
       return launch_streaming<256>(a, (float2*)ws, ws_bytes, st);
     case 512:
       return launch_streaming<512>(a, (float2*)ws, ws_bytes, st);
+    case 1024:
+      return launch_streaming<1024>(a, (float2*)ws, ws_bytes, st);
+    case 2048:
+      return launch_streaming<2048>(a, (float2*)ws, ws_bytes, st);
+    case 4096:
+      return launch_streaming<4096>(a, (float2*)ws, ws_bytes, st);
     default:
       return hipErrorInvalidValue;
   }

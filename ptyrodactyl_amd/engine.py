@@ -16,7 +16,13 @@ import torch
 
 from . import _native
 
-SUPPORTED_N = (32, 64, 128, 256, 512)
+SUPPORTED_N = (32, 64, 128, 256, 512, 1024, 2048, 4096)
+# windows of the checkpointed forward, the adjoint and the batched paths
+GRADIENT_N = (32, 64, 128, 256, 512)
+# windows of the fused virtual-detector epilogue
+DETECTOR_N = (32, 64, 128, 256, 512, 1024)
+# above 512 the default streaming batch is sized by bytes (waves of 8-128 MiB)
+LARGE_BATCH_BYTES = 256 << 20
 # Waves (positions x modes) per streaming batch for n >= 256; the batch's
 # waves live in a device workspace that should stay cache-resident.
 DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "512"))
@@ -38,7 +44,12 @@ def _check_cuda(t: torch.Tensor, name: str, dtype, ndim: int, device) -> None:
 
 def workspace_bytes(n: int, num_positions: int, num_modes: int, max_batch_waves: Optional[int] = None) -> int:
     lib = _native.load()
-    cap = DEFAULT_BATCH_WAVES if max_batch_waves is None else int(max_batch_waves)
+    if max_batch_waves is not None:
+        cap = int(max_batch_waves)
+    elif n <= 512:
+        cap = DEFAULT_BATCH_WAVES
+    else:
+        cap = max(int(num_modes), LARGE_BATCH_BYTES // (int(n) * int(n) * 8))
     return int(lib.pty_forward_workspace_bytes(int(n), int(num_positions), int(num_modes), cap))
 
 
@@ -146,7 +157,8 @@ def multislice_forward_detectors(slab_t: torch.Tensor, probe_k: torch.Tensor, we
     ``want_patterns`` the pattern stack is never written.
     """
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
-                                                                        shifts_px, prop)
+                                                                        shifts_px, prop, DETECTOR_N,
+                                                                        "virtual detectors")
     _check_cuda(masks, "masks", torch.uint8, 3, dev)
     d_count = masks.shape[0]
     _require(1 <= d_count <= MAX_DETECTORS and tuple(masks.shape[1:]) == (n, n),
@@ -220,7 +232,8 @@ def multislice_forward_persistent(slab_t: torch.Tensor, probe_k: torch.Tensor, w
     return patterns_out
 
 
-def _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop):
+def _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop, allowed=SUPPORTED_N,
+                            what: str = "the forward"):
     _require(isinstance(slab_t, torch.Tensor) and slab_t.is_cuda,
              "slab must be a GPU tensor (the multislice path has no CPU fallback)")
     dev = slab_t.device
@@ -228,7 +241,7 @@ def _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop):
     _check_cuda(probe_k, "probe_k", torch.complex64, 3, dev)
     m_count, n, n2 = probe_k.shape
     _require(n == n2, "probe window must be square")
-    _require(n in SUPPORTED_N, f"window size {n} is not one of {SUPPORTED_N}")
+    _require(n in allowed, f"window size {n} is not one of {allowed} ({what})")
     s_count, hs, ws = slab_t.shape
     _require(hs >= n and ws >= n, f"slab {hs}x{ws} is smaller than the {n}x{n} window")
     _check_cuda(weights, "weights", torch.float32, 1, dev)
@@ -254,7 +267,8 @@ def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px
     Returns (patterns (P, n, n) float32, checkpoints (P*M, (S-1)//k, n, n) complex64).
     """
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
-                                                                        shifts_px, prop)
+                                                                        shifts_px, prop, GRADIENT_N,
+                                                                        "checkpoints")
     _require(int(ckpt_every) >= 1, "ckpt_every must be >= 1")
     nc = num_checkpoints(s_count, int(ckpt_every))
     patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev)
@@ -285,7 +299,8 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
     Returns (grad_slab, grad_probe_k or None).
     """
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
-                                                                        shifts_px, prop)
+                                                                        shifts_px, prop, GRADIENT_N,
+                                                                        "the adjoint")
     k = int(ckpt_every)
     _require(k >= 1, "ckpt_every must be >= 1")
     nc = num_checkpoints(s_count, k)

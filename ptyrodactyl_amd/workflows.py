@@ -4,7 +4,7 @@ Mirrors ``src/ptyrodactyl/workflows/stem_4d.py``: ``crystal2stem4d``
 (``:191-550``) and ``crystal2stem4d_tiled`` (``:553-1007``) with the same
 arguments and defaults, and the memory estimate ``_estimate_memory_gb``
 (``:57-138``).  The multislice runs in this package's HIP kernels, whose
-windows are square powers of two from 32 to 512 pixels; a workflow whose grid
+windows are square powers of two from 32 to 4096 pixels; a workflow whose grid
 falls outside that set raises ``ValueError`` (choose the pixel size, padding
 or ``grid_pixels`` / ``fourier_pixels`` accordingly).  ``force_parallel``
 is accepted for signature compatibility: this build runs on one GPU, where the
@@ -23,6 +23,7 @@ from typing import Optional, Tuple
 
 import torch
 
+from . import engine
 from .atom_potentials import single_atom_potential
 from .parallelized import cbed_image_from_atoms, create_atomic_slice_data, stem4d_sharded
 from .physics import relativistic_wavelength_ang
@@ -30,7 +31,7 @@ from .simulations import _device_for, make_probe, probe_modes_to_distribution
 from .types import (STEM4D, CrystalData, _as_tensor, create_detector_config, create_microscope_config,
                     create_probe_modes, create_stem4d)
 
-SUPPORTED_GRIDS = (32, 64, 128, 256, 512)
+SUPPORTED_GRIDS = engine.SUPPORTED_N
 LARGE_POSITION_THRESHOLD = 100
 _POTENTIAL_EXTENT_ANG = 10.0
 

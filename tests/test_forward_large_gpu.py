@@ -1,0 +1,85 @@
+"""GPU parity of the streaming forward at 1024^2, 2048^2 and 4096^2 windows (three-pass row / column FFTs) against
+the complex128 oracle, vacuum closed forms at full size, and the size limits of the paths that stop at 512 / 1024.
+
+Tolerance: relative L2 error of the patterns <= 1e-5 (complex64 kernels against the complex128 oracle)."""
+
+import numpy as np
+import pytest
+
+from oracle import ms_oracle as o
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def _case(n, s_count, p_count, slab_hw, seed, modes):
+    rng = np.random.default_rng(seed)
+    slab = (rng.uniform(0.95, 1.0, (s_count,) + slab_hw)
+            * np.exp(1j * rng.uniform(0, 0.4, (s_count,) + slab_hw))).astype(np.complex64)
+    probes = np.stack([o.make_probe(300.0, 20.0 + 3 * m, 30.0 * m, 0.0, 0.0, (n, n), 10.0) for m in range(modes)])
+    origins = np.stack([rng.integers(0, slab_hw[0], p_count), rng.integers(0, slab_hw[1], p_count)], 1)
+    return slab, probes, origins.astype(np.int32), rng.uniform(-3.0, 3.0, (p_count, 2))
+
+
+def _hip(slab, probes, origins, shifts, n, **kw):
+    from ptyrodactyl_amd import engine, physics
+    dev = torch.device("cuda", 0)
+    m = probes.shape[0]
+    pk = torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous()
+    pat, _ = engine.multislice_forward(torch.tensor(slab, device=dev), pk,
+                                      torch.full((m,), 1.0 / m, dtype=torch.float32, device=dev),
+                                      torch.tensor(origins, device=dev), torch.tensor(shifts, device=dev),
+                                      physics.propagator_factors(n, 1.5, 300.0, 0.1, device=dev), **kw)
+    torch.cuda.synchronize()
+    return pat.cpu().numpy()
+
+
+@pytest.mark.parametrize("n, s_count, p_count, slab_hw, modes", [(1024, 4, 3, (1100, 1060), 2),
+                                                                 (2048, 3, 2, (2048, 2100), 1),
+                                                                 (4096, 2, 1, (4096, 4096), 1)])
+def test_large_windows_match_oracle(n, s_count, p_count, slab_hw, modes):
+    print(f"n={n} S={s_count} P={p_count} M={modes} slab={slab_hw}", flush=True)
+    slab, probes, origins, shifts = _case(n, s_count, p_count, slab_hw, n, modes)
+    got = _hip(slab, probes, origins, shifts, n)
+    prop = o.propagation_func(n, n, 1.5, 300.0, 0.1)
+    want = sum(o.window_stem(slab.astype(np.complex128), probes[m], origins, shifts, prop) for m in range(modes))
+    want = want / modes
+    err = rel_l2(got, want)
+    print(f"n={n}: rel-L2 {err:.2e}", flush=True)
+    assert err < TOL
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_large_window_vacuum_is_the_aperture_indicator(n):
+    """V = 0: every pattern is the fftshifted aperture indicator, independent of slices and shifts."""
+    probe = o.make_probe(300.0, 20.0, 40.0, 0.0, 0.0, (n, n), 10.0)
+    slab = np.ones((3, n, n), np.complex64)
+    got = _hip(slab, probe[None], np.zeros((2, 2), np.int32), np.asarray([[0.0, 0.0], [1.3, -2.7]]), n)
+    ind = np.fft.fftshift((np.abs(np.fft.fft2(probe)) > 0.5).astype(np.float64))
+    for p in range(2):
+        assert np.abs(got[p] - ind).max() < 1e-4
+        assert abs(got[p].sum() - ind.sum()) < 1e-4 * ind.sum()
+
+
+def test_paths_limited_to_smaller_windows_raise():
+    from ptyrodactyl_amd import engine, physics
+    dev = torch.device("cuda", 0)
+    n = 1024
+    slab = torch.ones((1, n, n), dtype=torch.complex64, device=dev)
+    pk = torch.ones((1, n, n), dtype=torch.complex64, device=dev)
+    w = torch.ones(1, dtype=torch.float32, device=dev)
+    org = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+    sh = torch.zeros((1, 2), dtype=torch.float64, device=dev)
+    prop = physics.propagator_factors(n, 1.0, 300.0, 0.1, device=dev)
+    with pytest.raises(ValueError, match="checkpoints"):
+        engine.multislice_forward_checkpointed(slab, pk, w, org, sh, prop, 1)
+    big = torch.ones((1, 2048, 2048), dtype=torch.complex64, device=dev)
+    with pytest.raises(ValueError, match="virtual detectors"):
+        engine.multislice_forward_detectors(big, big, w, org, sh, physics.propagator_factors(2048, 1.0, 300.0, 0.1,
+                                                                                             device=dev),
+                                            torch.ones((1, 2048, 2048), dtype=torch.uint8, device=dev))

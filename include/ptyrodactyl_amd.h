@@ -32,8 +32,8 @@ const char* pty_last_error(void);
 
 /*
  * Workspace (device bytes) that pty_forward needs to process `num_positions`
- * windows of n x n (n a power of two from 32 to 4096; checkpoints, adjoints and the batched entry points take
- * n <= 512, virtual detectors n <= 1024) with `num_modes` probe modes in one batch, capped at
+ * windows of n x n (n a power of two from 32 to 4096; the batched entry points take n <= 512, virtual
+ * detectors n <= 1024) with `num_modes` probe modes in one batch, capped at
  * `max_batch_waves` waves per batch (0 = no cap).  Windows n <= 128 run
  * LDS-resident and need none (returns 0).
  */

@@ -331,7 +331,7 @@ static size_t resident_adj_lds_bytes() {
 }
 
 // ---------------------------------------------------------------------------
-// Streaming adjoint (n = 256, 512): per-slice row / column passes over a
+// Streaming adjoint (n = 256 ... 4096): per-slice row / column passes over a
 // batch of waves in global scratch.
 // ---------------------------------------------------------------------------
 
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, Wave
   fill_twiddles(tw, N, tid, 256);
   for (int i = tid; i < N; i += 256) px[i] = g.prop_x[i];
   const TwTable twt{tw};
-  const RowBufWave rb{rows + lr * RS};
+  const typename RowBufFor<(T <= 64)>::type rb{rows + lr * RS};
   __syncthreads();
   const float2* srow = src.at(wl) + (size_t)y * N;
   float2 v[16];
@@ -614,8 +614,14 @@ static long long resident_slots(int n) {
   return (long long)device_cus() * per_cu;
 }
 
-// 512 waves of 256^2 (or 128 of 512^2) per streaming batch
-static long long streaming_batch(int n) { return n >= 512 ? 128 : 512; }
+// Waves per streaming batch: 512 of 256^2, 128 of 512^2, then 16 / 4 / 1 of 1024^2 / 2048^2 / 4096^2
+// (about 1.4 GiB of segment, work and contribution buffers at k = 4).
+static long long streaming_batch(int n) {
+  if (n <= 256) return 512;
+  if (n == 512) return 128;
+  const long long q = n / 1024;  // 1, 2, 4
+  return 16 / (q * q);
+}
 
 // Budget for the contribution buffer of the resident path: 4 GiB of the 288 GB of HBM.
 constexpr size_t CONTRIB_BUDGET = (size_t)4 << 30;
@@ -790,6 +796,12 @@ hipError_t launch_adjoint(const AdjParams& a, void* ws, size_t ws_bytes, hipStre
       return launch_adj_streaming<256>(a, ws, ws_bytes, st);
     case 512:
       return launch_adj_streaming<512>(a, ws, ws_bytes, st);
+    case 1024:
+      return launch_adj_streaming<1024>(a, ws, ws_bytes, st);
+    case 2048:
+      return launch_adj_streaming<2048>(a, ws, ws_bytes, st);
+    case 4096:
+      return launch_adj_streaming<4096>(a, ws, ws_bytes, st);
     default:
       return hipErrorInvalidValue;
   }

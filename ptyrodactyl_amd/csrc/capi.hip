@@ -36,8 +36,8 @@ bool valid_n(int n) {
   return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096;
 }
 
-// Windows the adjoint, the checkpointed forward and the batched paths support.
-bool gradient_n(int n) { return valid_n(n) && n <= 512; }
+// Windows the batched (descriptor) paths support.
+bool batched_n(int n) { return valid_n(n) && n <= 512; }
 
 int check_common(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
                  long long slab_h, long long slab_w, const void* probe_k, const float* weights,
@@ -154,8 +154,6 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
   int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
                         origins, shifts_px, prop_y, prop_x);
   if (rc) return rc;
-  if (!gradient_n(n))
-    return fail(PTY_ERR_INVALID, "window size n=%d: checkpoints and gradients take windows up to 512", n);
   if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
   if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
     return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
@@ -175,7 +173,7 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
 }
 
 size_t pty_adjoint_workspace_bytes(int n, int num_slices, int ckpt_every, long long num_positions, int num_modes) {
-  if (!gradient_n(n) || num_slices < 1 || ckpt_every < 1 || num_positions < 1 || num_modes < 1) return 0;
+  if (!valid_n(n) || num_slices < 1 || ckpt_every < 1 || num_positions < 1 || num_modes < 1) return 0;
   return pty::adjoint_workspace_bytes(n, num_slices, ckpt_every, num_positions * num_modes);
 }
 
@@ -188,8 +186,6 @@ int pty_adjoint(int n, int num_slices, long long num_positions, int num_modes, c
   int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
                         origins, shifts_px, prop_y, prop_x);
   if (rc) return rc;
-  if (!gradient_n(n))
-    return fail(PTY_ERR_INVALID, "window size n=%d: checkpoints and gradients take windows up to 512", n);
   if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
   if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
     return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
@@ -227,8 +223,6 @@ int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int nu
   int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
                         origins, shifts_px, prop_y, prop_x);
   if (rc) return rc;
-  if (!gradient_n(n))
-    return fail(PTY_ERR_INVALID, "window size n=%d: checkpoints and gradients take windows up to 512", n);
   if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
   if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
     return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
@@ -424,7 +418,7 @@ static_assert(sizeof(pty_position_desc) == sizeof(pty::PosDesc), "descriptor lay
 
 static int check_batched(int n, int num_slices, long long num_positions, const void* descs, const void* slab_arena,
                          const void* probe_arena, const void* prop_arena) {
-  if (!gradient_n(n))
+  if (!batched_n(n))
     return fail(PTY_ERR_INVALID, "window size n=%d: the batched paths take 32, 64, 128, 256, 512", n);
   if (num_slices < 1) return fail(PTY_ERR_INVALID, "num_slices must be >= 1 (got %d)", num_slices);
   if (num_positions < 1 || num_positions > (1LL << 30))
@@ -472,8 +466,6 @@ int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pt
   g_err[0] = '\0';
   int rc = check_batched(n, num_slices, num_positions, descs, slab_arena, probe_arena, prop_arena);
   if (rc) return rc;
-  if (!gradient_n(n))
-    return fail(PTY_ERR_INVALID, "window size n=%d: checkpoints and gradients take windows up to 512", n);
   if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
   if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
     return fail(PTY_ERR_INVALID, "checkpoint buffer is null but checkpoints are needed");

@@ -17,8 +17,8 @@ import torch
 from . import _native
 
 SUPPORTED_N = (32, 64, 128, 256, 512, 1024, 2048, 4096)
-# windows of the checkpointed forward, the adjoint and the batched paths
-GRADIENT_N = (32, 64, 128, 256, 512)
+# windows of the checkpointed forward and the adjoint
+GRADIENT_N = SUPPORTED_N
 # windows of the fused virtual-detector epilogue
 DETECTOR_N = (32, 64, 128, 256, 512, 1024)
 # above 512 the default streaming batch is sized by bytes (waves of 8-128 MiB)

@@ -19,8 +19,10 @@ TOL = 1e-5
 
 
 def rel_l2(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    """Relative L2 error; complex arrays keep their imaginary parts."""
+    a, b = np.asarray(a), np.asarray(b)
+    a = a.astype(np.result_type(a, np.float64))
+    b = b.astype(np.result_type(b, np.float64))
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 

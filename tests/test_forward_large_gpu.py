@@ -1,5 +1,6 @@
-"""GPU parity of the streaming forward at 1024^2, 2048^2 and 4096^2 windows (three-pass row / column FFTs) against
-the complex128 oracle, vacuum closed forms at full size, and the size limits of the paths that stop at 512 / 1024.
+"""GPU parity of the streaming forward and adjoint at 1024^2, 2048^2 and 4096^2 windows (three-pass row / column FFTs)
+against the complex128 oracle, vacuum closed forms at full size, and the size limits of the paths that stop at
+512 (batched) / 1024 (fused detectors).
 
 Tolerance: relative L2 error of the patterns <= 1e-5 (complex64 kernels against the complex128 oracle)."""
 
@@ -14,7 +15,9 @@ TOL = 1e-5
 
 
 def rel_l2(a, b):
-    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+    """Relative L2 error; complex arrays keep their imaginary parts."""
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a.astype(np.result_type(a, np.float64)) - b) / np.linalg.norm(b))
 
 
 def _case(n, s_count, p_count, slab_hw, seed, modes):
@@ -66,18 +69,30 @@ def test_large_window_vacuum_is_the_aperture_indicator(n):
         assert abs(got[p].sum() - ind.sum()) < 1e-4 * ind.sum()
 
 
+@pytest.mark.parametrize("n, s_count, k, p_count, slab_hw", [(1024, 3, 2, 2, (1024, 1060)), (2048, 2, 1, 1, (2048, 2048)),
+                                                              (4096, 2, 2, 1, (4096, 4096))])
+def test_large_window_adjoint_matches_oracle_vjp(n, s_count, k, p_count, slab_hw):
+    """Checkpointed forward + gathered adjoint at 1024^2 - 4096^2 against the oracle's VJP (slab and probe)."""
+    from test_adjoint_gpu import _case as adj_case, _gpu_grads, _oracle_grads
+    print(f"adjoint n={n} S={s_count} k={k} P={p_count}", flush=True)
+    slab, probes, origins, shifts, g = adj_case(n, s_count, p_count, slab_hw, seed=n + s_count)
+    _, g_slab, g_probe, w = _gpu_grads(slab, probes, origins, shifts, g, n, k)
+    ref_slab, ref_probe = _oracle_grads(slab, probes, origins, shifts, g, w, n)
+    e_slab, e_probe = rel_l2(g_slab, ref_slab), rel_l2(g_probe, ref_probe)
+    print(f"  n={n}: rel-L2 slab {e_slab:.2e} probe {e_probe:.2e}", flush=True)
+    assert e_slab < TOL and e_probe < TOL
+
+
 def test_paths_limited_to_smaller_windows_raise():
-    from ptyrodactyl_amd import engine, physics
+    from ptyrodactyl_amd import _native, engine, physics
     dev = torch.device("cuda", 0)
-    n = 1024
-    slab = torch.ones((1, n, n), dtype=torch.complex64, device=dev)
-    pk = torch.ones((1, n, n), dtype=torch.complex64, device=dev)
     w = torch.ones(1, dtype=torch.float32, device=dev)
     org = torch.zeros((1, 2), dtype=torch.int32, device=dev)
     sh = torch.zeros((1, 2), dtype=torch.float64, device=dev)
-    prop = physics.propagator_factors(n, 1.0, 300.0, 0.1, device=dev)
-    with pytest.raises(ValueError, match="checkpoints"):
-        engine.multislice_forward_checkpointed(slab, pk, w, org, sh, prop, 1)
+    lib = _native.load()
+    assert lib.pty_forward_batched(1024, 1, 1, None, None, None, None, None, None, None, 1, None, 0,
+                                   None) == _native.PTY_ERR_INVALID
+    assert b"batched paths" in lib.pty_last_error()
     big = torch.ones((1, 2048, 2048), dtype=torch.complex64, device=dev)
     with pytest.raises(ValueError, match="virtual detectors"):
         engine.multislice_forward_detectors(big, big, w, org, sh, physics.propagator_factors(2048, 1.0, 300.0, 0.1,

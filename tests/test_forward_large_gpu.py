@@ -98,3 +98,39 @@ def test_paths_limited_to_smaller_windows_raise():
         engine.multislice_forward_detectors(big, big, w, org, sh, physics.propagator_factors(2048, 1.0, 300.0, 0.1,
                                                                                              device=dev),
                                             torch.ones((1, 2048, 2048), dtype=torch.uint8, device=dev))
+
+
+def test_drop_in_stem_4d_and_its_potential_gradient_at_1024():
+    """The drop-in stem_4d on a 1024^2 grid (positions in A, as the reference applies them) against the oracle's
+    stem_4d, and its potential gradient (HIP adjoint through autograd) against central differences of the oracle
+    along a random direction."""
+    import ptyrodactyl_amd as pa
+    n, calib, dz, kv = 1024, 0.05, 1.5, 200.0
+    rng = np.random.default_rng(9)
+    pot = rng.uniform(0.0, 5.0, (n, n, 2))
+    positions = np.asarray([[10.0, 12.0], [25.3, 17.9]])
+    pot_t = torch.tensor(pot, requires_grad=True)
+    slices = pa.create_potential_slices(pot_t, dz, calib)
+    mic = pa.create_microscope_config(voltage_kv=kv, aperture_mrad=20.0, defocus_ang=30.0, probe_shape=(n, n))
+    det = pa.create_detector_config(real_space_calib_ang=calib, probe_calibration_pm=calib * 100.0,
+                                    scan_positions_px=torch.tensor(positions))
+    probe = pa.make_probe(mic, det)
+    pm = pa.create_probe_modes(probe[..., None], torch.tensor([1.0]), calib)
+    s4d = pa.stem_4d(slices, pm, mic, det)
+    modes = probe.detach().cpu().numpy()[..., None]
+
+    def oracle(v):
+        return o.stem_4d(v, dz, calib, modes, [1.0], calib, kv, positions, calib)["data"]
+
+    want = oracle(pot)
+    err = rel_l2(s4d.data.detach().cpu().numpy(), want)
+    print(f"drop-in stem_4d n=1024: rel-L2 {err:.2e}", flush=True)
+    assert err < TOL
+    g = rng.standard_normal(want.shape)
+    (s4d.data * torch.tensor(g, device=s4d.data.device, dtype=s4d.data.dtype)).sum().backward()
+    d = rng.standard_normal(pot.shape)
+    h = 1e-3
+    fd = (np.sum(g * oracle(pot + h * d)) - np.sum(g * oracle(pot - h * d))) / (2 * h)
+    got = float(np.sum(pot_t.grad.cpu().numpy() * d))
+    print(f"  directional derivative: HIP adjoint {got:.6e} oracle FD {fd:.6e}", flush=True)
+    assert abs(got - fd) < 1e-4 * abs(fd)

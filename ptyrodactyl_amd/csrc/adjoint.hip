@@ -388,7 +388,8 @@ __global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, Wave
   __shared__ float2 py[N];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
-  const int wl = blockIdx.x % nb, cblk = blockIdx.x / nb;
+  int wl, cblk;
+  column_block<NC>((int)blockIdx.x, nb, N / NC, wl, cblk);
   const long long w = w0 + wl;
   const int p = (int)(w / a.f.M);
   const int x = cblk * NC + gc;

@@ -27,6 +27,25 @@ __device__ __forceinline__ float2 shift_ramp(int kidx, int n, int m, float d) {
 }
 
 // Row buffer: a padded line in LDS, owned by a group inside one wavefront.
+// (unit, column block) of a column-pass workgroup.  Workgroups are dispatched
+// round-robin over the 8 XCDs.  With at most 2 columns per workgroup (n >= 2048)
+// the grid is remapped so that each XCD takes a contiguous run of column blocks:
+// workgroups that share the 128-byte lines of a row segment then meet in one
+// L2.  Otherwise consecutive blocks walk the waves first.  Both are bijections of
+// [0, nb * blocks_per_unit) when nb * blocks_per_unit is a multiple of 8.
+template <int NC>
+__device__ __forceinline__ void column_block(int b, int nb, int blocks_per_unit, int& unit, int& cblk) {
+  if constexpr (NC <= 2) {
+    const int total = nb * blocks_per_unit;
+    const int lin = (b % 8) * (total / 8) + b / 8;
+    unit = lin / blocks_per_unit;
+    cblk = lin % blocks_per_unit;
+  } else {
+    unit = b % nb;
+    cblk = b / nb;
+  }
+}
+
 struct RowBufWave {
   float2* row;
   __device__ __forceinline__ float2 ld(int e) const { return row[pad16(e)]; }

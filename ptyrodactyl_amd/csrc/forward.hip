@@ -269,8 +269,8 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
   __shared__ double red[4];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
-  const int unit = blockIdx.x % nb;  // wave (INIT/MID) or position (FINAL) inside the batch
-  const int cblk = blockIdx.x / nb;
+  int unit, cblk;  // wave (INIT/MID) or position (FINAL) inside the batch, column block
+  column_block<NC>((int)blockIdx.x, nb, N / NC, unit, cblk);
   const int x = cblk * NC + gc;
   const int pos = MODE >= COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M;
   const WaveCtx g = resolve_wave(a, pos);

@@ -16,6 +16,7 @@ as in the reference:
 
 from __future__ import annotations
 
+import math
 import os
 from enum import Enum
 from dataclasses import dataclass, field
@@ -41,6 +42,52 @@ def error_if(condition, message: str) -> None:
         raise RuntimeCheckError(message)
 
 
+class _ValueChecks:
+    """Tier-2 checks of one factory, in order.  Conditions are queued as callables, so nothing is computed when the
+    checks are disabled; when enabled, the device conditions are resolved with one synchronisation per device and
+    the first failing check raises with its message."""
+
+    def __init__(self):
+        self.on = runtime_checks_enabled()
+        self.items = []
+
+    def add(self, condition, message: str) -> "_ValueChecks":
+        if self.on:
+            self.items.append((condition, message))
+        return self
+
+    def finite(self, value: float, name: str) -> "_ValueChecks":
+        return self.add(lambda: not math.isfinite(float(value)), f"{name} must be finite")
+
+    def positive(self, value: float, name: str) -> "_ValueChecks":
+        self.finite(value, name)
+        return self.add(lambda: value <= 0, f"{name} must be positive")
+
+    def nonnegative(self, value: float, name: str) -> "_ValueChecks":
+        self.finite(value, name)
+        return self.add(lambda: value < 0, f"{name} must be non-negative")
+
+    def run(self) -> None:
+        if not self.on or not self.items:
+            return
+        values = [c() if callable(c) else c for c, _ in self.items]
+        flags = [None] * len(values)
+        by_device = {}
+        for i, v in enumerate(values):
+            if isinstance(v, torch.Tensor) and v.device.type != "cpu":
+                by_device.setdefault(v.device, []).append(i)
+        for idx in by_device.values():
+            got = torch.stack([values[i].reshape(()).to(torch.bool) for i in idx]).cpu().tolist()
+            for i, f in zip(idx, got):
+                flags[i] = f
+        for i, v in enumerate(values):
+            if flags[i] is None:
+                flags[i] = bool(v.item()) if isinstance(v, torch.Tensor) else bool(v)
+        for f, (_, message) in zip(flags, self.items):
+            if f:
+                raise RuntimeCheckError(message)
+
+
 def _as_tensor(x, dtype, device=None) -> torch.Tensor:
     if isinstance(x, torch.Tensor):
         return x.to(dtype=dtype, device=device if device is not None else x.device)
@@ -62,7 +109,7 @@ def _keep_grad(original, checked: float):
 
 
 def _checked_finite(value: float, name: str) -> float:
-    error_if(not torch.isfinite(torch.tensor(value)), f"{name} must be finite")
+    error_if(not math.isfinite(float(value)), f"{name} must be finite")
     return value
 
 
@@ -87,7 +134,7 @@ def _optional_shape(shape, name: str) -> Optional[Tuple[int, int]]:
     return vals
 
 
-def _optional_positions(pos, name: str) -> Optional[torch.Tensor]:
+def _optional_positions(pos, name: str, checks: "_ValueChecks") -> Optional[torch.Tensor]:
     if pos is None:
         return None
     t = _as_tensor(pos, torch.float64)
@@ -97,7 +144,7 @@ def _optional_positions(pos, name: str) -> Optional[torch.Tensor]:
         raise ValueError(f"{name} must be 2D")
     if t.shape[1] != 2:
         raise ValueError(f"{name} must have shape (P, 2)")
-    error_if(~torch.isfinite(t).all(), f"{name} contain non-finite values")
+    checks.add(lambda: ~torch.isfinite(t).all(), f"{name} contain non-finite values")
     return t
 
 
@@ -233,14 +280,13 @@ def create_detector_config(real_space_calib_ang, probe_calibration_pm=None, coll
         probe_calibration_pm, "probe_calibration_pm")
     inner = _scalar(collection_inner_mrad, "collection_inner_mrad")
     outer = _scalar(collection_outer_mrad, "collection_outer_mrad")
-    pos_px = _optional_positions(scan_positions_px, "scan_positions_px")
-    pos_ang = _optional_positions(scan_positions_ang, "scan_positions_ang")
+    checks = _ValueChecks()
+    pos_px = _optional_positions(scan_positions_px, "scan_positions_px", checks)
+    pos_ang = _optional_positions(scan_positions_ang, "scan_positions_ang", checks)
     shape = _optional_shape(scan_shape, "scan_shape")
-    _checked_positive(real_calib, "real_space_calib_ang")
-    _checked_positive(probe_calib, "probe_calibration_pm")
-    _checked_nonnegative(inner, "collection_inner_mrad")
-    _checked_nonnegative(outer, "collection_outer_mrad")
-    error_if(outer < inner, "collection_outer_mrad must be >= collection_inner_mrad")
+    checks.positive(real_calib, "real_space_calib_ang").positive(probe_calib, "probe_calibration_pm")
+    checks.nonnegative(inner, "collection_inner_mrad").nonnegative(outer, "collection_outer_mrad")
+    checks.add(outer < inner, "collection_outer_mrad must be >= collection_inner_mrad").run()
     return DetectorConfig(real_calib, probe_calib, inner, outer, pos_px, pos_ang, shape)
 
 
@@ -249,9 +295,9 @@ def create_calibrated_array(data_array, calib_y, calib_x, real_space) -> Calibra
     data = data_array if isinstance(data_array, torch.Tensor) else torch.as_tensor(data_array)
     if data.ndim != 2:
         raise ValueError("data_array must be 2D")
-    cy = _checked_positive(_scalar(calib_y, "calib_y"), "calib_y")
-    cx = _checked_positive(_scalar(calib_x, "calib_x"), "calib_x")
-    error_if(~torch.isfinite(data).all(), "data_array contain non-finite values")
+    cy, cx = _scalar(calib_y, "calib_y"), _scalar(calib_x, "calib_x")
+    checks = _ValueChecks().positive(cy, "calib_y").positive(cx, "calib_x")
+    checks.add(lambda: ~torch.isfinite(data).all(), "data_array contain non-finite values").run()
     return CalibratedArray(data, cy, cx, bool(real_space))
 
 
@@ -268,12 +314,13 @@ def create_probe_modes(modes, weights, calib) -> ProbeModes:
     if weights_t.shape != (modes_t.shape[2],):
         raise ValueError("weights must have shape (M,)")
     calib_f = _scalar(calib, "calib")
-    error_if(~torch.isfinite(torch.view_as_real(modes_t)).all(), "modes contain non-finite values")
-    error_if(~torch.isfinite(weights_t).all(), "weights contain non-finite values")
-    error_if((weights_t < 0).any(), "weights must be non-negative")
     wsum = weights_t.sum()
-    error_if(wsum <= torch.finfo(torch.float64).eps, "weights must sum to a positive value")
-    _checked_positive(calib_f, "calib")
+    checks = _ValueChecks()
+    checks.add(lambda: ~torch.isfinite(torch.view_as_real(modes_t)).all(), "modes contain non-finite values")
+    checks.add(lambda: ~torch.isfinite(weights_t).all(), "weights contain non-finite values")
+    checks.add(lambda: (weights_t < 0).any(), "weights must be non-negative")
+    checks.add(lambda: wsum <= torch.finfo(torch.float64).eps, "weights must sum to a positive value")
+    checks.positive(calib_f, "calib").run()
     return ProbeModes(modes_t, weights_t / wsum, calib_f)
 
 
@@ -285,9 +332,8 @@ def create_potential_slices(slices, slice_thickness, calib) -> PotentialSlices:
         raise ValueError("slices must be 3D")
     thick = _scalar(slice_thickness, "slice_thickness")
     calib_f = _scalar(calib, "calib")
-    error_if(~torch.isfinite(slices_t).all(), "slices contain non-finite values")
-    _checked_positive(thick, "slice_thickness")
-    _checked_positive(calib_f, "calib")
+    checks = _ValueChecks().add(lambda: ~torch.isfinite(slices_t).all(), "slices contain non-finite values")
+    checks.positive(thick, "slice_thickness").positive(calib_f, "calib").run()
     return PotentialSlices(slices_t, thick, calib_f)
 
 
@@ -349,9 +395,7 @@ def create_stem4d(data, real_space_calib, fourier_space_calib, scan_positions, v
     rc = _scalar(real_space_calib, "real_space_calib")
     fc = _scalar(fourier_space_calib, "fourier_space_calib")
     kv = _scalar(voltage_kv, "voltage_kv")
-    error_if(~torch.isfinite(data_t).all(), "data contain non-finite values")
-    error_if(~torch.isfinite(pos_t).all(), "scan_positions contain non-finite values")
-    _checked_positive(rc, "real_space_calib")
-    _checked_positive(fc, "fourier_space_calib")
-    _checked_positive(kv, "voltage_kv")
+    checks = _ValueChecks().add(lambda: ~torch.isfinite(data_t).all(), "data contain non-finite values")
+    checks.add(lambda: ~torch.isfinite(pos_t).all(), "scan_positions contain non-finite values")
+    checks.positive(rc, "real_space_calib").positive(fc, "fourier_space_calib").positive(kv, "voltage_kv").run()
     return STEM4D(data_t, rc, fc, pos_t, kv)

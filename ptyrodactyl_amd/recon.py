@@ -23,10 +23,15 @@ after the first update, then slot ``ii // save_every`` after iteration
 Precision: parameters and optimizer moments keep the promoted parameter
 dtype (complex128 for float64 inputs); the multislice itself runs in
 complex64 arithmetic on the GPU.
+
+With ``PTYRODACTYL_CAPTURE_ITERATIONS=1`` the iterations after the first
+replay a HIP graph of the forward and the gradients (MSE and MAE losses),
+with the same kernels on the same data as the eager loop.
 """
 
 from __future__ import annotations
 
+import os
 from typing import Callable, Dict, List, Tuple, Union
 
 import torch
@@ -34,7 +39,8 @@ import torch
 from .optim import OPTIMIZERS, OptaxRule, OptaxState, get_optimizer
 from .simulations import stem_4d
 from .types import (STEM4D, CalibratedArray, LossType, ProbeModes, create_calibrated_array, create_detector_config,
-                    create_microscope_config, create_potential_slices, create_probe_modes)
+                    create_microscope_config, create_potential_slices, create_probe_modes, error_if,
+                    suspended_checks)
 
 __all__ = ["OPTIMIZERS", "multi_slice_multi_modal", "single_slice_multi_modal", "single_slice_poscorrected",
            "single_slice_ptychography"]
@@ -90,6 +96,54 @@ def _device(experimental_data: STEM4D) -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def _capture_enabled() -> bool:
+    return os.environ.get("PTYRODACTYL_CAPTURE_ITERATIONS", "0").lower() in ("1", "true", "yes")
+
+
+def _loss_seed(model_output: torch.Tensor, experimental_data: torch.Tensor, loss_mode: LossType):
+    """dL/d(model output) of the MSE or MAE loss, elementwise in autograd's operation order, or None (RMSE needs
+    the reduced loss in its derivative)."""
+    r = model_output - experimental_data
+    inv_n = 1.0 / r.numel()
+    if loss_mode is LossType.MSE:
+        t = r * inv_n
+        return (t + t).to(model_output.dtype)
+    if loss_mode is LossType.MAE:
+        return torch.where(r >= 0, torch.full_like(r, inv_n), torch.full_like(r, -inv_n)).to(model_output.dtype)
+    return None
+
+
+class _CapturedGradient:
+    """The forward and the parameter gradients of one iteration recorded once in a HIP graph and replayed.
+
+    The parameters are read through views of their storage, which the eager Optax update changes in place
+    between replays. The graph is seeded with the elementwise loss derivative, so it holds no reductions. The
+    reported loss is reduced eagerly from the graph's static output. Value checks are off while recording, as
+    a graph may not synchronise."""
+
+    def __init__(self, forward, params, names, data, loss_mode):
+        self.names, self.data, self.loss_mode = names, data, loss_mode
+        with suspended_checks():
+            leaves = {k: params[k].detach().requires_grad_(True) for k in names}
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                for _ in range(2):  # warm-up on a side stream before recording; the parameters are not updated
+                    out = forward(**leaves)
+                    torch.autograd.grad(out, [leaves[k] for k in names], grad_outputs=_loss_seed(out, data, loss_mode),
+                                        allow_unused=True)
+            torch.cuda.current_stream().wait_stream(side)
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self.out = forward(**leaves)
+                self.grads = torch.autograd.grad(self.out, [leaves[k] for k in names],
+                                                 grad_outputs=_loss_seed(self.out, data, loss_mode), allow_unused=True)
+
+    def run(self):
+        self.graph.replay()
+        return self.grads, _reduce_loss(self.out, self.data, self.loss_mode).detach()
+
+
 def _optimise(forward: Callable[..., torch.Tensor], params: Dict[str, torch.Tensor],
               optimizers: Dict[str, OptaxRule], data: torch.Tensor, loss_mode: LossType, save_every: int,
               num_iterations: int) -> Tuple[Dict[str, torch.Tensor], Dict[str, torch.Tensor], List[float]]:
@@ -116,13 +170,28 @@ def _optimise(forward: Callable[..., torch.Tensor], params: Dict[str, torch.Tens
             for k in names:
                 snaps[k][..., slot] = params[k]
 
+    captured = None
     if num_iterations > 0:
         step()
         save(0)
+        if num_iterations > 1 and _capture_enabled() and loss_mode is not LossType.RMSE and \
+                next(iter(params.values())).is_cuda:
+            captured = _CapturedGradient(forward, params, names, data, loss_mode)
         for ii in range(1, num_iterations):
-            step()
+            if captured is None:
+                step()
+            else:
+                grads, loss = captured.run()
+                for k, g in zip(names, grads):
+                    _apply_optimizer_step(params[k], torch.zeros_like(params[k]) if g is None else g,
+                                          optimizers[k], states[k])
+                losses.append(loss)
             if ii % save_every == 0:
                 save(ii // save_every)
+    if captured is not None:  # the recorded iterations ran without value checks
+        for k in names:
+            error_if(~torch.isfinite(torch.view_as_real(params[k]) if params[k].is_complex() else params[k]).all(),
+                     f"{k} contains non-finite values")
     return params, snaps, [float(v) for v in losses]
 
 

@@ -16,8 +16,10 @@ as in the reference:
 
 from __future__ import annotations
 
+import contextlib
 import math
 import os
+import threading
 from enum import Enum
 from dataclasses import dataclass, field
 from typing import Optional, Tuple
@@ -29,8 +31,23 @@ class RuntimeCheckError(RuntimeError):
     """A deferred value check failed (the reference raises EquinoxRuntimeError)."""
 
 
+_CHECK_STATE = threading.local()
+
+
 def runtime_checks_enabled() -> bool:
+    if getattr(_CHECK_STATE, "suspended", 0):
+        return False
     return os.environ.get("PTYRODACTYL_DISABLE_RUNTIME_CHECKS", "0").lower() not in ("1", "true", "yes")
+
+
+@contextlib.contextmanager
+def suspended_checks():
+    """Tier-2 value checks off in this thread (for code recorded into a graph, which may not synchronise)."""
+    _CHECK_STATE.suspended = getattr(_CHECK_STATE, "suspended", 0) + 1
+    try:
+        yield
+    finally:
+        _CHECK_STATE.suspended -= 1
 
 
 def error_if(condition, message: str) -> None:

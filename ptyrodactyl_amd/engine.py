@@ -21,8 +21,9 @@ SUPPORTED_N = (32, 64, 128, 256, 512, 1024, 2048, 4096)
 GRADIENT_N = SUPPORTED_N
 # windows of the fused virtual-detector epilogue
 DETECTOR_N = (32, 64, 128, 256, 512, 1024)
-# above 512 the default streaming batch is sized by bytes (waves of 8-128 MiB)
-LARGE_BATCH_BYTES = 256 << 20
+# from 512 up the default streaming batch is sized by bytes: 192 MiB of waves measured fastest at 512^2
+# (96 waves: 64.4 ms vs 71.8 ms with 512) and 1024^2 (24 waves: 17.2 ms vs 19.6 ms with 32)
+LARGE_BATCH_BYTES = 192 << 20
 # Waves (positions x modes) per streaming batch for n >= 256; the batch's
 # waves live in a device workspace that should stay cache-resident.
 DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "512"))
@@ -46,7 +47,7 @@ def workspace_bytes(n: int, num_positions: int, num_modes: int, max_batch_waves:
     lib = _native.load()
     if max_batch_waves is not None:
         cap = int(max_batch_waves)
-    elif n <= 512:
+    elif n <= 256:
         cap = DEFAULT_BATCH_WAVES
     else:
         cap = max(int(num_modes), LARGE_BATCH_BYTES // (int(n) * int(n) * 8))

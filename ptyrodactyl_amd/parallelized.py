@@ -128,6 +128,10 @@ def stem4d_sharded(probe_modes: ProbeModes, sample: AtomicSliceData, microscope:
     dev = _device_for(probe_modes.modes, sample.atom_coords)
     n = _square_n(probe_modes.modes.shape[0], probe_modes.modes.shape[1])
     calib = float(detector.real_space_calib_ang)
+    if detector.scan_positions_ang.shape[0] == 0:  # empty scan: empty stack
+        pos0 = detector.scan_positions_ang.to(device=dev, dtype=torch.float64)
+        return create_stem4d(torch.empty((0, n, n), dtype=torch.float32, device=dev), calib, 1.0 / (n * calib), pos0,
+                             microscope.voltage_kv)
     pot = slice_potentials(sample, (n, n), calib, dev)
     slab = _transmission_slab(pot, microscope.voltage_kv)
     dz = float(sample.slice_z_bounds[0, 1] - sample.slice_z_bounds[0, 0])

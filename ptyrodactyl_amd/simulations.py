@@ -228,6 +228,11 @@ def stem_4d(pot_slice: PotentialSlices, beam: ProbeModes, microscope: Microscope
         raise ValueError("detector.scan_positions_px is required")
     dev = _device_for(pot_slice.slices, beam.modes, detector.scan_positions_px)
     positions = detector.scan_positions_px.to(device=dev, dtype=torch.float64)
+    if positions.shape[0] == 0:  # an empty scan maps to an empty stack, as vmap over zero positions does
+        n = _square_n(beam.modes.shape[0], beam.modes.shape[1])
+        calib = min(float(pot_slice.calib), float(beam.calib))
+        return create_stem4d(torch.empty((0, n, n), dtype=torch.float32, device=dev), detector.real_space_calib_ang,
+                             1.0 / (n * calib), positions * detector.real_space_calib_ang, microscope.voltage_kv)
     if not microscope.ensemble.is_empty():
         n = beam.modes.shape[0]
         calib = min(float(pot_slice.calib), float(beam.calib))

@@ -309,3 +309,16 @@ def test_stem4d_sharded_mode_weight_and_kernel_gradients():
         km[idx] -= h
         fd = (oracle_loss(kp) - oracle_loss(km)) / (2 * h)
         assert abs(float(gk[idx]) - fd) <= 1e-2 * abs(fd) + 1e-3 * float(gk.abs().max())
+
+
+def test_empty_scan_gives_an_empty_stack():
+    """vmap over zero positions returns an empty stack in the reference; no kernel is launched here."""
+    import ptyrodactyl_amd as pa
+    mic = pa.create_microscope_config(voltage_kv=100.0, aperture_mrad=20.0, probe_shape=(32, 32))
+    det = pa.create_detector_config(real_space_calib_ang=0.2, probe_calibration_pm=20.0,
+                                    scan_positions_px=torch.zeros((0, 2), dtype=torch.float64))
+    pm = pa.create_probe_modes(pa.make_probe(mic, det)[..., None], torch.tensor([1.0]), 0.2)
+    slices = pa.create_potential_slices(torch.zeros((32, 32, 2), dtype=torch.float64), 1.0, 0.2)
+    s4d = pa.stem_4d(slices, pm, mic, det)
+    assert tuple(s4d.data.shape) == (0, 32, 32) and tuple(s4d.scan_positions.shape) == (0, 2)
+    assert s4d.fourier_space_calib == pytest.approx(1 / 6.4)

@@ -56,15 +56,20 @@ def cpu_baseline(w, sample: int, cores: int):
     sel = np.linspace(0, len(w.origins) - 1, sample).astype(int)
     slab = w.slab_t  # complex64 input, arithmetic promoted to complex128 inside
     o.window_stem_batched(slab, w.probe, w.origins[sel[:2]], w.shifts_px[sel[:2]], prop, workers=cores)
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        o.window_stem_batched(slab, w.probe, w.origins[sel], w.shifts_px[sel], prop, workers=cores)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": round(sample / t, 3), "unit": "patterns/s", "cores": cores, "kind": "port",
+    def timed(dtype):
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            o.window_stem_batched(slab, w.probe, w.origins[sel], w.shifts_px[sel], prop, workers=cores, dtype=dtype)
+            times.append(time.perf_counter() - t0)
+        return sample / float(np.median(times))
+
+    c128 = timed(np.complex128)
+    c64 = timed(np.complex64)
+    return {"value": round(c128, 3), "unit": "patterns/s", "cores": cores, "kind": "port",
             "sample": f"{sample} of {len(w.origins)} positions (every {len(w.origins) // sample}th), complex128, "
-                      f"median of 3 after 1 warm-up; reference-algorithm CPU restatement (numpy/pocketfft)"}
+                      f"median of 3 after 1 warm-up; reference-algorithm CPU restatement (numpy/pocketfft)",
+            "complex64_value": round(c64, 3)}
 
 
 def detectors_bench(w, slab, pk, weights, origins, shifts, prop, steps: int):
@@ -291,7 +296,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-waves", type=int, default=None)
     ap.add_argument("--path", choices=("streaming", "persistent"), default="streaming",

@@ -489,19 +489,26 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
     out = torch.empty((p_count, n, n), dtype=torch.float64, device=dev)
     base = engine.arena_base(dev)
     sel_idx = torch.as_tensor(mode_idx, device=dev)
+    # The position shift (shift_beam_fourier at det_calib) and the jitter shift are both Fourier phase ramps, so
+    # every incident wave is formed in Fourier space as fft2(mode) * ramp_p * jitter_k; only a tilt (a real-space
+    # ramp) needs the inverse and forward transforms.
+    modes_k = torch.fft.fft2(modes, dim=(0, 1))  # (n, n, M)
+    qd = physics.fftfreq(n, float(det_calib), device=dev)
     for p0 in range(0, p_count, chunk):
         pc = min(chunk, p_count - p0)
         if positions is None:
-            shifted = modes[None]
+            kp = modes_k[None]
         else:
-            shifted = shift_beam_fourier(modes, positions[p0:p0 + pc].detach(), det_calib)
-        waves = shifted.index_select(3, sel_idx).permute(0, 3, 1, 2)  # (pc, K, n, n)
+            pp = positions[p0:p0 + pc].detach().to(device=dev, dtype=torch.float64)
+            phase = -2.0 * math.pi * (qd[None, :, None] * pp[:, 0, None, None] + qd[None, None, :] * pp[:, 1, None, None])
+            kp = modes_k[None] * torch.polar(torch.ones_like(phase), phase)[..., None]
+        waves_k = kp.index_select(3, sel_idx).permute(0, 3, 1, 2)  # (pc, K, n, n), Fourier space
         if jitter is not None:
-            waves = torch.fft.ifft2(torch.fft.fft2(waves) * jitter[None])
+            waves_k = waves_k * jitter[None]
         if tilts is not None:
-            waves = waves * tilts[None]
-        probe_k = torch.fft.fft2(waves).to(torch.complex64).reshape(pc * k_count, n, n).contiguous()
-        del waves
+            waves_k = torch.fft.fft2(torch.fft.ifft2(waves_k) * tilts[None])
+        probe_k = waves_k.to(torch.complex64).reshape(pc * k_count, n, n).contiguous()
+        del waves_k, kp
         patterns = torch.empty((pc * k_count, n, n), dtype=torch.float32, device=dev)
         amps = torch.empty((pc * k_count, n, n), dtype=torch.complex64, device=dev) if coherent else None
         rec = np.zeros(pc * k_count, dtype=_DESC_DTYPE)

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 3
+#define PTY_ABI_VERSION 4
 
 #define PTY_OK 0
 #define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
@@ -312,6 +312,48 @@ int pty_forward_persistent(int n, int num_slices, long long num_positions, const
                            const double* shifts_px, const void* prop_y, const void* prop_x, float* patterns,
                            int group_waves, int grid, void* workspace, size_t workspace_bytes, int* state,
                            void* stream);
+
+/*
+ * Grids of any size: h x w probe windows with 1 <= h, w <= 4096, square or not and of any
+ * factorisation (odd, prime, 15 x 32, 100 x 100, ...).  The reference's multislice takes any
+ * grid shape (src/ptyrodactyl/multislice/simulations.py:501-603 with jnp.fft.fft2; its tests
+ * use 8 x 8, 15 x 32 and 31 x 32 grids); the power-of-two entry points above cover only the
+ * square power-of-two windows.  One workgroup per position runs a mixed-radix transform in
+ * LDS; gradients accumulate with float32 atomics (not bitwise reproducible across runs).
+ *
+ * Device scratch for pty_forward_grid (adjoint = 0) or pty_adjoint_grid (adjoint = 1) with the
+ * default number of workgroups; any workspace of at least the bytes of one workgroup works
+ * (h * w * 8 for the forward, h * w * 8 * (min(ckpt_every, num_slices) + 2) for the adjoint),
+ * with fewer positions in flight.
+ */
+size_t pty_grid_workspace_bytes(int h, int w, int num_slices, int ckpt_every, long long num_positions, int adjoint);
+
+/*
+ * pty_forward / pty_forward_checkpointed on an h x w grid (same reference functions,
+ * simulations.py:501-603, 1130-1443).  Arrays as for pty_forward with n x n replaced by h x w;
+ * prop_y has h entries (exp(-i pi lambda dz qy^2) / h), prop_x has w (with / w).  patterns
+ * and amplitude may each be NULL (not both).  checkpoints (may be NULL): complex64
+ * [num_positions * num_modes][(S-1)/ckpt_every][h][w], the wave entering slices k, 2k, ...
+ */
+int pty_forward_grid(int h, int w, int num_slices, long long num_positions, int num_modes, const void* slab,
+                     long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                     const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                     float* patterns, void* amplitude, void* checkpoints, int ckpt_every, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/*
+ * pty_adjoint / pty_adjoint_amplitude on an h x w grid (the VJP of the same functions,
+ * src/ptyrodactyl/multislice/multislice_recon.py:406-434, 849-885, 1400-1436, 1915-1951).
+ * Exactly one of grad_patterns (float32 [P][h][w]) and grad_amplitude (complex64 [P][M][h][w]),
+ * both fftshifted, seeds it.  ACCUMULATES into grad_slab, grad_probe_k (may be NULL) and
+ * grad_shifts (float64 [P][2], may be NULL; summed per position in FP64 in a fixed order).
+ */
+int pty_adjoint_grid(int h, int w, int num_slices, long long num_positions, int num_modes, const void* slab,
+                     long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                     const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                     const float* grad_patterns, const void* grad_amplitude, const void* checkpoints, int ckpt_every,
+                     void* grad_slab, void* grad_probe_k, double* grad_shifts, void* workspace,
+                     size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -301,11 +301,12 @@ def shifted_probe(probe, shift_px):
     return np.fft.ifft2(np.fft.fft2(probe) * ramp)
 
 
-def window_of(slab, origin, n: int):
-    """(S, n, n) periodic window of ``slab`` (S, Hs, Ws) at (oy, ox)."""
+def window_of(slab, origin, n):
+    """(S, h, w) periodic window of ``slab`` (S, Hs, Ws) at (oy, ox); ``n`` is h = w or (h, w)."""
+    h, w = (n, n) if np.ndim(n) == 0 else n
     hs, ws = slab.shape[-2], slab.shape[-1]
-    rows = (int(origin[0]) + np.arange(n)) % hs
-    cols = (int(origin[1]) + np.arange(n)) % ws
+    rows = (int(origin[0]) + np.arange(h)) % hs
+    cols = (int(origin[1]) + np.arange(w)) % ws
     return slab[..., rows[:, None], cols[None, :]]
 
 
@@ -319,20 +320,20 @@ def window_stem(slab_t, probe, origins, shifts_px, prop, workers: int = 1,
     ``slab_t`` at integer ``origins[p]`` with the probe shifted by
     ``shifts_px[p]`` pixels through ``shift_beam_fourier``.
 
-    slab_t: (S, Hs, Ws) complex transmission; probe: (N, N) complex;
-    origins: (P, 2) int; shifts_px: (P, 2) float; prop: (N, N) complex.
-    Returns (P, N, N) float64 patterns (and amplitudes if requested).
+    slab_t: (S, Hs, Ws) complex transmission; probe: (h, w) complex;
+    origins: (P, 2) int; shifts_px: (P, 2) float; prop: (h, w) complex.
+    Returns (P, h, w) float64 patterns (and amplitudes if requested).
     """
-    n = probe.shape[-1]
+    h, w = probe.shape[-2], probe.shape[-1]
     origins = np.asarray(origins)
     shifts_px = np.asarray(shifts_px, np.float64)
-    out = np.empty((origins.shape[0], n, n), np.float64)
+    out = np.empty((origins.shape[0], h, w), np.float64)
     amps = [] if return_amplitude else None
     probe_k = _fft2(np.asarray(probe, np.complex128))
     for p in range(origins.shape[0]):
-        ramp = ramp_1d(n, shifts_px[p, 0])[:, None] * ramp_1d(n, shifts_px[p, 1])[None, :]
+        ramp = ramp_1d(h, shifts_px[p, 0])[:, None] * ramp_1d(w, shifts_px[p, 1])[None, :]
         wave0 = _ifft2(probe_k * ramp, workers=workers)
-        t_win = window_of(slab_t, origins[p], n)
+        t_win = window_of(slab_t, origins[p], (h, w))
         amp = multislice_amplitude(t_win, wave0, prop, workers=workers)
         out[p] = np.abs(amp) ** 2
         if return_amplitude:
@@ -380,22 +381,23 @@ def multislice_vjp(slab_t, probe, origins, shifts_px, prop, grad_intensity, want
     positions in its position-correcting reconstructors
     (``multislice_recon.py:849-885``).
 
-    grad_intensity: (P, N, N) real dL/dI.  Returns (grad_slab, grad_probe),
+    grad_intensity: (P, h, w) real dL/dI.  Returns (grad_slab, grad_probe),
     plus grad_shifts (P, 2) = dL/d(shift_y, shift_x) in pixels when
     ``want_shifts``.
     """
-    n = probe.shape[-1]
+    h, w = probe.shape[-2], probe.shape[-1]
     s_count = slab_t.shape[0]
     hs, ws = slab_t.shape[-2], slab_t.shape[-1]
     grad_slab = np.zeros(slab_t.shape, np.complex128)
     probe_k = np.fft.fft2(np.asarray(probe, np.complex128))
-    grad_probe_k = np.zeros((n, n), np.complex128)
+    grad_probe_k = np.zeros((h, w), np.complex128)
     grad_shifts = np.zeros((len(origins), 2))
-    kk = np.fft.fftfreq(n) * n  # Nyquist negative, as in ramp_1d
+    ky = np.fft.fftfreq(h) * h  # Nyquist negative, as in ramp_1d
+    kx = np.fft.fftfreq(w) * w
     for p in range(len(origins)):
-        ramp = ramp_1d(n, shifts_px[p][0])[:, None] * ramp_1d(n, shifts_px[p][1])[None, :]
+        ramp = ramp_1d(h, shifts_px[p][0])[:, None] * ramp_1d(w, shifts_px[p][1])[None, :]
         psi = [np.fft.ifft2(probe_k * ramp)]
-        t_win = window_of(slab_t, origins[p], n)
+        t_win = window_of(slab_t, origins[p], (h, w))
         for s in range(s_count):
             phi = psi[s] * t_win[s]
             if s != s_count - 1:
@@ -403,22 +405,22 @@ def multislice_vjp(slab_t, probe, origins, shifts_px, prop, grad_intensity, want
         amp = np.fft.fftshift(np.fft.fft2(phi))
         g_amp = 2.0 * grad_intensity[p] * amp
         # adjoint of fftshift(fft2): F^H = n^2 ifft2, applied after ifftshift
-        g_phi = np.fft.ifft2(np.fft.ifftshift(g_amp)) * (n * n)
-        rows = (int(origins[p][0]) + np.arange(n)) % hs
-        cols = (int(origins[p][1]) + np.arange(n)) % ws
+        g_phi = np.fft.ifft2(np.fft.ifftshift(g_amp)) * (h * w)
+        rows = (int(origins[p][0]) + np.arange(h)) % hs
+        cols = (int(origins[p][1]) + np.arange(w)) % ws
         for s in range(s_count - 1, -1, -1):
             if s != s_count - 1:
                 g_phi = np.fft.ifft2(np.fft.fft2(g_psi) * np.conj(prop))
             np.add.at(grad_slab[s], (rows[:, None], cols[None, :]), np.conj(psi[s]) * g_phi)
             g_psi = np.conj(t_win[s]) * g_phi
         # psi0 = ifft2(u), u = probe_k * ramp, probe_k = fft2(probe)
-        g_u = np.fft.fft2(g_psi) / (n * n)
+        g_u = np.fft.fft2(g_psi) / (h * w)
         grad_probe_k += np.conj(ramp) * g_u
-        # d ramp / d shift_y = ramp * (-2 pi i k_y / n); dL/ds = Re sum conj(g_u) du/ds
-        du = probe_k * ramp * (-2j * np.pi / n)
-        grad_shifts[p, 0] = np.real(np.sum(np.conj(g_u) * du * kk[:, None]))
-        grad_shifts[p, 1] = np.real(np.sum(np.conj(g_u) * du * kk[None, :]))
-    grad_probe = np.fft.ifft2(grad_probe_k) * (n * n)
+        # d ramp / d shift_y = ramp * (-2 pi i k_y / h); dL/ds = Re sum conj(g_u) du/ds
+        du = probe_k * ramp * (-2j * np.pi)
+        grad_shifts[p, 0] = np.real(np.sum(np.conj(g_u) * du * (ky / h)[:, None]))
+        grad_shifts[p, 1] = np.real(np.sum(np.conj(g_u) * du * (kx / w)[None, :]))
+    grad_probe = np.fft.ifft2(grad_probe_k) * (h * w)
     if want_shifts:
         return grad_slab, grad_probe, grad_shifts
     return grad_slab, grad_probe

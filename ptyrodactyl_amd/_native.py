@@ -19,7 +19,7 @@ from .build import LIB_PATH
 PTY_OK = 0
 PTY_ERR_INVALID = 1
 PTY_ERR_DEVICE = 2
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _lock = threading.Lock()
 _lib = None
@@ -64,6 +64,11 @@ SIGNATURES = {
     "pty_persistent_workspace_bytes": (c_size, [c_int, c_int]),
     "pty_forward_persistent": (c_int, [c_int, c_int, c_ll, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                        c_vp, c_int, c_int, c_vp, c_size, c_vp, c_vp]),
+    "pty_grid_workspace_bytes": (c_size, [c_int, c_int, c_int, c_int, c_ll, c_int]),
+    "pty_forward_grid": (c_int, [c_int, c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size, c_vp]),
+    "pty_adjoint_grid": (c_int, [c_int, c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
 }
 
 # numpy view of pty_position_desc (include/ptyrodactyl_amd.h); 96 bytes

@@ -20,6 +20,11 @@ from . import engine
 DEFAULT_CKPT_EVERY = 4
 
 
+def _pair(prop):
+    """(prop_y, prop_x): one factor serves both axes of a square grid."""
+    return tuple(prop) if isinstance(prop, (tuple, list)) else (prop, prop)
+
+
 class MultisliceFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, slab_t, probe_k, weights, origins, shifts_px, prop, ckpt_every):
@@ -32,14 +37,15 @@ class MultisliceFunction(torch.autograd.Function):
         else:
             patterns, ckpt = engine.multislice_forward_checkpointed(slab_c, probe_c, w_c, origins, shifts_px, prop,
                                                                     ckpt_every)
-        ctx.save_for_backward(slab_c, probe_c, w_c, origins, shifts_px.detach(), prop, ckpt)
+        ctx.save_for_backward(slab_c, probe_c, w_c, origins, shifts_px.detach(), *_pair(prop), ckpt)
         ctx.ckpt_every = ckpt_every
         ctx.weights_dtype = weights.dtype
         return patterns
 
     @staticmethod
     def backward(ctx, grad_patterns):
-        slab_t, probe_k, weights, origins, shifts_px, prop, ckpt = ctx.saved_tensors
+        slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x, ckpt = ctx.saved_tensors
+        prop = (prop_y, prop_x)
         g = grad_patterns.to(torch.float32).contiguous()
         g_slab = g_probe = g_w = g_shift = None
         if ctx.needs_input_grad[4]:
@@ -70,13 +76,14 @@ class MultisliceAmplitudeFunction(torch.autograd.Function):
         if ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[3]:
             _, ckpt = engine.multislice_forward_checkpointed(slab_c, probe_c, ones, origins, shifts_px, prop,
                                                              ckpt_every)
-        ctx.save_for_backward(slab_c, probe_c, ones, origins, shifts_px.detach(), prop, ckpt)
+        ctx.save_for_backward(slab_c, probe_c, ones, origins, shifts_px.detach(), *_pair(prop), ckpt)
         ctx.ckpt_every = ckpt_every
         return amp
 
     @staticmethod
     def backward(ctx, grad_amp):
-        slab_t, probe_k, ones, origins, shifts_px, prop, ckpt = ctx.saved_tensors
+        slab_t, probe_k, ones, origins, shifts_px, prop_y, prop_x, ckpt = ctx.saved_tensors
+        prop = (prop_y, prop_x)
         g_shift = None
         if ctx.needs_input_grad[3]:
             g_shift = torch.zeros(shifts_px.shape, dtype=torch.float64, device=shifts_px.device)

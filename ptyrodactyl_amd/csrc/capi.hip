@@ -11,6 +11,7 @@
 #include "adjoint.hpp"
 #include "aux_kernels.hpp"
 #include "forward.hpp"
+#include "grid.hpp"
 #include "optim.hpp"
 #include "potential.hpp"
 #include "xl2.hpp"
@@ -84,6 +85,48 @@ pty::FwdParams make_params(int n, int num_slices, long long num_positions, int n
   a.det_count = 0;
   return a;
 }
+
+int check_grid(int h, int w, int num_slices, long long num_positions, int num_modes, const void* slab,
+               long long slab_h, long long slab_w, const void* probe_k, const float* weights, const int32_t* origins,
+               const double* shifts, const void* prop_y, const void* prop_x) {
+  if (h < 1 || w < 1 || h > pty::GRID_MAX_EDGE || w > pty::GRID_MAX_EDGE)
+    return fail(PTY_ERR_INVALID, "grid %dx%d: each edge must be in [1, %d]", h, w, pty::GRID_MAX_EDGE);
+  if (num_slices < 1) return fail(PTY_ERR_INVALID, "num_slices must be >= 1 (got %d)", num_slices);
+  if (num_positions < 1 || num_positions > (1LL << 30))
+    return fail(PTY_ERR_INVALID, "num_positions must be in [1, 2^30] (got %lld)", num_positions);
+  if (num_modes < 1 || num_modes > 4096) return fail(PTY_ERR_INVALID, "num_modes must be in [1, 4096]");
+  if (slab_h < h || slab_w < w || slab_h > (1LL << 30) || slab_w > (1LL << 30))
+    return fail(PTY_ERR_INVALID, "slab extent %lldx%lld must be at least the window %dx%d", slab_h, slab_w, h, w);
+  if (!slab || !probe_k || !weights || !origins || !shifts || !prop_y || !prop_x)
+    return fail(PTY_ERR_INVALID, "null input pointer");
+  return PTY_OK;
+}
+
+pty::GridParams make_grid_params(int h, int w, int num_slices, long long num_positions, int num_modes,
+                                 const void* slab, long long slab_h, long long slab_w, const void* probe_k,
+                                 const float* weights, const int32_t* origins, const double* shifts,
+                                 const void* prop_y, const void* prop_x) {
+  pty::GridParams a;
+  memset(&a, 0, sizeof(a));
+  a.h = h;
+  a.w = w;
+  a.S = num_slices;
+  a.M = num_modes;
+  a.P = num_positions;
+  a.slab = (const float2*)slab;
+  a.slab_h = slab_h;
+  a.slab_w = slab_w;
+  a.probe_k = (const float2*)probe_k;
+  a.weights = weights;
+  a.origins = origins;
+  a.shifts = shifts;
+  a.prop_y = (const float2*)prop_y;
+  a.prop_x = (const float2*)prop_x;
+  a.ckpt_every = 1;
+  return a;
+}
+
+int grid_ckpt(int ckpt_every, int num_slices) { return ckpt_every < num_slices ? ckpt_every : num_slices; }
 
 }  // namespace
 
@@ -483,6 +526,72 @@ int pty_adjoint_batched(int n, int num_slices, long long num_positions, const pt
   a.grad_probe_k = (float2*)grad_probe_arena;
   return device_status(pty::launch_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream),
                        "pty_adjoint_batched");
+}
+
+size_t pty_grid_workspace_bytes(int h, int w, int num_slices, int ckpt_every, long long num_positions, int adjoint) {
+  if (h < 1 || w < 1 || h > pty::GRID_MAX_EDGE || w > pty::GRID_MAX_EDGE || num_slices < 1 || ckpt_every < 1 ||
+      num_positions < 1)
+    return 0;
+  return pty::grid_workspace_bytes(h, w, grid_ckpt(ckpt_every, num_slices), num_positions, adjoint != 0);
+}
+
+int pty_forward_grid(int h, int w, int num_slices, long long num_positions, int num_modes, const void* slab,
+                     long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                     const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                     float* patterns, void* amplitude, void* checkpoints, int ckpt_every, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_grid(h, w, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                      origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (!patterns && !amplitude) return fail(PTY_ERR_INVALID, "nothing to compute: patterns and amplitude are null");
+  if (checkpoints && ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
+  const size_t need = pty::grid_block_bytes(h, w, 1, false);
+  if (!workspace || workspace_bytes < need)
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
+                need);
+  pty::GridParams a = make_grid_params(h, w, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k,
+                                       mode_weights, origins, shifts_px, prop_y, prop_x);
+  a.patterns = patterns;
+  a.amplitude = (float2*)amplitude;
+  a.ckpt = (float2*)checkpoints;
+  a.ckpt_every = checkpoints ? ckpt_every : 1;
+  return device_status(pty::launch_grid_forward(a, workspace, workspace_bytes, (hipStream_t)stream),
+                       "pty_forward_grid");
+}
+
+int pty_adjoint_grid(int h, int w, int num_slices, long long num_positions, int num_modes, const void* slab,
+                     long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                     const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                     const float* grad_patterns, const void* grad_amplitude, const void* checkpoints, int ckpt_every,
+                     void* grad_slab, void* grad_probe_k, double* grad_shifts, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_grid(h, w, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                      origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (ckpt_every < 1) return fail(PTY_ERR_INVALID, "ckpt_every must be >= 1 (got %d)", ckpt_every);
+  if ((grad_patterns == nullptr) == (grad_amplitude == nullptr))
+    return fail(PTY_ERR_INVALID, "exactly one of grad_patterns and grad_amplitude must be given");
+  if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
+    return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
+                pty::num_checkpoints(num_slices, ckpt_every));
+  if (!grad_slab) return fail(PTY_ERR_INVALID, "grad_slab must be non-null");
+  const size_t need = pty::grid_block_bytes(h, w, grid_ckpt(ckpt_every, num_slices), true);
+  if (!workspace || workspace_bytes < need)
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
+                need);
+  pty::GridParams a = make_grid_params(h, w, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k,
+                                       mode_weights, origins, shifts_px, prop_y, prop_x);
+  a.grad_patterns = grad_patterns;
+  a.grad_amplitude = (const float2*)grad_amplitude;
+  a.ckpt = (float2*)checkpoints;
+  a.ckpt_every = ckpt_every;
+  a.grad_slab = (float2*)grad_slab;
+  a.grad_probe_k = (float2*)grad_probe_k;
+  a.grad_shifts = grad_shifts;
+  return device_status(pty::launch_grid_adjoint(a, workspace, workspace_bytes, (hipStream_t)stream),
+                       "pty_adjoint_grid");
 }
 
 }  // extern "C"

@@ -1,0 +1,517 @@
+// Multislice forward and adjoint on grids of any size h x w (1 to 4096 per
+// axis): the shapes the power-of-two kernels do not take, such as odd,
+// prime or non-square probe windows.
+//
+// One workgroup owns one scan position at a time (a grid-stride loop over
+// positions; workgroups never communicate).  The wave lives in the
+// workgroup's slice of the device workspace.  Each pass moves a group of rows
+// or columns through LDS, runs a mixed-radix Stockham transform on them there
+// and writes them back, with the pointwise products of the multislice (shift
+// ramp, transmission, propagator, |A|^2, and the adjoint's conjugate
+// products) applied in LDS between the transforms.  Per slice the wave makes
+// one column pass (inverse y transform, times T_s, forward y transform) and
+// one row pass (forward x transform, times the propagator, inverse x
+// transform), the same factorisation as the power-of-two streaming kernels.
+//
+// Stage of radix r after the radices so far multiply to ns (Stockham
+// autosort, one output per thread):
+//   out[e] = sum_q in[j + q L / r] * w^(q (lo + k ns)),  w = exp(-+2 pi i / (ns r)),
+//   with lo = e mod ns, k = (e / ns) mod r, j = (e / (ns r)) ns + lo.
+// Twiddles come from an exact table exp(-2 pi i e / L) built in FP64, so no
+// error accumulates across stages beyond the FP32 sums.
+#include <math.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "grid.hpp"
+
+namespace pty {
+namespace {
+
+constexpr int GT = 256;             // threads per workgroup
+constexpr int GROUP_ELEMS = 2048;   // elements per LDS line group
+constexpr int MAX_BLOCKS = 2048;    // workgroups per launch
+constexpr size_t WS_BUDGET = size_t(2) << 30;
+
+struct Axis {
+  int L;            // transform length
+  int ls;           // line stride in LDS (odd for even L, to spread banks)
+  int nst;          // stages
+  const int* radix; // LDS
+  const float2* tw; // LDS, exp(-2 pi i e / L)
+};
+
+__device__ __forceinline__ float2 gmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ float2 gmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float2 gscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// fftfreq bin of index kidx (Nyquist negative for even L, as numpy)
+__device__ __forceinline__ int signed_bin(int kidx, int L) { return kidx < (L + 1) / 2 ? kidx : kidx - L; }
+
+// Index of bin kidx in the fftshifted layout.
+__device__ __forceinline__ int shifted_index(int kidx, int L) {
+  const int i = kidx + L / 2;
+  return i >= L ? i - L : i;
+}
+
+// exp(-2 pi i k s / L) for the shift s = m + d pixels, with k m reduced modulo L first so that
+// large integer shifts lose no precision.
+__device__ __forceinline__ float2 ramp1(int kidx, int L, int m, float d) {
+  const int k = signed_bin(kidx, L);
+  long long km = ((long long)k * m) % L;
+  if (km < 0) km += L;
+  const float t = (float)km / (float)L + (float)k * d / (float)L;
+  float s, c;
+  sincospif(-2.0f * t, &s, &c);
+  return make_float2(c, s);
+}
+
+// Transforms `nl` lines held at stride ax.ls in `a` (ping-pong with `b`); returns the buffer that
+// holds the result.  Ends with a barrier.
+__device__ float2* fft_lines(float2* a, float2* b, int nl, const Axis& ax, bool inv) {
+  const int L = ax.L, ls = ax.ls;
+  const int total = nl * L;
+  int ns = 1;
+  for (int st = 0; st < ax.nst; ++st) {
+    const int r = ax.radix[st];
+    const int mr = ns * r;
+    const int tstep = L / mr;
+    const int lr = L / r;
+    for (int o = threadIdx.x; o < total; o += GT) {
+      const int line = o / L;
+      const int e = o - line * L;
+      const int lo = e % ns;
+      const int hi = e / ns;
+      const int k = hi % r;
+      const int j = (hi / r) * ns + lo;
+      const int step = lo + k * ns;
+      const float2* src = a + line * ls + j;
+      float re = 0.f, im = 0.f;
+      int ex = 0;
+      for (int q = 0; q < r; ++q) {
+        const float2 x = src[q * lr];
+        const float2 t = ax.tw[ex * tstep];
+        const float ti = inv ? -t.y : t.y;
+        re += x.x * t.x - x.y * ti;
+        im += x.x * ti + x.y * t.x;
+        ex += step;
+        if (ex >= mr) ex -= mr;
+      }
+      b[line * ls + e] = make_float2(re, im);
+    }
+    __syncthreads();
+    float2* t = a;
+    a = b;
+    b = t;
+    ns = mr;
+  }
+  return a;
+}
+
+enum { NONE = 0, FWD = 1, INV = 2 };
+
+// One pass over the `nlines` lines of one axis.  COLS: a line is a column of the h x w wave (the
+// element index is the row), and global accesses put consecutive threads on consecutive columns;
+// otherwise a line is a row.  load(line, e) -> value; transform fft1; mid(line, e, value) -> value
+// (if has_mid); transform fft2; store(line, e, value).
+template <bool COLS, class Load, class Mid, class Store>
+__device__ void line_pass(int nlines, int g_max, const Axis& ax, float2* bufa, float2* bufb, int fft1, bool has_mid,
+                          int fft2, Load load, Mid mid, Store store) {
+  const int L = ax.L, ls = ax.ls;
+  for (int l0 = 0; l0 < nlines; l0 += g_max) {
+    const int g = min(g_max, nlines - l0);
+    const int total = g * L;
+    for (int o = threadIdx.x; o < total; o += GT) {
+      int line, e;
+      if (COLS) {
+        line = o % g;
+        e = o / g;
+      } else {
+        line = o / L;
+        e = o - line * L;
+      }
+      bufa[line * ls + e] = load(l0 + line, e);
+    }
+    __syncthreads();
+    float2* cur = bufa;
+    float2* oth = bufb;
+    if (fft1 != NONE) {
+      cur = fft_lines(cur, oth, g, ax, fft1 == INV);
+      oth = cur == bufa ? bufb : bufa;
+    }
+    if (has_mid) {
+      for (int o = threadIdx.x; o < total; o += GT) {
+        int line, e;
+        if (COLS) {
+          line = o % g;
+          e = o / g;
+        } else {
+          line = o / L;
+          e = o - line * L;
+        }
+        cur[line * ls + e] = mid(l0 + line, e, cur[line * ls + e]);
+      }
+      __syncthreads();
+    }
+    if (fft2 != NONE) cur = fft_lines(cur, oth, g, ax, fft2 == INV);
+    for (int o = threadIdx.x; o < total; o += GT) {
+      int line, e;
+      if (COLS) {
+        line = o % g;
+        e = o / g;
+      } else {
+        line = o / L;
+        e = o - line * L;
+      }
+      store(l0 + line, e, cur[line * ls + e]);
+    }
+    __syncthreads();
+  }
+}
+
+struct Setup {
+  Axis ay, ax;
+  float2* bufa;
+  float2* bufb;
+};
+
+// LDS layout: twiddles along y [h], along x [w], then two ping-pong buffers of gcap elements.
+__device__ Setup setup(const GridParams& a, int* s_rad, float2* lds) {
+  Setup u;
+  float2* twy = lds;
+  float2* twx = lds + a.h;
+  u.bufa = twx + a.w;
+  u.bufb = u.bufa + a.gcap;
+  for (int e = threadIdx.x; e < a.h; e += GT) {
+    double s, c;
+    sincospi(-2.0 * (double)e / (double)a.h, &s, &c);
+    twy[e] = make_float2((float)c, (float)s);
+  }
+  for (int e = threadIdx.x; e < a.w; e += GT) {
+    double s, c;
+    sincospi(-2.0 * (double)e / (double)a.w, &s, &c);
+    twx[e] = make_float2((float)c, (float)s);
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < GRID_MAX_STAGES; ++i) {
+      s_rad[i] = a.ry[i];
+      s_rad[GRID_MAX_STAGES + i] = a.rx[i];
+    }
+  }
+  __syncthreads();
+  u.ay = Axis{a.h, a.lsy, a.ny, s_rad, twy};
+  u.ax = Axis{a.w, a.lsx, a.nx, s_rad + GRID_MAX_STAGES, twx};
+  return u;
+}
+
+struct Ident {
+  __device__ float2 operator()(int, int, float2 v) const { return v; }
+};
+
+struct Position {
+  int oy, ox;  // window origin reduced into the slab
+  int my, mx;  // integer shift modulo h, w
+  float dy, dx;
+};
+
+__device__ Position position(const GridParams& a, long long p) {
+  Position q;
+  q.oy = wrap_index(a.origins[2 * p], a.slab_h);
+  q.ox = wrap_index(a.origins[2 * p + 1], a.slab_w);
+  split_shift_px(a.shifts[2 * p], a.h, q.my, q.dy);
+  split_shift_px(a.shifts[2 * p + 1], a.w, q.mx, q.dx);
+  return q;
+}
+
+// Slab element of window pixel (y, x).
+__device__ __forceinline__ long long slab_index(const GridParams& a, const Position& q, int y, int x) {
+  long long ry = q.oy + y;
+  if (ry >= a.slab_h) ry -= a.slab_h;
+  long long rx = q.ox + x;
+  if (rx >= a.slab_w) rx -= a.slab_w;
+  return ry * a.slab_w + rx;
+}
+
+// Incident wave of mode m: rows of probe_k times the shift ramp (with the 1/(h w) of the inverse
+// transform), inverse transform along x; leaves (ky, x) in wb.
+__device__ void incident_rows(const GridParams& a, const Setup& u, const Position& q, int m, float2* wb) {
+  const int h = a.h, w = a.w;
+  const float inv_hw = 1.0f / ((float)h * (float)w);
+  const float2* pk = a.probe_k + (long long)m * h * w;
+  line_pass<false>(
+      h, a.gx, u.ax, u.bufa, u.bufb, INV, false, NONE,
+      [&](int ky, int kx) {
+        const float2 r = gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx));
+        return gmul(pk[(long long)ky * w + kx], gscale(r, inv_hw));
+      },
+      Ident{}, [&](int ky, int x, float2 v) { wb[(long long)ky * w + x] = v; });
+}
+
+// Rows of wb: forward along x, times f(ky, kx), inverse along x (the propagator step or its adjoint).
+template <bool CONJ>
+__device__ void propagate_rows(const GridParams& a, const Setup& u, float2* wb) {
+  const int w = a.w;
+  line_pass<false>(
+      a.h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
+      [&](int ky, int kx, float2 v) {
+        const float2 f = gmul(a.prop_y[ky], a.prop_x[kx]);
+        return CONJ ? gmulc(f, v) : gmul(v, f);
+      },
+      [&](int ky, int x, float2 v) { wb[(long long)ky * w + x] = v; });
+}
+
+template <bool CKPT>
+__global__ __launch_bounds__(GT) void k_grid_forward(GridParams a, float2* ws) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[2 * GRID_MAX_STAGES];
+  const Setup u = setup(a, s_rad, lds);
+  const int h = a.h, w = a.w;
+  const long long hw = (long long)h * w;
+  const long long sstride = a.slab_h * a.slab_w;
+  const int nc = num_checkpoints(a.S, a.ckpt_every);
+  float2* wb = ws + (long long)blockIdx.x * hw;
+  for (long long p = blockIdx.x; p < a.P; p += gridDim.x) {
+    const Position q = position(a, p);
+    for (int m = 0; m < a.M; ++m) {
+      incident_rows(a, u, q, m, wb);
+      for (int s = 0; s < a.S; ++s) {
+        const float2* ts = a.slab + (long long)s * sstride;
+        float2* ck = nullptr;
+        if (CKPT && s > 0 && s % a.ckpt_every == 0)
+          ck = a.ckpt + ((p * a.M + m) * nc + (s / a.ckpt_every - 1)) * hw;
+        // columns: to real space along y, checkpoint, times T_s, back along y
+        line_pass<true>(
+            w, a.gy, u.ay, u.bufa, u.bufb, INV, true, FWD, [&](int x, int ky) { return wb[(long long)ky * w + x]; },
+            [&](int x, int y, float2 v) {
+              if (ck) ck[(long long)y * w + x] = v;
+              return gmul(v, ts[slab_index(a, q, y, x)]);
+            },
+            [&](int x, int ky, float2 v) { wb[(long long)ky * w + x] = v; });
+        if (s + 1 < a.S) {
+          propagate_rows<false>(a, u, wb);
+        } else {
+          // rows: forward along x gives the detector amplitude A(ky, kx)
+          const float wm = a.weights[m];
+          float* pat = a.patterns ? a.patterns + p * hw : nullptr;
+          float2* amp = a.amplitude ? a.amplitude + (p * a.M + m) * hw : nullptr;
+          line_pass<false>(
+              h, a.gx, u.ax, u.bufa, u.bufb, FWD, false, NONE,
+              [&](int ky, int x) { return wb[(long long)ky * w + x]; }, Ident{},
+              [&](int ky, int kx, float2 v) {
+                const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
+                if (amp) amp[o] = v;
+                if (pat) {
+                  const float inten = wm * (v.x * v.x + v.y * v.y);
+                  pat[o] = m == 0 ? inten : pat[o] + inten;
+                }
+              });
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
+  __shared__ int s_rad[2 * GRID_MAX_STAGES];
+  __shared__ double s_red[GT / 64];
+  extern __shared__ float2 lds[];
+  const Setup u = setup(a, s_rad, lds);
+  const int h = a.h, w = a.w;
+  const long long hw = (long long)h * w;
+  const long long sstride = a.slab_h * a.slab_w;
+  const int k = a.ckpt_every;
+  const int nc = num_checkpoints(a.S, k);
+  const float inv_hw = 1.0f / ((float)h * (float)w);
+  float2* wb = ws + (long long)blockIdx.x * (k + 2) * hw;  // forward wave
+  float2* gb = wb + hw;                                      // gradient wave
+  float2* seg = gb + hw;                                     // waves entering the slices of one segment
+  const double two_pi = 6.283185307179586;
+  for (long long p = blockIdx.x; p < a.P; p += gridDim.x) {
+    const Position q = position(a, p);
+    double gsy = 0.0, gsx = 0.0;
+    for (int m = 0; m < a.M; ++m) {
+      const float2* pk = a.probe_k + (long long)m * hw;
+      const float wm = a.weights[m];
+      for (int c = nc; c >= 0; --c) {
+        const int s0 = c * k;
+        const int s1 = min(a.S, s0 + k);
+        // recompute the waves entering slices s0 .. s1-1 (and the detector seed in the last segment)
+        if (c == 0) incident_rows(a, u, q, m, wb);
+        for (int s = s0; s < s1; ++s) {
+          const bool from_ckpt = c > 0 && s == s0;
+          const bool advance = s + 1 < s1 || s == a.S - 1;
+          const float2* src = from_ckpt ? a.ckpt + ((p * a.M + m) * nc + (c - 1)) * hw : wb;
+          const float2* ts = a.slab + (long long)s * sstride;
+          float2* sg = seg + (long long)(s - s0) * hw;
+          line_pass<true>(
+              w, a.gy, u.ay, u.bufa, u.bufb, from_ckpt ? NONE : INV, true, advance ? FWD : NONE,
+              [&](int x, int e) { return src[(long long)e * w + x]; },
+              [&](int x, int y, float2 v) {
+                sg[(long long)y * w + x] = v;
+                return advance ? gmul(v, ts[slab_index(a, q, y, x)]) : v;
+              },
+              [&](int x, int ky, float2 v) {
+                if (advance) wb[(long long)ky * w + x] = v;
+              });
+          if (s + 1 < s1) {
+            propagate_rows<false>(a, u, wb);
+          } else if (s == a.S - 1) {
+            // detector amplitude -> seed dL/dA -> adjoint of the forward x transform, into gb
+            const float* gpat = a.grad_patterns ? a.grad_patterns + p * hw : nullptr;
+            const float2* gamp = a.grad_amplitude ? a.grad_amplitude + (p * a.M + m) * hw : nullptr;
+            line_pass<false>(
+                h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
+                [&](int ky, int kx, float2 amp) {
+                  const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
+                  if (gamp) return gamp[o];
+                  return gscale(amp, 2.0f * wm * gpat[o]);
+                },
+                [&](int ky, int x, float2 v) { gb[(long long)ky * w + x] = v; });
+          }
+        }
+        // back through slices s1-1 .. s0
+        for (int s = s1 - 1; s >= s0; --s) {
+          const float2* ts = a.slab + (long long)s * sstride;
+          float2* gs = a.grad_slab + (long long)s * sstride;
+          const float2* sg = seg + (long long)(s - s0) * hw;
+          line_pass<true>(
+              w, a.gy, u.ay, u.bufa, u.bufb, INV, true, FWD, [&](int x, int ky) { return gb[(long long)ky * w + x]; },
+              [&](int x, int y, float2 g) {
+                const long long si = slab_index(a, q, y, x);
+                const float2 cg = gmulc(sg[(long long)y * w + x], g);
+                unsafeAtomicAdd(&gs[si].x, cg.x);
+                unsafeAtomicAdd(&gs[si].y, cg.y);
+                return gmulc(ts[si], g);
+              },
+              [&](int x, int ky, float2 v) { gb[(long long)ky * w + x] = v; });
+          if (s > 0) {
+            propagate_rows<true>(a, u, gb);
+          } else {
+            // dL/d(probe_k) = conj(R) dL/du and dL/dshift = Re sum conj(dL/du) du/dshift, u = probe_k R
+            float2* gp = a.grad_probe_k ? a.grad_probe_k + (long long)m * hw : nullptr;
+            const bool want_shift = a.grad_shifts != nullptr;
+            line_pass<false>(
+                h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, NONE,
+                [&](int ky, int x) { return gb[(long long)ky * w + x]; },
+                [&](int ky, int kx, float2 gu) {
+                  const float2 r = gscale(gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx)), inv_hw);
+                  const long long o = (long long)ky * w + kx;
+                  if (gp) {
+                    const float2 cg = gmulc(r, gu);
+                    unsafeAtomicAdd(&gp[o].x, cg.x);
+                    unsafeAtomicAdd(&gp[o].y, cg.y);
+                  }
+                  if (want_shift) {
+                    // du/ds_y = u (-2 pi i ky / h): Re(conj(gu) u (-2 pi i f)) = 2 pi f Im(conj(gu) u)
+                    const float2 z = gmulc(gu, gmul(pk[o], r));
+                    gsy += two_pi * (double)signed_bin(ky, h) / (double)h * (double)z.y;
+                    gsx += two_pi * (double)signed_bin(kx, w) / (double)w * (double)z.y;
+                  }
+                  return gu;
+                },
+                [&](int, int, float2) {});
+          }
+        }
+      }
+    }
+    if (a.grad_shifts) {
+      const double ty = block_sum_fp64<GT>(gsy, s_red);
+      const double tx = block_sum_fp64<GT>(gsx, s_red);
+      if (threadIdx.x == 0) {
+        a.grad_shifts[2 * p] += ty;
+        a.grad_shifts[2 * p + 1] += tx;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+int factor_edge(int L, int* r) {
+  int n = 0, v = L;
+  while (v % 4 == 0) {
+    r[n++] = 4;
+    v /= 4;
+  }
+  for (int f = 2; v > 1; ++f)
+    while (v % f == 0) {
+      r[n++] = f;
+      v /= f;
+    }
+  return n;
+}
+
+int lds_stride(int L) { return L % 2 == 0 ? L + 1 : L; }
+
+size_t prepare(GridParams& a) {
+  for (int i = 0; i < GRID_MAX_STAGES; ++i) a.ry[i] = a.rx[i] = 1;
+  a.ny = factor_edge(a.h, a.ry);
+  a.nx = factor_edge(a.w, a.rx);
+  a.lsy = lds_stride(a.h);
+  a.lsx = lds_stride(a.w);
+  a.gy = std::max(1, std::min(GROUP_ELEMS / a.h, a.w));  // columns per group (line length h)
+  a.gx = std::max(1, std::min(GROUP_ELEMS / a.w, a.h));  // rows per group (line length w)
+  a.gcap = std::max(a.gy * a.lsy, a.gx * a.lsx);
+  return (size_t)(a.h + a.w + 2 * a.gcap) * sizeof(float2);
+}
+
+int clamp_ckpt(int k, int S) { return std::max(1, std::min(k, S)); }
+
+hipError_t set_lds(const void* fn, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+long long blocks_for(long long P, size_t per_block, size_t ws_bytes) {
+  const long long fit = per_block ? (long long)(ws_bytes / per_block) : 0;
+  return std::min(std::min(P, (long long)MAX_BLOCKS), fit);
+}
+
+}  // namespace
+
+size_t grid_block_bytes(int h, int w, int ckpt_every, bool adjoint) {
+  const size_t wave = (size_t)h * (size_t)w * sizeof(float2);
+  return adjoint ? wave * (size_t)(std::max(1, ckpt_every) + 2) : wave;
+}
+
+size_t grid_workspace_bytes(int h, int w, int ckpt_every, long long num_positions, bool adjoint) {
+  const size_t per = grid_block_bytes(h, w, ckpt_every, adjoint);
+  long long b = std::min(num_positions, (long long)MAX_BLOCKS);
+  b = std::min(b, std::max(1LL, (long long)(WS_BUDGET / per)));
+  return per * (size_t)std::max(1LL, b);
+}
+
+hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStream_t st) {
+  const bool ckpt = a.ckpt != nullptr;
+  a.ckpt_every = ckpt ? clamp_ckpt(a.ckpt_every, a.S) : 1;
+  const size_t lds = prepare(a);
+  const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, 1, false), ws_bytes);
+  if (nb < 1) return hipErrorInvalidValue;
+  const void* fn = ckpt ? (const void*)k_grid_forward<true> : (const void*)k_grid_forward<false>;
+  hipError_t e = set_lds(fn, lds);
+  if (e != hipSuccess) return e;
+  if (ckpt)
+    hipLaunchKernelGGL(k_grid_forward<true>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  else
+    hipLaunchKernelGGL(k_grid_forward<false>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_adjoint(GridParams a, void* ws, size_t ws_bytes, hipStream_t st) {
+  a.ckpt_every = clamp_ckpt(a.ckpt_every, a.S);
+  const size_t lds = prepare(a);
+  const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, a.ckpt_every, true), ws_bytes);
+  if (nb < 1) return hipErrorInvalidValue;
+  hipError_t e = set_lds((const void*)k_grid_adjoint, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_grid_adjoint, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  return hipGetLastError();
+}
+
+}  // namespace pty

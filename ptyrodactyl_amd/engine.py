@@ -27,6 +27,8 @@ LARGE_BATCH_BYTES = 192 << 20
 # Waves (positions x modes) per streaming batch for n >= 256; the batch's
 # waves live in a device workspace that should stay cache-resident.
 DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "512"))
+# largest edge of the runtime-size kernels (any h x w that is not a square power of two)
+GRID_MAX_EDGE = 4096
 
 
 def _require(cond: bool, msg: str) -> None:
@@ -41,6 +43,118 @@ def _check_cuda(t: torch.Tensor, name: str, dtype, ndim: int, device) -> None:
     _require(t.ndim == ndim, f"{name} must be {ndim}-D, got shape {tuple(t.shape)}")
     _require(t.is_contiguous(), f"{name} must be contiguous")
     _require(t.device == device, f"{name} is on {t.device}, expected {device}")
+
+
+def is_fast_grid(h: int, w: int) -> bool:
+    """Square power-of-two windows run on the dedicated kernels; every other h x w on the runtime-size ones."""
+    return h == w and h in SUPPORTED_N
+
+
+def _prop_pair(prop, h: int, w: int, dev):
+    """(prop_y, prop_x) from one factor (square grids) or a pair; each includes 1/h or 1/w."""
+    prop_y, prop_x = prop if isinstance(prop, (tuple, list)) else (prop, prop)
+    _check_cuda(prop_y, "prop_y", torch.complex64, 1, dev)
+    _check_cuda(prop_x, "prop_x", torch.complex64, 1, dev)
+    _require(prop_y.shape[0] == h and prop_x.shape[0] == w,
+             f"propagator factors must have {h} and {w} entries, got {prop_y.shape[0]} and {prop_x.shape[0]}")
+    return prop_y, prop_x
+
+
+def _grid_operands(slab_t, probe_k, weights, origins, shifts_px, prop):
+    _require(isinstance(slab_t, torch.Tensor) and slab_t.is_cuda,
+             "slab must be a GPU tensor (the multislice path has no CPU fallback)")
+    dev = slab_t.device
+    _check_cuda(slab_t, "slab", torch.complex64, 3, dev)
+    _check_cuda(probe_k, "probe_k", torch.complex64, 3, dev)
+    m_count, h, w = probe_k.shape
+    _require(1 <= h <= GRID_MAX_EDGE and 1 <= w <= GRID_MAX_EDGE,
+             f"grid {h}x{w}: each edge must be in [1, {GRID_MAX_EDGE}]")
+    s_count, hs, ws = slab_t.shape
+    _require(s_count >= 1, "at least one slice is required")
+    _require(hs >= h and ws >= w, f"slab {hs}x{ws} is smaller than the {h}x{w} window")
+    _check_cuda(weights, "weights", torch.float32, 1, dev)
+    _require(weights.shape[0] == m_count, "weights must have shape (M,)")
+    _check_cuda(origins, "origins", torch.int32, 2, dev)
+    _check_cuda(shifts_px, "shifts_px", torch.float64, 2, dev)
+    p_count = origins.shape[0]
+    _require(p_count >= 1 and origins.shape == (p_count, 2) and shifts_px.shape == (p_count, 2),
+             "origins and shifts must both have shape (P, 2) with P >= 1")
+    prop_y, prop_x = _prop_pair(prop, h, w, dev)
+    return dev, h, w, s_count, hs, ws, m_count, p_count, prop_y, prop_x
+
+
+def _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop, want_patterns, want_amplitude, patterns_out,
+                  ckpt_every):
+    """Forward on any h x w grid (``pty_forward_grid``); checkpoints when ckpt_every >= 1."""
+    dev, h, w, s_count, hs, ws, m_count, p_count, prop_y, prop_x = _grid_operands(slab_t, probe_k, weights, origins,
+                                                                                shifts_px, prop)
+    patterns = None
+    if want_patterns:
+        if patterns_out is not None:
+            _check_cuda(patterns_out, "patterns_out", torch.float32, 3, dev)
+            _require(tuple(patterns_out.shape) == (p_count, h, w), "patterns_out has the wrong shape")
+            patterns = patterns_out
+        else:
+            patterns = torch.empty((p_count, h, w), dtype=torch.float32, device=dev)
+    amplitude = torch.empty((p_count, m_count, h, w), dtype=torch.complex64, device=dev) if want_amplitude else None
+    ckpt = None
+    nc = num_checkpoints(s_count, ckpt_every) if ckpt_every else 0
+    if ckpt_every:
+        ckpt = torch.empty((p_count * m_count, nc, h, w), dtype=torch.complex64, device=dev)
+    lib = _native.load()
+    ws_bytes = int(lib.pty_grid_workspace_bytes(h, w, s_count, 1, p_count, 0))
+    workspace = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        status = lib.pty_forward_grid(
+            h, w, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
+            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop_y), _native.ptr(prop_x),
+            _native.ptr(patterns), _native.ptr(amplitude), _native.ptr(ckpt) if nc else 0, max(int(ckpt_every), 1),
+            _native.ptr(workspace), ws_bytes, _native.stream_handle(dev))
+    _native.check(status, "pty_forward_grid")
+    return patterns, amplitude, ckpt
+
+
+def _grid_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_patterns, ckpt, ckpt_every, grad_slab,
+                  grad_probe_k, want_probe, grad_amplitude, grad_shifts):
+    dev, h, w, s_count, hs, ws, m_count, p_count, prop_y, prop_x = _grid_operands(slab_t, probe_k, weights, origins,
+                                                                                shifts_px, prop)
+    k = int(ckpt_every)
+    _require(k >= 1, "ckpt_every must be >= 1")
+    nc = num_checkpoints(s_count, k)
+    if grad_amplitude is not None:
+        _check_cuda(grad_amplitude, "grad_amplitude", torch.complex64, 4, dev)
+        _require(tuple(grad_amplitude.shape) == (p_count, m_count, h, w), "grad_amplitude must have shape (P, M, h, w)")
+        grad_patterns = None
+    else:
+        _check_cuda(grad_patterns, "grad_patterns", torch.float32, 3, dev)
+        _require(tuple(grad_patterns.shape) == (p_count, h, w), "grad_patterns must have shape (P, h, w)")
+    if nc:
+        _check_cuda(ckpt, "checkpoints", torch.complex64, 4, dev)
+        _require(tuple(ckpt.shape) == (p_count * m_count, nc, h, w), "checkpoint buffer has the wrong shape")
+    if grad_slab is None:
+        grad_slab = torch.zeros_like(slab_t)
+    _check_cuda(grad_slab, "grad_slab", torch.complex64, 3, dev)
+    _require(grad_slab.shape == slab_t.shape, "grad_slab must match the slab")
+    if want_probe and grad_probe_k is None:
+        grad_probe_k = torch.zeros_like(probe_k)
+    if grad_probe_k is not None:
+        _check_cuda(grad_probe_k, "grad_probe_k", torch.complex64, 3, dev)
+        _require(grad_probe_k.shape == probe_k.shape, "grad_probe_k must match probe_k")
+    if grad_shifts is not None:
+        _check_cuda(grad_shifts, "grad_shifts", torch.float64, 2, dev)
+        _require(tuple(grad_shifts.shape) == (p_count, 2), "grad_shifts must have shape (P, 2)")
+    lib = _native.load()
+    ws_bytes = int(lib.pty_grid_workspace_bytes(h, w, s_count, k, p_count, 1))
+    workspace = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        status = lib.pty_adjoint_grid(
+            h, w, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
+            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop_y), _native.ptr(prop_x),
+            _native.ptr(grad_patterns), _native.ptr(grad_amplitude), _native.ptr(ckpt) if nc else 0, k,
+            _native.ptr(grad_slab), _native.ptr(grad_probe_k), _native.ptr(grad_shifts), _native.ptr(workspace),
+            ws_bytes, _native.stream_handle(dev))
+    _native.check(status, "pty_adjoint_grid")
+    return grad_slab, grad_probe_k
 
 
 def workspace_bytes(n: int, num_positions: int, num_modes: int, max_batch_waves: Optional[int] = None) -> int:
@@ -73,9 +187,12 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
     dev = slab_t.device
     _check_cuda(slab_t, "slab", torch.complex64, 3, dev)
     _check_cuda(probe_k, "probe_k", torch.complex64, 3, dev)
-    m_count, n, n2 = probe_k.shape
-    _require(n == n2, "probe window must be square")
-    _require(n in SUPPORTED_N, f"window size {n} is not one of {SUPPORTED_N}")
+    if not is_fast_grid(probe_k.shape[1], probe_k.shape[2]):
+        _require(want_patterns or want_amplitude, "nothing requested")
+        patterns, amplitude, _ = _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop,
+                                               want_patterns, want_amplitude, patterns_out, 0)
+        return patterns, amplitude
+    m_count, n, _ = probe_k.shape
     s_count, hs, ws = slab_t.shape
     _require(s_count >= 1, "at least one slice is required")
     _require(hs >= n and ws >= n, f"slab {hs}x{ws} is smaller than the {n}x{n} window")
@@ -87,8 +204,7 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
     _require(origins.shape == (p_count, 2) and shifts_px.shape == (p_count, 2),
              "origins and shifts must both have shape (P, 2)")
     _require(p_count >= 1, "at least one scan position is required")
-    _check_cuda(prop, "prop", torch.complex64, 1, dev)
-    _require(prop.shape[0] == n, "propagator factor must have n entries")
+    prop_y, prop_x = _prop_pair(prop, n, n, dev)
     _require(want_patterns or want_amplitude, "nothing requested")
 
     patterns = None
@@ -107,8 +223,8 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
     with torch.cuda.device(dev):
         status = lib.pty_forward(
             n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k),
-            _native.ptr(weights), _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop),
-            _native.ptr(prop), _native.ptr(patterns), _native.ptr(amplitude), _native.ptr(workspace),
+            _native.ptr(weights), _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop_y),
+            _native.ptr(prop_x), _native.ptr(patterns), _native.ptr(amplitude), _native.ptr(workspace),
             ws_bytes, _native.stream_handle(dev))
     _native.check(status, "pty_forward")
     return patterns, amplitude
@@ -172,7 +288,7 @@ def multislice_forward_detectors(slab_t: torch.Tensor, probe_k: torch.Tensor, we
     with torch.cuda.device(dev):
         status = lib.pty_forward_detectors(
             n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
-            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(origins), _native.ptr(shifts_px), *_prop_ptrs(prop),
             _native.ptr(patterns), d_count, _native.ptr(masks), _native.ptr(signals), _native.ptr(workspace),
             ws_bytes, _native.stream_handle(dev))
     _native.check(status, "pty_forward_detectors")
@@ -220,7 +336,7 @@ def multislice_forward_persistent(slab_t: torch.Tensor, probe_k: torch.Tensor, w
     with torch.cuda.device(dev):
         status = lib.pty_forward_persistent(
             n, s_count, p_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
-            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(origins), _native.ptr(shifts_px), *_prop_ptrs(prop),
             _native.ptr(patterns_out), g, int(grid), _native.ptr(workspace), ws_bytes, _native.ptr(state),
             _native.stream_handle(dev))
     _native.check(status, "pty_forward_persistent")
@@ -252,9 +368,13 @@ def _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop, 
     p_count = origins.shape[0]
     _require(p_count >= 1 and origins.shape == (p_count, 2) and shifts_px.shape == (p_count, 2),
              "origins and shifts must both have shape (P, 2) with P >= 1")
-    _check_cuda(prop, "prop", torch.complex64, 1, dev)
-    _require(prop.shape[0] == n, "propagator factor must have n entries")
+    _prop_pair(prop, n, n, dev)
     return dev, n, s_count, hs, ws, m_count, p_count
+
+
+def _prop_ptrs(prop):
+    prop_y, prop_x = prop if isinstance(prop, (tuple, list)) else (prop, prop)
+    return _native.ptr(prop_y), _native.ptr(prop_x)
 
 
 def num_checkpoints(num_slices: int, ckpt_every: int) -> int:
@@ -267,6 +387,12 @@ def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px
 
     Returns (patterns (P, n, n) float32, checkpoints (P*M, (S-1)//k, n, n) complex64).
     """
+    if isinstance(probe_k, torch.Tensor) and probe_k.ndim == 3 and not is_fast_grid(probe_k.shape[1],
+                                                                                     probe_k.shape[2]):
+        _require(int(ckpt_every) >= 1, "ckpt_every must be >= 1")
+        patterns, _, ckpt = _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop, True, False, None,
+                                          int(ckpt_every))
+        return patterns, ckpt
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
                                                                         shifts_px, prop, GRADIENT_N,
                                                                         "checkpoints")
@@ -280,7 +406,7 @@ def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px
     with torch.cuda.device(dev):
         status = lib.pty_forward_checkpointed(
             n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
-            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(origins), _native.ptr(shifts_px), *_prop_ptrs(prop),
             _native.ptr(patterns), _native.ptr(ckpt) if nc else 0, int(ckpt_every), _native.ptr(workspace),
             ws_bytes, _native.stream_handle(dev))
     _native.check(status, "pty_forward_checkpointed")
@@ -299,6 +425,10 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
     ``grad_shifts`` ((P, 2) float64, may be None) accumulates dL/d(shifts_px).
     Returns (grad_slab, grad_probe_k or None).
     """
+    if isinstance(probe_k, torch.Tensor) and probe_k.ndim == 3 and not is_fast_grid(probe_k.shape[1],
+                                                                                     probe_k.shape[2]):
+        return _grid_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_patterns, ckpt, ckpt_every,
+                             grad_slab, grad_probe_k, want_probe, grad_amplitude, grad_shifts)
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
                                                                         shifts_px, prop, GRADIENT_N,
                                                                         "the adjoint")
@@ -334,7 +464,7 @@ def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_
         seed = grad_amplitude if grad_amplitude is not None else grad_patterns
         status = fn(
             n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
-            _native.ptr(origins), _native.ptr(shifts_px), _native.ptr(prop), _native.ptr(prop),
+            _native.ptr(origins), _native.ptr(shifts_px), *_prop_ptrs(prop),
             _native.ptr(seed), _native.ptr(ckpt) if nc else 0, k, _native.ptr(grad_slab),
             _native.ptr(grad_probe_k), _native.ptr(grad_shifts), _native.ptr(workspace), ws_bytes,
             _native.stream_handle(dev))

@@ -18,7 +18,7 @@ from typing import Optional
 import torch
 
 from . import engine, physics
-from .simulations import _device_for, _probe_k, _square_n
+from .simulations import _device_for, _grid_shape, _probe_k
 from .types import (STEM4D, DetectorConfig, MicroscopeConfig, ProbeModes, _as_tensor, create_stem4d, error_if)
 
 
@@ -97,11 +97,11 @@ def cbed_amplitude_from_atoms(beam: torch.Tensor, atom_coords, atom_types, slice
     sample = create_atomic_slice_data(atom_coords, atom_types, slice_z_bounds, atom_potentials, atom_mask)
     modes = beam if beam.ndim == 3 else beam[..., None]
     dev = _device_for(modes, sample.atom_coords)
-    n = _square_n(modes.shape[0], modes.shape[1])
+    h, w = _grid_shape(modes.shape[0], modes.shape[1])
     calib = float(calib_ang)
-    slab = _transmission_slab(slice_potentials(sample, (n, n), calib, dev), float(voltage_kv))
+    slab = _transmission_slab(slice_potentials(sample, (h, w), calib, dev), float(voltage_kv))
     dz = float(sample.slice_z_bounds[0, 1] - sample.slice_z_bounds[0, 0])
-    prop = physics.propagator_factors(n, dz, float(voltage_kv), calib, device=dev)
+    prop = physics.propagator_pair(h, w, dz, float(voltage_kv), calib, device=dev)
     pk = _probe_k(modes, dev)
     ones = torch.ones(pk.shape[0], dtype=torch.float32, device=dev)
     zeros = torch.zeros((1, 2), dtype=torch.int32, device=dev)
@@ -126,19 +126,19 @@ def stem4d_sharded(probe_modes: ProbeModes, sample: AtomicSliceData, microscope:
     if detector.scan_positions_ang is None:
         raise ValueError("detector.scan_positions_ang is required")
     dev = _device_for(probe_modes.modes, sample.atom_coords)
-    n = _square_n(probe_modes.modes.shape[0], probe_modes.modes.shape[1])
+    h, w = _grid_shape(probe_modes.modes.shape[0], probe_modes.modes.shape[1])
     calib = float(detector.real_space_calib_ang)
     if detector.scan_positions_ang.shape[0] == 0:  # empty scan: empty stack
         pos0 = detector.scan_positions_ang.to(device=dev, dtype=torch.float64)
-        return create_stem4d(torch.empty((0, n, n), dtype=torch.float32, device=dev), calib, 1.0 / (n * calib), pos0,
+        return create_stem4d(torch.empty((0, h, w), dtype=torch.float32, device=dev), calib, 1.0 / (h * calib), pos0,
                              microscope.voltage_kv)
-    pot = slice_potentials(sample, (n, n), calib, dev)
+    pot = slice_potentials(sample, (h, w), calib, dev)
     slab = _transmission_slab(pot, microscope.voltage_kv)
     dz = float(sample.slice_z_bounds[0, 1] - sample.slice_z_bounds[0, 0])
     pos = detector.scan_positions_ang.to(device=dev, dtype=torch.float64)
     shifts = physics.shift_to_pixels(pos, calib)
     origins = torch.zeros((pos.shape[0], 2), dtype=torch.int32, device=dev)
-    prop = physics.propagator_factors(n, dz, microscope.voltage_kv, calib, device=dev)
+    prop = physics.propagator_pair(h, w, dz, microscope.voltage_kv, calib, device=dev)
     weights = probe_modes.weights.to(device=dev, dtype=torch.float32).contiguous()
     pk = _probe_k(probe_modes.modes, dev)
     if torch.is_grad_enabled() and (slab.requires_grad or pk.requires_grad or weights.requires_grad
@@ -148,4 +148,4 @@ def stem4d_sharded(probe_modes: ProbeModes, sample: AtomicSliceData, microscope:
         patterns = autograd.multislice_patterns(slab, pk, weights, origins, shifts, prop)
     else:
         patterns, _ = engine.multislice_forward(slab, pk, weights, origins, shifts, prop)
-    return create_stem4d(patterns, calib, 1.0 / (n * calib), pos, microscope.voltage_kv)
+    return create_stem4d(patterns, calib, 1.0 / (h * calib), pos, microscope.voltage_kv)

@@ -62,6 +62,15 @@ def propagator_factors(n: int, thickness_ang: float, voltage_kv: float, calib_an
     return f.to(torch.complex64).contiguous()
 
 
+def propagator_pair(imsize_y: int, imsize_x: int, thickness_ang: float, voltage_kv: float, calib_ang: float,
+                    device=None):
+    """(Py / h, Px / w) of ``propagation_func`` on an h x w grid; one shared tensor when h == w."""
+    fy = propagator_factors(imsize_y, thickness_ang, voltage_kv, calib_ang, device=device)
+    if int(imsize_x) == int(imsize_y):
+        return fy, fy
+    return fy, propagator_factors(imsize_x, thickness_ang, voltage_kv, calib_ang, device=device)
+
+
 def shift_to_pixels(positions_ang: torch.Tensor, calib_ang: float) -> torch.Tensor:
     """Probe shifts in pixels for ``shift_beam_fourier``'s Angstrom positions."""
     return (positions_ang.to(torch.float64) / float(calib_ang)).contiguous()

@@ -146,18 +146,19 @@ def _probe_k(modes: torch.Tensor, device) -> torch.Tensor:
     return torch.fft.fft2(m).to(torch.complex64).contiguous()
 
 
-def _square_n(h: int, w: int) -> int:
-    if h != w:
-        raise ValueError(f"the multislice kernels need a square grid, got {h}x{w}")
-    if h not in engine.SUPPORTED_N:
-        raise ValueError(f"grid size {h} is not one of {engine.SUPPORTED_N}")
-    return h
+def _grid_shape(h: int, w: int):
+    """(h, w) of a probe grid the kernels take: any edge from 1 to 4096 (square powers of two from 32 run on
+    the dedicated kernels, every other shape on the runtime-size ones)."""
+    h, w = int(h), int(w)
+    if not (1 <= h <= engine.GRID_MAX_EDGE and 1 <= w <= engine.GRID_MAX_EDGE):
+        raise ValueError(f"grid {h}x{w}: each edge must be in [1, {engine.GRID_MAX_EDGE}]")
+    return h, w
 
 
 def _run_positions(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig,
                    shifts_px: torch.Tensor, want_amplitude: bool, device):
-    n = _square_n(beam.modes.shape[0], beam.modes.shape[1])
-    if tuple(pot_slices.slices.shape[:2]) != (n, n):
+    h, w = _grid_shape(beam.modes.shape[0], beam.modes.shape[1])
+    if tuple(pot_slices.slices.shape[:2]) != (h, w):
         raise ValueError("beam.modes spatial shape must match pot_slices.slices")
     calib = min(float(pot_slices.calib), float(beam.calib))
     slab = _slab_from_potential(pot_slices, microscope.voltage_kv, device)
@@ -165,7 +166,7 @@ def _run_positions(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Mi
     weights = beam.weights.to(device=device, dtype=torch.float32).contiguous()
     p_count = shifts_px.shape[0]
     origins = torch.zeros((p_count, 2), dtype=torch.int32, device=device)
-    prop = physics.propagator_factors(n, pot_slices.slice_thickness, microscope.voltage_kv, calib, device=device)
+    prop = physics.propagator_pair(h, w, pot_slices.slice_thickness, microscope.voltage_kv, calib, device=device)
     shifts = shifts_px.to(device).contiguous()
     if torch.is_grad_enabled() and (slab.requires_grad or pk.requires_grad or weights.requires_grad
                                     or shifts.requires_grad):
@@ -174,10 +175,10 @@ def _run_positions(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Mi
         # the transmission, probe_k and the shifts in pixels
         from . import autograd
         if want_amplitude:
-            return (None, autograd.multislice_amplitudes(slab, pk, origins, shifts, prop)), calib, n
-        return (autograd.multislice_patterns(slab, pk, weights, origins, shifts, prop), None), calib, n
+            return (None, autograd.multislice_amplitudes(slab, pk, origins, shifts, prop)), calib, h
+        return (autograd.multislice_patterns(slab, pk, weights, origins, shifts, prop), None), calib, h
     return engine.multislice_forward(slab, pk, weights, origins, shifts, prop,
-                                     want_patterns=True, want_amplitude=want_amplitude), calib, n
+                                     want_patterns=True, want_amplitude=want_amplitude), calib, h
 
 
 def cbed_amplitude(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig) -> torch.Tensor:
@@ -229,10 +230,10 @@ def stem_4d(pot_slice: PotentialSlices, beam: ProbeModes, microscope: Microscope
     dev = _device_for(pot_slice.slices, beam.modes, detector.scan_positions_px)
     positions = detector.scan_positions_px.to(device=dev, dtype=torch.float64)
     if positions.shape[0] == 0:  # an empty scan maps to an empty stack, as vmap over zero positions does
-        n = _square_n(beam.modes.shape[0], beam.modes.shape[1])
+        h, w = _grid_shape(beam.modes.shape[0], beam.modes.shape[1])
         calib = min(float(pot_slice.calib), float(beam.calib))
-        return create_stem4d(torch.empty((0, n, n), dtype=torch.float32, device=dev), detector.real_space_calib_ang,
-                             1.0 / (n * calib), positions * detector.real_space_calib_ang, microscope.voltage_kv)
+        return create_stem4d(torch.empty((0, h, w), dtype=torch.float32, device=dev), detector.real_space_calib_ang,
+                             1.0 / (h * calib), positions * detector.real_space_calib_ang, microscope.voltage_kv)
     if not microscope.ensemble.is_empty():
         n = beam.modes.shape[0]
         calib = min(float(pot_slice.calib), float(beam.calib))
@@ -424,14 +425,17 @@ _DESC_DTYPE = np.dtype(_native.POSITION_DESC_FIELDS)
 
 def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig,
                        positions: Optional[torch.Tensor], det_calib: float, dev) -> torch.Tensor:
-    """(P, n, n) float64 ensemble-reduced patterns; P = 1 without positions (``cbed_image``).
+    """(P, h, w) float64 ensemble-reduced patterns; P = 1 without positions (``cbed_image``).
 
     Every (position, ensemble sample) pair becomes one wave of a batched HIP launch: its incident wave is
     built in complex128 in the reference's order (position shift at ``det_calib``, mode selection, jitter
     shift and tilt at the cbed calibration), its transmission and propagator are those of the sample's
-    energy.  Incoherent axes are summed from the kernel's intensities, coherent axes from its amplitudes."""
-    n = _square_n(beam.modes.shape[0], beam.modes.shape[1])
-    if tuple(pot_slices.slices.shape[:2]) != (n, n):
+    energy.  Incoherent axes are summed from the kernel's intensities, coherent axes from its amplitudes.
+    Grids other than the square powers of two run each energy's waves as the modes of one runtime-size
+    launch (amplitudes per wave)."""
+    h, w = _grid_shape(beam.modes.shape[0], beam.modes.shape[1])
+    fast = engine.is_fast_grid(h, w)
+    if tuple(pot_slices.slices.shape[:2]) != (h, w):
         raise ValueError("beam.modes spatial shape must match pot_slices.slices")
     calib = min(float(pot_slices.calib), float(beam.calib))
     axes = _ensemble_axes_for_cbed(microscope.ensemble, beam)
@@ -456,24 +460,25 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
     v = pot_slices.slices if pot_slices.slices.ndim == 3 else pot_slices.slices[..., None]
     v = v.detach().to(device=dev, dtype=torch.float64)
     s_count = v.shape[2]
-    slabs = torch.empty((len(energies), s_count, n, n), dtype=torch.complex64, device=dev)
-    props = torch.empty((len(energies), 2 * n), dtype=torch.complex64, device=dev)
+    slabs = torch.empty((len(energies), s_count, h, w), dtype=torch.complex64, device=dev)
+    props = torch.empty((len(energies), h + w), dtype=torch.complex64, device=dev)
     kvs = [float(microscope.voltage_kv) + e / 1000.0 for e in energies]
     for g, kv in enumerate(kvs):
         t = torch.polar(torch.ones_like(v), v * physics.interaction_sigma(kv))
         slabs[g] = t.permute(2, 0, 1).to(torch.complex64)
-        f = physics.propagator_factors(n, pot_slices.slice_thickness, kv, calib, device=dev)
-        props[g, :n] = f
-        props[g, n:] = f
-    q = physics.fftfreq(n, calib, device=dev)
+        fy, fx = physics.propagator_pair(h, w, pot_slices.slice_thickness, kv, calib, device=dev)
+        props[g, :h] = fy
+        props[g, h:] = fx
+    qy = physics.fftfreq(h, calib, device=dev)
+    qx = physics.fftfreq(w, calib, device=dev)
     jitter = None
     if bool((dpos != 0).any()):
-        jitter = torch.polar(torch.ones((k_count, n, n), dtype=torch.float64, device=dev),
-                             -2.0 * math.pi * (q[None, :, None] * dpos[:, 0, None, None]
-                                               + q[None, None, :] * dpos[:, 1, None, None]))
+        jitter = torch.polar(torch.ones((k_count, h, w), dtype=torch.float64, device=dev),
+                             -2.0 * math.pi * (qy[None, :, None] * dpos[:, 0, None, None]
+                                               + qx[None, None, :] * dpos[:, 1, None, None]))
     tilts = None
     if bool((tilt != 0).any()):
-        tilts = torch.stack([_tilt_ramp(n, n, tilt[k], kvs[group[k]], calib, dev) for k in range(k_count)])
+        tilts = torch.stack([_tilt_ramp(h, w, tilt[k], kvs[group[k]], calib, dev) for k in range(k_count)])
     weights = None
     if not coherent:
         grids = torch.meshgrid(*[torch.arange(c) for c in counts], indexing="ij")
@@ -485,48 +490,66 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
     modes = beam.modes if beam.modes.ndim == 3 else beam.modes[..., None]
     modes = modes.detach().to(device=dev, dtype=torch.complex128)
     p_count = 1 if positions is None else positions.shape[0]
-    chunk = max(1, min(p_count, _ENSEMBLE_CHUNK_BYTES // (k_count * n * n * 80)))
-    out = torch.empty((p_count, n, n), dtype=torch.float64, device=dev)
+    chunk = max(1, min(p_count, _ENSEMBLE_CHUNK_BYTES // (k_count * h * w * 80)))
+    out = torch.empty((p_count, h, w), dtype=torch.float64, device=dev)
     base = engine.arena_base(dev)
     sel_idx = torch.as_tensor(mode_idx, device=dev)
     # The position shift (shift_beam_fourier at det_calib) and the jitter shift are both Fourier phase ramps, so
     # every incident wave is formed in Fourier space as fft2(mode) * ramp_p * jitter_k; only a tilt (a real-space
     # ramp) needs the inverse and forward transforms.
-    modes_k = torch.fft.fft2(modes, dim=(0, 1))  # (n, n, M)
-    qd = physics.fftfreq(n, float(det_calib), device=dev)
+    modes_k = torch.fft.fft2(modes, dim=(0, 1))  # (h, w, M)
+    qdy = physics.fftfreq(h, float(det_calib), device=dev)
+    qdx = physics.fftfreq(w, float(det_calib), device=dev)
     for p0 in range(0, p_count, chunk):
         pc = min(chunk, p_count - p0)
         if positions is None:
             kp = modes_k[None]
         else:
             pp = positions[p0:p0 + pc].detach().to(device=dev, dtype=torch.float64)
-            phase = -2.0 * math.pi * (qd[None, :, None] * pp[:, 0, None, None] + qd[None, None, :] * pp[:, 1, None, None])
+            phase = -2.0 * math.pi * (qdy[None, :, None] * pp[:, 0, None, None]
+                                      + qdx[None, None, :] * pp[:, 1, None, None])
             kp = modes_k[None] * torch.polar(torch.ones_like(phase), phase)[..., None]
-        waves_k = kp.index_select(3, sel_idx).permute(0, 3, 1, 2)  # (pc, K, n, n), Fourier space
+        waves_k = kp.index_select(3, sel_idx).permute(0, 3, 1, 2)  # (pc, K, h, w), Fourier space
         if jitter is not None:
             waves_k = waves_k * jitter[None]
         if tilts is not None:
             waves_k = torch.fft.fft2(torch.fft.ifft2(waves_k) * tilts[None])
-        probe_k = waves_k.to(torch.complex64).reshape(pc * k_count, n, n).contiguous()
+        probe_k = waves_k.to(torch.complex64).reshape(pc * k_count, h, w).contiguous()
         del waves_k, kp
-        patterns = torch.empty((pc * k_count, n, n), dtype=torch.float32, device=dev)
-        amps = torch.empty((pc * k_count, n, n), dtype=torch.complex64, device=dev) if coherent else None
-        rec = np.zeros(pc * k_count, dtype=_DESC_DTYPE)
         wave_group = np.tile(np.asarray(group, dtype=np.int64), pc)
-        rec["slab_offset"] = engine.element_offset(slabs, base) + wave_group * (s_count * n * n)
-        rec["slab_h"] = n
-        rec["slab_w"] = n
-        rec["pattern_offset"] = engine.element_offset(patterns, base) + np.arange(pc * k_count) * n * n
-        rec["probe_offset"] = engine.element_offset(probe_k, base) + np.arange(pc * k_count) * n * n
-        rec["prop_offset"] = engine.element_offset(props, base) + wave_group * (2 * n)
-        descs = torch.from_numpy(rec.view(np.uint8)).to(dev)
-        engine.forward_batched(n, s_count, descs, base, amplitudes=amps)
+        if fast:
+            n = h
+            patterns = torch.empty((pc * k_count, n, n), dtype=torch.float32, device=dev)
+            amps = torch.empty((pc * k_count, n, n), dtype=torch.complex64, device=dev) if coherent else None
+            rec = np.zeros(pc * k_count, dtype=_DESC_DTYPE)
+            rec["slab_offset"] = engine.element_offset(slabs, base) + wave_group * (s_count * n * n)
+            rec["slab_h"] = n
+            rec["slab_w"] = n
+            rec["pattern_offset"] = engine.element_offset(patterns, base) + np.arange(pc * k_count) * n * n
+            rec["probe_offset"] = engine.element_offset(probe_k, base) + np.arange(pc * k_count) * n * n
+            rec["prop_offset"] = engine.element_offset(props, base) + wave_group * (2 * n)
+            descs = torch.from_numpy(rec.view(np.uint8)).to(dev)
+            engine.forward_batched(n, s_count, descs, base, amplitudes=amps)
+        else:
+            # each energy: its waves as the probe modes of one position at the origin, amplitudes per wave
+            amps = torch.empty((pc * k_count, h, w), dtype=torch.complex64, device=dev)
+            zero_o = torch.zeros((1, 2), dtype=torch.int32, device=dev)
+            zero_s = torch.zeros((1, 2), dtype=torch.float64, device=dev)
+            for g in range(len(energies)):
+                sel = torch.as_tensor(np.nonzero(wave_group == g)[0], device=dev)
+                pk_g = probe_k.index_select(0, sel).contiguous()
+                ones = torch.ones(pk_g.shape[0], dtype=torch.float32, device=dev)
+                _, amp_g = engine.multislice_forward(slabs[g], pk_g, ones, zero_o, zero_s,
+                                                     (props[g, :h].contiguous(), props[g, h:].contiguous()),
+                                                     want_patterns=False, want_amplitude=True)
+                amps.index_copy_(0, sel, amp_g[0])
+            patterns = amps.abs().square() if not coherent else None
         if coherent:
-            a = amps.to(torch.complex128).reshape(pc, *counts, n, n)
+            a = amps.to(torch.complex128).reshape(pc, *counts, h, w)
             out[p0:p0 + pc] = torch.stack([_reduce_one(axes, a[i]) for i in range(pc)])
         else:
             out[p0:p0 + pc] = torch.einsum("k,pkhw->phw", weights,
-                                           patterns.to(torch.float64).reshape(pc, k_count, n, n))
+                                           patterns.to(torch.float64).reshape(pc, k_count, h, w))
     return out
 
 
@@ -542,16 +565,16 @@ def stem_4d_windows(slab_t: torch.Tensor, probe: torch.Tensor, origins: torch.Te
 
     The pattern at position p equals ``cbed_image`` on the periodic window of
     ``slab_t`` (S, Hs, Ws complex64) at integer ``origins[p]`` with the probe
-    (n, n) or (M, n, n) shifted by ``shifts_px[p]`` pixels.  Returns
-    (P, n, n) float32 fftshifted patterns.
+    (h, w) or (M, h, w) shifted by ``shifts_px[p]`` pixels.  Returns
+    (P, h, w) float32 fftshifted patterns.
     """
     dev = slab_t.device
     pr = probe if probe.ndim == 3 else probe[None]
     pk = torch.fft.fft2(pr.to(device=dev, dtype=torch.complex128)).to(torch.complex64).contiguous()
-    m_count, n = pk.shape[0], pk.shape[-1]
+    m_count = pk.shape[0]
     w = torch.full((m_count,), 1.0 / m_count, dtype=torch.float32, device=dev) if weights is None else \
         (weights / weights.sum()).to(device=dev, dtype=torch.float32).contiguous()
-    prop = physics.propagator_factors(n, slice_thickness, voltage_kv, calib_ang, device=dev)
+    prop = physics.propagator_pair(pk.shape[-2], pk.shape[-1], slice_thickness, voltage_kv, calib_ang, device=dev)
     patterns, _ = engine.multislice_forward(slab_t.contiguous(), pk, w, origins.to(dev, torch.int32).contiguous(),
                                             shifts_px.to(dev, torch.float64).contiguous(), prop)
     return patterns

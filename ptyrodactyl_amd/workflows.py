@@ -3,9 +3,10 @@
 Mirrors ``src/ptyrodactyl/workflows/stem_4d.py``: ``crystal2stem4d``
 (``:191-550``) and ``crystal2stem4d_tiled`` (``:553-1007``) with the same
 arguments and defaults, and the memory estimate ``_estimate_memory_gb``
-(``:57-138``).  The multislice runs in this package's HIP kernels, whose
-windows are square powers of two from 32 to 4096 pixels; a workflow whose grid
-falls outside that set raises ``ValueError`` (choose the pixel size, padding
+(``:57-138``).  The multislice runs in this package's HIP kernels, which take
+grids of any shape with edges up to 4096 pixels (square powers of two on the
+dedicated kernels, every other shape on the runtime-size ones); a workflow
+whose grid exceeds that raises ``ValueError`` (choose the pixel size, padding
 or ``grid_pixels`` / ``fourier_pixels`` accordingly).  ``force_parallel``
 is accepted for signature compatibility: this build runs on one GPU, where the
 reference's parallel and serial branches compute the same patterns.
@@ -31,7 +32,9 @@ from .simulations import _device_for, make_probe, probe_modes_to_distribution
 from .types import (STEM4D, CrystalData, _as_tensor, create_detector_config, create_microscope_config,
                     create_probe_modes, create_stem4d)
 
+# square grids that run on the dedicated power-of-two kernels; any other h x w up to GRID_MAX_EDGE also runs
 SUPPORTED_GRIDS = engine.SUPPORTED_N
+GRID_MAX_EDGE = engine.GRID_MAX_EDGE
 LARGE_POSITION_THRESHOLD = 100
 _POTENTIAL_EXTENT_ANG = 10.0
 
@@ -116,9 +119,9 @@ def _element_types(numbers: torch.Tensor):
 
 
 def _check_grid(name: str, h: int, w: int) -> None:
-    if h != w or h not in SUPPORTED_GRIDS:
-        raise ValueError(f"{name}: the multislice grid is {h} x {w} pixels; the GPU kernels take square grids of "
-                         f"{SUPPORTED_GRIDS} pixels")
+    if not (1 <= h <= GRID_MAX_EDGE and 1 <= w <= GRID_MAX_EDGE):
+        raise ValueError(f"{name}: the multislice grid is {h} x {w} pixels; the GPU kernels take edges of 1 to "
+                         f"{GRID_MAX_EDGE} pixels")
 
 
 def _probe_modes(microscope, detector, num_modes: int, pixel: float, device):

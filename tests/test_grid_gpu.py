@@ -1,0 +1,245 @@
+"""GPU parity of the runtime-size multislice kernels (any h x w grid) against the complex128 oracle.
+
+The reference's multislice takes any grid shape (jnp.fft.fft2 in
+``src/ptyrodactyl/multislice/simulations.py:501-603``; its tests run 8 x 8, 15 x 32 and 31 x 32
+grids).  Shapes other than the square powers of two run on ``pty_forward_grid`` /
+``pty_adjoint_grid``.  Tolerances: relative L2 <= 1e-5 for patterns, amplitudes and the slab,
+probe and shift gradients (complex64 kernels with FP32 sums against the complex128 oracle);
+<= 1e-6 where two GPU paths compute the same quantity.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import ensemble_oracle as eo
+from oracle import ms_oracle as o
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = 1e-5
+DEV = "cuda"
+
+SHAPES = [(8, 8), (15, 32), (31, 32), (48, 40), (1, 12), (9, 1), (97, 61), (100, 100), (243, 200)]
+
+
+def rel_l2(a, b):
+    """Relative L2 error; complex arrays keep their imaginary parts."""
+    a, b = np.asarray(a), np.asarray(b)
+    a = a.astype(np.result_type(a, np.float64))
+    b = b.astype(np.result_type(b, np.float64))
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _case(h, w, s_count, p_count, seed, modes=1):
+    rng = np.random.default_rng(seed)
+    hs, ws = h + 5, w + 3
+    slab = (rng.uniform(0.95, 1.0, (s_count, hs, ws)) * np.exp(1j * rng.uniform(0, 0.4, (s_count, hs, ws))))
+    slab = slab.astype(np.complex64)
+    probes = (rng.standard_normal((modes, h, w)) + 1j * rng.standard_normal((modes, h, w))) / np.sqrt(h * w)
+    origins = np.stack([rng.integers(0, hs, p_count), rng.integers(0, ws, p_count)], 1).astype(np.int32)
+    shifts = rng.uniform(-3.0, 3.0, (p_count, 2))
+    weights = rng.uniform(0.2, 1.0, modes)
+    return slab, probes, origins, shifts, weights / weights.sum()
+
+
+def _operands(slab, probes, origins, shifts, weights, dz=1.5, kv=300.0, calib=0.1):
+    from ptyrodactyl_amd import physics
+    h, w = probes.shape[-2:]
+    pk = torch.fft.fft2(torch.tensor(probes, device=DEV)).to(torch.complex64).contiguous()
+    return (torch.tensor(slab, device=DEV), pk, torch.tensor(weights, dtype=torch.float32, device=DEV),
+            torch.tensor(origins, device=DEV), torch.tensor(shifts, device=DEV),
+            physics.propagator_pair(h, w, dz, kv, calib, device=DEV))
+
+
+def _oracle_prop(h, w, dz=1.5, kv=300.0, calib=0.1):
+    return o.propagation_func(h, w, dz, kv, calib)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_grid_forward_matches_oracle(shape):
+    from ptyrodactyl_amd import engine
+    h, w = shape
+    modes = 2 if h * w <= 4096 else 1
+    slab, probes, origins, shifts, weights = _case(h, w, 4, 3, seed=h * 1000 + w, modes=modes)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    assert not engine.is_fast_grid(h, w)
+    pat, amp = engine.multislice_forward(*ops, want_amplitude=True)
+    torch.cuda.synchronize()
+    prop = _oracle_prop(h, w)
+    want_pat, want_amp = 0, []
+    for m in range(modes):
+        p_m, a_m = o.window_stem(slab.astype(np.complex128), probes[m], origins, shifts, prop, return_amplitude=True)
+        want_pat = want_pat + weights[m] * p_m
+        want_amp.append(a_m)
+    want_amp = np.stack(want_amp, axis=1)
+    e_pat, e_amp = rel_l2(pat.cpu().numpy(), want_pat), rel_l2(amp.cpu().numpy(), want_amp)
+    print(f"grid forward {h}x{w} (4 slices, 3 positions, {modes} modes): patterns {e_pat:.2e}, amplitudes "
+          f"{e_amp:.2e}", flush=True)
+    assert pat.shape == (3, h, w) and amp.shape == (3, modes, h, w)
+    assert e_pat < TOL and e_amp < TOL
+
+
+def test_grid_kernels_equal_power_of_two_kernels_on_square_windows():
+    """The runtime-size kernels and the dedicated ones compute the same patterns on a 64 x 64 window."""
+    from ptyrodactyl_amd import engine
+    slab, probes, origins, shifts, weights = _case(64, 64, 5, 6, seed=7, modes=2)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    fast, _ = engine.multislice_forward(*ops)
+    grid, _, _ = engine._grid_forward(*ops, True, False, None, 0)
+    torch.cuda.synchronize()
+    err = rel_l2(grid.cpu().numpy(), fast.cpu().numpy())
+    print(f"grid vs power-of-two kernels at 64x64: rel-L2 {err:.2e}", flush=True)
+    assert err < 1e-6
+
+
+def test_grid_checkpointed_forward_equals_plain_and_stores_entering_waves():
+    from ptyrodactyl_amd import engine
+    h, w, s_count, k = 15, 32, 7, 3
+    slab, probes, origins, shifts, weights = _case(h, w, s_count, 4, seed=11, modes=2)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    plain, _ = engine.multislice_forward(*ops)
+    pat, ckpt = engine.multislice_forward_checkpointed(*ops, k)
+    torch.cuda.synchronize()
+    assert torch.equal(plain, pat)
+    assert ckpt.shape == (4 * 2, 2, h, w)
+    # checkpoint j of wave (p, m) = the wave entering slice (j + 1) k: the oracle's multislice of the first
+    # (j + 1) k slices before the final transform
+    prop = _oracle_prop(h, w)
+    probe_k = np.fft.fft2(probes.astype(np.complex128))
+    for p in range(4):
+        for m in range(2):
+            ramp = o.ramp_1d(h, shifts[p, 0])[:, None] * o.ramp_1d(w, shifts[p, 1])[None, :]
+            wave = np.fft.ifft2(probe_k[m] * ramp)
+            t_win = o.window_of(slab.astype(np.complex128), origins[p], (h, w))
+            for s in range(2 * k):
+                wave = np.fft.ifft2(np.fft.fft2(wave * t_win[s]) * prop)
+                if (s + 1) % k == 0:
+                    got = ckpt[p * 2 + m, (s + 1) // k - 1].cpu().numpy()
+                    assert rel_l2(got, wave) < TOL
+
+
+@pytest.mark.parametrize("shape,k,modes", [((8, 8), 2, 1), ((15, 32), 3, 2), ((31, 32), 4, 1), ((1, 12), 2, 1),
+                                           ((97, 61), 2, 1), ((100, 100), 4, 2)])
+def test_grid_adjoint_matches_oracle(shape, k, modes):
+    from ptyrodactyl_amd import engine
+    h, w = shape
+    s_count, p_count = 5, 3
+    slab, probes, origins, shifts, weights = _case(h, w, s_count, p_count, seed=31 * h + w, modes=modes)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    g = np.random.default_rng(5).standard_normal((p_count, h, w))
+    _, ckpt = engine.multislice_forward_checkpointed(*ops, k)
+    g_shift = torch.zeros((p_count, 2), dtype=torch.float64, device=DEV)
+    g_slab, g_pk = engine.multislice_adjoint(*ops, torch.tensor(g, dtype=torch.float32, device=DEV), ckpt, k,
+                                             grad_shifts=g_shift)
+    # probe_k = fft2(probe)  =>  dL/dprobe = F^H dL/dprobe_k = h w ifft2(.)
+    g_probe = (torch.fft.ifft2(g_pk.to(torch.complex128)) * (h * w)).cpu().numpy()
+    torch.cuda.synchronize()
+    prop = _oracle_prop(h, w)
+    want_slab, want_shift, want_probe = 0, 0, []
+    for m in range(modes):
+        gs, gp, gsh = o.multislice_vjp(slab.astype(np.complex128), probes[m], origins, shifts, prop,
+                                       weights[m] * g, want_shifts=True)
+        want_slab, want_shift = want_slab + gs, want_shift + gsh
+        want_probe.append(gp)
+    errs = (rel_l2(g_slab.cpu().numpy(), want_slab), rel_l2(g_probe, np.stack(want_probe)),
+            rel_l2(g_shift.cpu().numpy(), want_shift))
+    print(f"grid adjoint {h}x{w} (k={k}, {modes} modes): slab {errs[0]:.2e}, probe {errs[1]:.2e}, "
+          f"shifts {errs[2]:.2e}", flush=True)
+    assert max(errs) < TOL
+
+
+def test_grid_amplitude_seed_equals_intensity_seed():
+    """dL/dI = g seeds the same gradient as dL/dA = 2 g A (one mode of weight 1)."""
+    from ptyrodactyl_amd import engine
+    h, w, k = 24, 36, 2
+    slab, probes, origins, shifts, _ = _case(h, w, 4, 3, seed=13)
+    ops = _operands(slab, probes, origins, shifts, np.ones(1))
+    g = torch.tensor(np.random.default_rng(6).standard_normal((3, h, w)), dtype=torch.float32, device=DEV)
+    _, amp = engine.multislice_forward(*ops, want_patterns=False, want_amplitude=True)
+    _, ckpt = engine.multislice_forward_checkpointed(*ops, k)
+    a_slab, a_pk = engine.multislice_adjoint(*ops, None, ckpt, k, grad_amplitude=(2.0 * g[:, None] * amp).contiguous())
+    i_slab, i_pk = engine.multislice_adjoint(*ops, g, ckpt, k)
+    torch.cuda.synchronize()
+    e1 = rel_l2(a_slab.cpu().numpy(), i_slab.cpu().numpy())
+    e2 = rel_l2(a_pk.cpu().numpy(), i_pk.cpu().numpy())
+    print(f"amplitude vs intensity seed {h}x{w}: slab {e1:.2e}, probe {e2:.2e}", flush=True)
+    assert e1 < 1e-6 and e2 < 1e-6
+
+
+def test_dropin_stem4d_and_cbed_on_non_square_grids():
+    import ptyrodactyl_amd as pa
+    rng = np.random.default_rng(17)
+    h, w, calib, kv = 24, 40, 0.25, 200.0
+    pot = rng.uniform(0, 10, (h, w, 3))
+    probe = o.make_probe(kv, 24.0, 4.0, 0.0, 0.0, (h, w), calib * 100)
+    pos = np.array([[0.7, 1.9], [-2.0, 0.33], [3.1, -0.45]])
+    mic = pa.create_microscope_config(voltage_kv=kv, aperture_mrad=24.0, defocus_ang=4.0, probe_shape=(h, w))
+    det = pa.create_detector_config(real_space_calib_ang=calib, scan_positions_px=torch.tensor(pos))
+    beam = pa.create_probe_modes(torch.tensor(probe[..., None], device=DEV), torch.tensor([1.0]), calib)
+    made = pa.make_probe(mic, pa.create_detector_config(real_space_calib_ang=calib)).cpu().numpy()
+    assert rel_l2(made, probe) < 1e-12
+    pots = pa.create_potential_slices(torch.tensor(pot, device=DEV), 1.1, calib)
+    got = pa.stem_4d(pots, beam, mic, det)
+    want = o.stem_4d(pot, 1.1, calib, probe[..., None], [1.0], calib, kv, pos, calib)
+    err = rel_l2(got.data.cpu().numpy(), want["data"])
+    print(f"drop-in stem_4d {h}x{w}: rel-L2 {err:.2e}", flush=True)
+    assert got.data.shape == (3, h, w) and err < TOL
+    assert got.fourier_space_calib == pytest.approx(want["fourier_space_calib"], rel=1e-14)
+    cb = pa.cbed_image(pots, beam, mic)
+    want_cb, fcal = o.cbed_image(pot, 1.1, calib, probe[..., None], [1.0], calib, kv)
+    assert rel_l2(cb.data_array.cpu().numpy(), want_cb) < TOL
+    assert cb.calib_y == pytest.approx(fcal, rel=1e-14)
+
+
+def test_dropin_gradient_wrt_potential_on_odd_grid():
+    """dL/dV through stem_4d on a 21 x 27 grid against central differences of the complex128 oracle."""
+    import ptyrodactyl_amd as pa
+    rng = np.random.default_rng(19)
+    h, w, calib, kv = 21, 27, 0.25, 200.0
+    pot = rng.uniform(0, 30, (h, w, 3))
+    probe = o.make_probe(kv, 25.0, 10.0, 0.0, 0.0, (h, w), calib * 100)
+    pos = np.array([[1.0, 2.0], [3.5, 0.5]])
+    mic = pa.create_microscope_config(voltage_kv=kv, aperture_mrad=25.0, defocus_ang=10.0, probe_shape=(h, w))
+    det = pa.create_detector_config(real_space_calib_ang=calib, scan_positions_px=torch.tensor(pos))
+    beam = pa.create_probe_modes(torch.tensor(probe[..., None], device=DEV), torch.tensor([1.0]), calib)
+    wts = rng.standard_normal((2, h, w))
+    v = torch.tensor(pot, device=DEV, requires_grad=True)
+    out = pa.stem_4d(pa.create_potential_slices(v, 1.5, calib), beam, mic, det)
+    (g_v,) = torch.autograd.grad(torch.sum(out.data.double() * torch.tensor(wts, device=DEV)), v)
+    g_v = g_v.cpu().numpy()
+
+    def oracle_loss(p):
+        return float(np.sum(wts * o.stem_4d(p, 1.5, calib, probe[..., None], [1.0], calib, kv, pos, calib)["data"]))
+
+    eps = 1e-3
+    worst = 0.0
+    for idx in [(3, 4, 0), (10, 13, 1), (20, 26, 2), (0, 0, 1)]:
+        pp, pm = pot.copy(), pot.copy()
+        pp[idx] += eps
+        pm[idx] -= eps
+        fd = (oracle_loss(pp) - oracle_loss(pm)) / (2 * eps)
+        worst = max(worst, abs(g_v[idx] - fd) / (abs(fd) + 1e-3 * np.abs(g_v).max()))
+    print(f"drop-in dL/dV {h}x{w}: worst relative difference to oracle FD {worst:.2e}", flush=True)
+    assert worst < 1e-3
+
+
+def test_ensemble_cbed_image_on_non_square_grid():
+    from ptyrodactyl_amd import producers
+    import ptyrodactyl_amd as pa
+    rng = np.random.default_rng(23)
+    h, w, calib, kv = 20, 24, 0.3, 120.0
+    pot = rng.uniform(0, 25, (h, w, 3))
+    modes = np.stack([o.make_probe(kv, 22.0, 20.0 + 15 * m, 0.0, 0.0, (h, w), calib * 100.0) for m in range(2)], 2)
+    weights = np.array([0.6, 0.4])
+    ens = pa.create_ensemble_axes(position_jitter=producers.position_jitter_to_distribution(0.25, 2),
+                                  coherence=producers.coherence_to_distribution(0.8, 0.3, 2))
+    mic = pa.create_microscope_config(kv, 22.0, ensemble=ens)
+    pots = pa.create_potential_slices(torch.tensor(pot, device=DEV), 1.2, calib)
+    beam = pa.create_probe_modes(torch.tensor(modes, device=DEV), torch.tensor(weights), calib)
+    got = pa.cbed_image(pots, beam, mic).data_array.cpu().numpy()
+    want = eo.cbed_image(pot, 1.2, calib, modes, weights, calib, kv, eo.position_jitter(0.25, 2),
+                         eo.coherence(0.8, 0.3, 2))
+    err = rel_l2(got, want)
+    print(f"ensemble cbed_image {h}x{w} (2 modes x 4 jitter x 8 coherence): rel-L2 {err:.2e}", flush=True)
+    assert got.shape == (h, w) and err < TOL

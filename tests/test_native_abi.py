@@ -81,3 +81,31 @@ def test_potential_builders_validate_on_host():
     assert b"num_atoms" in lib.pty_last_error()
     assert lib.pty_potential_slices_fourier(2, 1, 8, 8, 3, None, None, None, None, None) == _native.PTY_ERR_INVALID
     assert b"null pointer" in lib.pty_last_error()
+
+
+def test_grid_entry_points_validate_on_host():
+    """Runtime-size entry points: edge limits, seed exclusivity and workspace sizes, checked before any launch."""
+    from ptyrodactyl_amd import _native
+    if not _native.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    lib = _native.load()
+    args = (1, 1, 1, None, 100, 100, None, None, None, None, None, None)
+    rc = lib.pty_forward_grid(5000, 100, *args, None, None, None, 1, None, 0, None)
+    assert rc == _native.PTY_ERR_INVALID and b"each edge must be in [1, 4096]" in lib.pty_last_error()
+    rc = lib.pty_forward_grid(0, 100, *args, None, None, None, 1, None, 0, None)
+    assert rc == _native.PTY_ERR_INVALID
+    rc = lib.pty_forward_grid(100, 100, *args, None, None, None, 1, None, 0, None)
+    assert rc == _native.PTY_ERR_INVALID and b"null input pointer" in lib.pty_last_error()
+    # one forward workgroup holds one h x w complex64 wave; the adjoint k + 2 of them (k clamped to S)
+    assert lib.pty_grid_workspace_bytes(15, 32, 8, 4, 1, 0) == 15 * 32 * 8
+    assert lib.pty_grid_workspace_bytes(15, 32, 8, 4, 3, 1) == 3 * 15 * 32 * 8 * 6
+    assert lib.pty_grid_workspace_bytes(15, 32, 2, 4, 1, 1) == 15 * 32 * 8 * 4
+    assert lib.pty_grid_workspace_bytes(15, 32, 8, 4, 10 ** 6, 0) == 2048 * 15 * 32 * 8
+    assert lib.pty_grid_workspace_bytes(4097, 32, 8, 4, 1, 0) == 0
+    fake = 16  # any non-null address: validation fails before it is used
+    rc = lib.pty_adjoint_grid(15, 32, 1, 1, 1, fake, 15, 32, fake, fake, fake, fake, fake, fake, fake, fake, None, 1,
+                              fake, None, None, fake, 1 << 20, None)
+    assert rc == _native.PTY_ERR_INVALID and b"exactly one of grad_patterns and grad_amplitude" in lib.pty_last_error()
+    rc = lib.pty_adjoint_grid(15, 32, 1, 1, 1, fake, 15, 32, fake, fake, fake, fake, fake, fake, fake, None, None, 1,
+                              fake, None, None, fake, 100, None)
+    assert rc == _native.PTY_ERR_INVALID and b"workspace" in lib.pty_last_error()

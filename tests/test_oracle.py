@@ -344,3 +344,98 @@ def test_golden_stem4d_sharded_matches_reference_capture():
     pos = np.array([[12.0, 12.0], [12.0, 20.0], [20.0, 12.0], [20.0, 20.0]]) * 0.2
     got = o.stem4d_from_atoms(probe[..., None], [1.0], coords, types, bounds, kern, 100.0, 0.2, pos)
     assert np.max(np.abs(got - ref["stem4d_sharded"])) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Non-square and odd grids: the windowed oracle on h x w windows
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape", [(15, 32), (7, 5), (12, 9)])
+def test_window_stem_equals_reference_stem4d_on_non_square_grids(shape):
+    """The windowed restatement on an h x w window = the reference-level stem_4d (shape-generic numpy) at the
+    same positions; odd edges exercise the fftfreq sign convention of the shift ramp."""
+    rng = np.random.default_rng(21)
+    h, w = shape
+    calib, kv = 0.25, 200.0
+    pot = rng.uniform(0, 10, (h, w, 3))
+    probe = o.make_probe(kv, 24.0, 4.0, 0.0, 0.0, (h, w), calib * 100)
+    pos = np.array([[0.7, 1.9], [-2.0, 0.33], [3.1, -0.45]])
+    ref = o.stem_4d(pot, 1.1, calib, probe[..., None], [1.0], calib, kv, pos, calib)["data"]
+    slab = np.stack([o.transmission_func(pot[..., s], kv) for s in range(3)])
+    prop = o.propagation_func(h, w, 1.1, kv, calib)
+    got = o.window_stem(slab, probe, np.zeros((3, 2), int), pos / calib, prop)
+    assert got.shape == (3, h, w)
+    assert np.allclose(ref, got, atol=1e-12)
+
+
+def test_bruteforce_dft_multislice_non_square():
+    """DFT-matrix multislice on a 6 x 9 grid, propagator from its own formula, shift applied as a DFT phase."""
+    rng = np.random.default_rng(22)
+    h, w, s_count, dz, kv, calib = 6, 9, 3, 1.3, 160.0, 0.35
+    slab = np.exp(1j * rng.uniform(0, 1.0, (s_count, h, w)))
+    probe = rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))
+    shift = (0.6, -1.3)
+    lam = o.wavelength_ang(kv)
+    ky = np.array([k if k < (h + 1) // 2 else k - h for k in range(h)])
+    kx = np.array([k if k < (w + 1) // 2 else k - w for k in range(w)])
+    prop = np.exp(-1j * np.pi * lam * dz * ((ky / (h * calib))[:, None] ** 2 + (kx / (w * calib))[None, :] ** 2))
+    fy, fx = _dft_matrix(h, -1), _dft_matrix(w, -1)
+    fyi, fxi = _dft_matrix(h, 1) / h, _dft_matrix(w, 1) / w
+    ramp = np.exp(-2j * np.pi * (ky[:, None] * shift[0] / h + kx[None, :] * shift[1] / w))
+    wave = fyi @ ((fy @ probe @ fx.T) * ramp) @ fxi.T
+    for s in range(s_count):
+        wave = wave * slab[s]
+        if s != s_count - 1:
+            wave = fyi @ ((fy @ wave @ fx.T) * prop) @ fxi.T
+    amp = fy @ wave @ fx.T
+    expected = np.abs(np.roll(amp, (h // 2, w // 2), axis=(0, 1))) ** 2
+    got = o.window_stem(slab, probe, np.zeros((1, 2), int), np.array([shift]),
+                        o.propagation_func(h, w, dz, kv, calib))[0]
+    assert np.allclose(got, expected, atol=1e-10)
+
+
+def _small_problem_hw(seed, h, w, s_count=3, p_count=3):
+    rng = np.random.default_rng(seed)
+    slab = np.exp(1j * rng.uniform(0, 1.0, (s_count, h + 3, w + 4))) * rng.uniform(0.9, 1.0, (s_count, h + 3, w + 4))
+    probe = rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))
+    origins = rng.integers(0, 12, (p_count, 2))
+    shifts = rng.uniform(-2, 2, (p_count, 2))
+    prop = o.propagation_func(h, w, 1.3, 200.0, 0.3)
+    c = rng.standard_normal((p_count, h, w))
+    return slab, probe, origins, shifts, prop, c
+
+
+def test_vjp_non_square_matches_central_differences():
+    slab, probe, origins, shifts, prop, c = _small_problem_hw(23, 5, 7)
+    g_slab, g_probe, g_shift = o.multislice_vjp(slab, probe, origins, shifts, prop, c, want_shifts=True)
+
+    def loss(sl, pr, sh):
+        return float(np.sum(c * o.window_stem(sl, pr, origins, sh, prop)))
+
+    eps = 1e-6
+    rng = np.random.default_rng(24)
+    for _ in range(6):
+        idx = tuple(rng.integers(0, d) for d in slab.shape)
+        for direction in (1.0, 1j):
+            sp, sm = slab.copy(), slab.copy()
+            sp[idx] += eps * direction
+            sm[idx] -= eps * direction
+            fd = (loss(sp, probe, shifts) - loss(sm, probe, shifts)) / (2 * eps)
+            expected = g_slab[idx].real if direction == 1.0 else g_slab[idx].imag
+            assert fd == pytest.approx(expected, rel=1e-5, abs=1e-7)
+    for _ in range(4):
+        idx = tuple(rng.integers(0, d) for d in probe.shape)
+        for direction in (1.0, 1j):
+            pp, pm = probe.copy(), probe.copy()
+            pp[idx] += eps * direction
+            pm[idx] -= eps * direction
+            fd = (loss(slab, pp, shifts) - loss(slab, pm, shifts)) / (2 * eps)
+            expected = g_probe[idx].real if direction == 1.0 else g_probe[idx].imag
+            assert fd == pytest.approx(expected, rel=1e-5, abs=1e-7)
+    for p in range(len(origins)):
+        for ax in range(2):
+            sp, sm = shifts.copy(), shifts.copy()
+            sp[p, ax] += eps
+            sm[p, ax] -= eps
+            fd = (loss(slab, probe, sp) - loss(slab, probe, sm)) / (2 * eps)
+            assert fd == pytest.approx(g_shift[p, ax], rel=1e-5, abs=1e-7)

@@ -61,7 +61,8 @@ def test_workflow_argument_validation():
     with pytest.raises(ValueError, match=r"fourier_pixels \(64\) must be >= grid_pixels \(128\)"):
         pa.crystal2stem4d_tiled(crystal, [[1.0, 1.0]], 300.0, 20.0, tile_size_ang=5.0, grid_pixels=128,
                                 pixel_size_ang=0.1, fourier_pixels=64)
-    with pytest.raises(ValueError, match="the GPU kernels take square grids"):
+    with pytest.raises(ValueError, match="the GPU kernels take edges of 1 to 4096"):
         pa.crystal2stem4d_tiled(crystal, [[1.0, 1.0]], 300.0, 20.0, grid_pixels=5000)   # FFT grid 5000
-    with pytest.raises(ValueError, match="the GPU kernels take square grids"):
-        pa.crystal2stem4d(crystal, [[1.0, 1.0]], 300.0, 20.0)          # 0.02 A default: 550 x 500 pixels
+    with pytest.raises(ValueError, match="the GPU kernels take edges of 1 to 4096"):
+        pa.crystal2stem4d(crystal, [[1.0, 1.0]], 300.0, 20.0,
+                          real_space_pixel_size_ang=0.002)             # 5500 x 5000 pixels

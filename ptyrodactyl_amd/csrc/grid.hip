@@ -14,11 +14,13 @@
 // transform), the same factorisation as the power-of-two streaming kernels.
 //
 // Stage of radix r after the radices so far multiply to ns (Stockham
-// autosort, one output per thread):
+// autosort):
 //   out[e] = sum_q in[j + q L / r] * w^(q (lo + k ns)),  w = exp(-+2 pi i / (ns r)),
 //   with lo = e mod ns, k = (e / ns) mod r, j = (e / (ns r)) ns + lo.
-// Twiddles come from an exact table exp(-2 pi i e / L) built in FP64, so no
-// error accumulates across stages beyond the FP32 sums.
+// Radices 2 to 5 run one butterfly (all r outputs of one j) per thread with
+// constant-coefficient r-point DFTs; larger prime radices one output per
+// thread.  Twiddles come from an exact table exp(-2 pi i e / L) built in
+// FP64, so no error accumulates across stages beyond the FP32 sums.
 #include <math.h>
 
 #include <algorithm>
@@ -51,6 +53,38 @@ __device__ __forceinline__ float2 gmulc(float2 a, float2 b) {
 }
 __device__ __forceinline__ float2 gscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
+// q = x / d, r = x mod d for 0 <= x < 2^24 and d >= 1: the float estimate x * (1/d) is within one
+// of the quotient, and the remainder test corrects it.
+__device__ __forceinline__ void divmod(int x, int d, float rd, int& q, int& r) {
+  q = (int)((float)x * rd);
+  r = x - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+}
+
+// (major, minor) of o = major * mod + minor, stepped by the workgroup size without divisions.
+struct Walk {
+  int major, minor, dmaj, dmin, mod;
+  __device__ Walk(int o, int m) : mod(m) {
+    const float rm = 1.0f / (float)m;
+    divmod(o, m, rm, major, minor);
+    divmod(GT, m, rm, dmaj, dmin);
+  }
+  __device__ __forceinline__ void step() {
+    major += dmaj;
+    minor += dmin;
+    if (minor >= mod) {
+      minor -= mod;
+      ++major;
+    }
+  }
+};
+
 // fftfreq bin of index kidx (Nyquist negative for even L, as numpy)
 __device__ __forceinline__ int signed_bin(int kidx, int L) { return kidx < (L + 1) / 2 ? kidx : kidx - L; }
 
@@ -72,46 +106,151 @@ __device__ __forceinline__ float2 ramp1(int kidx, int L, int m, float d) {
   return make_float2(c, s);
 }
 
+// In-register DFT of R = 2, 3, 4 or 5 points, y_k = sum_q x_q exp(-+2 pi i q k / R) (INV: +).
+template <int R, bool INV>
+__device__ __forceinline__ void small_dft(float2* v) {
+  if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = make_float2(a.x + b.x, a.y + b.y);
+    v[1] = make_float2(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    const float2 s02 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const float2 d02 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const float2 d13 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+    // forward: y1 = d02 - i d13, y3 = d02 + i d13 (inverse swaps them)
+    const float2 mi = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+    v[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+    v[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+    v[1] = make_float2(d02.x + mi.x, d02.y + mi.y);
+    v[3] = make_float2(d02.x - mi.x, d02.y - mi.y);
+  } else if constexpr (R == 3) {
+    constexpr float S3 = 0.86602540378443865f;  // sin(2 pi / 3)
+    const float2 s12 = make_float2(v[1].x + v[2].x, v[1].y + v[2].y);
+    const float2 d12 = make_float2(v[1].x - v[2].x, v[1].y - v[2].y);
+    const float2 t = make_float2(v[0].x - 0.5f * s12.x, v[0].y - 0.5f * s12.y);
+    // forward: u = -i sin(2 pi/3) (x1 - x2)
+    const float2 u = INV ? make_float2(-S3 * d12.y, S3 * d12.x) : make_float2(S3 * d12.y, -S3 * d12.x);
+    v[0] = make_float2(v[0].x + s12.x, v[0].y + s12.y);
+    v[1] = make_float2(t.x + u.x, t.y + u.y);
+    v[2] = make_float2(t.x - u.x, t.y - u.y);
+  } else {
+    static_assert(R == 5, "radix");
+    // cos and sin of 2 pi m / 5, m = 0..4
+    constexpr float C[5] = {1.0f, 0.30901699437494745f, -0.80901699437494745f, -0.80901699437494745f,
+                            0.30901699437494745f};
+    constexpr float S[5] = {0.0f, 0.95105651629515357f, 0.58778525229247314f, -0.58778525229247314f,
+                            -0.95105651629515357f};
+    float2 y[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      float re = 0.f, im = 0.f;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const int m = (q * k) % 5;
+        const float c = C[m], sn = INV ? S[m] : -S[m];
+        re += v[q].x * c - v[q].y * sn;
+        im += v[q].x * sn + v[q].y * c;
+      }
+      y[k] = make_float2(re, im);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = y[k];
+  }
+}
+
+// One Stockham stage of radix R in {2, 3, 4, 5}: a thread takes a butterfly (line, j), j < L / R,
+// twiddles its R inputs by w^(q lo) (w = exp(-+2 pi i / (ns R)), lo = j mod ns), runs the R-point
+// DFT and writes outputs (j / ns) ns R + lo + k ns.
+template <int R, bool INV>
+__device__ void butterfly_stage(const float2* a, float2* b, int nl, int L, int ls, int ns, const float2* tw) {
+  const int nb = L / R;
+  const int tstep = L / (ns * R);
+  const int total = nl * nb;
+  const float rns = 1.0f / (float)ns;
+  Walk wk(threadIdx.x, nb);
+  for (int t = threadIdx.x; t < total; t += GT, wk.step()) {
+    const int line = wk.major;
+    const int j = wk.minor;
+    int jb, lo;
+    divmod(j, ns, rns, jb, lo);
+    const float2* src = a + line * ls + j;
+    float2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const float2 x = src[q * nb];
+      const float2 w = tw[q * lo * tstep];
+      const float wi = INV ? -w.y : w.y;
+      v[q] = make_float2(x.x * w.x - x.y * wi, x.x * wi + x.y * w.x);
+    }
+    small_dft<R, INV>(v);
+    float2* dst = b + line * ls + jb * ns * R + lo;
+#pragma unroll
+    for (int k = 0; k < R; ++k) dst[k * ns] = v[k];
+  }
+}
+
+// Stage of any radix r, one output per thread:
+//   out[e] = sum_q in[j + q L / r] w^(q (lo + k ns)), lo = e mod ns, k = (e / ns) mod r,
+//   j = (e / (ns r)) ns + lo.
+__device__ void generic_stage(const float2* a, float2* b, int nl, int L, int ls, int ns, int r, const float2* tw,
+                              bool inv) {
+  const int mr = ns * r;
+  const int tstep = L / mr;
+  const int lr = L / r;
+  const int total = nl * L;
+  const float rns = 1.0f / (float)ns, rr = 1.0f / (float)r;
+  Walk wk(threadIdx.x, L);
+  for (int o = threadIdx.x; o < total; o += GT, wk.step()) {
+    const int line = wk.major;
+    const int e = wk.minor;
+    int hi, lo, jr, k;
+    divmod(e, ns, rns, hi, lo);
+    divmod(hi, r, rr, jr, k);
+    const int j = jr * ns + lo;
+    const int step = lo + k * ns;
+    const float2* src = a + line * ls + j;
+    float re = 0.f, im = 0.f;
+    int ex = 0;
+    for (int q = 0; q < r; ++q) {
+      const float2 x = src[q * lr];
+      const float2 t = tw[ex * tstep];
+      const float ti = inv ? -t.y : t.y;
+      re += x.x * t.x - x.y * ti;
+      im += x.x * ti + x.y * t.x;
+      ex += step;
+      if (ex >= mr) ex -= mr;
+    }
+    b[line * ls + e] = make_float2(re, im);
+  }
+}
+
 // Transforms `nl` lines held at stride ax.ls in `a` (ping-pong with `b`); returns the buffer that
 // holds the result.  Ends with a barrier.
-__device__ float2* fft_lines(float2* a, float2* b, int nl, const Axis& ax, bool inv) {
+template <bool INV>
+__device__ float2* fft_lines_dir(float2* a, float2* b, int nl, const Axis& ax) {
   const int L = ax.L, ls = ax.ls;
-  const int total = nl * L;
   int ns = 1;
   for (int st = 0; st < ax.nst; ++st) {
     const int r = ax.radix[st];
-    const int mr = ns * r;
-    const int tstep = L / mr;
-    const int lr = L / r;
-    for (int o = threadIdx.x; o < total; o += GT) {
-      const int line = o / L;
-      const int e = o - line * L;
-      const int lo = e % ns;
-      const int hi = e / ns;
-      const int k = hi % r;
-      const int j = (hi / r) * ns + lo;
-      const int step = lo + k * ns;
-      const float2* src = a + line * ls + j;
-      float re = 0.f, im = 0.f;
-      int ex = 0;
-      for (int q = 0; q < r; ++q) {
-        const float2 x = src[q * lr];
-        const float2 t = ax.tw[ex * tstep];
-        const float ti = inv ? -t.y : t.y;
-        re += x.x * t.x - x.y * ti;
-        im += x.x * ti + x.y * t.x;
-        ex += step;
-        if (ex >= mr) ex -= mr;
-      }
-      b[line * ls + e] = make_float2(re, im);
+    switch (r) {
+      case 2: butterfly_stage<2, INV>(a, b, nl, L, ls, ns, ax.tw); break;
+      case 3: butterfly_stage<3, INV>(a, b, nl, L, ls, ns, ax.tw); break;
+      case 4: butterfly_stage<4, INV>(a, b, nl, L, ls, ns, ax.tw); break;
+      case 5: butterfly_stage<5, INV>(a, b, nl, L, ls, ns, ax.tw); break;
+      default: generic_stage(a, b, nl, L, ls, ns, r, ax.tw, INV); break;
     }
     __syncthreads();
     float2* t = a;
     a = b;
     b = t;
-    ns = mr;
+    ns *= r;
   }
   return a;
+}
+
+__device__ float2* fft_lines(float2* a, float2* b, int nl, const Axis& ax, bool inv) {
+  return inv ? fft_lines_dir<true>(a, b, nl, ax) : fft_lines_dir<false>(a, b, nl, ax);
 }
 
 enum { NONE = 0, FWD = 1, INV = 2 };
@@ -127,16 +266,13 @@ __device__ void line_pass(int nlines, int g_max, const Axis& ax, float2* bufa, f
   for (int l0 = 0; l0 < nlines; l0 += g_max) {
     const int g = min(g_max, nlines - l0);
     const int total = g * L;
-    for (int o = threadIdx.x; o < total; o += GT) {
-      int line, e;
-      if (COLS) {
-        line = o % g;
-        e = o / g;
-      } else {
-        line = o / L;
-        e = o - line * L;
+    {
+      Walk wk(threadIdx.x, COLS ? g : L);
+      for (int o = threadIdx.x; o < total; o += GT, wk.step()) {
+        const int line = COLS ? wk.minor : wk.major;
+        const int e = COLS ? wk.major : wk.minor;
+        bufa[line * ls + e] = load(l0 + line, e);
       }
-      bufa[line * ls + e] = load(l0 + line, e);
     }
     __syncthreads();
     float2* cur = bufa;
@@ -146,30 +282,22 @@ __device__ void line_pass(int nlines, int g_max, const Axis& ax, float2* bufa, f
       oth = cur == bufa ? bufb : bufa;
     }
     if (has_mid) {
-      for (int o = threadIdx.x; o < total; o += GT) {
-        int line, e;
-        if (COLS) {
-          line = o % g;
-          e = o / g;
-        } else {
-          line = o / L;
-          e = o - line * L;
-        }
+      Walk wk(threadIdx.x, COLS ? g : L);
+      for (int o = threadIdx.x; o < total; o += GT, wk.step()) {
+        const int line = COLS ? wk.minor : wk.major;
+        const int e = COLS ? wk.major : wk.minor;
         cur[line * ls + e] = mid(l0 + line, e, cur[line * ls + e]);
       }
       __syncthreads();
     }
     if (fft2 != NONE) cur = fft_lines(cur, oth, g, ax, fft2 == INV);
-    for (int o = threadIdx.x; o < total; o += GT) {
-      int line, e;
-      if (COLS) {
-        line = o % g;
-        e = o / g;
-      } else {
-        line = o / L;
-        e = o - line * L;
+    {
+      Walk wk(threadIdx.x, COLS ? g : L);
+      for (int o = threadIdx.x; o < total; o += GT, wk.step()) {
+        const int line = COLS ? wk.minor : wk.major;
+        const int e = COLS ? wk.major : wk.minor;
+        store(l0 + line, e, cur[line * ls + e]);
       }
-      store(l0 + line, e, cur[line * ls + e]);
     }
     __syncthreads();
   }

@@ -262,6 +262,59 @@ def potential_bench(steps: int = 3):
         ff._TABLES.update(saved)
 
 
+def grid_bench(steps: int = 3, cases=((100, 100, 16, 1024), (250, 250, 16, 512), (550, 500, 16, 128))):
+    """Forward and gradient on grids that are not square powers of two (the runtime-size kernels): per case
+    (h, w, slices, positions) a random transmission slab 64 pixels larger than the window, one probe mode,
+    random origins and shifts.  Algorithmic bytes per pattern as for the headline: S h w 8 + h w 8 + h w 4;
+    the gradient step (checkpoints every 4 slices, adjoint with slab, probe and shift gradients) is timed at the
+    first case."""
+    import torch
+    from ptyrodactyl_amd import engine, physics
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(0)
+    out = []
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    for i, (h, w, s_count, p_count) in enumerate(cases):
+        gen = torch.Generator(device=dev).manual_seed(i)
+        slab = torch.polar(torch.ones((s_count, h + 64, w + 64), device=dev),
+                           0.4 * torch.rand((s_count, h + 64, w + 64), device=dev, generator=gen)).contiguous()
+        pk = torch.fft.fft2(torch.randn((1, h, w), dtype=torch.complex64, device=dev, generator=gen)).contiguous()
+        wt = torch.ones(1, dtype=torch.float32, device=dev)
+        org = torch.tensor(rng.integers(0, 64, (p_count, 2)), dtype=torch.int32, device=dev)
+        sh = torch.tensor(rng.uniform(-2, 2, (p_count, 2)), device=dev)
+        prop = physics.propagator_pair(h, w, 1.0, 300.0, 0.1, device=dev)
+        ms = timed(lambda: engine.multislice_forward(slab, pk, wt, org, sh, prop))
+        bytes_per = (s_count * h * w * 8 + h * w * 8 + h * w * 4)
+        row = {"grid": [h, w], "slices": s_count, "positions": p_count, "forward_ms": round(ms, 3),
+               "patterns_per_s": round(p_count / (ms * 1e-3), 1),
+               "algorithmic_gbs": round(bytes_per * p_count / (ms * 1e-3) / 1e9, 1)}
+        if i == 0:
+            g = torch.randn((p_count, h, w), device=dev, generator=gen)
+
+            def grad_step():
+                _, ck = engine.multislice_forward_checkpointed(slab, pk, wt, org, sh, prop, 4)
+                gs = torch.zeros((p_count, 2), dtype=torch.float64, device=dev)
+                engine.multislice_adjoint(slab, pk, wt, org, sh, prop, g, ck, 4, grad_shifts=gs)
+
+            gms = timed(grad_step)
+            row["gradient_ms"] = round(gms, 3)
+            row["gradient_patterns_per_s"] = round(p_count / (gms * 1e-3), 1)
+        out.append(row)
+        print(f"# grid {row}", file=sys.stderr, flush=True)
+    return out
+
+
 def membw_bench(sizes_mib=(256, 512, 1024, 4096), reps: int = 20):
     """Achievable HBM bandwidth on this box: torch device copies (read + write) and reductions (read) of
     float32 buffers, timed with events; the practical ceiling the streaming passes are compared with."""
@@ -307,7 +360,7 @@ def main() -> int:
     ap.add_argument("--serving-qps", type=float, default=60.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
-    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw", "potential"),
+    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw", "potential", "grid"),
                     default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
                          "config1: 128^2 x 20 slices, 256 positions (the reference's CPU-runnable case)")
@@ -333,6 +386,13 @@ def main() -> int:
         print(json.dumps({"metric": "potential slices/s, 1024^2 x 64 slab", "value": res["slices_per_s"],
                           "unit": "slices/s", "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
                           "data": "synthetic", "potential_builder": res}), flush=True)
+        return 0
+    if args.workload == "grid":
+        res = grid_bench(max(args.steps, 1))
+        print(json.dumps({"metric": "4D-STEM patterns/s on grids other than square powers of two",
+                          "value": res[0]["patterns_per_s"], "unit": "patterns/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "complex64", "data": "synthetic", "grid": res}),
+              flush=True)
         return 0
     if args.workload == "membw":
         res = membw_bench()

@@ -92,22 +92,23 @@ def test_public_reconstruction_descends_complex_beam_loss(monkeypatch, optimizer
     assert final_loss < initial_loss
 
 
-def _problem(n=32, p_count=4, modes=1, seed=0):
-    """Synthetic data from the oracle forward and perturbed starting guesses."""
+def _problem(n=32, p_count=4, modes=1, seed=0, w=None):
+    """Synthetic data from the oracle forward and perturbed starting guesses on an n x w grid (w = n)."""
     rng = np.random.default_rng(seed)
+    h, w = n, (n if w is None else w)
     calib, kv, dz = 0.5, 80.0, 1.0
-    pot_true = rng.uniform(0, 40, (n, n))
-    beam_true = (rng.standard_normal((n, n, modes)) + 1j * rng.standard_normal((n, n, modes))) * 0.1
+    pot_true = rng.uniform(0, 40, (h, w))
+    beam_true = (rng.standard_normal((h, w, modes)) + 1j * rng.standard_normal((h, w, modes))) * 0.1
     weights = np.array([0.7, 0.3][:modes]) if modes > 1 else np.array([1.0])
     pos_true = rng.uniform(-1.5, 1.5, (p_count, 2))
     data = 0
-    prop = o.propagation_func(n, n, dz, kv, calib)
+    prop = o.propagation_func(h, w, dz, kv, calib)
     trans = np.exp(1j * o.interaction_sigma(kv) * pot_true)[None]
     for m in range(modes):
         data = data + weights[m] / weights.sum() * o.window_stem(trans, beam_true[..., m],
                                                                  np.zeros((p_count, 2), int), pos_true / calib, prop)
-    pot0 = (pot_true + rng.standard_normal((n, n)) * 4).astype(np.complex128)
-    beam0 = beam_true * (1 + 0.1 * rng.standard_normal((n, n, modes)))
+    pot0 = (pot_true + rng.standard_normal((h, w)) * 4).astype(np.complex128)
+    beam0 = beam_true * (1 + 0.1 * rng.standard_normal((h, w, modes)))
     pos0 = pos_true + rng.uniform(-0.2, 0.2, (p_count, 2))
     return dict(n=n, calib=calib, kv=kv, dz=dz, data=data, pot0=pot0, beam0=beam0, pos0=pos0, weights=weights)
 
@@ -165,6 +166,26 @@ def test_single_slice_poscorrected_matches_oracle():
     e_snap = _rel_change_error(out[5].cpu().numpy()[..., 1], pb["pos0"], ref[5][..., 1])
     print(f"poscorrected: pot {e_pot:.2e} beam {e_beam:.2e} pos {e_pos:.2e} snapshot {e_snap:.2e}", flush=True)
     assert max(e_pot, e_beam, e_pos, e_snap) < TOL_RECON
+
+
+def test_poscorrected_on_non_square_odd_grid_matches_oracle():
+    """The reconstructors take any grid shape, as the reference's do: a 21 x 30 grid runs on the runtime-size
+    kernels (forward, adjoint and position gradient)."""
+    import ptyrodactyl_amd as pa
+    pb = _problem(n=21, w=30, seed=4)
+    exp = _experimental(pb)
+    pot = pa.create_calibrated_array(torch.tensor(pb["pot0"]), 0.5, 0.5, True)
+    beam = pa.create_calibrated_array(torch.tensor(pb["beam0"][..., 0]), 0.5, 0.5, True)
+    out = pa.single_slice_poscorrected(exp, pot, beam, pb["dz"], save_every=2, num_iterations=4,
+                                       learning_rate=torch.tensor([0.01, 0.02]), loss_type="mse",
+                                       optimizer_name="adam")
+    ref = _oracle(pb, 0.01, 0.02, "mse", "adam", True, 4, 2)
+    e_pot = _rel_change_error(out[0].data_array.cpu(), pb["pot0"], ref[0])
+    e_beam = _rel_change_error(out[1].data_array.cpu(), pb["beam0"][..., 0], ref[1][..., 0])
+    e_pos = _rel_change_error(out[2].cpu(), pb["pos0"], ref[2])
+    print(f"poscorrected 21x30: pot {e_pot:.2e} beam {e_beam:.2e} pos {e_pos:.2e}", flush=True)
+    assert out[0].data_array.shape == (21, 30)
+    assert max(e_pot, e_beam, e_pos) < TOL_RECON
 
 
 def test_single_slice_multi_modal_matches_oracle():

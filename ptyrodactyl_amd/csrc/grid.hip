@@ -23,6 +23,8 @@
 // FP64, so no error accumulates across stages beyond the FP32 sums.
 #include <math.h>
 
+#include <stdlib.h>
+
 #include <algorithm>
 
 #include "common.hpp"
@@ -35,6 +37,7 @@ constexpr int GT = 256;             // threads per workgroup
 constexpr int GROUP_ELEMS = 2048;   // elements per LDS line group
 constexpr int MAX_BLOCKS = 2048;    // workgroups per launch
 constexpr size_t WS_BUDGET = size_t(2) << 30;
+constexpr long long DEFAULT_STREAM_MIN_PIXELS = 1LL << 14;
 
 struct Axis {
   int L;            // transform length
@@ -255,16 +258,16 @@ __device__ float2* fft_lines(float2* a, float2* b, int nl, const Axis& ax, bool 
 
 enum { NONE = 0, FWD = 1, INV = 2 };
 
-// One pass over the `nlines` lines of one axis.  COLS: a line is a column of the h x w wave (the
+// One pass over lines [l_begin, l_end) of one axis.  COLS: a line is a column of the h x w wave (the
 // element index is the row), and global accesses put consecutive threads on consecutive columns;
 // otherwise a line is a row.  load(line, e) -> value; transform fft1; mid(line, e, value) -> value
 // (if has_mid); transform fft2; store(line, e, value).
 template <bool COLS, class Load, class Mid, class Store>
-__device__ void line_pass(int nlines, int g_max, const Axis& ax, float2* bufa, float2* bufb, int fft1, bool has_mid,
-                          int fft2, Load load, Mid mid, Store store) {
+__device__ void line_pass(int l_begin, int l_end, int g_max, const Axis& ax, float2* bufa, float2* bufb, int fft1,
+                          bool has_mid, int fft2, Load load, Mid mid, Store store) {
   const int L = ax.L, ls = ax.ls;
-  for (int l0 = 0; l0 < nlines; l0 += g_max) {
-    const int g = min(g_max, nlines - l0);
+  for (int l0 = l_begin; l0 < l_end; l0 += g_max) {
+    const int g = min(g_max, l_end - l0);
     const int total = g * L;
     {
       Walk wk(threadIdx.x, COLS ? g : L);
@@ -367,14 +370,18 @@ __device__ __forceinline__ long long slab_index(const GridParams& a, const Posit
   return ry * a.slab_w + rx;
 }
 
+// Pieces of the forward shared by the per-position and the streaming kernels; each works on lines
+// [l0, l1) of one wave wb (h x w complex64, (ky, x) layout between passes).
+
 // Incident wave of mode m: rows of probe_k times the shift ramp (with the 1/(h w) of the inverse
-// transform), inverse transform along x; leaves (ky, x) in wb.
-__device__ void incident_rows(const GridParams& a, const Setup& u, const Position& q, int m, float2* wb) {
+// transform), inverse transform along x.
+__device__ void incident_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
+                              const Position& q, int m, float2* wb) {
   const int h = a.h, w = a.w;
   const float inv_hw = 1.0f / ((float)h * (float)w);
   const float2* pk = a.probe_k + (long long)m * h * w;
   line_pass<false>(
-      h, a.gx, u.ax, u.bufa, u.bufb, INV, false, NONE,
+      l0, l1, a.gx, ax, ba, bb, INV, false, NONE,
       [&](int ky, int kx) {
         const float2 r = gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx));
         return gmul(pk[(long long)ky * w + kx], gscale(r, inv_hw));
@@ -382,12 +389,13 @@ __device__ void incident_rows(const GridParams& a, const Setup& u, const Positio
       Ident{}, [&](int ky, int x, float2 v) { wb[(long long)ky * w + x] = v; });
 }
 
-// Rows of wb: forward along x, times f(ky, kx), inverse along x (the propagator step or its adjoint).
+// Rows: forward along x, times Py Px (or its conjugate), inverse along x.
 template <bool CONJ>
-__device__ void propagate_rows(const GridParams& a, const Setup& u, float2* wb) {
+__device__ void propagate_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
+                               float2* wb) {
   const int w = a.w;
   line_pass<false>(
-      a.h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
+      l0, l1, a.gx, ax, ba, bb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
       [&](int ky, int kx, float2 v) {
         const float2 f = gmul(a.prop_y[ky], a.prop_x[kx]);
         return CONJ ? gmulc(f, v) : gmul(v, f);
@@ -395,55 +403,149 @@ __device__ void propagate_rows(const GridParams& a, const Setup& u, float2* wb) 
       [&](int ky, int x, float2 v) { wb[(long long)ky * w + x] = v; });
 }
 
-template <bool CKPT>
+// Columns of slice s: to real space along y, checkpoint (if ck), times T_s, back along y.
+__device__ void slice_cols(const GridParams& a, const Axis& ay, float2* ba, float2* bb, int l0, int l1,
+                           const Position& q, int s, float2* wb, float2* ck) {
+  const int w = a.w;
+  const float2* ts = a.slab + (long long)s * a.slab_h * a.slab_w;
+  line_pass<true>(
+      l0, l1, a.gy, ay, ba, bb, INV, true, FWD, [&](int x, int ky) { return wb[(long long)ky * w + x]; },
+      [&](int x, int y, float2 v) {
+        if (ck) ck[(long long)y * w + x] = v;
+        return gmul(v, ts[slab_index(a, q, y, x)]);
+      },
+      [&](int x, int ky, float2 v) { wb[(long long)ky * w + x] = v; });
+}
+
+// Rows after the last slice: forward along x gives the detector amplitude A(ky, kx) of mode m at
+// position p; writes it fftshifted and adds w_m |A|^2 to the pattern (stores it for m = 0).
+__device__ void final_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
+                           long long p, int m, float2* wb) {
+  const int h = a.h, w = a.w;
+  const long long hw = (long long)h * w;
+  const float wm = a.weights[m];
+  float* pat = a.patterns ? a.patterns + p * hw : nullptr;
+  float2* amp = a.amplitude ? a.amplitude + (p * a.M + m) * hw : nullptr;
+  line_pass<false>(
+      l0, l1, a.gx, ax, ba, bb, FWD, false, NONE, [&](int ky, int x) { return wb[(long long)ky * w + x]; }, Ident{},
+      [&](int ky, int kx, float2 v) {
+        const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
+        if (amp) amp[o] = v;
+        if (pat) {
+          const float inten = wm * (v.x * v.x + v.y * v.y);
+          pat[o] = m == 0 ? inten : pat[o] + inten;
+        }
+      });
+}
+
+__device__ __forceinline__ float2* checkpoint_of(const GridParams& a, long long p, int m, int s) {
+  if (!a.ckpt || s == 0 || s % a.ckpt_every != 0) return nullptr;
+  const int nc = num_checkpoints(a.S, a.ckpt_every);
+  return a.ckpt + ((p * a.M + m) * nc + (s / a.ckpt_every - 1)) * (long long)a.h * a.w;
+}
+
 __global__ __launch_bounds__(GT) void k_grid_forward(GridParams a, float2* ws) {
   extern __shared__ float2 lds[];
   __shared__ int s_rad[2 * GRID_MAX_STAGES];
   const Setup u = setup(a, s_rad, lds);
   const int h = a.h, w = a.w;
   const long long hw = (long long)h * w;
-  const long long sstride = a.slab_h * a.slab_w;
-  const int nc = num_checkpoints(a.S, a.ckpt_every);
   float2* wb = ws + (long long)blockIdx.x * hw;
   for (long long p = blockIdx.x; p < a.P; p += gridDim.x) {
     const Position q = position(a, p);
     for (int m = 0; m < a.M; ++m) {
-      incident_rows(a, u, q, m, wb);
+      incident_rows(a, u.ax, u.bufa, u.bufb, 0, h, q, m, wb);
       for (int s = 0; s < a.S; ++s) {
-        const float2* ts = a.slab + (long long)s * sstride;
-        float2* ck = nullptr;
-        if (CKPT && s > 0 && s % a.ckpt_every == 0)
-          ck = a.ckpt + ((p * a.M + m) * nc + (s / a.ckpt_every - 1)) * hw;
-        // columns: to real space along y, checkpoint, times T_s, back along y
-        line_pass<true>(
-            w, a.gy, u.ay, u.bufa, u.bufb, INV, true, FWD, [&](int x, int ky) { return wb[(long long)ky * w + x]; },
-            [&](int x, int y, float2 v) {
-              if (ck) ck[(long long)y * w + x] = v;
-              return gmul(v, ts[slab_index(a, q, y, x)]);
-            },
-            [&](int x, int ky, float2 v) { wb[(long long)ky * w + x] = v; });
-        if (s + 1 < a.S) {
-          propagate_rows<false>(a, u, wb);
-        } else {
-          // rows: forward along x gives the detector amplitude A(ky, kx)
-          const float wm = a.weights[m];
-          float* pat = a.patterns ? a.patterns + p * hw : nullptr;
-          float2* amp = a.amplitude ? a.amplitude + (p * a.M + m) * hw : nullptr;
-          line_pass<false>(
-              h, a.gx, u.ax, u.bufa, u.bufb, FWD, false, NONE,
-              [&](int ky, int x) { return wb[(long long)ky * w + x]; }, Ident{},
-              [&](int ky, int kx, float2 v) {
-                const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
-                if (amp) amp[o] = v;
-                if (pat) {
-                  const float inten = wm * (v.x * v.x + v.y * v.y);
-                  pat[o] = m == 0 ? inten : pat[o] + inten;
-                }
-              });
-        }
+        slice_cols(a, u.ay, u.bufa, u.bufb, 0, w, q, s, wb, checkpoint_of(a, p, m, s));
+        if (s + 1 < a.S)
+          propagate_rows<false>(a, u.ax, u.bufa, u.bufb, 0, h, wb);
+        else
+          final_rows(a, u.ax, u.bufa, u.bufb, 0, h, p, m, wb);
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Streaming forward for large grids: every pass is one launch over (wave, line group) workgroups, so
+// that a few positions still fill the device.  The twiddle tables are built once in the workspace.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(GT) void k_grid_twiddles(int h, int w, float2* tw) {
+  const int e = blockIdx.x * GT + threadIdx.x;
+  if (e >= h + w) return;
+  const int L = e < h ? h : w;
+  const int i = e < h ? e : e - h;
+  double sn, cs;
+  sincospi(-2.0 * (double)i / (double)L, &sn, &cs);
+  tw[e] = make_float2((float)cs, (float)sn);
+}
+
+// Axis of one streaming pass: its twiddle table copied into LDS, then the two line buffers.
+template <bool Y>
+__device__ Axis stream_axis(const GridParams& a, const float2* twg, float2* lds, int* s_rad, float2*& ba,
+                            float2*& bb) {
+  const int L = Y ? a.h : a.w;
+  const int ls = Y ? a.lsy : a.lsx;
+  const int g = Y ? a.gy : a.gx;
+  for (int e = threadIdx.x; e < L; e += GT) lds[e] = twg[(Y ? 0 : a.h) + e];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < GRID_MAX_STAGES; ++i) s_rad[i] = Y ? a.ry[i] : a.rx[i];
+  }
+  __syncthreads();
+  ba = lds + L;
+  bb = ba + g * ls;
+  return Axis{L, ls, Y ? a.ny : a.nx, s_rad, lds};
+}
+
+// Blocks b = group * nw + wave.
+__global__ __launch_bounds__(GT) void k_gs_incident(GridParams a, const float2* twg, float2* waves, long long p0,
+                                                     int nw) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  float2 *ba, *bb;
+  const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
+  const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
+  const long long p = p0 + wl / a.M;
+  incident_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p), wl % a.M,
+                waves + (long long)wl * a.h * a.w);
+}
+
+__global__ __launch_bounds__(GT) void k_gs_cols(GridParams a, const float2* twg, float2* waves, long long p0, int nw,
+                                                 int s) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  float2 *ba, *bb;
+  const Axis ay = stream_axis<true>(a, twg, lds, s_rad, ba, bb);
+  const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gy;
+  const long long p = p0 + wl / a.M;
+  const int m = wl % a.M;
+  slice_cols(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p), s, waves + (long long)wl * a.h * a.w,
+             checkpoint_of(a, p, m, s));
+}
+
+__global__ __launch_bounds__(GT) void k_gs_rows(GridParams a, const float2* twg, float2* waves, int nw) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  float2 *ba, *bb;
+  const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
+  const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
+  propagate_rows<false>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), waves + (long long)wl * a.h * a.w);
+}
+
+// Blocks b = group * np + position; each block runs the modes of its position in order, so the pattern
+// sum over modes has one writer per pixel.
+__global__ __launch_bounds__(GT) void k_gs_final(GridParams a, const float2* twg, float2* waves, long long p0,
+                                                  int np) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  float2 *ba, *bb;
+  const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
+  const int pl = blockIdx.x % np, l0 = (blockIdx.x / np) * a.gx;
+  for (int m = 0; m < a.M; ++m)
+    final_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), p0 + pl, m,
+               waves + ((long long)pl * a.M + m) * a.h * a.w);
 }
 
 __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
@@ -471,7 +573,7 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
         const int s0 = c * k;
         const int s1 = min(a.S, s0 + k);
         // recompute the waves entering slices s0 .. s1-1 (and the detector seed in the last segment)
-        if (c == 0) incident_rows(a, u, q, m, wb);
+        if (c == 0) incident_rows(a, u.ax, u.bufa, u.bufb, 0, h, q, m, wb);
         for (int s = s0; s < s1; ++s) {
           const bool from_ckpt = c > 0 && s == s0;
           const bool advance = s + 1 < s1 || s == a.S - 1;
@@ -479,7 +581,7 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
           const float2* ts = a.slab + (long long)s * sstride;
           float2* sg = seg + (long long)(s - s0) * hw;
           line_pass<true>(
-              w, a.gy, u.ay, u.bufa, u.bufb, from_ckpt ? NONE : INV, true, advance ? FWD : NONE,
+              0, w, a.gy, u.ay, u.bufa, u.bufb, from_ckpt ? NONE : INV, true, advance ? FWD : NONE,
               [&](int x, int e) { return src[(long long)e * w + x]; },
               [&](int x, int y, float2 v) {
                 sg[(long long)y * w + x] = v;
@@ -489,13 +591,13 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
                 if (advance) wb[(long long)ky * w + x] = v;
               });
           if (s + 1 < s1) {
-            propagate_rows<false>(a, u, wb);
+            propagate_rows<false>(a, u.ax, u.bufa, u.bufb, 0, h, wb);
           } else if (s == a.S - 1) {
             // detector amplitude -> seed dL/dA -> adjoint of the forward x transform, into gb
             const float* gpat = a.grad_patterns ? a.grad_patterns + p * hw : nullptr;
             const float2* gamp = a.grad_amplitude ? a.grad_amplitude + (p * a.M + m) * hw : nullptr;
             line_pass<false>(
-                h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
+                0, h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
                 [&](int ky, int kx, float2 amp) {
                   const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
                   if (gamp) return gamp[o];
@@ -510,7 +612,7 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
           float2* gs = a.grad_slab + (long long)s * sstride;
           const float2* sg = seg + (long long)(s - s0) * hw;
           line_pass<true>(
-              w, a.gy, u.ay, u.bufa, u.bufb, INV, true, FWD, [&](int x, int ky) { return gb[(long long)ky * w + x]; },
+              0, w, a.gy, u.ay, u.bufa, u.bufb, INV, true, FWD, [&](int x, int ky) { return gb[(long long)ky * w + x]; },
               [&](int x, int y, float2 g) {
                 const long long si = slab_index(a, q, y, x);
                 const float2 cg = gmulc(sg[(long long)y * w + x], g);
@@ -520,13 +622,13 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
               },
               [&](int x, int ky, float2 v) { gb[(long long)ky * w + x] = v; });
           if (s > 0) {
-            propagate_rows<true>(a, u, gb);
+            propagate_rows<true>(a, u.ax, u.bufa, u.bufb, 0, h, gb);
           } else {
             // dL/d(probe_k) = conj(R) dL/du and dL/dshift = Re sum conj(dL/du) du/dshift, u = probe_k R
             float2* gp = a.grad_probe_k ? a.grad_probe_k + (long long)m * hw : nullptr;
             const bool want_shift = a.grad_shifts != nullptr;
             line_pass<false>(
-                h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, NONE,
+                0, h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, NONE,
                 [&](int ky, int x) { return gb[(long long)ky * w + x]; },
                 [&](int ky, int kx, float2 gu) {
                   const float2 r = gscale(gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx)), inv_hw);
@@ -596,6 +698,15 @@ hipError_t set_lds(const void* fn, size_t lds) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// Twiddle tables of the streaming forward at the start of its workspace, 256-byte aligned.
+size_t twiddle_bytes(int h, int w) { return (((size_t)(h + w) * sizeof(float2)) + 255) & ~(size_t)255; }
+
+// Grids of at least this many pixels run the streaming forward (PTY_GRID_STREAM_MIN_PIXELS overrides).
+long long stream_min_pixels() {
+  const char* v = getenv("PTY_GRID_STREAM_MIN_PIXELS");
+  return v ? atoll(v) : DEFAULT_STREAM_MIN_PIXELS;
+}
+
 long long blocks_for(long long P, size_t per_block, size_t ws_bytes) {
   const long long fit = per_block ? (long long)(ws_bytes / per_block) : 0;
   return std::min(std::min(P, (long long)MAX_BLOCKS), fit);
@@ -612,22 +723,52 @@ size_t grid_workspace_bytes(int h, int w, int ckpt_every, long long num_position
   const size_t per = grid_block_bytes(h, w, ckpt_every, adjoint);
   long long b = std::min(num_positions, (long long)MAX_BLOCKS);
   b = std::min(b, std::max(1LL, (long long)(WS_BUDGET / per)));
-  return per * (size_t)std::max(1LL, b);
+  return per * (size_t)std::max(1LL, b) + (adjoint ? 0 : twiddle_bytes(h, w));
 }
 
 hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStream_t st) {
-  const bool ckpt = a.ckpt != nullptr;
-  a.ckpt_every = ckpt ? clamp_ckpt(a.ckpt_every, a.S) : 1;
+  a.ckpt_every = a.ckpt ? clamp_ckpt(a.ckpt_every, a.S) : 1;
   const size_t lds = prepare(a);
+  const long long hw = (long long)a.h * a.w;
+  const size_t twb = twiddle_bytes(a.h, a.w);
+  const size_t per_pos = (size_t)a.M * (size_t)hw * sizeof(float2);
+  if (hw >= stream_min_pixels() && ws_bytes >= twb + per_pos) {
+    // streaming: one launch per pass over (wave, line group) workgroups
+    float2* tw = (float2*)ws;
+    float2* waves = (float2*)((char*)ws + twb);
+    const long long bpos = std::min(a.P, (long long)((ws_bytes - twb) / per_pos));
+    const size_t lds_rows = (size_t)(a.w + 2 * a.gx * a.lsx) * sizeof(float2);
+    const size_t lds_cols = (size_t)(a.h + 2 * a.gy * a.lsy) * sizeof(float2);
+    hipError_t e = set_lds((const void*)k_gs_incident, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gs_final, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gs_cols, lds_cols);
+    if (e != hipSuccess) return e;
+    const long long row_groups = (a.h + a.gx - 1) / a.gx;
+    const long long col_groups = (a.w + a.gy - 1) / a.gy;
+    hipLaunchKernelGGL(k_grid_twiddles, dim3((unsigned)((a.h + a.w + GT - 1) / GT)), dim3(GT), 0, st, a.h, a.w, tw);
+    for (long long p0 = 0; p0 < a.P; p0 += bpos) {
+      const int np = (int)std::min(bpos, a.P - p0);
+      const int nw = np * a.M;
+      hipLaunchKernelGGL(k_gs_incident, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, p0,
+                         nw);
+      for (int s = 0; s < a.S; ++s) {
+        hipLaunchKernelGGL(k_gs_cols, dim3((unsigned)(nw * col_groups)), dim3(GT), lds_cols, st, a, tw, waves, p0,
+                           nw, s);
+        if (s + 1 < a.S)
+          hipLaunchKernelGGL(k_gs_rows, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, nw);
+        else
+          hipLaunchKernelGGL(k_gs_final, dim3((unsigned)(np * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, p0,
+                             np);
+      }
+    }
+    return hipGetLastError();
+  }
   const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, 1, false), ws_bytes);
   if (nb < 1) return hipErrorInvalidValue;
-  const void* fn = ckpt ? (const void*)k_grid_forward<true> : (const void*)k_grid_forward<false>;
-  hipError_t e = set_lds(fn, lds);
+  hipError_t e = set_lds((const void*)k_grid_forward, lds);
   if (e != hipSuccess) return e;
-  if (ckpt)
-    hipLaunchKernelGGL(k_grid_forward<true>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
-  else
-    hipLaunchKernelGGL(k_grid_forward<false>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  hipLaunchKernelGGL(k_grid_forward, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
   return hipGetLastError();
 }
 

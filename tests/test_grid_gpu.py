@@ -243,3 +243,28 @@ def test_ensemble_cbed_image_on_non_square_grid():
     err = rel_l2(got, want)
     print(f"ensemble cbed_image {h}x{w} (2 modes x 4 jitter x 8 coherence): rel-L2 {err:.2e}", flush=True)
     assert got.shape == (h, w) and err < TOL
+
+
+def test_streaming_forward_matches_oracle_and_per_position_kernel(monkeypatch):
+    """Grids of 2^14 pixels or more run one launch per pass over (wave, line group) workgroups.  Each line is
+    transformed by the same code with the same grouping as in the per-position kernel, so the two agree
+    bitwise; both match the oracle."""
+    from ptyrodactyl_amd import engine
+    h, w, s_count, k = 257, 256, 4, 3
+    slab, probes, origins, shifts, weights = _case(h, w, s_count, 3, seed=41, modes=2)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    pat_s, amp_s = engine.multislice_forward(*ops, want_amplitude=True)
+    pat_sc, ckpt_s = engine.multislice_forward_checkpointed(*ops, k)
+    monkeypatch.setenv("PTY_GRID_STREAM_MIN_PIXELS", str(1 << 40))   # force the per-position kernel
+    pat_p, amp_p = engine.multislice_forward(*ops, want_amplitude=True)
+    pat_pc, ckpt_p = engine.multislice_forward_checkpointed(*ops, k)
+    torch.cuda.synchronize()
+    assert torch.equal(pat_s, pat_p) and torch.equal(amp_s, amp_p)
+    assert torch.equal(pat_sc, pat_s) and torch.equal(pat_pc, pat_p) and torch.equal(ckpt_s, ckpt_p)
+    prop = _oracle_prop(h, w)
+    want = 0
+    for m in range(2):
+        want = want + weights[m] * o.window_stem(slab.astype(np.complex128), probes[m], origins, shifts, prop)
+    err = rel_l2(pat_s.cpu().numpy(), want)
+    print(f"streaming grid forward {h}x{w} (2 modes): rel-L2 {err:.2e}", flush=True)
+    assert err < TOL

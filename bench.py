@@ -265,9 +265,9 @@ def potential_bench(steps: int = 3):
 def grid_bench(steps: int = 3, cases=((100, 100, 16, 1024), (250, 250, 16, 512), (550, 500, 16, 128))):
     """Forward and gradient on grids that are not square powers of two (the runtime-size kernels): per case
     (h, w, slices, positions) a random transmission slab 64 pixels larger than the window, one probe mode,
-    random origins and shifts.  Algorithmic bytes per pattern as for the headline: S h w 8 + h w 8 + h w 4;
-    the gradient step (checkpoints every 4 slices, adjoint with slab, probe and shift gradients) is timed at the
-    first case."""
+    random origins and shifts.  Algorithmic bytes per pattern as for the headline: S h w 8 + h w 8 + h w 4.
+    The gradient step is the checkpointed forward (every 4 slices) and the adjoint with slab, probe and shift
+    gradients."""
     import torch
     from ptyrodactyl_amd import engine, physics
     dev = torch.device("cuda", 0)
@@ -299,17 +299,16 @@ def grid_bench(steps: int = 3, cases=((100, 100, 16, 1024), (250, 250, 16, 512),
         row = {"grid": [h, w], "slices": s_count, "positions": p_count, "forward_ms": round(ms, 3),
                "patterns_per_s": round(p_count / (ms * 1e-3), 1),
                "algorithmic_gbs": round(bytes_per * p_count / (ms * 1e-3) / 1e9, 1)}
-        if i == 0:
-            g = torch.randn((p_count, h, w), device=dev, generator=gen)
+        g = torch.randn((p_count, h, w), device=dev, generator=gen)
 
-            def grad_step():
-                _, ck = engine.multislice_forward_checkpointed(slab, pk, wt, org, sh, prop, 4)
-                gs = torch.zeros((p_count, 2), dtype=torch.float64, device=dev)
-                engine.multislice_adjoint(slab, pk, wt, org, sh, prop, g, ck, 4, grad_shifts=gs)
+        def grad_step():
+            _, ck = engine.multislice_forward_checkpointed(slab, pk, wt, org, sh, prop, 4)
+            gs = torch.zeros((p_count, 2), dtype=torch.float64, device=dev)
+            engine.multislice_adjoint(slab, pk, wt, org, sh, prop, g, ck, 4, grad_shifts=gs)
 
-            gms = timed(grad_step)
-            row["gradient_ms"] = round(gms, 3)
-            row["gradient_patterns_per_s"] = round(p_count / (gms * 1e-3), 1)
+        gms = timed(grad_step)
+        row["gradient_ms"] = round(gms, 3)
+        row["gradient_patterns_per_s"] = round(p_count / (gms * 1e-3), 1)
         out.append(row)
         print(f"# grid {row}", file=sys.stderr, flush=True)
     return out

@@ -499,17 +499,16 @@ __device__ Axis stream_axis(const GridParams& a, const float2* twg, float2* lds,
   return Axis{L, ls, Y ? a.ny : a.nx, s_rad, lds};
 }
 
-// Blocks b = group * nw + wave.
-__global__ __launch_bounds__(GT) void k_gs_incident(GridParams a, const float2* twg, float2* waves, long long p0,
-                                                     int nw) {
+// Blocks b = group * nw + wave; wave wl lives at waves + wl * stride.
+__global__ __launch_bounds__(GT) void k_gs_incident(GridParams a, const float2* twg, float2* waves, long long stride,
+                                                     long long p0, int nw) {
   extern __shared__ float2 lds[];
   __shared__ int s_rad[GRID_MAX_STAGES];
   float2 *ba, *bb;
   const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
   const long long p = p0 + wl / a.M;
-  incident_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p), wl % a.M,
-                waves + (long long)wl * a.h * a.w);
+  incident_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p), wl % a.M, waves + (long long)wl * stride);
 }
 
 __global__ __launch_bounds__(GT) void k_gs_cols(GridParams a, const float2* twg, float2* waves, long long p0, int nw,
@@ -525,13 +524,15 @@ __global__ __launch_bounds__(GT) void k_gs_cols(GridParams a, const float2* twg,
              checkpoint_of(a, p, m, s));
 }
 
-__global__ __launch_bounds__(GT) void k_gs_rows(GridParams a, const float2* twg, float2* waves, int nw) {
+template <bool CONJ>
+__global__ __launch_bounds__(GT) void k_gs_rows(GridParams a, const float2* twg, float2* waves, long long stride,
+                                                 int nw) {
   extern __shared__ float2 lds[];
   __shared__ int s_rad[GRID_MAX_STAGES];
   float2 *ba, *bb;
   const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
-  propagate_rows<false>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), waves + (long long)wl * a.h * a.w);
+  propagate_rows<CONJ>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), waves + (long long)wl * stride);
 }
 
 // Blocks b = group * np + position; each block runs the modes of its position in order, so the pattern
@@ -548,6 +549,104 @@ __global__ __launch_bounds__(GT) void k_gs_final(GridParams a, const float2* twg
                waves + ((long long)pl * a.M + m) * a.h * a.w);
 }
 
+// Pieces of the adjoint shared by the per-position and the streaming kernels.
+
+// Recompute pass over columns of slice s: the wave entering slice s, from a checkpoint (real space) or
+// from wb ((ky, x) layout, inverse along y), is stored in sg; if `advance`, times T_s and forward along y
+// into wb.
+__device__ void recompute_cols(const GridParams& a, const Axis& ay, float2* ba, float2* bb, int l0, int l1,
+                               const Position& q, int s, const float2* src, bool from_ckpt, bool advance, float2* sg,
+                               float2* wb) {
+  const int w = a.w;
+  const float2* ts = a.slab + (long long)s * a.slab_h * a.slab_w;
+  line_pass<true>(
+      l0, l1, a.gy, ay, ba, bb, from_ckpt ? NONE : INV, true, advance ? FWD : NONE,
+      [&](int x, int e) { return src[(long long)e * w + x]; },
+      [&](int x, int y, float2 v) {
+        sg[(long long)y * w + x] = v;
+        return advance ? gmul(v, ts[slab_index(a, q, y, x)]) : v;
+      },
+      [&](int x, int ky, float2 v) {
+        if (advance) wb[(long long)ky * w + x] = v;
+      });
+}
+
+// Rows after the last slice: detector amplitude, seed dL/dA (from dL/dI or given), adjoint of the forward
+// x transform into gb.
+__device__ void seed_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1, long long p,
+                          int m, const float2* wb, float2* gb) {
+  const int h = a.h, w = a.w;
+  const long long hw = (long long)h * w;
+  const float wm = a.weights[m];
+  const float* gpat = a.grad_patterns ? a.grad_patterns + p * hw : nullptr;
+  const float2* gamp = a.grad_amplitude ? a.grad_amplitude + (p * a.M + m) * hw : nullptr;
+  line_pass<false>(
+      l0, l1, a.gx, ax, ba, bb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
+      [&](int ky, int kx, float2 amp) {
+        const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
+        if (gamp) return gamp[o];
+        return gscale(amp, 2.0f * wm * gpat[o]);
+      },
+      [&](int ky, int x, float2 v) { gb[(long long)ky * w + x] = v; });
+}
+
+// Back through slice s over columns: dL/dphi (inverse along y), dL/dT_s += conj(psi_s) dL/dphi (float32
+// atomics), dL/dpsi_s = conj(T_s) dL/dphi, forward along y into gb.
+__device__ void back_cols(const GridParams& a, const Axis& ay, float2* ba, float2* bb, int l0, int l1,
+                          const Position& q, int s, const float2* sg, float2* gb) {
+  const int w = a.w;
+  const long long sstride = a.slab_h * a.slab_w;
+  const float2* ts = a.slab + (long long)s * sstride;
+  float2* gs = a.grad_slab + (long long)s * sstride;
+  line_pass<true>(
+      l0, l1, a.gy, ay, ba, bb, INV, true, FWD, [&](int x, int ky) { return gb[(long long)ky * w + x]; },
+      [&](int x, int y, float2 g) {
+        const long long si = slab_index(a, q, y, x);
+        const float2 cg = gmulc(sg[(long long)y * w + x], g);
+        unsafeAtomicAdd(&gs[si].x, cg.x);
+        unsafeAtomicAdd(&gs[si].y, cg.y);
+        return gmulc(ts[si], g);
+      },
+      [&](int x, int ky, float2 v) { gb[(long long)ky * w + x] = v; });
+}
+
+// Rows of dL/dpsi_0, forward along x: dL/d(probe_k) += conj(R) dL/du (float32 atomics) and this thread's
+// share of dL/dshift = Re sum conj(dL/du) du/dshift, u = probe_k R.
+__device__ void probe_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
+                           const Position& q, int m, const float2* gb, double& gsy, double& gsx) {
+  const int h = a.h, w = a.w;
+  const long long hw = (long long)h * w;
+  const float inv_hw = 1.0f / ((float)h * (float)w);
+  const double two_pi = 6.283185307179586;
+  const float2* pk = a.probe_k + (long long)m * hw;
+  float2* gp = a.grad_probe_k ? a.grad_probe_k + (long long)m * hw : nullptr;
+  const bool want_shift = a.grad_shifts != nullptr;
+  line_pass<false>(
+      l0, l1, a.gx, ax, ba, bb, FWD, true, NONE, [&](int ky, int x) { return gb[(long long)ky * w + x]; },
+      [&](int ky, int kx, float2 gu) {
+        const float2 r = gscale(gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx)), inv_hw);
+        const long long o = (long long)ky * w + kx;
+        if (gp) {
+          const float2 cg = gmulc(r, gu);
+          unsafeAtomicAdd(&gp[o].x, cg.x);
+          unsafeAtomicAdd(&gp[o].y, cg.y);
+        }
+        if (want_shift) {
+          // du/ds_y = u (-2 pi i ky / h): Re(conj(gu) u (-2 pi i f)) = 2 pi f Im(conj(gu) u)
+          const float2 z = gmulc(gu, gmul(pk[o], r));
+          gsy += two_pi * (double)signed_bin(ky, h) / (double)h * (double)z.y;
+          gsx += two_pi * (double)signed_bin(kx, w) / (double)w * (double)z.y;
+        }
+        return gu;
+      },
+      [&](int, int, float2) {});
+}
+
+__device__ __forceinline__ const float2* checkpoint_in(const GridParams& a, long long p, int m, int c) {
+  const int nc = num_checkpoints(a.S, a.ckpt_every);
+  return a.ckpt + ((p * a.M + m) * nc + (c - 1)) * (long long)a.h * a.w;
+}
+
 __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
   __shared__ int s_rad[2 * GRID_MAX_STAGES];
   __shared__ double s_red[GT / 64];
@@ -555,20 +654,15 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
   const Setup u = setup(a, s_rad, lds);
   const int h = a.h, w = a.w;
   const long long hw = (long long)h * w;
-  const long long sstride = a.slab_h * a.slab_w;
   const int k = a.ckpt_every;
   const int nc = num_checkpoints(a.S, k);
-  const float inv_hw = 1.0f / ((float)h * (float)w);
   float2* wb = ws + (long long)blockIdx.x * (k + 2) * hw;  // forward wave
   float2* gb = wb + hw;                                      // gradient wave
   float2* seg = gb + hw;                                     // waves entering the slices of one segment
-  const double two_pi = 6.283185307179586;
   for (long long p = blockIdx.x; p < a.P; p += gridDim.x) {
     const Position q = position(a, p);
     double gsy = 0.0, gsx = 0.0;
     for (int m = 0; m < a.M; ++m) {
-      const float2* pk = a.probe_k + (long long)m * hw;
-      const float wm = a.weights[m];
       for (int c = nc; c >= 0; --c) {
         const int s0 = c * k;
         const int s1 = min(a.S, s0 + k);
@@ -577,77 +671,20 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
         for (int s = s0; s < s1; ++s) {
           const bool from_ckpt = c > 0 && s == s0;
           const bool advance = s + 1 < s1 || s == a.S - 1;
-          const float2* src = from_ckpt ? a.ckpt + ((p * a.M + m) * nc + (c - 1)) * hw : wb;
-          const float2* ts = a.slab + (long long)s * sstride;
-          float2* sg = seg + (long long)(s - s0) * hw;
-          line_pass<true>(
-              0, w, a.gy, u.ay, u.bufa, u.bufb, from_ckpt ? NONE : INV, true, advance ? FWD : NONE,
-              [&](int x, int e) { return src[(long long)e * w + x]; },
-              [&](int x, int y, float2 v) {
-                sg[(long long)y * w + x] = v;
-                return advance ? gmul(v, ts[slab_index(a, q, y, x)]) : v;
-              },
-              [&](int x, int ky, float2 v) {
-                if (advance) wb[(long long)ky * w + x] = v;
-              });
-          if (s + 1 < s1) {
+          recompute_cols(a, u.ay, u.bufa, u.bufb, 0, w, q, s, from_ckpt ? checkpoint_in(a, p, m, c) : wb, from_ckpt,
+                         advance, seg + (long long)(s - s0) * hw, wb);
+          if (s + 1 < s1)
             propagate_rows<false>(a, u.ax, u.bufa, u.bufb, 0, h, wb);
-          } else if (s == a.S - 1) {
-            // detector amplitude -> seed dL/dA -> adjoint of the forward x transform, into gb
-            const float* gpat = a.grad_patterns ? a.grad_patterns + p * hw : nullptr;
-            const float2* gamp = a.grad_amplitude ? a.grad_amplitude + (p * a.M + m) * hw : nullptr;
-            line_pass<false>(
-                0, h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
-                [&](int ky, int kx, float2 amp) {
-                  const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
-                  if (gamp) return gamp[o];
-                  return gscale(amp, 2.0f * wm * gpat[o]);
-                },
-                [&](int ky, int x, float2 v) { gb[(long long)ky * w + x] = v; });
-          }
+          else if (s == a.S - 1)
+            seed_rows(a, u.ax, u.bufa, u.bufb, 0, h, p, m, wb, gb);
         }
         // back through slices s1-1 .. s0
         for (int s = s1 - 1; s >= s0; --s) {
-          const float2* ts = a.slab + (long long)s * sstride;
-          float2* gs = a.grad_slab + (long long)s * sstride;
-          const float2* sg = seg + (long long)(s - s0) * hw;
-          line_pass<true>(
-              0, w, a.gy, u.ay, u.bufa, u.bufb, INV, true, FWD, [&](int x, int ky) { return gb[(long long)ky * w + x]; },
-              [&](int x, int y, float2 g) {
-                const long long si = slab_index(a, q, y, x);
-                const float2 cg = gmulc(sg[(long long)y * w + x], g);
-                unsafeAtomicAdd(&gs[si].x, cg.x);
-                unsafeAtomicAdd(&gs[si].y, cg.y);
-                return gmulc(ts[si], g);
-              },
-              [&](int x, int ky, float2 v) { gb[(long long)ky * w + x] = v; });
-          if (s > 0) {
+          back_cols(a, u.ay, u.bufa, u.bufb, 0, w, q, s, seg + (long long)(s - s0) * hw, gb);
+          if (s > 0)
             propagate_rows<true>(a, u.ax, u.bufa, u.bufb, 0, h, gb);
-          } else {
-            // dL/d(probe_k) = conj(R) dL/du and dL/dshift = Re sum conj(dL/du) du/dshift, u = probe_k R
-            float2* gp = a.grad_probe_k ? a.grad_probe_k + (long long)m * hw : nullptr;
-            const bool want_shift = a.grad_shifts != nullptr;
-            line_pass<false>(
-                0, h, a.gx, u.ax, u.bufa, u.bufb, FWD, true, NONE,
-                [&](int ky, int x) { return gb[(long long)ky * w + x]; },
-                [&](int ky, int kx, float2 gu) {
-                  const float2 r = gscale(gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx)), inv_hw);
-                  const long long o = (long long)ky * w + kx;
-                  if (gp) {
-                    const float2 cg = gmulc(r, gu);
-                    unsafeAtomicAdd(&gp[o].x, cg.x);
-                    unsafeAtomicAdd(&gp[o].y, cg.y);
-                  }
-                  if (want_shift) {
-                    // du/ds_y = u (-2 pi i ky / h): Re(conj(gu) u (-2 pi i f)) = 2 pi f Im(conj(gu) u)
-                    const float2 z = gmulc(gu, gmul(pk[o], r));
-                    gsy += two_pi * (double)signed_bin(ky, h) / (double)h * (double)z.y;
-                    gsx += two_pi * (double)signed_bin(kx, w) / (double)w * (double)z.y;
-                  }
-                  return gu;
-                },
-                [&](int, int, float2) {});
-          }
+          else
+            probe_rows(a, u.ax, u.bufa, u.bufb, 0, h, q, m, gb, gsy, gsx);
         }
       }
     }
@@ -661,6 +698,94 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
       __syncthreads();
     }
   }
+}
+
+// Streaming adjoint: per wave the workspace holds (k + 2) h w elements (forward wave, gradient wave, k
+// segment waves); every pass is one launch over (wave, line group) workgroups.  Blocks b = group * nw + wave.
+
+__global__ __launch_bounds__(GT) void k_gsa_recompute(GridParams a, const float2* twg, float2* base, long long p0,
+                                                       int nw, int s, int c) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  float2 *ba, *bb;
+  const Axis ay = stream_axis<true>(a, twg, lds, s_rad, ba, bb);
+  const long long hw = (long long)a.h * a.w;
+  const int k = a.ckpt_every;
+  const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gy;
+  const long long p = p0 + wl / a.M;
+  const int m = wl % a.M;
+  const int s0 = c * k, s1 = min(a.S, s0 + k);
+  const bool from_ckpt = c > 0 && s == s0;
+  const bool advance = s + 1 < s1 || s == a.S - 1;
+  float2* wb = base + (long long)wl * (k + 2) * hw;
+  recompute_cols(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p), s,
+                 from_ckpt ? checkpoint_in(a, p, m, c) : wb, from_ckpt, advance, wb + (2 + s - s0) * hw, wb);
+}
+
+__global__ __launch_bounds__(GT) void k_gsa_seed(GridParams a, const float2* twg, float2* base, long long p0,
+                                                  int nw) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  float2 *ba, *bb;
+  const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
+  const long long hw = (long long)a.h * a.w;
+  const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
+  float2* wb = base + (long long)wl * (a.ckpt_every + 2) * hw;
+  seed_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), p0 + wl / a.M, wl % a.M, wb, wb + hw);
+}
+
+__global__ __launch_bounds__(GT) void k_gsa_back(GridParams a, const float2* twg, float2* base, long long p0, int nw,
+                                                  int s, int s0) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  float2 *ba, *bb;
+  const Axis ay = stream_axis<true>(a, twg, lds, s_rad, ba, bb);
+  const long long hw = (long long)a.h * a.w;
+  const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gy;
+  float2* wb = base + (long long)wl * (a.ckpt_every + 2) * hw;
+  back_cols(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p0 + wl / a.M), s, wb + (2 + s - s0) * hw,
+            wb + hw);
+}
+
+// Probe gradient rows; with shift gradients each block writes its FP64 partial sums to
+// partials[2 b], partials[2 b + 1].
+__global__ __launch_bounds__(GT) void k_gsa_probe(GridParams a, const float2* twg, float2* base, long long p0, int nw,
+                                                   double* partials) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_rad[GRID_MAX_STAGES];
+  __shared__ double s_red[GT / 64];
+  float2 *ba, *bb;
+  const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
+  const long long hw = (long long)a.h * a.w;
+  const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
+  float2* wb = base + (long long)wl * (a.ckpt_every + 2) * hw;
+  double gsy = 0.0, gsx = 0.0;
+  probe_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p0 + wl / a.M), wl % a.M, wb + hw, gsy, gsx);
+  if (a.grad_shifts) {
+    const double ty = block_sum_fp64<GT>(gsy, s_red);
+    const double tx = block_sum_fp64<GT>(gsx, s_red);
+    if (threadIdx.x == 0) {
+      partials[2 * (long long)blockIdx.x] = ty;
+      partials[2 * (long long)blockIdx.x + 1] = tx;
+    }
+  }
+}
+
+// dL/dshift of each position of the batch: its partials summed over row groups and modes in a fixed order.
+__global__ __launch_bounds__(GT) void k_gsa_shift_reduce(GridParams a, const double* partials, long long p0, int np,
+                                                          int row_groups) {
+  const int pl = blockIdx.x * GT + threadIdx.x;
+  if (pl >= np) return;
+  const int nw = np * a.M;
+  double sy = 0.0, sx = 0.0;
+  for (int g = 0; g < row_groups; ++g)
+    for (int m = 0; m < a.M; ++m) {
+      const long long b = (long long)g * nw + (long long)pl * a.M + m;
+      sy += partials[2 * b];
+      sx += partials[2 * b + 1];
+    }
+  a.grad_shifts[2 * (p0 + pl)] += sy;
+  a.grad_shifts[2 * (p0 + pl) + 1] += sx;
 }
 
 int factor_edge(int L, int* r) {
@@ -723,7 +848,12 @@ size_t grid_workspace_bytes(int h, int w, int ckpt_every, long long num_position
   const size_t per = grid_block_bytes(h, w, ckpt_every, adjoint);
   long long b = std::min(num_positions, (long long)MAX_BLOCKS);
   b = std::min(b, std::max(1LL, (long long)(WS_BUDGET / per)));
-  return per * (size_t)std::max(1LL, b) + (adjoint ? 0 : twiddle_bytes(h, w));
+  b = std::max(1LL, b);
+  if (!adjoint) return per * (size_t)b + twiddle_bytes(h, w);
+  // the streaming adjoint also keeps the twiddle tables and (row groups) x 2 FP64 shift partials per position
+  const int gx = std::max(1, std::min(GROUP_ELEMS / w, h));
+  const size_t row_groups = (size_t)((h + gx - 1) / gx);
+  return (per + row_groups * 16) * (size_t)b + twiddle_bytes(h, w) + 256;
 }
 
 hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStream_t st) {
@@ -740,7 +870,7 @@ hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStrea
     const size_t lds_rows = (size_t)(a.w + 2 * a.gx * a.lsx) * sizeof(float2);
     const size_t lds_cols = (size_t)(a.h + 2 * a.gy * a.lsy) * sizeof(float2);
     hipError_t e = set_lds((const void*)k_gs_incident, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<false>, lds_rows);
     if (e == hipSuccess) e = set_lds((const void*)k_gs_final, lds_rows);
     if (e == hipSuccess) e = set_lds((const void*)k_gs_cols, lds_cols);
     if (e != hipSuccess) return e;
@@ -750,13 +880,14 @@ hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStrea
     for (long long p0 = 0; p0 < a.P; p0 += bpos) {
       const int np = (int)std::min(bpos, a.P - p0);
       const int nw = np * a.M;
-      hipLaunchKernelGGL(k_gs_incident, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, p0,
-                         nw);
+      hipLaunchKernelGGL(k_gs_incident, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, hw,
+                         p0, nw);
       for (int s = 0; s < a.S; ++s) {
         hipLaunchKernelGGL(k_gs_cols, dim3((unsigned)(nw * col_groups)), dim3(GT), lds_cols, st, a, tw, waves, p0,
                            nw, s);
         if (s + 1 < a.S)
-          hipLaunchKernelGGL(k_gs_rows, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, nw);
+          hipLaunchKernelGGL(k_gs_rows<false>, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves,
+                             hw, nw);
         else
           hipLaunchKernelGGL(k_gs_final, dim3((unsigned)(np * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, p0,
                              np);
@@ -775,7 +906,61 @@ hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStrea
 hipError_t launch_grid_adjoint(GridParams a, void* ws, size_t ws_bytes, hipStream_t st) {
   a.ckpt_every = clamp_ckpt(a.ckpt_every, a.S);
   const size_t lds = prepare(a);
-  const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, a.ckpt_every, true), ws_bytes);
+  const int k = a.ckpt_every;
+  const long long hw = (long long)a.h * a.w;
+  const long long row_groups = (a.h + a.gx - 1) / a.gx;
+  const long long col_groups = (a.w + a.gy - 1) / a.gy;
+  const size_t twb = twiddle_bytes(a.h, a.w);
+  // per position: M waves of (k + 2) h w elements and M x row_groups pairs of shift partial sums
+  const size_t per_pos = (size_t)a.M * ((size_t)(k + 2) * (size_t)hw * sizeof(float2) + (size_t)row_groups * 16);
+  if (hw >= stream_min_pixels() && ws_bytes >= twb + 256 + per_pos) {
+    const long long bpos = std::min(a.P, (long long)((ws_bytes - twb - 256) / per_pos));
+    float2* tw = (float2*)ws;
+    double* partials = (double*)((char*)ws + twb);
+    const size_t waves_off = (twb + (size_t)bpos * a.M * row_groups * 16 + 255) & ~(size_t)255;
+    float2* base = (float2*)((char*)ws + waves_off);
+    const long long stride = (long long)(k + 2) * hw;
+    const size_t lds_rows = (size_t)(a.w + 2 * a.gx * a.lsx) * sizeof(float2);
+    const size_t lds_cols = (size_t)(a.h + 2 * a.gy * a.lsy) * sizeof(float2);
+    hipError_t e = set_lds((const void*)k_gs_incident, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<false>, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<true>, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gsa_seed, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gsa_probe, lds_rows);
+    if (e == hipSuccess) e = set_lds((const void*)k_gsa_recompute, lds_cols);
+    if (e == hipSuccess) e = set_lds((const void*)k_gsa_back, lds_cols);
+    if (e != hipSuccess) return e;
+    const int nc = num_checkpoints(a.S, k);
+    hipLaunchKernelGGL(k_grid_twiddles, dim3((unsigned)((a.h + a.w + GT - 1) / GT)), dim3(GT), 0, st, a.h, a.w, tw);
+    for (long long p0 = 0; p0 < a.P; p0 += bpos) {
+      const int np = (int)std::min(bpos, a.P - p0);
+      const int nw = np * a.M;
+      const dim3 rows((unsigned)(nw * row_groups)), cols((unsigned)(nw * col_groups));
+      for (int c = nc; c >= 0; --c) {
+        const int s0 = c * k, s1 = std::min(a.S, s0 + k);
+        if (c == 0) hipLaunchKernelGGL(k_gs_incident, rows, dim3(GT), lds_rows, st, a, tw, base, stride, p0, nw);
+        for (int s = s0; s < s1; ++s) {
+          hipLaunchKernelGGL(k_gsa_recompute, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, c);
+          if (s + 1 < s1)
+            hipLaunchKernelGGL(k_gs_rows<false>, rows, dim3(GT), lds_rows, st, a, tw, base, stride, nw);
+          else if (s == a.S - 1)
+            hipLaunchKernelGGL(k_gsa_seed, rows, dim3(GT), lds_rows, st, a, tw, base, p0, nw);
+        }
+        for (int s = s1 - 1; s >= s0; --s) {
+          hipLaunchKernelGGL(k_gsa_back, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, s0);
+          if (s > 0)
+            hipLaunchKernelGGL(k_gs_rows<true>, rows, dim3(GT), lds_rows, st, a, tw, base + hw, stride, nw);
+          else
+            hipLaunchKernelGGL(k_gsa_probe, rows, dim3(GT), lds_rows, st, a, tw, base, p0, nw, partials);
+        }
+      }
+      if (a.grad_shifts)
+        hipLaunchKernelGGL(k_gsa_shift_reduce, dim3((unsigned)((np + GT - 1) / GT)), dim3(GT), 0, st, a,
+                           (const double*)partials, p0, np, (int)row_groups);
+    }
+    return hipGetLastError();
+  }
+  const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, k, true), ws_bytes);
   if (nb < 1) return hipErrorInvalidValue;
   hipError_t e = set_lds((const void*)k_grid_adjoint, lds);
   if (e != hipSuccess) return e;

@@ -268,3 +268,37 @@ def test_streaming_forward_matches_oracle_and_per_position_kernel(monkeypatch):
     err = rel_l2(pat_s.cpu().numpy(), want)
     print(f"streaming grid forward {h}x{w} (2 modes): rel-L2 {err:.2e}", flush=True)
     assert err < TOL
+
+
+def test_streaming_adjoint_matches_oracle_and_per_position_kernel(monkeypatch):
+    """The adjoint on grids of 2^14 pixels or more runs one launch per pass; it matches the oracle's VJP and the
+    per-position adjoint (float32 atomics in both: equal to rounding)."""
+    from ptyrodactyl_amd import engine
+    h, w, s_count, k, p_count, modes = 150, 130, 5, 2, 3, 2
+    slab, probes, origins, shifts, weights = _case(h, w, s_count, p_count, seed=43, modes=modes)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    g = torch.tensor(np.random.default_rng(9).standard_normal((p_count, h, w)), dtype=torch.float32, device=DEV)
+
+    def grads():
+        _, ckpt = engine.multislice_forward_checkpointed(*ops, k)
+        g_shift = torch.zeros((p_count, 2), dtype=torch.float64, device=DEV)
+        g_slab, g_pk = engine.multislice_adjoint(*ops, g, ckpt, k, grad_shifts=g_shift)
+        torch.cuda.synchronize()
+        return g_slab.cpu().numpy(), g_pk.cpu().numpy(), g_shift.cpu().numpy()
+
+    s_slab, s_pk, s_shift = grads()
+    monkeypatch.setenv("PTY_GRID_STREAM_MIN_PIXELS", str(1 << 40))   # the per-position kernel
+    p_slab, p_pk, p_shift = grads()
+    prop = _oracle_prop(h, w)
+    want_slab, want_shift, want_probe = 0, 0, []
+    for m in range(modes):
+        gs, gp, gsh = o.multislice_vjp(slab.astype(np.complex128), probes[m], origins, shifts, prop,
+                                       weights[m] * g.cpu().numpy().astype(np.float64), want_shifts=True)
+        want_slab, want_shift = want_slab + gs, want_shift + gsh
+        want_probe.append(gp)
+    s_probe = np.fft.ifft2(s_pk.astype(np.complex128)) * (h * w)
+    errs = (rel_l2(s_slab, want_slab), rel_l2(s_probe, np.stack(want_probe)), rel_l2(s_shift, want_shift))
+    print(f"streaming grid adjoint {h}x{w} (k={k}, {modes} modes): slab {errs[0]:.2e}, probe {errs[1]:.2e}, "
+          f"shifts {errs[2]:.2e}", flush=True)
+    assert max(errs) < TOL
+    assert rel_l2(s_slab, p_slab) < 1e-6 and rel_l2(s_pk, p_pk) < 1e-6 and rel_l2(s_shift, p_shift) < 1e-6

@@ -302,3 +302,19 @@ def test_streaming_adjoint_matches_oracle_and_per_position_kernel(monkeypatch):
           f"shifts {errs[2]:.2e}", flush=True)
     assert max(errs) < TOL
     assert rel_l2(s_slab, p_slab) < 1e-6 and rel_l2(s_pk, p_pk) < 1e-6 and rel_l2(s_shift, p_shift) < 1e-6
+
+
+def test_checked_wrappers_are_transparent_on_odd_grid():
+    """checked_stem_4d / checked_cbed_image validate and then return the bare functions' results bitwise
+    (the reference's test_checked.py:209-324), here on a 15 x 32 grid."""
+    import ptyrodactyl_amd as pa
+    rng = np.random.default_rng(29)
+    h, w, calib, kv = 15, 32, 0.25, 200.0
+    pot = pa.create_potential_slices(torch.tensor(rng.uniform(0, 10, (h, w, 2)), device=DEV), 1.0, calib)
+    mic = pa.create_microscope_config(voltage_kv=kv, aperture_mrad=24.0, probe_shape=(h, w))
+    probe = pa.make_probe(mic, pa.create_detector_config(real_space_calib_ang=calib))
+    beam = pa.create_probe_modes(probe[..., None].to(DEV), torch.tensor([1.0]), calib)
+    det = pa.create_detector_config(real_space_calib_ang=calib,
+                                    scan_positions_px=torch.tensor([[1.0, 2.0], [3.0, 4.0]], dtype=torch.float64))
+    assert torch.equal(pa.checked_stem_4d(pot, beam, mic, det).data, pa.stem_4d(pot, beam, mic, det).data)
+    assert torch.equal(pa.checked_cbed_image(pot, beam, mic).data_array, pa.cbed_image(pot, beam, mic).data_array)

@@ -20,7 +20,7 @@ torch = pytest.importorskip("torch")
 TOL = 1e-5
 DEV = "cuda"
 
-SHAPES = [(8, 8), (15, 32), (31, 32), (48, 40), (1, 12), (9, 1), (97, 61), (100, 100), (243, 200)]
+SHAPES = [(8, 8), (15, 32), (31, 32), (48, 40), (1, 12), (9, 1), (22, 33), (97, 61), (100, 100), (243, 200)]
 
 
 def rel_l2(a, b):

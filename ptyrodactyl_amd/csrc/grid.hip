@@ -451,9 +451,16 @@ __global__ __launch_bounds__(GT) void k_grid_forward(GridParams a, float2* ws) {
   const int h = a.h, w = a.w;
   const long long hw = (long long)h * w;
   float2* wb = ws + (long long)blockIdx.x * hw;
-  for (long long p = blockIdx.x; p < a.P; p += gridDim.x) {
+  // With a pattern output a workgroup runs all modes of its position (one writer per pattern pixel);
+  // amplitudes alone make every (position, mode) wave an independent task.
+  const bool per_wave = a.patterns == nullptr;
+  const long long tasks = per_wave ? a.P * a.M : a.P;
+  for (long long t = blockIdx.x; t < tasks; t += gridDim.x) {
+    const long long p = per_wave ? t / a.M : t;
+    const int m0 = per_wave ? (int)(t % a.M) : 0;
+    const int m1 = per_wave ? m0 + 1 : a.M;
     const Position q = position(a, p);
-    for (int m = 0; m < a.M; ++m) {
+    for (int m = m0; m < m1; ++m) {
       incident_rows(a, u.ax, u.bufa, u.bufb, 0, h, q, m, wb);
       for (int s = 0; s < a.S; ++s) {
         slice_cols(a, u.ay, u.bufa, u.bufb, 0, w, q, s, wb, checkpoint_of(a, p, m, s));
@@ -895,7 +902,8 @@ hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStrea
     }
     return hipGetLastError();
   }
-  const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, 1, false), ws_bytes);
+  const long long tasks = a.patterns ? a.P : a.P * a.M;
+  const long long nb = blocks_for(tasks, grid_block_bytes(a.h, a.w, 1, false), ws_bytes);
   if (nb < 1) return hipErrorInvalidValue;
   hipError_t e = set_lds((const void*)k_grid_forward, lds);
   if (e != hipSuccess) return e;

@@ -102,7 +102,9 @@ def _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop, want_patte
     if ckpt_every:
         ckpt = torch.empty((p_count * m_count, nc, h, w), dtype=torch.complex64, device=dev)
     lib = _native.load()
-    ws_bytes = int(lib.pty_grid_workspace_bytes(h, w, s_count, 1, p_count, 0))
+    # without a pattern output every (position, mode) wave is its own task: size the workspace for all of them
+    tasks = p_count if want_patterns else p_count * m_count
+    ws_bytes = int(lib.pty_grid_workspace_bytes(h, w, s_count, 1, tasks, 0))
     workspace = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
         status = lib.pty_forward_grid(

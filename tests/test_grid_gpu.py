@@ -318,3 +318,22 @@ def test_checked_wrappers_are_transparent_on_odd_grid():
                                     scan_positions_px=torch.tensor([[1.0, 2.0], [3.0, 4.0]], dtype=torch.float64))
     assert torch.equal(pa.checked_stem_4d(pot, beam, mic, det).data, pa.stem_4d(pot, beam, mic, det).data)
     assert torch.equal(pa.checked_cbed_image(pot, beam, mic).data_array, pa.cbed_image(pot, beam, mic).data_array)
+
+
+def test_amplitude_only_forward_runs_modes_as_independent_waves():
+    """Without a pattern output the per-position kernel spreads (position, mode) waves over workgroups (the
+    ensemble path sends many waves as the modes of one position); amplitudes equal the pattern-path ones."""
+    from ptyrodactyl_amd import engine
+    h, w, modes = 36, 44, 24
+    slab, probes, origins, shifts, weights = _case(h, w, 3, 1, seed=47, modes=modes)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    _, amp_only = engine.multislice_forward(*ops, want_patterns=False, want_amplitude=True)
+    pat, amp = engine.multislice_forward(*ops, want_amplitude=True)
+    torch.cuda.synchronize()
+    assert torch.equal(amp_only, amp)
+    inten = (amp.abs() ** 2 * torch.tensor(weights, dtype=torch.float32, device=DEV)[None, :, None, None]).sum(1)
+    assert rel_l2(pat.cpu().numpy(), inten.cpu().numpy()) < 1e-6
+    prop = _oracle_prop(h, w)
+    want = np.stack([o.window_stem(slab.astype(np.complex128), probes[m], origins, shifts, prop,
+                                   return_amplitude=True)[1] for m in range(modes)], axis=1)
+    assert rel_l2(amp_only.cpu().numpy(), want) < TOL

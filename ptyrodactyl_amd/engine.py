@@ -236,7 +236,8 @@ class CapturedForward:
     """A forward over fixed operands captured once in a HIP graph and replayed.
 
     The launch sequence of :func:`multislice_forward` (one launch per position
-    group for n <= 128, 2S + 2 launches per batch for n >= 256) is recorded on
+    group for n <= 128, 2S + 2 launches per batch for n >= 256; one launch, or
+    2S + 2 per batch when streaming, on the runtime-size grids) is recorded on
     a capture stream after one eager warm-up call, together with its
     workspace; :meth:`replay` re-runs it on the current stream with whatever
     the operand tensors hold at that time (update them in place between
@@ -245,10 +246,13 @@ class CapturedForward:
     """
 
     def __init__(self, slab_t, probe_k, weights, origins, shifts_px, prop, max_batch_waves: Optional[int] = None):
-        _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop)
+        if is_fast_grid(probe_k.shape[-2], probe_k.shape[-1]):
+            _check_forward_operands(slab_t, probe_k, weights, origins, shifts_px, prop)
+        else:
+            _grid_operands(slab_t, probe_k, weights, origins, shifts_px, prop)
         self.operands = (slab_t, probe_k, weights, origins, shifts_px, prop)
-        p_count, n = origins.shape[0], probe_k.shape[-1]
-        self.patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=slab_t.device)
+        p_count, h, w = origins.shape[0], probe_k.shape[-2], probe_k.shape[-1]
+        self.patterns = torch.empty((p_count, h, w), dtype=torch.float32, device=slab_t.device)
         self.max_batch_waves = max_batch_waves
         multislice_forward(*self.operands, patterns_out=self.patterns, max_batch_waves=max_batch_waves)  # warm-up
         torch.cuda.synchronize(slab_t.device)

@@ -337,3 +337,21 @@ def test_amplitude_only_forward_runs_modes_as_independent_waves():
     want = np.stack([o.window_stem(slab.astype(np.complex128), probes[m], origins, shifts, prop,
                                    return_amplitude=True)[1] for m in range(modes)], axis=1)
     assert rel_l2(amp_only.cpu().numpy(), want) < TOL
+
+
+@pytest.mark.parametrize("shape", [(30, 44), (150, 130)])
+def test_captured_forward_on_runtime_size_grid_replays_bitwise(shape):
+    """engine.CapturedForward on the runtime-size kernels (per-position at 30 x 44, streaming at 150 x 130):
+    replays after in-place operand updates, with eager work between them, equal the eager forward bitwise."""
+    from ptyrodactyl_amd import engine
+    h, w = shape
+    slab, probes, origins, shifts, weights = _case(h, w, 3, 5, seed=53, modes=2)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    cap = engine.CapturedForward(*ops)
+    for i in range(3):
+        ops[4].add_(0.25)  # move the scan in place
+        ops[0].mul_(torch.polar(torch.ones((), device=DEV), torch.tensor(0.01 * (i + 1), device=DEV)))
+        replay = cap.replay().clone()
+        eager, _ = engine.multislice_forward(*ops)
+        torch.cuda.synchronize()
+        assert torch.equal(replay, eager), f"replay {i} differs"

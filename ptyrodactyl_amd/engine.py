@@ -177,12 +177,14 @@ def multislice_forward(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: tor
                        max_batch_waves: Optional[int] = None) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor]]:
     """Windowed multislice forward on the GPU.
 
-    slab_t (S, Hs, Ws) complex64 transmission; probe_k (M, n, n) complex64
+    slab_t (S, Hs, Ws) complex64 transmission; probe_k (M, h, w) complex64
     fft2 of the probe modes; weights (M,) float32; origins (P, 2) int32;
-    shifts_px (P, 2) float64; prop (n,) complex64 separable propagator
-    factor including 1/n.  Returns (patterns (P, n, n) float32 fftshifted,
-    amplitude (P, M, n, n) complex64 fftshifted) with None for outputs not
-    requested.
+    shifts_px (P, 2) float64; prop the separable propagator factor including
+    1/n ((n,) complex64 on a square grid) or a pair (prop_y (h,), prop_x (w,)).
+    Returns (patterns (P, h, w) float32 fftshifted, amplitude (P, M, h, w)
+    complex64 fftshifted) with None for outputs not requested.  Square
+    power-of-two windows run on ``pty_forward`` (``max_batch_waves`` caps its
+    streaming batch); any other h x w (edges up to 4096) on ``pty_forward_grid``.
     """
     _require(isinstance(slab_t, torch.Tensor) and slab_t.is_cuda,
              "slab must be a GPU tensor (the multislice path has no CPU fallback)")

@@ -191,3 +191,17 @@ def test_annular_detector_host_errors():
                                     collection_outer_mrad=1000.0, scan_shape=(2, 3))
     with pytest.raises(ValueError, match="GPU tensor"):
         pa.annular_detector(stem, det)
+
+
+def test_grid_edges_beyond_the_kernels_raise_before_any_device_work():
+    """Any grid shape with edges up to 4096 is accepted; larger ones raise ValueError from the host checks
+    (before any device work: this runs without a GPU)."""
+    import ptyrodactyl_amd as pa
+    from ptyrodactyl_amd import simulations
+    assert simulations._grid_shape(15, 32) == (15, 32)
+    assert simulations._grid_shape(1, 4096) == (1, 4096)
+    with pytest.raises(ValueError, match=r"each edge must be in \[1, 4096\]"):
+        simulations._grid_shape(4097, 8)
+    with pytest.raises(ValueError, match="each edge must be in"):
+        pa.crystal2stem4d(pa.create_crystal_data([[0.0, 0.0, 0.0], [3.0, 2.0, 1.0]], [6, 14]), [[1.0, 1.0]],
+                          300.0, 20.0, real_space_pixel_size_ang=0.002)

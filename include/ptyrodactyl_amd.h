@@ -324,8 +324,10 @@ int pty_forward_persistent(int n, int num_slices, long long num_positions, const
  * Device scratch for pty_forward_grid (adjoint = 0) or pty_adjoint_grid (adjoint = 1) with the
  * default number of workgroups; any workspace of at least the bytes of one workgroup works
  * (h * w * 8 for the forward, h * w * 8 * (min(ckpt_every, num_slices) + 2) for the adjoint),
- * with fewer positions in flight.  A forward without a pattern output runs every (position,
- * mode) wave as its own task: pass num_positions * num_modes to size it for all of them.
+ * with fewer positions in flight.  Pass num_positions * num_modes: a forward without a pattern
+ * output runs every (position, mode) wave as its own task, and the streaming kernels (grids of
+ * 2^14 pixels or more) keep all num_modes waves of a position in flight, so a workspace sized
+ * for fewer waves than num_modes falls back to one workgroup per position.
  */
 size_t pty_grid_workspace_bytes(int h, int w, int num_slices, int ckpt_every, long long num_positions, int adjoint);
 

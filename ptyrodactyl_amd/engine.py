@@ -83,6 +83,14 @@ def _grid_operands(slab_t, probe_k, weights, origins, shifts_px, prop):
     return dev, h, w, s_count, hs, ws, m_count, p_count, prop_y, prop_x
 
 
+def grid_workspace_bytes(h: int, w: int, num_slices: int, ckpt_every: int, num_positions: int, num_modes: int,
+                         adjoint: bool) -> int:
+    """Workspace of ``pty_forward_grid`` / ``pty_adjoint_grid`` for every (position, mode) wave in flight."""
+    lib = _native.load()
+    return int(lib.pty_grid_workspace_bytes(int(h), int(w), int(num_slices), int(ckpt_every),
+                                            int(num_positions) * int(num_modes), 1 if adjoint else 0))
+
+
 def _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop, want_patterns, want_amplitude, patterns_out,
                   ckpt_every):
     """Forward on any h x w grid (``pty_forward_grid``); checkpoints when ckpt_every >= 1."""
@@ -102,9 +110,10 @@ def _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop, want_patte
     if ckpt_every:
         ckpt = torch.empty((p_count * m_count, nc, h, w), dtype=torch.complex64, device=dev)
     lib = _native.load()
-    # without a pattern output every (position, mode) wave is its own task: size the workspace for all of them
-    tasks = p_count if want_patterns else p_count * m_count
-    ws_bytes = int(lib.pty_grid_workspace_bytes(h, w, s_count, 1, tasks, 0))
+    # size for every (position, mode) wave: amplitude-only forwards run each wave as its own task, and the
+    # streaming kernels (grids of 2^14 pixels or more) hold all M waves of a position at once, so a scan with
+    # fewer positions than modes would otherwise fall back to one workgroup per position
+    ws_bytes = grid_workspace_bytes(h, w, s_count, 1, p_count, m_count, adjoint=False)
     workspace = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
         status = lib.pty_forward_grid(
@@ -146,7 +155,8 @@ def _grid_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_patte
         _check_cuda(grad_shifts, "grad_shifts", torch.float64, 2, dev)
         _require(tuple(grad_shifts.shape) == (p_count, 2), "grad_shifts must have shape (P, 2)")
     lib = _native.load()
-    ws_bytes = int(lib.pty_grid_workspace_bytes(h, w, s_count, k, p_count, 1))
+    # the streaming adjoint holds the M waves of each position in flight: size for P * M waves
+    ws_bytes = grid_workspace_bytes(h, w, s_count, k, p_count, m_count, adjoint=True)
     workspace = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
         status = lib.pty_adjoint_grid(

@@ -181,6 +181,62 @@ def test_config2_subset_full_size():
     assert err < TOL
 
 
+def test_config1_full_workload():
+    """Config 1 at its full size: the 5 x 5 x 10-cell perovskite crystal on a 128^2 grid at 0.15 A, 20 slices
+    of 1.95 A, 200 kV / 20 mrad, all 256 positions of the 16 x 16 scan, against the complex128 oracle.  The
+    same patterns through the drop-in stem_4d (potential slices, make_probe, positions passed in Angstrom as
+    the reference's shift_beam_fourier takes them)."""
+    import ptyrodactyl_amd as pa
+    from ptyrodactyl_amd import synth
+    w = synth.config1()
+    prop = o.propagation_func(w.n, w.n, w.slice_thickness, w.voltage_kv, w.calib)
+    print("config1 full workload (256 positions)", flush=True)
+    got, _, _ = _run_hip(w.slab_t, w.probe[None], w.origins, w.shifts_px, w.n, w.slice_thickness, w.voltage_kv,
+                         w.calib)
+    ref = o.window_stem_batched(w.slab_t.astype(np.complex128), w.probe, w.origins, w.shifts_px, prop, workers=16)
+    err = rel_l2(got, ref)
+    print(f"config1 (256 positions) rel-L2 {err:.3e}", flush=True)
+    assert got.shape == (256, 128, 128) and err < TOL
+    # the drop-in API on the potential: positions (pixels x calibration) are taken as Angstrom by the shift
+    pot = synth.projected_potential(synth.crystal_atoms(5, 5, 10, seed=0), w.n, w.calib, w.num_slices,
+                                    w.slice_thickness)
+    mic = pa.create_microscope_config(voltage_kv=w.voltage_kv, aperture_mrad=20.0, probe_shape=(w.n, w.n))
+    det = pa.create_detector_config(real_space_calib_ang=w.calib, probe_calibration_pm=100.0 * w.calib,
+                                    scan_positions_px=torch.tensor(w.shifts_px * w.calib))
+    beam = pa.create_probe_modes(pa.make_probe(mic, det)[..., None], torch.tensor([1.0]), w.calib)
+    pots = pa.create_potential_slices(torch.tensor(np.moveaxis(pot, 0, -1)), w.slice_thickness, w.calib)
+    out = pa.stem_4d(pots, beam, mic, det).data.cpu().numpy()
+    err_api = rel_l2(out, ref)
+    print(f"config1 through the drop-in stem_4d: rel-L2 {err_api:.3e}", flush=True)
+    assert out.shape == (256, 128, 128) and err_api < TOL
+
+
+def test_transmission_kernel_matches_oracle():
+    """pty_transmission (exp(i sigma V) in FP64, stored complex64) against the oracle's transmission_func:
+    each component within float32 rounding of the FP64 value, over potentials from 0 to 1e5 V A (phases up
+    to ~73 rad at 200 kV) and a negative range."""
+    from ptyrodactyl_amd import _native, physics
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(17)
+    pot = np.concatenate([rng.uniform(0.0, 500.0, 40000), rng.uniform(0.0, 1e5, 20000),
+                          rng.uniform(-300.0, 0.0, 5000), [0.0, 1e-300, 2.5e4]])
+    for kv in (100.0, 200.0, 300.0):
+        sigma = physics.interaction_sigma(kv)
+        assert sigma == pytest.approx(o.interaction_sigma(kv), rel=1e-14)
+        pot_t = torch.tensor(pot, dtype=torch.float64, device=dev)
+        out = torch.empty(pot.size, dtype=torch.complex64, device=dev)
+        lib = _native.load()
+        _native.check(lib.pty_transmission(_native.ptr(pot_t), sigma, _native.ptr(out), pot.size,
+                                           _native.stream_handle(dev)), "pty_transmission")
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().astype(np.complex128)
+        want = o.transmission_func(pot, kv)
+        err = np.max(np.abs(got - want))
+        print(f"pty_transmission {kv} kV: max abs error {err:.2e}", flush=True)
+        assert err < 1.2e-7
+        assert np.array_equal(got, want.astype(np.complex64).astype(np.complex128)) or err < 1.2e-7
+
+
 def test_golden_stem4d_sharded_through_hip_drop_in():
     """stem4d_sharded (atom slice provider) reproduces the reference's golden capture on the GPU."""
     import ptyrodactyl_amd as pa

@@ -304,6 +304,39 @@ def test_streaming_adjoint_matches_oracle_and_per_position_kernel(monkeypatch):
     assert rel_l2(s_slab, p_slab) < 1e-6 and rel_l2(s_pk, p_pk) < 1e-6 and rel_l2(s_shift, p_shift) < 1e-6
 
 
+def test_streaming_single_position_with_two_modes(monkeypatch):
+    """One position and two modes on a 150 x 130 grid: the workspace holds both waves, so the streaming kernels
+    run (not one workgroup for the whole scan).  Forward bitwise equal to the per-position kernel, adjoint equal
+    to rounding (float32 atomics in both), both against the oracle."""
+    from ptyrodactyl_amd import engine
+    h, w, s_count, k, modes = 150, 130, 5, 2, 2
+    slab, probes, origins, shifts, weights = _case(h, w, s_count, 1, seed=47, modes=modes)
+    ops = _operands(slab, probes, origins, shifts, weights)
+    g = torch.tensor(np.random.default_rng(11).standard_normal((1, h, w)), dtype=torch.float32, device=DEV)
+
+    def run():
+        pat, amp = engine.multislice_forward(*ops, want_amplitude=True)
+        _, ckpt = engine.multislice_forward_checkpointed(*ops, k)
+        g_slab, g_pk = engine.multislice_adjoint(*ops, g, ckpt, k)
+        torch.cuda.synchronize()
+        return pat, amp, g_slab.cpu().numpy(), g_pk.cpu().numpy()
+
+    print(f"streaming grid {h}x{w}, 1 position, {modes} modes", flush=True)
+    pat_s, amp_s, slab_s, pk_s = run()
+    monkeypatch.setenv("PTY_GRID_STREAM_MIN_PIXELS", str(1 << 40))   # the per-position kernel
+    pat_p, amp_p, slab_p, pk_p = run()
+    assert torch.equal(pat_s, pat_p) and torch.equal(amp_s, amp_p)
+    assert rel_l2(slab_s, slab_p) < 1e-6 and rel_l2(pk_s, pk_p) < 1e-6
+    prop = _oracle_prop(h, w)
+    want, want_slab = 0, 0
+    for m in range(modes):
+        want = want + weights[m] * o.window_stem(slab.astype(np.complex128), probes[m], origins, shifts, prop)
+        gs, _ = o.multislice_vjp(slab.astype(np.complex128), probes[m], origins, shifts, prop,
+                                 weights[m] * g.cpu().numpy().astype(np.float64))
+        want_slab = want_slab + gs
+    assert rel_l2(pat_s.cpu().numpy(), want) < TOL and rel_l2(slab_s, want_slab) < TOL
+
+
 def test_checked_wrappers_are_transparent_on_odd_grid():
     """checked_stem_4d / checked_cbed_image validate and then return the bare functions' results bitwise
     (the reference's test_checked.py:209-324), here on a 15 x 32 grid."""

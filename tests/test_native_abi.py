@@ -112,3 +112,19 @@ def test_grid_entry_points_validate_on_host():
     rc = lib.pty_adjoint_grid(15, 32, 1, 1, 1, fake, 15, 32, fake, fake, fake, fake, fake, fake, fake, None, None, 1,
                               fake, None, None, fake, 100, None)
     assert rc == _native.PTY_ERR_INVALID and b"workspace" in lib.pty_last_error()
+
+
+def test_grid_workspace_holds_every_mode_of_a_position():
+    """The streaming runtime-size kernels (grids of 2^14 pixels or more) keep the M waves of a position in
+    flight: the engine's workspace for one position and two modes must hold them (plus the twiddle tables), or
+    the launcher falls back to one workgroup per position."""
+    from ptyrodactyl_amd import _native, engine
+    if not _native.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    h, w, s_count, k = 150, 130, 5, 2
+    wave = h * w * 8
+    twiddles = ((h + w) * 8 + 255) // 256 * 256
+    fwd = engine.grid_workspace_bytes(h, w, s_count, 1, 1, 2, adjoint=False)
+    assert fwd >= twiddles + 2 * wave
+    adj = engine.grid_workspace_bytes(h, w, s_count, k, 1, 2, adjoint=True)
+    assert adj >= twiddles + 256 + 2 * (k + 2) * wave

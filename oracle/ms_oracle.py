@@ -367,6 +367,36 @@ def window_stem_batched(slab_t, probe, origins, shifts_px, prop, workers: int = 
     return (np.abs(amp) ** 2).astype(np.float64)
 
 
+def window_stem_perturbed(slab_t, probe, origins, shifts_px, prop, index, deltas, workers: int = 1):
+    """``window_stem_batched`` of ``slab_t`` with ``deltas[j]`` added to the one element ``index`` = (s, y, x),
+    for every j: (len(deltas), P, h, w) patterns.  The waves entering slice s do not depend on the
+    perturbation, so they are computed once and shared by all variants (central differences in the
+    real and imaginary directions need four variants of one pixel)."""
+    n = probe.shape[-1]
+    s_pix, y_pix, x_pix = (int(v) for v in index)
+    origins = np.asarray(origins)
+    shifts_px = np.asarray(shifts_px, np.float64)
+    probe_k = _fft2(np.asarray(probe, np.complex128))
+    ramps = np.stack([ramp_1d(n, sy)[:, None] * ramp_1d(n, sx)[None, :] for sy, sx in shifts_px])
+    wave = _ifft2(probe_k[None] * ramps, workers=workers)
+    s_count = slab_t.shape[0]
+    for s in range(s_pix):
+        wave = wave * np.stack([window_of(slab_t[s], o, n) for o in origins])
+        wave = _ifft2(_fft2(wave, workers=workers) * prop, workers=workers)
+    out = []
+    for d in deltas:
+        sl = np.array(slab_t[s_pix:], np.complex128)
+        sl[0, y_pix, x_pix] += d
+        w = wave.copy()
+        for s in range(sl.shape[0]):
+            w = w * np.stack([window_of(sl[s], o, n) for o in origins])
+            if s_pix + s != s_count - 1:
+                w = _ifft2(_fft2(w, workers=workers) * prop, workers=workers)
+        amp = np.fft.fftshift(_fft2(w, workers=workers), axes=(-2, -1))
+        out.append(np.abs(amp) ** 2)
+    return np.stack(out)
+
+
 # ---------------------------------------------------------------------------
 # Gradient (reverse mode) of the windowed forward, complex128.
 # ---------------------------------------------------------------------------

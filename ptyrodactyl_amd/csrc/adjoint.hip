@@ -26,16 +26,16 @@ enum : int { C_PROP = 0, C_ADJPROP = 1, C_IFFT_PROBE = 2, C_FINAL_ADJ = 3, C_PRO
 
 // Row work shared by both strategies: v holds row y of one wave in the
 // strided distribution (thread t owns x = t + T q).
-template <int N, int PRE, int POST, class Buf>
+template <int N, int PRE, int POST, class Buf, class TW, class FX>
 __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, int pw, const AdjParams& a,
                                         const WaveCtx& g, float2* seg_row, bool seg_store, float2* gt_row,
-                                        const Buf& rb, const TwTable& twt, const float2* px) {
+                                        const Buf& rb, const TW& twt, const FX& px) {
   constexpr int T = N / 16;
   if constexpr (PRE == PRE_IFFT) line_fft<N, true>(v, t, rb, twt);
   if constexpr (PRE == PRE_ADJPROP) {
     line_fft<N, false>(v, t, rb, twt);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = cmulc(v[q], px[t + T * q]);
+    for (int q = 0; q < 16; ++q) v[q] = px.mulc(v[q], t + T * q);
     line_fft<N, true>(v, t, rb, twt);
   }
   if (seg_store) {
@@ -87,7 +87,7 @@ __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, in
   if constexpr (POST == POST_PROP) {
     line_fft<N, false>(v, t, rb, twt);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], px[t + T * q]);
+    for (int q = 0; q < 16; ++q) v[q] = px.mul(v[q], t + T * q);
     line_fft<N, true>(v, t, rb, twt);
   }
   if constexpr (POST == POST_FFT) line_fft<N, false>(v, t, rb, twt);
@@ -95,9 +95,9 @@ __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, in
 
 // Column work: v holds column x (thread t owns ky = t + T q).  Returns false
 // when the result was consumed (probe gradient) and must not be stored.
-template <int N, int OP, class Buf>
+template <int N, int OP, class Buf, class TW, class FX>
 __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long w, const AdjParams& a,
-                                        const WaveCtx& g, const Buf& cb, const TwTable& twt, const float2* py,
+                                        const WaveCtx& g, const Buf& cb, const TW& twt, const FX& py,
                                         const float2* ramp_y, const float2* ramp_x) {
   constexpr int T = N / 16;
   const int p = (int)(w / a.f.M), m = (int)(w % a.f.M);
@@ -116,7 +116,7 @@ __device__ __forceinline__ bool col_ops(float2 (&v)[16], int t, int x, long long
     line_fft<N, false>(v, t, cb, twt);
     if constexpr (OP == C_PROP || OP == C_ADJPROP) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = OP == C_PROP ? cmul(v[q], py[t + T * q]) : cmulc(v[q], py[t + T * q]);
+      for (int q = 0; q < 16; ++q) v[q] = OP == C_PROP ? py.mul(v[q], t + T * q) : py.mulc(v[q], t + T * q);
       line_fft<N, true>(v, t, cb, twt);
       return true;
     } else if constexpr (OP == C_FINAL_ADJ) {
@@ -173,9 +173,9 @@ struct ColBufRes {
 template <int N>
 struct ResShared {
   float2* wave;
-  const float2* tw;
-  const float2* py;
-  const float2* px;
+  TwTable tw;
+  FixTable py;
+  FixTable px;
   const float2* ramp_y;
   const float2* ramp_x;
 };
@@ -193,7 +193,7 @@ __device__ __forceinline__ void res_row(const ResShared<N>& sh, const AdjParams&
   const int lane = tid & 63, wid = tid >> 6;
   const int rt = lane % T, ry = wid * G + lane / T;
   const RowBufWave rb{sh.wave + pad16(ry) * RS};
-  const TwTable twt{sh.tw};
+  const TwTable twt = sh.tw;
   float2 v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = rb.row[pad16(rt + T * q)];
@@ -214,7 +214,7 @@ __device__ __forceinline__ void res_col(const ResShared<N>& sh, const AdjParams&
   const int ct = lane / G, cx = wid * G + lane % G;
   float2* col = sh.wave + pad16(cx);
   const ColBufRes cb{col, RS};
-  const TwTable twt{sh.tw};
+  const TwTable twt = sh.tw;
   float2 v[16];
   if constexpr (OP != C_IFFT_PROBE) {
 #pragma unroll
@@ -263,9 +263,12 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
   float2* px = py + N;
   float2* ramp_y = px + N;
   float2* ramp_x = ramp_y + N;
+  float2* twf = ramp_x + N;  // modulus-correcting low parts of tw, py, px
+  float2* pyf = twf + N;
+  float2* pxf = pyf + N;
   const int tid = threadIdx.x;
-  fill_twiddles(tw, N, tid, NT);
-  const ResShared<N> sh{wave, tw, py, px, ramp_y, ramp_x};
+  fill_twiddles(tw, twf, N, tid, NT);
+  const ResShared<N> sh{wave, TwTable{tw, twf}, FixTable{py, pyf}, FixTable{px, pxf}, ramp_y, ramp_x};
   const int S = a.f.S, k = a.k;
   const int nc = num_checkpoints(S, k), C = nc + 1;
   float2* seg = scratch + (size_t)blockIdx.x * (k + 1) * N * N;
@@ -275,9 +278,9 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
     const WaveCtx g = resolve_wave(a.f, p);
     float2* gtw = a.gt_contrib ? a.gt_contrib + (size_t)(w - a.w_begin) * a.contrib_slices * N * N : nullptr;
     __syncthreads();
+    stage_factors(py, pyf, g.prop_y, N, tid, NT);
+    stage_factors(px, pxf, g.prop_x, N, tid, NT);
     for (int i = tid; i < N; i += NT) {
-      py[i] = g.prop_y[i];
-      px[i] = g.prop_x[i];
       ramp_y[i] = shift_ramp(i, N, g.my, g.dy);
       ramp_x[i] = shift_ramp(i, N, g.mx, g.dx);
     }
@@ -327,7 +330,7 @@ __global__ __launch_bounds__(N* N / 16) void k_adjoint_resident(AdjParams a, flo
 
 template <int N>
 static size_t resident_adj_lds_bytes() {
-  return sizeof(float2) * ((size_t)(N + N / 16) * (N + N / 16) + 5 * N);
+  return sizeof(float2) * ((size_t)(N + N / 16) * (N + N / 16) + 8 * N);
 }
 
 // ---------------------------------------------------------------------------
@@ -346,17 +349,21 @@ __global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, Wave
                                                   int nb, int s, int pw, int seg_idx, int seg_store) {
   constexpr int T = N / 16, RPB = 256 / T, RS = N + N / 16;
   __shared__ float2 rows[RPB * RS];
+  constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 tw[N];
   __shared__ float2 px[N];
+  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 pxf[DF ? N : 1];
   const int tid = threadIdx.x;
   const int wl = blockIdx.x % nb, rblk = blockIdx.x / nb;
   const long long w = w0 + wl;
   const int p = (int)(w / a.f.M);
   const int t = tid % T, lr = tid / T, y = rblk * RPB + lr;
   const WaveCtx g = resolve_wave(a.f, p);
-  fill_twiddles(tw, N, tid, 256);
-  for (int i = tid; i < N; i += 256) px[i] = g.prop_x[i];
-  const TwTable twt{tw};
+  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  stage_factors<DF>(px, pxf, g.prop_x, N, tid, 256);
+  const TwTab<DF> twt{tw, twf};
+  const FixTab<DF> pxt{px, pxf};
   const typename RowBufFor<(T <= 64)>::type rb{rows + lr * RS};
   __syncthreads();
   const float2* srow = src.at(wl) + (size_t)y * N;
@@ -366,7 +373,7 @@ __global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, Wave
   float2* seg_row = segb.base ? segb.at(wl) + ((size_t)seg_idx * N + y) * N : nullptr;
   // contributions of the k slices of the current segment: [wave][slice - segment start][n][n]
   float2* gt_row = a.gt_contrib ? a.gt_contrib + (((size_t)wl * a.contrib_slices + seg_idx) * N + y) * N : nullptr;
-  row_ops<N, PRE, POST>(v, t, y, s, pw, a, g, seg_row, seg_store != 0, gt_row, rb, twt, px);
+  row_ops<N, PRE, POST>(v, t, y, s, pw, a, g, seg_row, seg_store != 0, gt_row, rb, twt, pxt);
   float2* drow = dst.at(wl) + (size_t)y * N;
 #pragma unroll
   for (int q = 0; q < 16; ++q) drow[t + T * q] = v[q];
@@ -384,8 +391,11 @@ template <int N, int OP>
 __global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, WaveBuf dst, long long w0, int nb) {
   constexpr int T = N / 16, NC = 256 / T, PS = N + N / 16;
   __shared__ float2 buf[PS * NC];
+  constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 tw[N];
   __shared__ float2 py[N];
+  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 pyf[DF ? N : 1];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
   int wl, cblk;
@@ -394,9 +404,10 @@ __global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, Wave
   const int p = (int)(w / a.f.M);
   const int x = cblk * NC + gc;
   const WaveCtx g = resolve_wave(a.f, p);
-  fill_twiddles(tw, N, tid, 256);
-  for (int i = tid; i < N; i += 256) py[i] = g.prop_y[i];
-  const TwTable twt{tw};
+  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
+  const TwTab<DF> twt{tw, twf};
+  const FixTab<DF> pyt{py, pyf};
   const ColBufBlk<NC> cb{buf + gc};
   __syncthreads();
   float2 v[16];
@@ -405,7 +416,7 @@ __global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, Wave
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = scol[(size_t)(t + T * q) * N];
   }
-  if (col_ops<N, OP>(v, t, x, w, a, g, cb, twt, py, nullptr, nullptr)) {
+  if (col_ops<N, OP>(v, t, x, w, a, g, cb, twt, pyt, nullptr, nullptr)) {
     float2* dcol = dst.at(wl) + x;
 #pragma unroll
     for (int q = 0; q < 16; ++q) dcol[(size_t)(t + T * q) * N] = v[q];

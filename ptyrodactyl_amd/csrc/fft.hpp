@@ -25,6 +25,22 @@ PTY_HD float2 cmulc(float2 a, float2 b) {
   return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 PTY_HD float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// Modulus correction of a complex64 factor whose exact value has modulus r (r2 = r * r): the real d
+// with |z (1 + d)| = r to about 1e-15.  The tables below keep z * d as a second float2 (a double-float
+// pair with z) and multiply with cmul_df, so the correction survives the float32 rounding of the
+// product (scaling an already rounded product by 1 + d would round back to the same value).  Each float32 component carries a rounding error of up to half
+// an ulp, so |z| misses r by up to ~6e-8; the factors of a unitary operator (the FFT twiddles, the
+// Fresnel propagator) are applied the same way to every position and slice, so that modulus error
+// is systematic and grows with the number of transforms, while a phase error of the same size is
+// harmless.  A factor whose modulus differs from r by more than rounding explains is not a rounded
+// unit-modulus value and is used as given (d = 0).
+__device__ __forceinline__ float modulus_fix(float2 z, double r2) {
+  const double m2 = (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+  if (!(m2 > 0.0)) return 0.f;
+  const double d = 0.5 * (r2 - m2) / m2;
+  return fabs(d) < 2.5e-7 ? (float)d : 0.f;
+}
 PTY_HD float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 PTY_HD float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 // a * (-i) and a * (+i)
@@ -215,8 +231,8 @@ struct Plan<4096> {
 };
 
 // One Stockham pass.  NS = product of the radices of earlier passes.
-// Buf provides ld(e), st(e, v) and sync() for the group's scratch; TW(m)
-// returns W_L^m = exp(-2 pi i m / L).
+// Buf provides ld(e), st(e, v) and sync() for the group's scratch; TW::apply<INV>(x, m)
+// returns x * W_L^m (x * conj(W_L^m) when INV), W_L^m = exp(-2 pi i m / L).
 template <int L, int E, int R, int NS, bool INV, bool FIRST, bool LAST, class Buf, class TW>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[E], int t, Buf& buf, const TW& tw) {
   constexpr int T = L / E;
@@ -232,12 +248,12 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[E], int t, Buf& buf, c
   for (int b = 0; b < NB; ++b) {
     const int j = t + T * b;
     if constexpr (!FIRST) {
-      const int jm = j % NS;
+      // opaque twiddle index: the forward and inverse transforms of a line read the same entries, and
+      // reusing them across the pointwise multiply between the two would keep 45 values live
+      int jm = j % NS;
+      asm volatile("" : "+v"(jm));
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        float2 w = tw(jm * r * (L / (NS * R)));
-        x[b][r] = INV ? cmulc(x[b][r], w) : cmul(x[b][r], w);
-      }
+      for (int r = 1; r < R; ++r) x[b][r] = tw.template apply<INV>(x[b][r], jm * r * (L / (NS * R)));
     }
     Dft<R, INV>::run(x[b]);
   }
@@ -273,18 +289,75 @@ __device__ __forceinline__ void line_fft(float2 (&v)[16], int t, Buf& buf, const
   }
 }
 
-// Twiddle lookup from an LDS table of W_L^m.
-struct TwTable {
-  const float2* tab;
-  __device__ __forceinline__ float2 operator()(int m) const { return tab[m]; }
-};
+// z * (hi + lo) and z * conj(hi + lo) for a double-float complex factor.
+PTY_HD float2 cmul_pair(float2 z, float2 hi, float2 lo) { return cmul_df(z, hi.x, lo.x, hi.y, lo.y); }
+PTY_HD float2 cmulc_pair(float2 z, float2 hi, float2 lo) { return cmul_df(z, hi.x, lo.x, -hi.y, -lo.y); }
 
-// Fill an LDS table with W_L^m = exp(-2 pi i m / L), m in [0, L).
-__device__ __forceinline__ void fill_twiddles(float2* tab, int L, int tid, int nthreads) {
+// Lines up to this length (the LDS-resident kernels) carry the modulus-correcting low parts of their
+// twiddle and propagator tables (two more float2 tables in LDS, two more FMAs per multiply).  The
+// streaming kernels (n >= 256) keep plain float32 tables: there the extra LDS lowers occupancy from
+// four to three workgroups per CU and the extra work costs 15 % of the 256^2 x 64-slice forward,
+// while their patterns already stay within 3.2e-6 of the complex128 oracle.
+constexpr int DF_MAX_LINE = 128;
+
+// Twiddle multiply from LDS tables of W_L^m (float32) and, when DF, the modulus-correcting low parts.
+template <bool DF>
+struct TwTab {
+  const float2* tab;
+  const float2* lo;
+  template <bool INV>
+  __device__ __forceinline__ float2 apply(float2 x, int m) const {
+    if constexpr (DF) return INV ? cmulc_pair(x, tab[m], lo[m]) : cmul_pair(x, tab[m], lo[m]);
+    else return INV ? cmulc(x, tab[m]) : cmul(x, tab[m]);
+  }
+};
+using TwTable = TwTab<true>;
+
+// Fill LDS tables with W_L^m = exp(-2 pi i m / L), m in [0, L), and (DF) the low parts that make
+// each rounded entry unit-modulus.
+template <bool DF = true>
+__device__ __forceinline__ void fill_twiddles(float2* tab, float2* lo, int L, int tid, int nthreads) {
   for (int m = tid; m < L; m += nthreads) {
     float s, c;
     sincospif(-2.0f * (float)m / (float)L, &s, &c);
-    tab[m] = make_float2(c, s);
+    const float2 w = make_float2(c, s);
+    tab[m] = w;
+    if constexpr (DF) {
+      const float d = modulus_fix(w, 1.0);
+      lo[m] = make_float2(w.x * d, w.y * d);
+    }
+  }
+}
+
+// A complex64 factor table of modulus r (the separable Fresnel propagator, r = 1/n) staged in LDS
+// with (DF) the modulus-correcting low parts of its entries.
+template <bool DF>
+struct FixTab {
+  const float2* z;
+  const float2* lo;
+  __device__ __forceinline__ float2 mul(float2 v, int i) const {
+    if constexpr (DF) return cmul_pair(v, z[i], lo[i]);
+    else return cmul(v, z[i]);
+  }
+  __device__ __forceinline__ float2 mulc(float2 v, int i) const {
+    if constexpr (DF) return cmulc_pair(v, z[i], lo[i]);
+    else return cmulc(v, z[i]);
+  }
+};
+using FixTable = FixTab<true>;
+
+// Copy n propagator entries (modulus 1/n each) from global memory into LDS with (DF) their low parts.
+template <bool DF = true>
+__device__ __forceinline__ void stage_factors(float2* z, float2* lo, const float2* src, int n, int tid,
+                                              int nthreads) {
+  const double r2 = 1.0 / ((double)n * (double)n);
+  for (int i = tid; i < n; i += nthreads) {
+    const float2 v = src[i];
+    z[i] = v;
+    if constexpr (DF) {
+      const float d = modulus_fix(v, r2);
+      lo[i] = make_float2(v.x * d, v.y * d);
+    }
   }
 }
 

@@ -58,21 +58,25 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
   float2* px = py + N;
   float2* ramp_y = px + N;
   float2* ramp_x = ramp_y + N;
-  double* red = (double*)(ramp_x + N);  // NT / 64 doubles for the detector sums
+  float2* twf = ramp_x + N;  // modulus-correcting low parts of tw, py, px
+  float2* pyf = twf + N;
+  float2* pxf = pyf + N;
+  double* red = (double*)(pxf + N);  // NT / 64 doubles for the detector sums
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const int p = blockIdx.x;
   const WaveCtx g = resolve_wave(a, p);
-  fill_twiddles(tw, N, tid, NT);
+  fill_twiddles(tw, twf, N, tid, NT);
+  stage_factors(py, pyf, g.prop_y, N, tid, NT);
+  stage_factors(px, pxf, g.prop_x, N, tid, NT);
   for (int i = tid; i < N; i += NT) {
-    py[i] = g.prop_y[i];
-    px[i] = g.prop_x[i];
     ramp_y[i] = shift_ramp(i, N, g.my, g.dy);
     ramp_x[i] = shift_ramp(i, N, g.mx, g.dx);
   }
-  const TwTable twt{tw};
+  const TwTable twt{tw, twf};
+  const FixTable pyt{py, pyf}, pxt{px, pxf};
   __syncthreads();
 
   for (int m = 0; m < a.M; ++m) {
@@ -108,7 +112,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
         line_fft<N, false>(v, rt, rb, twt);
         if (s < a.S - 1) {
 #pragma unroll
-          for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], px[rt + T * q]);
+          for (int q = 0; q < 16; ++q) v[q] = pxt.mul(v[q], rt + T * q);
           line_fft<N, true>(v, rt, rb, twt);
         }
         wave_sync();
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
           line_fft<N, false>(v, ct, cb, twt);
           if (s < a.S - 1) {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], py[ct + T * q]);
+            for (int q = 0; q < 16; ++q) v[q] = pyt.mul(v[q], ct + T * q);
             line_fft<N, true>(v, ct, cb, twt);
           }
         }
@@ -185,7 +189,8 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
 
 template <int N>
 static size_t resident_lds_bytes() {
-  return sizeof(float2) * ((size_t)(N + N / 16) * (N + N / 16) + 5 * N) + sizeof(double) * (N * N / 16 / 64);
+  return sizeof(float2) * ((size_t)(N + N / 16) * (N + N / 16) + 8 * N) +
+         sizeof(double) * (N * N / 16 / 64);
 }
 
 // ---------------------------------------------------------------------------
@@ -200,8 +205,11 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   constexpr int RPB = 256 / T;  // rows per workgroup
   constexpr int RS = N + N / 16;
   __shared__ float2 rows[RPB * RS];
+  constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 tw[N];
   __shared__ float2 px[N];
+  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 pxf[DF ? N : 1];
   const int tid = threadIdx.x;
   const int wl = blockIdx.x % nb;  // consecutive blocks -> different waves (XCD round robin)
   const int rblk = blockIdx.x / nb;
@@ -210,10 +218,10 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   const int t = tid % T, lr = tid / T;
   const int y = rblk * RPB + lr;
   const WaveCtx g = resolve_wave(a, p);
-  fill_twiddles(tw, N, tid, 256);
-  if (!(FLAGS & ROW_LAST))
-    for (int i = tid; i < N; i += 256) px[i] = g.prop_x[i];
-  const TwTable twt{tw};
+  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  if (!(FLAGS & ROW_LAST)) stage_factors<DF>(px, pxf, g.prop_x, N, tid, 256);
+  const TwTab<DF> twt{tw, twf};
+  const FixTab<DF> pxt{px, pxf};
   const typename RowBufFor<(T <= 64)>::type rb{rows + lr * RS};
   __syncthreads();
 
@@ -240,7 +248,7 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   line_fft<N, false>(v, t, rb, twt);
   if (!(FLAGS & ROW_LAST)) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], px[t + T * q]);
+    for (int q = 0; q < 16; ++q) v[q] = pxt.mul(v[q], t + T * q);
     line_fft<N, true>(v, t, rb, twt);
   }
 #pragma unroll
@@ -264,8 +272,11 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
   constexpr int NC = 256 / T;  // columns per workgroup
   constexpr int PS = N + N / 16;
   __shared__ float2 buf[PS * NC];
+  constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 tw[N];
   __shared__ float2 py[N];
+  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 pyf[DF ? N : 1];
   __shared__ double red[4];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
@@ -274,10 +285,10 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
   const int x = cblk * NC + gc;
   const int pos = MODE >= COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M;
   const WaveCtx g = resolve_wave(a, pos);
-  fill_twiddles(tw, N, tid, 256);
-  if (MODE == COL_MID)
-    for (int i = tid; i < N; i += 256) py[i] = g.prop_y[i];
-  const TwTable twt{tw};
+  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  if (MODE == COL_MID) stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
+  const TwTab<DF> twt{tw, twf};
+  const FixTab<DF> pyt{py, pyf};
   const ColBufBlock<NC> cb{buf + gc};
   __syncthreads();
 
@@ -302,7 +313,7 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
     for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
     line_fft<N, false>(v, t, cb, twt);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], py[t + T * q]);
+    for (int q = 0; q < 16; ++q) v[q] = pyt.mul(v[q], t + T * q);
     line_fft<N, true>(v, t, cb, twt);
 #pragma unroll
     for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];

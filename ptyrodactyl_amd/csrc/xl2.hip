@@ -73,6 +73,9 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
   __shared__ float2 tw[N];
   __shared__ float2 pvx[N];
   __shared__ float2 pvy[N];
+  __shared__ float2 twf[N];
+  __shared__ float2 pvxf[N];
+  __shared__ float2 pvyf[N];
   __shared__ long long s_ticket;
   __shared__ int s_abort;
 
@@ -82,13 +85,12 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
   int* ticket = st;
   int* done = st + XL2_LINE;  // done[(slot * XL2_MAX_GROUP + wl) * XL2_LINE]: completed tasks of one wave
   int* err = state;
-  fill_twiddles(tw, N, tid, XL2_THREADS);
+  fill_twiddles(tw, twf, N, tid, XL2_THREADS);
   // without position descriptors every wave shares one propagator
-  for (int i = tid; i < N; i += XL2_THREADS) {
-    pvx[i] = a.prop_x[i];
-    pvy[i] = a.prop_y[i];
-  }
-  const TwTable twt{tw};
+  stage_factors(pvx, pvxf, a.prop_x, N, tid, XL2_THREADS);
+  stage_factors(pvy, pvyf, a.prop_y, N, tid, XL2_THREADS);
+  const TwTable twt{tw, twf};
+  const FixTable pxt{pvx, pvxf}, pyt{pvy, pvyf};
 
   const int S = a.S;
   const long long groups = (waves + g - 1) / g;
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
         line_fft<N, false>(v, t16, rb, twt);
         if (!last) {
 #pragma unroll
-          for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], pvx[t16 + T * q]);
+          for (int q = 0; q < 16; ++q) v[q] = pxt.mul(v[q], t16 + T * q);
           line_fft<N, true>(v, t16, rb, twt);
         }
 #pragma unroll
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
           line_fft<N, false>(v, t16, cb, twt);
           if (!fin) {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], pvy[t16 + T * q]);
+            for (int q = 0; q < 16; ++q) v[q] = pyt.mul(v[q], t16 + T * q);
             line_fft<N, true>(v, t16, cb, twt);
 #pragma unroll
             for (int q = 0; q < 16; ++q) wcol[(size_t)(t16 + T * q) * N] = v[q];

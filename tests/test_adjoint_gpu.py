@@ -140,12 +140,16 @@ def test_gradient_matches_central_differences():
 def test_config3_full_gradient():
     """Gradient workload at full size: 16 x 512^2 object, 128^2 probe, 32 x 32 jittered scan,
     sqrt-intensity loss against Poisson data, checkpoints every 4 slices.  GPU gradients w.r.t. all
-    slices and the probe against the oracle's reverse mode (rel-L2) and central FD (h = 1e-3)."""
+    slices and the probe against the oracle's reverse mode end to end (the loss evaluated on the GPU's
+    complex64 patterns against the oracle's complex128 ones), at relative L2 <= 1e-5, and against
+    central differences (h = 1e-3) of the complex128 oracle loss on 64 seeded slab pixels, real and
+    imaginary.  eps = 1e-6 keeps the loss derivative finite where I = 0; it is about 1e-4 of the intensity
+    one electron contributes at the 1e5-electron dose (each pattern sums to ~1200)."""
     from ptyrodactyl_amd import autograd, physics, synth
     w = synth.config3()
     n = w.n
     prop = o.propagation_func(n, n, w.slice_thickness, w.voltage_kv, w.calib)
-    clean = o.window_stem_batched(w.object_true, w.probe, w.origins, w.shifts_px, prop, workers=8)
+    clean = o.window_stem_batched(w.object_true, w.probe, w.origins, w.shifts_px, prop, workers=16)
     measured = synth.poisson_measure(clean)
     eps = 1e-6
     dev = torch.device("cuda", 0)
@@ -163,14 +167,15 @@ def test_config3_full_gradient():
     g_obj, g_probe = obj.grad.cpu().numpy(), probe.grad.cpu().numpy()
     # oracle reverse mode with the same upstream gradient dL/dI = (sqrt(I+eps) - sqrt(M)) / sqrt(I+eps)
     inten = o.window_stem_batched(w.object_t.astype(np.complex128), w.probe, w.origins, w.shifts_px, prop,
-                                  workers=8)
+                                  workers=16)
     root = np.sqrt(inten + eps)
     gi = (root - np.sqrt(measured)) / root
     print("config3 oracle reverse mode", flush=True)
     r_obj, r_probe = o.multislice_vjp(w.object_t.astype(np.complex128), w.probe, w.origins, w.shifts_px, prop, gi)
+    e_pat = rel_l2(pat.detach().double().cpu().numpy(), inten)
     e_obj, e_probe = rel_l2(g_obj, r_obj), rel_l2(g_probe, r_probe)
-    print(f"config3 end-to-end gradient rel-L2 (loss at complex64 vs complex128 patterns): slices {e_obj:.2e}"
-          f"  probe {e_probe:.2e}", flush=True)
+    print(f"config3 patterns rel-L2 {e_pat:.2e}; end-to-end gradient rel-L2: slices {e_obj:.2e}  probe {e_probe:.2e}",
+          flush=True)
     # adjoint alone: the oracle's reverse mode fed the GPU run's own dL/dI
     pat64 = pat.detach().double().cpu().numpy()
     root_g = np.sqrt(pat64 + eps)
@@ -179,31 +184,26 @@ def test_config3_full_gradient():
     f_obj, f_probe = rel_l2(g_obj, a_obj), rel_l2(g_probe, a_probe)
     print(f"config3 adjoint rel-L2 (same dL/dI): slices {f_obj:.2e}  probe {f_probe:.2e}", flush=True)
     assert f_obj < TOL and f_probe < TOL
-    # end to end the sqrt-intensity loss amplifies the complex64 pattern error (it differences sqrt(I)
-    # against Poisson data); the bound below is the measured complex64 floor with a 2x margin
-    assert e_obj < TOL and e_probe < 6e-5
+    assert e_obj < TOL and e_probe < TOL
 
-    # central differences of the complex128 oracle loss on random slab pixels (real and imaginary)
+    # central differences of the complex128 oracle loss on 64 seeded slab pixels (real and imaginary)
     rng = np.random.default_rng(64)
     h = 1e-3
     base = w.object_t.astype(np.complex128)
     worst = 0.0
-    for _ in range(8):
+    scale = np.abs(g_obj).max()
+    for i in range(64):
         s, y, x = rng.integers(0, w.num_slices), rng.integers(64, 448), rng.integers(64, 448)
         covers = np.nonzero(((y - w.origins[:, 0]) % 512 < n) & ((x - w.origins[:, 1]) % 512 < n))[0]
-
-        def loss_at(sl):
-            i = o.window_stem_batched(sl, w.probe, w.origins[covers], w.shifts_px[covers], prop, workers=8)
-            return float(np.sum((np.sqrt(i + eps) - np.sqrt(measured[covers])) ** 2))
-
-        for direction in (1.0, 1j):
-            sp, sm = base.copy(), base.copy()
-            sp[s, y, x] += h * direction
-            sm[s, y, x] -= h * direction
-            fd = (loss_at(sp) - loss_at(sm)) / (2 * h)
-            got = g_obj[s, y, x].real if direction == 1.0 else g_obj[s, y, x].imag
-            worst = max(worst, abs(got - fd) / (abs(fd) + 1e-3 * np.abs(g_obj).max()))
-    print(f"config3 FD check: worst relative deviation {worst:.2e} over 16 directional derivatives", flush=True)
+        pats = o.window_stem_perturbed(base, w.probe, w.origins[covers], w.shifts_px[covers], prop, (s, y, x),
+                                       (h, -h, 1j * h, -1j * h), workers=16)
+        losses = np.sum((np.sqrt(pats + eps) - np.sqrt(measured[covers])[None]) ** 2, axis=(1, 2, 3))
+        fd_re = (losses[0] - losses[1]) / (2 * h)
+        fd_im = (losses[2] - losses[3]) / (2 * h)
+        for got, fd in ((g_obj[s, y, x].real, fd_re), (g_obj[s, y, x].imag, fd_im)):
+            worst = max(worst, abs(got - fd) / (abs(fd) + 1e-3 * scale))
+    print(f"config3 FD check: worst relative deviation {worst:.2e} over 128 directional derivatives "
+          f"(64 pixels)", flush=True)
     assert worst < 1e-2
 
 

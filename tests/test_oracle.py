@@ -209,6 +209,20 @@ def test_vjp_matches_central_differences():
             assert fd == pytest.approx(expected, rel=1e-5, abs=1e-7)
 
 
+def test_window_stem_perturbed_equals_forward_of_perturbed_slab():
+    """The shared-prefix variants equal full forwards of the perturbed slab (pixels in the first, a middle and
+    the last slice, so the prefix is empty, partial and complete)."""
+    slab, probe, origins, shifts, prop, _ = _small_problem(seed=12)
+    for index in ((0, 3, 4), (1, 11, 0), (2, 5, 9)):
+        deltas = (1e-3, -1e-3, 2e-3j, 0.25 - 0.5j)
+        got = o.window_stem_perturbed(slab, probe, origins, shifts, prop, index, deltas)
+        for j, d in enumerate(deltas):
+            sl = slab.astype(np.complex128).copy()
+            sl[index] += d
+            want = o.window_stem_batched(sl, probe, origins, shifts, prop)
+            assert np.allclose(got[j], want, rtol=1e-12, atol=1e-14)
+
+
 def test_vjp_matches_torch_autograd_twin():
     torch = pytest.importorskip("torch")
     slab, probe, origins, shifts, prop, c = _small_problem(seed=8)

@@ -7,8 +7,12 @@
 * Each reconstructor against the oracle loop (``oracle/recon_oracle.py``,
   pinned by the reference's fixed-seed scalars) on a 32x32 grid, including
   snapshots.  The GPU multislice runs in complex64, so the parameter changes
-  agree with the complex128 oracle to about 1e-6 of their size; the bound is
-  1e-4.
+  agree with the complex128 oracle to about 1e-5 of their size in relative L2;
+  the bound is 5e-5.  The max-norm of the difference is printed beside it but
+  not bounded: adam's first steps move each element by about lr * g / |g|, so
+  an element whose gradient is near zero has its direction set by the
+  complex64 rounding of that gradient, and the largest elementwise deviation
+  is governed by the smallest gradients, not by the multislice's accuracy.
 """
 
 import zlib
@@ -24,7 +28,7 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 DEV = "cuda"
-TOL_RECON = 1e-4
+TOL_RECON = 5e-5
 
 
 @pytest.mark.parametrize("name", ["adam", "adagrad", "rmsprop"])
@@ -122,7 +126,10 @@ def _experimental(pb):
 def _rel_change_error(got, start, want):
     got, start, want = (np.asarray(a) for a in (got, start, want))
     d_want = want - start
-    return float(np.abs((got - start) - d_want).max() / np.abs(d_want).max())
+    d = (got - start) - d_want
+    print(f"    change error: rel-L2 {np.linalg.norm(d) / np.linalg.norm(d_want):.2e}  "
+          f"(max-norm {np.abs(d).max() / np.abs(d_want).max():.2e})", flush=True)
+    return float(np.linalg.norm(d) / np.linalg.norm(d_want))
 
 
 def _oracle(pb, lr, pos_lr, loss, opt, update_positions, iters, save_every, modes=None):

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 4
+#define PTY_ABI_VERSION 5
 
 #define PTY_OK 0
 #define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
@@ -77,6 +77,25 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
                              const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
                              float* patterns, void* checkpoints, int ckpt_every, void* workspace,
                              size_t workspace_bytes, void* stream);
+
+/*
+ * pty_forward without a workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference: it is this library's own addition for those callers.  The
+ * streaming waves (n >= 256) live in a static device array of PTY_NOALLOC_SCRATCH_BYTES that
+ * every GPU the library runs on holds until the process ends; positions are processed in
+ * batches that fit it.  One position's num_modes waves must fit (n * n * 8 * num_modes bytes:
+ * n <= 1024 for any mode count up to 8, n = 2048 with at most 2 modes, never n = 4096);
+ * larger requests fail with PTY_ERR_INVALID.  The call allocates nothing.  All calls on one
+ * GPU share that scratch, so calls that overlap on one GPU (different streams or host
+ * threads without ordering) race: order them on one stream.  Results equal pty_forward's.
+ */
+#define PTY_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+int pty_forward_noalloc(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                        long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                        const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                        float* patterns, void* amplitude, void* stream);
 
 #define PTY_MAX_DETECTORS 4
 

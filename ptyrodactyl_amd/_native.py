@@ -19,7 +19,7 @@ from .build import LIB_PATH
 PTY_OK = 0
 PTY_ERR_INVALID = 1
 PTY_ERR_DEVICE = 2
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _lock = threading.Lock()
 _lib = None
@@ -37,6 +37,8 @@ SIGNATURES = {
     "pty_forward_workspace_bytes": (c_size, [c_int, c_ll, c_int, c_ll]),
     "pty_forward": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
                             c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "pty_forward_noalloc": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
+                                    c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pty_forward_checkpointed": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size, c_vp]),
     "pty_adjoint_workspace_bytes": (c_size, [c_int, c_int, c_int, c_ll, c_int]),

@@ -7,6 +7,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "../../include/ptyrodactyl_amd.h"
 #include "adjoint.hpp"
 #include "aux_kernels.hpp"
@@ -16,9 +18,18 @@
 #include "potential.hpp"
 #include "xl2.hpp"
 
+// Streaming-wave scratch of pty_forward_noalloc.  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code `scratch` is only a stand-in for the symbol: kernels get
+// the per-device address that hipGetSymbolAddress returns (noalloc_scratch below), never the name.
+static __device__ __align__(256) unsigned char scratch[PTY_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local char g_err[512] = "";
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
 
 int fail(int code, const char* fmt, ...) {
   va_list ap;
@@ -128,6 +139,26 @@ pty::GridParams make_grid_params(int h, int w, int num_slices, long long num_pos
 
 int grid_ckpt(int ckpt_every, int num_slices) { return ckpt_every < num_slices ? ckpt_every : num_slices; }
 
+// Device address of the current device's `scratch`, looked up once per device (with that device current).
+int noalloc_scratch(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return device_status(e, "pty_forward_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(PTY_ERR_DEVICE, "pty_forward_noalloc: device %d is beyond the %d devices supported", dev,
+                NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return device_status(e, "pty_forward_noalloc: hipGetSymbolAddress(scratch)");
+    if (!p) return fail(PTY_ERR_DEVICE, "pty_forward_noalloc: hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PTY_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -159,6 +190,35 @@ int pty_forward(int n, int num_slices, long long num_positions, int num_modes, c
   a.patterns = patterns;
   a.amplitude = (float2*)amplitude;
   return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_forward");
+}
+
+int pty_forward_noalloc(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
+                        long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
+                        const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
+                        float* patterns, void* amplitude, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                        origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (!patterns && !amplitude) return fail(PTY_ERR_INVALID, "nothing to compute: patterns and amplitude are null");
+  const size_t need = pty::forward_workspace_bytes(n, 1, num_modes, 0);
+  if (need > PTY_NOALLOC_SCRATCH_BYTES)
+    return fail(PTY_ERR_INVALID,
+                "pty_forward_noalloc: one position of %d mode(s) at n=%d needs %zu bytes of scratch, above the "
+                "%zu-byte static scratch (use pty_forward with a workspace)",
+                num_modes, n, need, (size_t)PTY_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  if (need > 0) {
+    rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    ws_bytes = PTY_NOALLOC_SCRATCH_BYTES;
+  }
+  pty::FwdParams a = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k,
+                                 mode_weights, origins, shifts_px, prop_y, prop_x);
+  a.patterns = patterns;
+  a.amplitude = (float2*)amplitude;
+  return device_status(pty::launch_forward(a, ws, ws_bytes, (hipStream_t)stream), "pty_forward_noalloc");
 }
 
 int pty_forward_detectors(int n, int num_slices, long long num_positions, int num_modes, const void* slab,

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 5
+#define PTY_ABI_VERSION 6
 
 #define PTY_OK 0
 #define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
@@ -68,14 +68,15 @@ int pty_forward(int n, int num_slices, long long num_positions, int num_modes, c
 /*
  * Forward that also stores the wave entering slices k, 2k, ... (< S) for the
  * adjoint: checkpoints complex64 [num_positions * num_modes][(S-1)/k][n][n].
- * Same multislice as pty_forward; the reference has no checkpointing (XLA
+ * Same multislice as pty_forward, with the patterns and/or the amplitudes
+ * (each may be NULL, not both); the reference has no checkpointing (XLA
  * keeps every slice's residuals under jax.value_and_grad,
  * src/ptyrodactyl/multislice/multislice_recon.py:1915-1951).
  */
 int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
                              long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
                              const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
-                             float* patterns, void* checkpoints, int ckpt_every, void* workspace,
+                             float* patterns, void* amplitude, void* checkpoints, int ckpt_every, void* workspace,
                              size_t workspace_bytes, void* stream);
 
 /*
@@ -96,6 +97,32 @@ int pty_forward_noalloc(int n, int num_slices, long long num_positions, int num_
                         long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
                         const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
                         float* patterns, void* amplitude, void* stream);
+
+/*
+ * Workspace (device bytes) of pty_jvp: a primal and a tangent wave, n * n * 16 bytes, for each of the
+ * num_positions * num_modes waves in one batch, capped at max_batch_waves waves (0 = no cap).
+ */
+size_t pty_jvp_workspace_bytes(int n, long long num_positions, int num_modes, long long max_batch_waves);
+
+/*
+ * Forward-mode multislice: the Jacobian-vector product of pty_forward's patterns (and amplitudes).
+ * Replaces the jax.jvp that jvp_operator and jtj_operator build through the forward model for the
+ * Gauss-Newton and Levenberg-Marquardt solvers (src/ptyrodactyl/jacobian/operators.py:30-185,
+ * solvers.py:387-521).  Every wave carries its tangent through the slices:
+ * d phi_s = d psi_s T_s + psi_s d T_s, d psi_s+1 = Prop(d phi_s), dI = sum_m w_m 2 Re(conj(A_m) dA_m).
+ *   d_slab      complex64 tangent of the slab (the slab's shape), or NULL (zero)
+ *   d_probe_k   complex64 [num_modes][n][n] tangent of probe_k, or NULL (zero)
+ *   d_shifts_px float64 [num_positions][2] tangent of the probe shifts (pixels), or NULL (zero)
+ *   patterns, amplitude     primal outputs as in pty_forward (each may be NULL); pty_forward's to rounding
+ *   d_patterns  float32 [num_positions][n][n] fftshifted dI, or NULL
+ *   d_amplitude complex64 [num_positions][num_modes][n][n] fftshifted dA_m, or NULL (not both NULL)
+ * Windows n = 32 ... 512 (powers of two); workspace of at least one position (pty_jvp_workspace_bytes).
+ */
+int pty_jvp(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
+            long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+            const double* shifts_px, const void* prop_y, const void* prop_x, const void* d_slab,
+            const void* d_probe_k, const double* d_shifts_px, float* patterns, void* amplitude, float* d_patterns,
+            void* d_amplitude, void* workspace, size_t workspace_bytes, void* stream);
 
 #define PTY_MAX_DETECTORS 4
 

@@ -30,9 +30,16 @@ Parity pins (see ``tests/test_oracle.py``):
 * ``window_stem``: equals ``stem_4d`` when the slab is one window tiled
   periodically; equals ``stem_4d`` on an explicitly cropped window.
 * ``aberration12`` / ``probe_from_aberrations``: reduces to the reference
-  ``aberration`` (C1/C3 terms) when only C1 and C3 are set.
-* ``lm_*``: closed-form pins from
-  ``tests/test_ptyrodactyl/test_jacobian/test_solvers.py:53-226``.
+  ``aberration`` (C1/C3 terms) when only C1 and C3 are set; every other
+  coefficient by the properties its name fixes (radial order 2/3/4, m-fold
+  rotational symmetry with a sign flip under a rotation by pi/m, the
+  Krivanek-form prefactor of its maximum, and the orientation of its real and
+  imaginary parts).
+* ``lm_step_dense`` / ``lm_gain_ratio`` / ``lm_update_damping`` and the loop of
+  ``lm_aberration_fit``: the closed forms of
+  ``tests/test_ptyrodactyl/test_jacobian/test_solvers.py:53-226`` (GN step 1,
+  LM step 4/(4+lambda), damping/3 on an accepted quadratic step, frozen state
+  when already converged), and the fit recovers known coefficients.
 """
 
 from __future__ import annotations
@@ -533,15 +540,21 @@ def probe_from_aberrations(n: int, calib_ang: float, voltage_kv: float,
 # ---------------------------------------------------------------------------
 
 def lm_step_dense(jac, residual, damping):
-    """One LM step on a dense real Jacobian: solve (J^T J + lambda I) s = -J^T r.
-
-    Gain ratio rho = (|r|^2 - |r_new|^2) / (<s,g> - 1/2 <s, JtJ s>) with
-    g = -J^T r (``solvers.py:480-489``).  Returns (step, JtJ, g).
-    """
+    """One LM step on a dense real Jacobian (``solvers.py:450-474``): g = J^T r with r = f(theta) - data,
+    s solves (J^T J + lambda I) s = g, and the candidate is theta - s.  Returns (s, J^T J, g)."""
     jtj = jac.T @ jac
-    g = -(jac.T @ residual)
+    g = jac.T @ residual
     step = np.linalg.solve(jtj + damping * np.eye(jtj.shape[0]), g)
     return step, jtj, g
+
+
+def lm_gain_ratio(rss: float, rss_new: float, step, g, jtj) -> float:
+    """(|r|^2 - |r_new|^2) / (<s, g> - 1/2 <s, J^T J s> + 1e-12) (``solvers.py:480-489``).
+
+    The numerator is a reduction of |r|^2 and the denominator that of |r|^2 / 2 under the quadratic
+    model, so an exact quadratic step gives 2 (the reference's factor, kept)."""
+    predicted = float(step @ g - 0.5 * step @ (jtj @ step))
+    return (float(rss) - float(rss_new)) / (predicted + 1e-12)
 
 
 def lm_update_damping(rho: float, damping: float) -> float:
@@ -604,10 +617,10 @@ def jax_uniform_f64(seed: int, shape):
 # ---------------------------------------------------------------------------
 # FP64 Levenberg-Marquardt over the twelve probe aberrations (the oracle run
 # the GPU mixed-precision fit is compared with).  Same update rules as
-# ``levenberg_marquardt_step`` (``jacobian/solvers.py:387-521``): step from
-# (J^T J + lambda I) s = J^T r, candidate theta - s, gain ratio
-# (|r|^2 - |r_new|^2) / (<s, g> - 1/2 <s, J^T J s> + 1e-12), lambda/3 above
-# 0.75, 2 lambda below 0.25, accept when the ratio is positive.
+# ``levenberg_marquardt_step`` (``jacobian/solvers.py:387-521``), through
+# ``lm_step_dense`` (step from (J^T J + lambda I) s = J^T r, candidate
+# theta - s), ``lm_gain_ratio`` and ``lm_update_damping`` (lambda/3 above
+# 0.75, 2 lambda below 0.25); accept when the ratio is positive.
 # ---------------------------------------------------------------------------
 
 def aberration_basis12(n: int, calib_ang: float, voltage_kv: float):
@@ -678,14 +691,11 @@ def lm_aberration_fit(slab_t, origins, shifts_px, data, coeffs_init, calib_ang, 
         jac = 2.0 * np.real(np.conj(a0)[:, None] * amp[:, 1:])          # (P, 12, n, n)
         r = np.abs(a0) ** 2 - data_u
         jmat = jac.transpose(1, 0, 2, 3).reshape(12, -1)
-        jtj = jmat @ jmat.T
-        g = jmat @ r.ravel()
         rss = float(np.sum(r**2))
-        step = np.linalg.solve(jtj + damping * np.eye(12), g)
-        predicted = step @ g - 0.5 * step @ (jtj @ step)
+        step, jtj, g = lm_step_dense(jmat.T, r.ravel(), damping)
         cand = theta - step
         rss_new = rss_of(cand)
-        gain = (rss - rss_new) / (predicted + 1e-12)
+        gain = lm_gain_ratio(rss, rss_new, step, g, jtj)
         damping = lm_update_damping(gain, damping)
         if gain > 0:
             theta, norm = cand, np.sqrt(rss_new)

@@ -19,7 +19,7 @@ from .build import LIB_PATH
 PTY_OK = 0
 PTY_ERR_INVALID = 1
 PTY_ERR_DEVICE = 2
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _lock = threading.Lock()
 _lib = None
@@ -39,8 +39,11 @@ SIGNATURES = {
                             c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "pty_forward_noalloc": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
                                     c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pty_jvp_workspace_bytes": (c_size, [c_int, c_ll, c_int, c_ll]),
+    "pty_jvp": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "pty_forward_checkpointed": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp,
-                                         c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size, c_vp]),
+                                         c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_size, c_vp]),
     "pty_adjoint_workspace_bytes": (c_size, [c_int, c_int, c_int, c_ll, c_int]),
     "pty_adjoint": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),

@@ -1,17 +1,24 @@
-"""torch.autograd.Function around the HIP forward and adjoint kernels.
+"""torch.autograd.Function around the HIP forward, adjoint and JVP kernels.
 
 ``multislice_patterns(slab_t, probe_k, weights, ...)`` is differentiable
 w.r.t. the complex transmission slab, probe_k (fft2 of the probe), the
 mode weights and the probe shifts (pixels); ``multislice_amplitudes``
 (per-mode complex detector amplitudes) w.r.t. the slab, probe_k and the
-shifts.  Gradients follow torch's convention for complex leaves
-(dL/dRe + i dL/dIm), which is the complex conjugate of what ``jax.grad``
-returns for the reference (``src/ptyrodactyl/multislice/multislice_recon.py:195-207``
+shifts.  Both run in reverse mode (``backward``, the HIP adjoint) and in
+forward mode (``jvp``, the HIP tangent recurrence ``pty_jvp``), and both
+work under ``torch.func`` (``vjp``, ``jvp``, ``grad``), which the
+Gauss-Newton and Levenberg-Marquardt solvers use the way the reference
+uses ``jax.jvp`` / ``jax.vjp`` (``src/ptyrodactyl/jacobian/operators.py:30-185``).
+Gradients follow torch's convention for complex leaves (dL/dRe + i dL/dIm),
+which is the complex conjugate of what ``jax.grad`` returns for the
+reference (``src/ptyrodactyl/multislice/multislice_recon.py:195-207``
 conjugates it before Optax for that reason).  The integer window origins
 are piecewise constant and get no gradient.
 """
 
 from __future__ import annotations
+
+import contextlib
 
 import torch
 
@@ -25,82 +32,220 @@ def _pair(prop):
     return tuple(prop) if isinstance(prop, (tuple, list)) else (prop, prop)
 
 
+def _empty_like_ckpt(ref: torch.Tensor) -> torch.Tensor:
+    return torch.empty((0,), dtype=torch.complex64, device=ref.device)
+
+
+def _transformed(t) -> bool:
+    """True for a tensor that a torch.func transform (vjp, jvp, grad) wraps, or that carries a forward-mode
+    tangent: such tensors have no storage the kernels could read, so they go through the Functions below."""
+    if not isinstance(t, torch.Tensor):
+        return False
+    try:
+        if torch._C._functorch.is_functorch_wrapped_tensor(t):
+            return True
+    except (AttributeError, RuntimeError):
+        pass
+    try:
+        return torch.autograd.forward_ad.unpack_dual(t).tangent is not None
+    except RuntimeError:
+        return False
+
+
+def _plain(t):
+    """The tensor a torch.func transform wraps (its innermost value).  The kernels read raw memory, so the
+    backward and jvp rules below hand them plain tensors; the results are first-order derivatives (the
+    kernels' derivatives are not differentiated again)."""
+    if not isinstance(t, torch.Tensor):
+        return t
+    try:
+        while torch._C._functorch.is_functorch_wrapped_tensor(t):
+            t = torch._C._functorch.get_unwrapped(t)
+    except AttributeError:
+        pass
+    return t
+
+
+def _kernel_scope():
+    """A scope in which torch ops on plain tensors stay plain: inside a torch.func transform every op would
+    otherwise wrap its result again at the transform's level, and the kernels cannot read wrapped tensors."""
+    disable = getattr(torch._C, "_DisableFuncTorch", None)
+    return disable() if disable is not None else contextlib.nullcontext()
+
+
+def differentiable(*tensors) -> bool:
+    """Whether a multislice over these operands must run through the differentiable Functions: grad mode with
+    an operand that requires grad, or an operand under a torch.func transform or with a forward-mode tangent."""
+    if torch.is_grad_enabled() and any(isinstance(t, torch.Tensor) and t.requires_grad for t in tensors):
+        return True
+    return any(_transformed(t) for t in tensors)
+
+
+def _jvp_supported(probe_k: torch.Tensor) -> bool:
+    m_count, h, w = probe_k.shape
+    return h == w and h in engine.JVP_N
+
+
 class MultisliceFunction(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, slab_t, probe_k, weights, origins, shifts_px, prop, ckpt_every):
-        need_field = ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[4]
-        slab_c, probe_c = slab_t.detach().contiguous(), probe_k.detach().contiguous()
-        w_c = weights.detach().to(torch.float32).contiguous()
-        if not need_field:
-            patterns, _ = engine.multislice_forward(slab_c, probe_c, w_c, origins, shifts_px, prop)
-            ckpt = None
-        else:
-            patterns, ckpt = engine.multislice_forward_checkpointed(slab_c, probe_c, w_c, origins, shifts_px, prop,
-                                                                    ckpt_every)
-        ctx.save_for_backward(slab_c, probe_c, w_c, origins, shifts_px.detach(), *_pair(prop), ckpt)
-        ctx.ckpt_every = ckpt_every
-        ctx.weights_dtype = weights.dtype
-        return patterns
+    """(P, n, n) float32 patterns; the second output holds the adjoint's checkpoints (not differentiable)."""
 
     @staticmethod
-    def backward(ctx, grad_patterns):
-        slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x, ckpt = ctx.saved_tensors
-        prop = (prop_y, prop_x)
-        g = grad_patterns.to(torch.float32).contiguous()
-        g_slab = g_probe = g_w = g_shift = None
-        if ctx.needs_input_grad[4]:
-            g_shift = torch.zeros(shifts_px.shape, dtype=torch.float64, device=shifts_px.device)
-        if ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[4]:
-            g_slab, g_probe = engine.multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, g, ckpt,
-                                                        ctx.ckpt_every, want_probe=ctx.needs_input_grad[1],
-                                                        grad_shifts=g_shift)
-        if ctx.needs_input_grad[2]:
-            # I = sum_m w_m |A_m|^2  =>  dL/dw_m = sum over positions and pixels of g |A_m|^2
-            _, amp = engine.multislice_forward(slab_t, probe_k, weights, origins, shifts_px, prop,
-                                               want_patterns=False, want_amplitude=True)
-            inten = amp.real.double() ** 2 + amp.imag.double() ** 2
-            g_w = (g.double()[:, None] * inten).sum(dim=(0, 2, 3)).to(ctx.weights_dtype)
-        return (g_slab if ctx.needs_input_grad[0] else None, g_probe, g_w, None, g_shift, None, None)
+    def forward(slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x, ckpt_every, want_ckpt):
+        slab_c, probe_c = slab_t.detach().contiguous(), probe_k.detach().contiguous()
+        w_c = weights.detach().to(torch.float32).contiguous()
+        prop = (prop_y.detach(), prop_x.detach())
+        if want_ckpt:
+            patterns, ckpt = engine.multislice_forward_checkpointed(slab_c, probe_c, w_c, origins, shifts_px.detach(),
+                                                                    prop, ckpt_every)
+            if ckpt is None:
+                ckpt = _empty_like_ckpt(slab_c)
+        else:
+            patterns, _ = engine.multislice_forward(slab_c, probe_c, w_c, origins, shifts_px.detach(), prop)
+            ckpt = _empty_like_ckpt(slab_c)
+        return patterns, ckpt
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x, ckpt_every, want_ckpt = inputs
+        _, ckpt = output
+        ctx.mark_non_differentiable(ckpt)
+        slab_c, probe_c = slab_t.detach().contiguous(), probe_k.detach().contiguous()
+        w_c = weights.detach().to(torch.float32).contiguous()
+        ctx.save_for_backward(slab_c, probe_c, w_c, origins, shifts_px.detach(), prop_y.detach(), prop_x.detach(),
+                              ckpt)
+        ctx.save_for_forward(slab_c, probe_c, w_c, origins, shifts_px.detach(), prop_y.detach(), prop_x.detach())
+        ctx.ckpt_every = int(ckpt_every)
+        ctx.has_ckpt = bool(want_ckpt)
+        ctx.weights_dtype = weights.dtype
+
+    @staticmethod
+    def backward(ctx, grad_patterns, _grad_ckpt):
+        with _kernel_scope():
+            saved = (_plain(t) for t in ctx.saved_tensors)
+            slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x, ckpt = saved
+            if not ctx.has_ckpt:
+                raise RuntimeError("multislice_patterns ran without checkpoints (grad mode was off): no reverse mode")
+            prop = (prop_y, prop_x)
+            g = _plain(grad_patterns).to(torch.float32).contiguous()
+            need_slab, need_probe, need_w = ctx.needs_input_grad[0], ctx.needs_input_grad[1], ctx.needs_input_grad[2]
+            need_shift = ctx.needs_input_grad[4]
+            g_slab = g_probe = g_w = g_shift = None
+            if need_shift:
+                g_shift = torch.zeros(shifts_px.shape, dtype=torch.float64, device=shifts_px.device)
+            if need_slab or need_probe or need_shift:
+                ck = ckpt if ckpt.numel() else None
+                g_slab, g_probe = engine.multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, g, ck,
+                                                            ctx.ckpt_every, want_probe=need_probe, grad_shifts=g_shift)
+            if need_w:
+                # I = sum_m w_m |A_m|^2  =>  dL/dw_m = sum over positions and pixels of g |A_m|^2
+                _, amp = engine.multislice_forward(slab_t, probe_k, weights, origins, shifts_px, prop,
+                                                   want_patterns=False, want_amplitude=True)
+                inten = amp.real.double() ** 2 + amp.imag.double() ** 2
+                g_w = (g.double()[:, None] * inten).sum(dim=(0, 2, 3)).to(ctx.weights_dtype)
+            return (g_slab if need_slab else None, g_probe if need_probe else None, g_w, None, g_shift, None, None,
+                    None, None)
+
+    @staticmethod
+    def jvp(ctx, d_slab, d_probe_k, d_weights, _d_origins, d_shifts, _d_prop_y, _d_prop_x, _d_k, _d_want):
+        with _kernel_scope():
+            slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x = (_plain(t) for t in ctx.saved_tensors)
+            d_slab, d_probe_k, d_weights, d_shifts = (_plain(t) for t in (d_slab, d_probe_k, d_weights, d_shifts))
+            if not _jvp_supported(probe_k):
+                raise ValueError("forward-mode multislice (pty_jvp) takes square windows n = 32 ... 512")
+            prop = (prop_y, prop_x)
+            want_amp = d_weights is not None
+            _, amp, d_pat, _ = engine.multislice_jvp(
+                slab_t, probe_k, weights, origins, shifts_px, prop,
+                d_slab.to(torch.complex64).contiguous() if d_slab is not None else None,
+                d_probe_k.to(torch.complex64).contiguous() if d_probe_k is not None else None,
+                d_shifts.to(torch.float64).contiguous() if d_shifts is not None else None,
+                want_amplitude=want_amp)
+            if want_amp:
+                inten = amp.real ** 2 + amp.imag ** 2
+                d_pat = d_pat + (d_weights.to(torch.float32)[None, :, None, None] * inten).sum(dim=1)
+            return d_pat, None
 
 
 class MultisliceAmplitudeFunction(torch.autograd.Function):
     """Per-mode fftshifted complex amplitudes (P, M, n, n), differentiable w.r.t. the slab, probe_k and shifts."""
 
     @staticmethod
-    def forward(ctx, slab_t, probe_k, origins, shifts_px, prop, ckpt_every):
+    def forward(slab_t, probe_k, origins, shifts_px, prop_y, prop_x, ckpt_every, want_ckpt):
         slab_c, probe_c = slab_t.detach().contiguous(), probe_k.detach().contiguous()
         ones = torch.ones(probe_c.shape[0], dtype=torch.float32, device=probe_c.device)
-        _, amp = engine.multislice_forward(slab_c, probe_c, ones, origins, shifts_px, prop, want_patterns=False,
-                                           want_amplitude=True)
-        ckpt = None
-        if ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[3]:
-            _, ckpt = engine.multislice_forward_checkpointed(slab_c, probe_c, ones, origins, shifts_px, prop,
-                                                             ckpt_every)
-        ctx.save_for_backward(slab_c, probe_c, ones, origins, shifts_px.detach(), *_pair(prop), ckpt)
-        ctx.ckpt_every = ckpt_every
-        return amp
+        prop = (prop_y.detach(), prop_x.detach())
+        ckpt = _empty_like_ckpt(slab_c)
+        if want_ckpt:
+            amp, ck = engine.multislice_amplitude_checkpointed(slab_c, probe_c, ones, origins, shifts_px.detach(),
+                                                               prop, ckpt_every)
+            if ck is not None:
+                ckpt = ck
+        else:
+            _, amp = engine.multislice_forward(slab_c, probe_c, ones, origins, shifts_px.detach(), prop,
+                                               want_patterns=False, want_amplitude=True)
+        return amp, ckpt
 
     @staticmethod
-    def backward(ctx, grad_amp):
-        slab_t, probe_k, ones, origins, shifts_px, prop_y, prop_x, ckpt = ctx.saved_tensors
-        prop = (prop_y, prop_x)
-        g_shift = None
-        if ctx.needs_input_grad[3]:
-            g_shift = torch.zeros(shifts_px.shape, dtype=torch.float64, device=shifts_px.device)
-        g_slab, g_probe = engine.multislice_adjoint(slab_t, probe_k, ones, origins, shifts_px, prop, None, ckpt,
-                                                    ctx.ckpt_every, want_probe=ctx.needs_input_grad[1],
-                                                    grad_amplitude=grad_amp.to(torch.complex64).contiguous(),
-                                                    grad_shifts=g_shift)
-        return (g_slab if ctx.needs_input_grad[0] else None, g_probe, None, g_shift, None, None)
+    def setup_context(ctx, inputs, output):
+        slab_t, probe_k, origins, shifts_px, prop_y, prop_x, ckpt_every, want_ckpt = inputs
+        _, ckpt = output
+        ctx.mark_non_differentiable(ckpt)
+        slab_c, probe_c = slab_t.detach().contiguous(), probe_k.detach().contiguous()
+        ones = torch.ones(probe_c.shape[0], dtype=torch.float32, device=probe_c.device)
+        ctx.save_for_backward(slab_c, probe_c, ones, origins, shifts_px.detach(), prop_y.detach(), prop_x.detach(),
+                              ckpt)
+        ctx.save_for_forward(slab_c, probe_c, ones, origins, shifts_px.detach(), prop_y.detach(), prop_x.detach())
+        ctx.ckpt_every = int(ckpt_every)
+        ctx.has_ckpt = bool(want_ckpt)
+
+    @staticmethod
+    def backward(ctx, grad_amp, _grad_ckpt):
+        with _kernel_scope():
+            slab_t, probe_k, ones, origins, shifts_px, prop_y, prop_x, ckpt = (_plain(t) for t in ctx.saved_tensors)
+            grad_amp = _plain(grad_amp)
+            if not ctx.has_ckpt:
+                raise RuntimeError("multislice_amplitudes ran without checkpoints (grad mode was off): no reverse mode")
+            prop = (prop_y, prop_x)
+            g_shift = None
+            if ctx.needs_input_grad[3]:
+                g_shift = torch.zeros(shifts_px.shape, dtype=torch.float64, device=shifts_px.device)
+            ck = ckpt if ckpt.numel() else None
+            g_slab, g_probe = engine.multislice_adjoint(slab_t, probe_k, ones, origins, shifts_px, prop, None, ck,
+                                                        ctx.ckpt_every, want_probe=ctx.needs_input_grad[1],
+                                                        grad_amplitude=grad_amp.to(torch.complex64).contiguous(),
+                                                        grad_shifts=g_shift)
+            return (g_slab if ctx.needs_input_grad[0] else None, g_probe if ctx.needs_input_grad[1] else None, None,
+                    g_shift, None, None, None, None)
+
+    @staticmethod
+    def jvp(ctx, d_slab, d_probe_k, _d_origins, d_shifts, _d_prop_y, _d_prop_x, _d_k, _d_want):
+        with _kernel_scope():
+            slab_t, probe_k, ones, origins, shifts_px, prop_y, prop_x = (_plain(t) for t in ctx.saved_tensors)
+            d_slab, d_probe_k, d_shifts = (_plain(t) for t in (d_slab, d_probe_k, d_shifts))
+            if not _jvp_supported(probe_k):
+                raise ValueError("forward-mode multislice (pty_jvp) takes square windows n = 32 ... 512")
+            _, _, _, d_amp = engine.multislice_jvp(
+                slab_t, probe_k, ones, origins, shifts_px, (prop_y, prop_x),
+                d_slab.to(torch.complex64).contiguous() if d_slab is not None else None,
+                d_probe_k.to(torch.complex64).contiguous() if d_probe_k is not None else None,
+                d_shifts.to(torch.float64).contiguous() if d_shifts is not None else None,
+                want_d_patterns=False, want_d_amplitude=True)
+            return d_amp, None
 
 
 def multislice_patterns(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: torch.Tensor, origins: torch.Tensor,
-                        shifts_px: torch.Tensor, prop: torch.Tensor, ckpt_every: int = DEFAULT_CKPT_EVERY):
-    """Differentiable (P, n, n) float32 patterns of the windowed multislice."""
-    return MultisliceFunction.apply(slab_t, probe_k, weights, origins, shifts_px, prop, int(ckpt_every))
+                        shifts_px: torch.Tensor, prop, ckpt_every: int = DEFAULT_CKPT_EVERY):
+    """Differentiable (P, n, n) float32 patterns of the windowed multislice (reverse and forward mode)."""
+    prop_y, prop_x = _pair(prop)
+    patterns, _ = MultisliceFunction.apply(slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x,
+                                           int(ckpt_every), torch.is_grad_enabled())
+    return patterns
 
 
 def multislice_amplitudes(slab_t: torch.Tensor, probe_k: torch.Tensor, origins: torch.Tensor,
-                          shifts_px: torch.Tensor, prop: torch.Tensor, ckpt_every: int = DEFAULT_CKPT_EVERY):
+                          shifts_px: torch.Tensor, prop, ckpt_every: int = DEFAULT_CKPT_EVERY):
     """Differentiable (P, M, n, n) complex64 fftshifted amplitudes of the windowed multislice."""
-    return MultisliceAmplitudeFunction.apply(slab_t, probe_k, origins, shifts_px, prop, int(ckpt_every))
+    prop_y, prop_x = _pair(prop)
+    amp, _ = MultisliceAmplitudeFunction.apply(slab_t, probe_k, origins, shifts_px, prop_y, prop_x,
+                                               int(ckpt_every), torch.is_grad_enabled())
+    return amp

@@ -14,6 +14,7 @@
 #include "aux_kernels.hpp"
 #include "forward.hpp"
 #include "grid.hpp"
+#include "jvp.hpp"
 #include "optim.hpp"
 #include "potential.hpp"
 #include "xl2.hpp"
@@ -221,6 +222,41 @@ int pty_forward_noalloc(int n, int num_slices, long long num_positions, int num_
   return device_status(pty::launch_forward(a, ws, ws_bytes, (hipStream_t)stream), "pty_forward_noalloc");
 }
 
+size_t pty_jvp_workspace_bytes(int n, long long num_positions, int num_modes, long long max_batch_waves) {
+  if (!batched_n(n) || num_positions < 1 || num_modes < 1) return 0;
+  return pty::jvp_workspace_bytes(n, num_positions, num_modes, max_batch_waves);
+}
+
+int pty_jvp(int n, int num_slices, long long num_positions, int num_modes, const void* slab, long long slab_h,
+            long long slab_w, const void* probe_k, const float* mode_weights, const int32_t* origins,
+            const double* shifts_px, const void* prop_y, const void* prop_x, const void* d_slab,
+            const void* d_probe_k, const double* d_shifts_px, float* patterns, void* amplitude, float* d_patterns,
+            void* d_amplitude, void* workspace, size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
+                        origins, shifts_px, prop_y, prop_x);
+  if (rc) return rc;
+  if (!batched_n(n)) return fail(PTY_ERR_INVALID, "pty_jvp takes windows n = 32 ... 512 (got n=%d)", n);
+  if (!d_patterns && !d_amplitude)
+    return fail(PTY_ERR_INVALID, "nothing to compute: d_patterns and d_amplitude are null");
+  const size_t need = pty::jvp_workspace_bytes(n, 1, num_modes, 0);
+  if (!workspace || workspace_bytes < need)
+    return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
+                need);
+  pty::JvpParams a;
+  a.f = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights, origins,
+                    shifts_px, prop_y, prop_x);
+  a.f.patterns = patterns;
+  a.f.amplitude = (float2*)amplitude;
+  a.d_slab = (const float2*)d_slab;
+  a.d_probe_k = (const float2*)d_probe_k;
+  a.d_shifts = d_shifts_px;
+  a.d_patterns = d_patterns;
+  a.d_amplitude = (float2*)d_amplitude;
+  a.w0 = 0;
+  return device_status(pty::launch_jvp(a, workspace, workspace_bytes, (hipStream_t)stream), "pty_jvp");
+}
+
 int pty_forward_detectors(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
                           long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
                           const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
@@ -251,7 +287,7 @@ int pty_forward_detectors(int n, int num_slices, long long num_positions, int nu
 int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
                              long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,
                              const int32_t* origins, const double* shifts_px, const void* prop_y, const void* prop_x,
-                             float* patterns, void* checkpoints, int ckpt_every, void* workspace,
+                             float* patterns, void* amplitude, void* checkpoints, int ckpt_every, void* workspace,
                              size_t workspace_bytes, void* stream) {
   g_err[0] = '\0';
   int rc = check_common(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k, mode_weights,
@@ -261,7 +297,7 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
   if (pty::num_checkpoints(num_slices, ckpt_every) > 0 && !checkpoints)
     return fail(PTY_ERR_INVALID, "checkpoint buffer is null but %d checkpoints are needed",
                 pty::num_checkpoints(num_slices, ckpt_every));
-  if (!patterns) return fail(PTY_ERR_INVALID, "patterns is null");
+  if (!patterns && !amplitude) return fail(PTY_ERR_INVALID, "patterns and amplitude are both null");
   const size_t need = pty::forward_workspace_bytes(n, 1, num_modes, 0);
   if (need > 0 && (!workspace || workspace_bytes < need))
     return fail(PTY_ERR_INVALID, "workspace of %zu bytes is below the %zu bytes one position needs", workspace_bytes,
@@ -269,6 +305,7 @@ int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int
   pty::FwdParams a = make_params(n, num_slices, num_positions, num_modes, slab, slab_h, slab_w, probe_k,
                                  mode_weights, origins, shifts_px, prop_y, prop_x);
   a.patterns = patterns;
+  a.amplitude = (float2*)amplitude;
   a.ckpt = (float2*)checkpoints;
   a.ckpt_every = ckpt_every;
   return device_status(pty::launch_forward(a, workspace, workspace_bytes, (hipStream_t)stream),

@@ -73,9 +73,10 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
   __shared__ float2 tw[N];
   __shared__ float2 pvx[N];
   __shared__ float2 pvy[N];
-  __shared__ float2 twf[N];
-  __shared__ float2 pvxf[N];
-  __shared__ float2 pvyf[N];
+  constexpr bool DF = N <= DF_MAX_LINE;  // the same tables as the per-pass kernels of this size
+  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 pvxf[DF ? N : 1];
+  __shared__ float2 pvyf[DF ? N : 1];
   __shared__ long long s_ticket;
   __shared__ int s_abort;
 
@@ -85,12 +86,12 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
   int* ticket = st;
   int* done = st + XL2_LINE;  // done[(slot * XL2_MAX_GROUP + wl) * XL2_LINE]: completed tasks of one wave
   int* err = state;
-  fill_twiddles(tw, twf, N, tid, XL2_THREADS);
+  fill_twiddles<DF>(tw, twf, N, tid, XL2_THREADS);
   // without position descriptors every wave shares one propagator
-  stage_factors(pvx, pvxf, a.prop_x, N, tid, XL2_THREADS);
-  stage_factors(pvy, pvyf, a.prop_y, N, tid, XL2_THREADS);
-  const TwTable twt{tw, twf};
-  const FixTable pxt{pvx, pvxf}, pyt{pvy, pvyf};
+  stage_factors<DF>(pvx, pvxf, a.prop_x, N, tid, XL2_THREADS);
+  stage_factors<DF>(pvy, pvyf, a.prop_y, N, tid, XL2_THREADS);
+  const TwTab<DF> twt{tw, twf};
+  const FixTab<DF> pxt{pvx, pvxf}, pyt{pvy, pvyf};
 
   const int S = a.S;
   const long long groups = (waves + g - 1) / g;

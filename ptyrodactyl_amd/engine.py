@@ -405,18 +405,33 @@ def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px
 
     Returns (patterns (P, n, n) float32, checkpoints (P*M, (S-1)//k, n, n) complex64).
     """
+    patterns, _, ckpt = _forward_with_checkpoints(slab_t, probe_k, weights, origins, shifts_px, prop, ckpt_every,
+                                                  True, False, max_batch_waves)
+    return patterns, ckpt
+
+
+def multislice_amplitude_checkpointed(slab_t, probe_k, weights, origins, shifts_px, prop, ckpt_every: int,
+                                      max_batch_waves: Optional[int] = None):
+    """The per-mode fftshifted amplitudes (P, M, n, n) complex64 and the adjoint's checkpoints from one forward."""
+    _, amplitude, ckpt = _forward_with_checkpoints(slab_t, probe_k, weights, origins, shifts_px, prop, ckpt_every,
+                                                   False, True, max_batch_waves)
+    return amplitude, ckpt
+
+
+def _forward_with_checkpoints(slab_t, probe_k, weights, origins, shifts_px, prop, ckpt_every, want_patterns,
+                              want_amplitude, max_batch_waves):
     if isinstance(probe_k, torch.Tensor) and probe_k.ndim == 3 and not is_fast_grid(probe_k.shape[1],
                                                                                      probe_k.shape[2]):
         _require(int(ckpt_every) >= 1, "ckpt_every must be >= 1")
-        patterns, _, ckpt = _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop, True, False, None,
-                                          int(ckpt_every))
-        return patterns, ckpt
+        return _grid_forward(slab_t, probe_k, weights, origins, shifts_px, prop, want_patterns, want_amplitude, None,
+                             int(ckpt_every))
     dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
                                                                         shifts_px, prop, GRADIENT_N,
                                                                         "checkpoints")
     _require(int(ckpt_every) >= 1, "ckpt_every must be >= 1")
     nc = num_checkpoints(s_count, int(ckpt_every))
-    patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev)
+    patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev) if want_patterns else None
+    amplitude = torch.empty((p_count, m_count, n, n), dtype=torch.complex64, device=dev) if want_amplitude else None
     ckpt = torch.empty((p_count * m_count, nc, n, n), dtype=torch.complex64, device=dev)
     ws_bytes = workspace_bytes(n, p_count, m_count, max_batch_waves)
     workspace = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev) if ws_bytes else None
@@ -425,10 +440,58 @@ def multislice_forward_checkpointed(slab_t, probe_k, weights, origins, shifts_px
         status = lib.pty_forward_checkpointed(
             n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
             _native.ptr(origins), _native.ptr(shifts_px), *_prop_ptrs(prop),
-            _native.ptr(patterns), _native.ptr(ckpt) if nc else 0, int(ckpt_every), _native.ptr(workspace),
-            ws_bytes, _native.stream_handle(dev))
+            _native.ptr(patterns), _native.ptr(amplitude), _native.ptr(ckpt) if nc else 0, int(ckpt_every),
+            _native.ptr(workspace), ws_bytes, _native.stream_handle(dev))
     _native.check(status, "pty_forward_checkpointed")
-    return patterns, ckpt
+    return patterns, amplitude, ckpt
+
+
+JVP_N = (32, 64, 128, 256, 512)
+# default JVP batch: 256 MiB of (primal, tangent) wave pairs
+JVP_BATCH_BYTES = 256 << 20
+
+
+def multislice_jvp(slab_t, probe_k, weights, origins, shifts_px, prop, d_slab: Optional[torch.Tensor] = None,
+                   d_probe_k: Optional[torch.Tensor] = None, d_shifts: Optional[torch.Tensor] = None, *,
+                   want_patterns: bool = False, want_amplitude: bool = False, want_d_patterns: bool = True,
+                   want_d_amplitude: bool = False, max_batch_waves: Optional[int] = None):
+    """Forward-mode multislice (``pty_jvp``): tangents of the patterns / amplitudes for tangents of the slab,
+    probe_k and shifts (each None = zero).
+
+    Operands as for :func:`multislice_forward` (square power-of-two windows up to 512).  Returns
+    (patterns, amplitude, d_patterns, d_amplitude), None for outputs not requested; the primal outputs equal
+    :func:`multislice_forward`'s to rounding, the tangents are complex64 / float32 in the fftshifted layout.
+    """
+    dev, n, s_count, hs, ws, m_count, p_count = _check_forward_operands(slab_t, probe_k, weights, origins,
+                                                                        shifts_px, prop, JVP_N, "the JVP")
+    _require(want_d_patterns or want_d_amplitude, "no tangent output requested")
+    if d_slab is not None:
+        _check_cuda(d_slab, "d_slab", torch.complex64, 3, dev)
+        _require(d_slab.shape == slab_t.shape, "d_slab must match the slab")
+    if d_probe_k is not None:
+        _check_cuda(d_probe_k, "d_probe_k", torch.complex64, 3, dev)
+        _require(d_probe_k.shape == probe_k.shape, "d_probe_k must match probe_k")
+    if d_shifts is not None:
+        _check_cuda(d_shifts, "d_shifts", torch.float64, 2, dev)
+        _require(tuple(d_shifts.shape) == (p_count, 2), "d_shifts must have shape (P, 2)")
+    patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev) if want_patterns else None
+    amplitude = torch.empty((p_count, m_count, n, n), dtype=torch.complex64, device=dev) if want_amplitude else None
+    d_patterns = torch.empty((p_count, n, n), dtype=torch.float32, device=dev) if want_d_patterns else None
+    d_amplitude = (torch.empty((p_count, m_count, n, n), dtype=torch.complex64, device=dev)
+                   if want_d_amplitude else None)
+    cap = int(max_batch_waves) if max_batch_waves is not None else max(m_count, JVP_BATCH_BYTES // (16 * n * n))
+    lib = _native.load()
+    ws_bytes = int(lib.pty_jvp_workspace_bytes(n, p_count, m_count, cap))
+    workspace = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        status = lib.pty_jvp(
+            n, s_count, p_count, m_count, _native.ptr(slab_t), hs, ws, _native.ptr(probe_k), _native.ptr(weights),
+            _native.ptr(origins), _native.ptr(shifts_px), *_prop_ptrs(prop), _native.ptr(d_slab),
+            _native.ptr(d_probe_k), _native.ptr(d_shifts), _native.ptr(patterns), _native.ptr(amplitude),
+            _native.ptr(d_patterns), _native.ptr(d_amplitude), _native.ptr(workspace), ws_bytes,
+            _native.stream_handle(dev))
+    _native.check(status, "pty_jvp")
+    return patterns, amplitude, d_patterns, d_amplitude
 
 
 def multislice_adjoint(slab_t, probe_k, weights, origins, shifts_px, prop, grad_patterns, ckpt, ckpt_every: int,

@@ -53,12 +53,16 @@ def _leaves(tree):
 
 
 def tree_dot(a, b) -> torch.Tensor:
-    """Real Hermitian inner product Re sum conj(a) b over all leaves (float64)."""
-    total = torch.zeros((), dtype=torch.float64)
+    """Real Hermitian inner product Re sum conj(a) b over all leaves, accumulated in float64 / complex128 on the
+    leaves' device (one host read per call)."""
+    total = None
     for x, y in zip(_leaves(a), _leaves(b)):
-        v = torch.sum(torch.conj(x) * y) if x.is_complex() else torch.sum(x * y)
-        total = total + (v.real if v.is_complex() else v).to(torch.float64).cpu()
-    return total
+        if x.is_complex() or y.is_complex():
+            v = torch.sum(torch.conj(x.to(torch.complex128)) * y.to(torch.complex128)).real
+        else:
+            v = torch.sum(x.to(torch.float64) * y.to(torch.float64))
+        total = v if total is None else total + v.to(total.device)
+    return total.cpu() if total is not None else torch.zeros((), dtype=torch.float64)
 
 
 def tree_add(a, b):

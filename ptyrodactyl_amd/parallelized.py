@@ -141,10 +141,9 @@ def stem4d_sharded(probe_modes: ProbeModes, sample: AtomicSliceData, microscope:
     prop = physics.propagator_pair(h, w, dz, microscope.voltage_kv, calib, device=dev)
     weights = probe_modes.weights.to(device=dev, dtype=torch.float32).contiguous()
     pk = _probe_k(probe_modes.modes, dev)
-    if torch.is_grad_enabled() and (slab.requires_grad or pk.requires_grad or weights.requires_grad
-                                    or shifts.requires_grad):
+    from . import autograd
+    if autograd.differentiable(slab, pk, weights, shifts):
         # differentiable w.r.t. the modes, their weights, the atom kernels and the positions (HIP adjoint)
-        from . import autograd
         patterns = autograd.multislice_patterns(slab, pk, weights, origins, shifts, prop)
     else:
         patterns, _ = engine.multislice_forward(slab, pk, weights, origins, shifts, prop)

@@ -168,12 +168,11 @@ def _run_positions(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Mi
     origins = torch.zeros((p_count, 2), dtype=torch.int32, device=device)
     prop = physics.propagator_pair(h, w, pot_slices.slice_thickness, microscope.voltage_kv, calib, device=device)
     shifts = shifts_px.to(device).contiguous()
-    if torch.is_grad_enabled() and (slab.requires_grad or pk.requires_grad or weights.requires_grad
-                                    or shifts.requires_grad):
+    from . import autograd
+    if autograd.differentiable(slab, pk, weights, shifts):
         # differentiable path: the potential, probe modes and positions reach the kernels through torch
-        # (exp(i sigma V), fft2, positions / calibration) and the HIP adjoint supplies the gradient w.r.t.
-        # the transmission, probe_k and the shifts in pixels
-        from . import autograd
+        # (exp(i sigma V), fft2, positions / calibration); the HIP adjoint supplies the gradient and the HIP
+        # tangent recurrence the forward-mode derivative w.r.t. the transmission, probe_k and the shifts
         if want_amplitude:
             return (None, autograd.multislice_amplitudes(slab, pk, origins, shifts, prop)), calib, h
         return (autograd.multislice_patterns(slab, pk, weights, origins, shifts, prop), None), calib, h

@@ -339,6 +339,113 @@ def test_aberration12_astigmatism_symmetry():
     assert o.aberration12(qx, -qy, lam, coeffs) == pytest.approx(-a)
 
 
+# (index, radial order k in lambda |q|, fold m, prefactor of the Krivanek form chi = (2 pi / lambda) Re[...])
+ABERRATION_TERMS = {
+    "C1": (0, 2, 0, 1 / 2), "A1a": (1, 2, 2, 1 / 2), "A1b": (2, 2, 2, 1 / 2), "B2a": (3, 3, 1, 1.0),
+    "B2b": (4, 3, 1, 1.0), "C3": (5, 4, 0, 1 / 4), "A2a": (6, 3, 3, 1 / 3), "A2b": (7, 3, 3, 1 / 3),
+    "S3a": (8, 4, 2, 1.0), "S3b": (9, 4, 2, 1.0), "A3a": (10, 4, 4, 1 / 4), "A3b": (11, 4, 4, 1 / 4),
+}
+
+
+def _rotate(qy, qx, phi):
+    """Counter-clockwise rotation of (qx, qy) by phi (omega = lambda (qx + i qy) -> omega e^{i phi})."""
+    return qx * np.sin(phi) + qy * np.cos(phi), qx * np.cos(phi) - qy * np.sin(phi)
+
+
+@pytest.mark.parametrize("name", list(ABERRATION_TERMS))
+def test_aberration12_term_order_symmetry_prefactor_orientation(name):
+    """Each coefficient alone, by the properties its name fixes: chi scales as (lambda |q|)^k (k = 2 for C1, A1;
+    3 for B2, A2; 4 for C3, S3, A3); it is m-fold symmetric (C1, C3 round; B2 1-, A1 and S3 2-, A2 3-, A3
+    4-fold) and changes sign under a rotation by pi/m; its maximum over the azimuth at a radius is
+    (2 pi / lambda) c (lambda |q|)^k times the prefactor (1/2 C1, A1; 1/3 A2; 1/4 C3, A3; 1 B2, S3); the real
+    part (suffix a) peaks on the +qx axis and the imaginary part (suffix b) at the azimuth pi / (2 m)."""
+    idx, k, m, pref = ABERRATION_TERMS[name]
+    lam = o.wavelength_ang(300.0)
+    c = 37.0
+    coeffs = np.zeros(12)
+    coeffs[idx] = c
+    rng = np.random.default_rng(idx)
+    qy, qx = rng.uniform(-0.8, 0.8, 50), rng.uniform(-0.8, 0.8, 50)
+    chi = o.aberration12(qy, qx, lam, coeffs)
+    assert np.allclose(o.aberration12(1.7 * qy, 1.7 * qx, lam, coeffs), 1.7 ** k * chi, rtol=1e-12, atol=1e-12)
+    if m == 0:
+        assert np.allclose(o.aberration12(*_rotate(qy, qx, 0.7), lam, coeffs), chi, rtol=1e-12, atol=1e-12)
+    else:
+        assert np.allclose(o.aberration12(*_rotate(qy, qx, 2 * np.pi / m), lam, coeffs), chi, rtol=1e-10, atol=1e-12)
+        assert np.allclose(o.aberration12(*_rotate(qy, qx, np.pi / m), lam, coeffs), -chi, rtol=1e-10, atol=1e-12)
+    # azimuthal profile on a circle |q| = 0.5
+    theta = np.linspace(0.0, 2 * np.pi, 7201)[:-1]
+    prof = o.aberration12(0.5 * np.sin(theta), 0.5 * np.cos(theta), lam, coeffs)
+    peak = (2 * np.pi / lam) * c * (lam * 0.5) ** k * pref
+    assert prof.max() == pytest.approx(peak, rel=1e-12)
+    if m:
+        want_theta = 0.0 if name.endswith("a") else np.pi / (2 * m)
+        assert o.aberration12(0.5 * np.sin(want_theta), 0.5 * np.cos(want_theta), lam, coeffs) == \
+            pytest.approx(peak, rel=1e-12)
+
+
+def test_lm_step_rules_closed_forms():
+    """The reference's closed forms (test_solvers.py:131-226) through the oracle's step functions, on
+    f(p) = 2 p with data 2 from p = 0: the GN step (lambda = 0) lands on 1; the LM step on 4 / (4 + lambda) with
+    residual norm 2 lambda / (4 + lambda); on this quadratic the gain ratio is exactly 2 (the reference divides a
+    reduction of |r|^2 by the model's reduction of |r|^2 / 2) and the damping drops to lambda / 3; the damping
+    rule's thresholds are strict."""
+    jac = np.array([[2.0]])
+    r0 = np.array([2.0 * 0.0 - 2.0])
+    step, _, _ = o.lm_step_dense(jac, r0, 0.0)
+    assert 0.0 - step[0] == pytest.approx(1.0, abs=1e-15)
+    for lam in (1e-3, 0.25, 3.0):
+        step, jtj, g = o.lm_step_dense(jac, r0, lam)
+        cand = 0.0 - step[0]
+        assert cand == pytest.approx(4.0 / (4.0 + lam), rel=1e-15)
+        r1 = 2.0 * cand - 2.0
+        assert abs(r1) == pytest.approx(2.0 * lam / (4.0 + lam), rel=1e-12)
+        rho = o.lm_gain_ratio(float(r0 @ r0), r1 * r1, step, g, jtj)
+        assert rho == pytest.approx(2.0, rel=1e-9)
+        assert o.lm_update_damping(rho, lam) == pytest.approx(lam / 3.0)
+    assert o.lm_update_damping(0.5, 0.2) == 0.2
+    assert o.lm_update_damping(0.1, 0.2) == pytest.approx(0.4)
+    assert o.lm_update_damping(0.75, 0.2) == 0.2 and o.lm_update_damping(0.25, 0.2) == 0.2
+    assert o.lm_update_damping(-1.0, 0.2) == pytest.approx(0.4)
+
+
+def _small_aberration_problem(offset):
+    n, calib, kv, aperture, dz = 32, 0.3, 300.0, 20.0, 1.5
+    rng = np.random.default_rng(31)
+    slab = np.exp(1j * rng.uniform(0.0, 0.6, (2, 40, 40)))
+    origins = rng.integers(0, 40, (5, 2))
+    shifts = np.zeros((5, 2))
+    true = np.array([40.0, 8.0, -5.0, 60.0, 30.0, 2e3, 150.0, -90.0, 400.0, 300.0, 900.0, -700.0])
+    pk = o.probe_k_from_coeffs(n, calib, kv, aperture, true)
+    prop = o.propagation_func(n, n, dz, kv, calib)
+    amp = o.window_amplitudes(slab, pk[None], origins, shifts, prop)[:, 0]
+    data = np.fft.fftshift(np.abs(amp) ** 2, axes=(-2, -1))
+    init = true * (1.0 + offset * np.linspace(-1.0, 1.0, 12))
+    return dict(slab=slab, origins=origins, shifts=shifts, data=data, init=init, true=true, calib=calib, kv=kv,
+                aperture=aperture, dz=dz)
+
+
+def test_lm_aberration_fit_freezes_when_converged():
+    """Started at the coefficients that produced the data, the residual is 0 below the tolerance: the loop
+    takes no step and returns the start (test_solvers.py:198-226)."""
+    pb = _small_aberration_problem(0.0)
+    coeffs, hist = o.lm_aberration_fit(pb["slab"], pb["origins"], pb["shifts"], pb["data"], pb["init"], pb["calib"],
+                                       pb["kv"], pb["aperture"], pb["dz"], iterations=5, tolerance=1e-8)
+    assert len(hist) == 1 and hist[0] < 1e-8
+    assert np.allclose(coeffs, pb["init"], rtol=1e-15, atol=0)
+
+
+def test_lm_aberration_fit_recovers_known_coefficients():
+    """From +-3 % offsets the FP64 loop returns the twelve coefficients that produced noise-free data (a wrong
+    step sign or gain-ratio rule diverges or stalls), with a non-increasing residual history."""
+    pb = _small_aberration_problem(0.03)
+    coeffs, hist = o.lm_aberration_fit(pb["slab"], pb["origins"], pb["shifts"], pb["data"], pb["init"], pb["calib"],
+                                       pb["kv"], pb["aperture"], pb["dz"], iterations=25, tolerance=1e-12)
+    assert all(b <= a * (1 + 1e-12) for a, b in zip(hist, hist[1:]))
+    assert hist[-1] < 1e-6 * hist[0]
+    assert np.allclose(coeffs, pb["true"], rtol=1e-5)
+
+
 def test_threefry_known_answer():
     """Random123 known-answer vector for Threefry-2x32-20 (key=0, ctr=0)."""
     x0, x1 = o.threefry2x32((0, 0), np.array([0], np.uint32), np.array([0], np.uint32))

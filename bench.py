@@ -314,6 +314,31 @@ def grid_bench(steps: int = 3, cases=((100, 100, 16, 1024), (250, 250, 16, 512),
     return out
 
 
+def calibration_kernels(sizes_mib=(64, 2048), reps: int = 5):
+    """Kernels of known traffic for calibrating the TCC byte counters under rocprofv3 --pmc: per size, `reps`
+    launches each of a read-only reduction (a.sum(): n * 4 bytes read), a write-only fill (b.fill_(): n * 4
+    bytes written) and a copy (b.copy_(a): n * 4 read + n * 4 written).  64 MiB stays in the 256 MB
+    Infinity Cache between launches, 2 GiB does not.  Prints the launch plan; timing is incidental."""
+    import torch
+    dev = torch.device("cuda", 0)
+    plan = []
+    for mib in sizes_mib:
+        n = mib * (1 << 20) // 4
+        a = torch.ones(n, device=dev)
+        b = torch.empty_like(a)
+        s = torch.empty((), device=dev)
+        for _ in range(reps):
+            s.copy_(a.sum())
+        for _ in range(reps):
+            b.fill_(2.0)
+        for _ in range(reps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        plan.append({"mib": mib, "bytes": n * 4, "reps": reps, "order": ["sum (read)", "fill (write)", "copy"]})
+        del a, b
+    return plan
+
+
 def membw_bench(sizes_mib=(256, 512, 1024, 4096), reps: int = 20):
     """Achievable HBM bandwidth on this box: torch device copies (read + write) and reductions (read) of
     float32 buffers, timed with events; the practical ceiling the streaming passes are compared with."""
@@ -359,7 +384,11 @@ def main() -> int:
     ap.add_argument("--serving-qps", type=float, default=60.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
-    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw", "potential", "grid"),
+    ap.add_argument("--only-forward", action="store_true",
+                    help="time the headline forward only (no detector, graph, gradient, potential, serving or CPU "
+                         "legs): the run rocprofv3 counter passes profile")
+    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw", "potential", "grid",
+                                           "calib"),
                     default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
                          "config1: 128^2 x 20 slices, 256 positions (the reference's CPU-runnable case)")
@@ -392,6 +421,9 @@ def main() -> int:
                           "value": res[0]["patterns_per_s"], "unit": "patterns/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "complex64", "data": "synthetic", "grid": res}),
               flush=True)
+        return 0
+    if args.workload == "calib":
+        print(json.dumps({"workload": "counter_calibration", "plan": calibration_kernels()}), flush=True)
         return 0
     if args.workload == "membw":
         res = membw_bench()
@@ -493,6 +525,9 @@ def main() -> int:
                      "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
                      "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
     }
+    if args.only_forward:
+        print(json.dumps(result), flush=True)
+        return 0
     if args.workload == "config2" and not persistent:
         result["virtual_detectors"] = detectors_bench(w, slab, pk, weights, origins, shifts, prop, args.steps)
         result["small_request_graph"] = captured_small_bench(w, slab, pk, weights, origins, shifts, prop)

@@ -95,6 +95,7 @@ pty::FwdParams make_params(int n, int num_slices, long long num_positions, int n
   a.det_masks = nullptr;
   a.det_out = nullptr;
   a.det_count = 0;
+  a.max_lanes = 0;
   return a;
 }
 
@@ -219,6 +220,7 @@ int pty_forward_noalloc(int n, int num_slices, long long num_positions, int num_
                                  mode_weights, origins, shifts_px, prop_y, prop_x);
   a.patterns = patterns;
   a.amplitude = (float2*)amplitude;
+  a.max_lanes = 1;  // the caller's stream only: no library streams or events are created
   return device_status(pty::launch_forward(a, ws, ws_bytes, (hipStream_t)stream), "pty_forward_noalloc");
 }
 

@@ -41,6 +41,7 @@ struct FwdParams {
   const unsigned char* det_masks;  // [D][n][n] 0/1 virtual-detector masks (fftshifted layout), or null
   double* det_out;                 // [P][D] sum over each mask of the pattern intensity, or null
   int det_count;                   // D, at most MAX_DETECTORS
+  int max_lanes;                   // streaming launch lanes allowed (0 = the launcher's default, 1 = one stream)
 };
 
 constexpr int MAX_DETECTORS = 4;

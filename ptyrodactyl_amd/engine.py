@@ -24,9 +24,9 @@ DETECTOR_N = (32, 64, 128, 256, 512, 1024)
 # from 512 up the default streaming batch is sized by bytes: 192 MiB of waves measured fastest at 512^2
 # (96 waves: 64.4 ms vs 71.8 ms with 512) and 1024^2 (24 waves: 17.2 ms vs 19.6 ms with 32)
 LARGE_BATCH_BYTES = 192 << 20
-# Waves (positions x modes) per streaming batch for n >= 256; the batch's
-# waves live in a device workspace that should stay cache-resident.
-DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "512"))
+# Waves (positions x modes) per streaming batch at n = 256; the batch's waves live in a device workspace that
+# should stay cache-resident.  The library runs it as three lanes of 128 waves on their own streams.
+DEFAULT_BATCH_WAVES = int(os.environ.get("PTY_MAX_BATCH_WAVES", "384"))
 # largest edge of the runtime-size kernels (any h x w that is not a square power of two)
 GRID_MAX_EDGE = 4096
 

@@ -350,6 +350,67 @@ class AberrationFit:
     history: list
 
 
+class _AberrationModel:
+    """The pieces of the aberration LM shared by the coefficient-only and the joint fit: the scaled basis
+    (a unit step of a scaled coefficient moves chi by at most 1 rad inside the aperture), the FP64
+    normal-equation and solve buffers, and the coefficient step."""
+
+    def __init__(self, n, p_count, calib_ang, voltage_kv, aperture_mrad, slice_thickness, device):
+        self.n, self.p_count, self.dev = int(n), int(p_count), device
+        basis = aberration_basis(n, calib_ang, voltage_kv, device=device)
+        self.aperture = aperture_mask(n, calib_ang, voltage_kv, aperture_mrad, device=device)
+        self.unit = 1.0 / torch.clamp(torch.amax(torch.abs(basis) * self.aperture, dim=(1, 2)), min=1e-300)
+        self.sbasis = basis * self.unit[:, None, None]     # d chi / d theta_j
+        self.q = self.sbasis.shape[0]
+        self.prop = physics.propagator_factors(n, slice_thickness, voltage_kv, calib_ang, device=device)
+        self.ones = torch.ones(self.q + 1, dtype=torch.float32, device=device)
+        self.one = torch.ones(1, dtype=torch.float32, device=device)
+        self.lib = _native.load()
+        self.ws_bytes = int(self.lib.pty_normal_equations_workspace_bytes(p_count, n, self.q))
+        self.ne_ws = torch.empty(max(self.ws_bytes, 8) // 8, dtype=torch.float64, device=device)
+        self.jtj = torch.empty((self.q, self.q), dtype=torch.float64, device=device)
+        self.jtr = torch.empty(self.q, dtype=torch.float64, device=device)
+        self.rss_buf = torch.empty(1, dtype=torch.float64, device=device)
+        self.step = torch.empty(self.q, dtype=torch.float64, device=device)
+        self.model = torch.empty(1, dtype=torch.float64, device=device)
+        self.status = torch.empty(1, dtype=torch.int32, device=device)
+        self.stream = _native.stream_handle(device)
+
+    def probe_k(self, theta) -> torch.Tensor:
+        return probe_k_from_chi(self.aperture, torch.einsum("j,jyx->yx", theta, self.sbasis))
+
+    def rss(self, slab_t, theta, origins, shifts_px, data32) -> torch.Tensor:
+        pk = self.probe_k(theta).to(torch.complex64)[None].contiguous()
+        pat, _ = engine.multislice_forward(slab_t, pk, self.one, origins, shifts_px, self.prop)
+        return torch.sum((pat.to(torch.float64) - data32.to(torch.float64)) ** 2)
+
+    def coefficient_step(self, slab_t, theta, damping, origins, shifts_px, data32):
+        """One LM step on the coefficients: 13-mode complex64 forward (the probe and its 12 directional
+        derivatives) for the Jacobian columns J_j = 2 Re(conj(A) dA_j), FP64 J^T J / J^T r / |r|^2
+        (``pty_normal_equations``) and the FP64 damped solve (``pty_damped_solve``) on the GPU, the
+        reference's gain ratio, damping rule and acceptance.  Returns (theta, rss, damping)."""
+        lib = self.lib
+        pk0 = self.probe_k(theta)
+        modes = torch.cat([pk0[None], (-1j) * self.sbasis * pk0[None]], dim=0).to(torch.complex64).contiguous()
+        _, amp = engine.multislice_forward(slab_t, modes, self.ones, origins, shifts_px, self.prop,
+                                           want_patterns=False, want_amplitude=True)
+        _native.check(lib.pty_normal_equations(_native.ptr(amp), _native.ptr(data32), self.p_count, self.n, self.q,
+                                               _native.ptr(self.ne_ws), self.ws_bytes, _native.ptr(self.jtj),
+                                               _native.ptr(self.jtr), _native.ptr(self.rss_buf), self.stream),
+                      "pty_normal_equations")
+        _native.check(lib.pty_damped_solve(_native.ptr(self.jtj), _native.ptr(self.jtr), float(damping), self.q,
+                                           _native.ptr(self.step), _native.ptr(self.model), _native.ptr(self.status),
+                                           self.stream), "pty_damped_solve")
+        candidate = theta - self.step
+        rss_new = self.rss(slab_t, candidate, origins, shifts_px, data32)
+        rss = self.rss_buf[0].clone()
+        gain = float((rss - rss_new) / (self.model[0] + 1e-12))
+        damping = _lm_damping_update(gain, float(damping))
+        if gain > 0.0:
+            return candidate, rss_new, damping
+        return theta, rss, damping
+
+
 def fit_probe_aberrations(slab_t: torch.Tensor, origins: torch.Tensor, shifts_px: torch.Tensor,
                           data: torch.Tensor, coeffs_init, *, calib_ang: float, voltage_kv: float,
                           aperture_mrad: float, slice_thickness: float, iterations: int = 30,
@@ -366,63 +427,115 @@ def fit_probe_aberrations(slab_t: torch.Tensor, origins: torch.Tensor, shifts_px
     ``levenberg_marquardt_step`` (``solvers.py:387-521``).
     """
     dev = slab_t.device
-    n = int(data.shape[-1])
-    p_count = int(origins.shape[0])
-    basis = aberration_basis(n, calib_ang, voltage_kv, device=dev)
-    aperture = aperture_mask(n, calib_ang, voltage_kv, aperture_mrad, device=dev)
-    unit = 1.0 / torch.clamp(torch.amax(torch.abs(basis) * aperture, dim=(1, 2)), min=1e-300)
-    sbasis = basis * unit[:, None, None]                   # d chi / d theta_j
-    q_count = sbasis.shape[0]
-    theta = torch.as_tensor(coeffs_init, dtype=torch.float64, device=dev) / unit
-    prop = physics.propagator_factors(n, slice_thickness, voltage_kv, calib_ang, device=dev)
-    ones = torch.ones(q_count + 1, dtype=torch.float32, device=dev)
-    one = torch.ones(1, dtype=torch.float32, device=dev)
+    model = _AberrationModel(int(data.shape[-1]), int(origins.shape[0]), calib_ang, voltage_kv, aperture_mrad,
+                             slice_thickness, dev)
+    theta = torch.as_tensor(coeffs_init, dtype=torch.float64, device=dev) / model.unit
     data32 = data.to(torch.float32).contiguous()
-    lib = _native.load()
-    ws_bytes = int(lib.pty_normal_equations_workspace_bytes(p_count, n, q_count))
-    ne_ws = torch.empty(max(ws_bytes, 8) // 8, dtype=torch.float64, device=dev)
-    jtj = torch.empty((q_count, q_count), dtype=torch.float64, device=dev)
-    jtr = torch.empty(q_count, dtype=torch.float64, device=dev)
-    rss = torch.empty(1, dtype=torch.float64, device=dev)
-    step = torch.empty(q_count, dtype=torch.float64, device=dev)
-    model = torch.empty(1, dtype=torch.float64, device=dev)
-    status = torch.empty(1, dtype=torch.int32, device=dev)
-    stream = _native.stream_handle(dev)
-
-    def chi_of(th):
-        return torch.einsum("j,jyx->yx", th, sbasis)
-
-    def rss_of(th) -> torch.Tensor:
-        pk = probe_k_from_chi(aperture, chi_of(th)).to(torch.complex64)[None].contiguous()
-        pat, _ = engine.multislice_forward(slab_t, pk, one, origins, shifts_px, prop)
-        return torch.sum((pat.to(torch.float64) - data32.to(torch.float64)) ** 2)
-
     damping = float(damping_init)
-    res_norm = float(torch.sqrt(rss_of(theta)))
+    res_norm = float(torch.sqrt(model.rss(slab_t, theta, origins, shifts_px, data32)))
     history = [res_norm]
     it = 0
     for _ in range(iterations):
         if res_norm < tolerance:
             break
-        pk0 = probe_k_from_chi(aperture, chi_of(theta))
-        modes = torch.cat([pk0[None], (-1j) * sbasis * pk0[None]], dim=0).to(torch.complex64).contiguous()
-        _, amp = engine.multislice_forward(slab_t, modes, ones, origins, shifts_px, prop, want_patterns=False,
-                                           want_amplitude=True)
-        _native.check(lib.pty_normal_equations(_native.ptr(amp), _native.ptr(data32), p_count, n, q_count,
-                                               _native.ptr(ne_ws), ws_bytes, _native.ptr(jtj), _native.ptr(jtr),
-                                               _native.ptr(rss), stream), "pty_normal_equations")
-        _native.check(lib.pty_damped_solve(_native.ptr(jtj), _native.ptr(jtr), damping, q_count, _native.ptr(step),
-                                           _native.ptr(model), _native.ptr(status), stream), "pty_damped_solve")
-        candidate = theta - step
-        rss_new = rss_of(candidate)
-        gain = float((rss[0] - rss_new) / (model[0] + 1e-12))
-        damping = _lm_damping_update(gain, damping)
-        if gain > 0.0:
-            theta = candidate
-            res_norm = float(torch.sqrt(rss_new))
-        else:
-            res_norm = float(torch.sqrt(rss[0]))
+        theta, rss, damping = model.coefficient_step(slab_t, theta, damping, origins, shifts_px, data32)
+        res_norm = float(torch.sqrt(rss))
         it += 1
         history.append(res_norm)
-    return AberrationFit(coefficients=(theta * unit).detach(), residual_norm=res_norm, damping=damping,
+    return AberrationFit(coefficients=(theta * model.unit).detach(), residual_norm=res_norm, damping=damping,
                          iterations=it, history=history)
+
+
+@dataclass
+class JointFit:
+    coefficients: torch.Tensor      # (12,) float64, Angstrom
+    object: torch.Tensor            # (H, W) complex128, the object slice every slice repeats
+    residual_norm: float
+    damping: tuple                  # (coefficients, object)
+    iterations: int
+    history: list                   # [(|r| after the coefficient step, |r| after the object step)]
+
+
+def _cg_fixed(apply_a, rhs, iterations: int):
+    """The reference's conjugate gradient (``solvers.py:62-180``) from x0 = 0 for a fixed number of iterations,
+    complex128 vectors on the device and FP64 scalars."""
+    x = torch.zeros_like(rhs)
+    r = rhs.clone()
+    p = r.clone()
+    rr = tree_dot(r, r)
+    for _ in range(int(iterations)):
+        ap = apply_a(p)
+        alpha = float(rr / (tree_dot(p, ap) + 1e-12))
+        x = x + alpha * p
+        r = r - alpha * ap
+        rr_new = tree_dot(r, r)
+        beta = float(rr_new / (rr + 1e-12))
+        p = r + beta * p
+        rr = rr_new
+    return x
+
+
+def fit_aberrations_and_object(object_init: torch.Tensor, num_slices: int, origins: torch.Tensor,
+                               shifts_px: torch.Tensor, data: torch.Tensor, coeffs_init, *, calib_ang: float,
+                               voltage_kv: float, aperture_mrad: float, slice_thickness: float,
+                               iterations: int = 30, damping_coeffs: float = 1e-3, damping_object: float = 1e-3,
+                               cg_iterations: int = 3, ckpt_every: int = 4) -> JointFit:
+    """Joint reconstruction of the 12 probe aberrations and an object slice that every one of ``num_slices``
+    slices repeats, from 4D-STEM intensities (block-coordinate Levenberg-Marquardt).
+
+    Every iteration takes one LM step on the coefficients with the object fixed (complex64 Jacobian columns,
+    FP64 normal equations and damped solve on the GPU, as :func:`fit_probe_aberrations`), then one LM step on
+    the object with the coefficients fixed, matrix-free as the reference's ``levenberg_marquardt_step``
+    (``src/ptyrodactyl/jacobian/solvers.py:387-521``): g = J^T r from the HIP adjoint, ``cg_iterations`` steps
+    of the reference's conjugate gradient on (J^T J + lambda I) s = g with J^T J v = adjoint(JVP(v)) (``pty_jvp``
+    then ``pty_adjoint``, the object tangent moving every slice), candidate O - s, the reference's gain ratio,
+    damping rule and acceptance.  CG vectors are complex128 and every inner product and residual norm FP64
+    on the GPU; the multislice, its JVP and its adjoint run in complex64.
+    """
+    dev = object_init.device
+    n = int(data.shape[-1])
+    model = _AberrationModel(n, int(origins.shape[0]), calib_ang, voltage_kv, aperture_mrad, slice_thickness, dev)
+    theta = torch.as_tensor(coeffs_init, dtype=torch.float64, device=dev) / model.unit
+    obj = object_init.to(torch.complex128).clone()
+    data32 = data.to(torch.float32).contiguous()
+    lam_c, lam_o = float(damping_coeffs), float(damping_object)
+    k = int(ckpt_every)
+    s_count = int(num_slices)
+
+    def slab_of(o_):
+        return o_.to(torch.complex64)[None].expand(s_count, -1, -1).contiguous()
+
+    slab = slab_of(obj)
+    rss = model.rss(slab, theta, origins, shifts_px, data32)
+    history = [(float(torch.sqrt(rss)), float(torch.sqrt(rss)))]
+    for _ in range(iterations):
+        theta, rss, lam_c = model.coefficient_step(slab, theta, lam_c, origins, shifts_px, data32)
+        after_c = float(torch.sqrt(rss))
+        pk = model.probe_k(theta).to(torch.complex64)[None].contiguous()
+        pat, ckpt = engine.multislice_forward_checkpointed(slab, pk, model.one, origins, shifts_px, model.prop, k)
+        resid = (pat - data32).contiguous()
+
+        def vjp(u):
+            g_slab, _ = engine.multislice_adjoint(slab, pk, model.one, origins, shifts_px, model.prop, u, ckpt, k,
+                                                  want_probe=False)
+            return g_slab.to(torch.complex128).sum(dim=0)
+
+        def jtj(v):
+            d_slab = v.to(torch.complex64)[None].expand(s_count, -1, -1).contiguous()
+            _, _, d_pat, _ = engine.multislice_jvp(slab, pk, model.one, origins, shifts_px, model.prop, d_slab)
+            return vjp(d_pat)
+
+        g = vjp(resid)
+        step = _cg_fixed(lambda v: jtj(v) + lam_o * v, g, cg_iterations)
+        cand = obj - step
+        cand_slab = slab_of(cand)
+        rss_new = model.rss(cand_slab, theta, origins, shifts_px, data32)
+        predicted = tree_dot(step, g) - 0.5 * tree_dot(step, jtj(step))
+        gain = float((rss.cpu() - rss_new.cpu()) / (predicted + 1e-12))
+        lam_o = _lm_damping_update(gain, lam_o)
+        if gain > 0.0:
+            obj, slab, rss = cand, cand_slab, rss_new
+        history.append((after_c, float(torch.sqrt(rss))))
+    return JointFit(coefficients=(theta * model.unit).detach(), object=obj.detach(),
+                    residual_norm=float(torch.sqrt(rss)), damping=(lam_c, lam_o), iterations=iterations,
+                    history=history)

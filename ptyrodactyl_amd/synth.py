@@ -148,11 +148,14 @@ class AberrationWorkload:
     shifts_px: np.ndarray    # (P, 2) float64
     coeffs_true: np.ndarray  # (12,) Angstrom
     coeffs_init: np.ndarray  # (12,) seeded +-20 % offsets
+    object_true: np.ndarray = None  # (H, W) complex64 object slice (each of the S slices repeats it)
+    object_init: np.ndarray = None  # (H, W) complex64 start of the joint fit: seeded +-5 % amplitude noise
 
 
 def config4(scan: int = 24, n: int = 128, object_px: int = 256, num_slices: int = 8) -> AberrationWorkload:
     """Aberration fit: one 256^2 object slice (config-1 generator, 0.15 A) applied at each of 8 slices,
-    128^2 windows on a scan x scan raster, 300 kV / 20 mrad; initial coefficients offset by seeded +-20 %."""
+    128^2 windows on a scan x scan raster, 300 kV / 20 mrad; initial coefficients offset by seeded +-20 %,
+    and for the joint fit an initial object with seeded +-5 % amplitude noise."""
     calib, kv, dz = 0.15, 300.0, 1.95
     cells = int(math.ceil(object_px * calib / LATTICE_A))
     atoms = crystal_atoms(cells, cells, 1, seed=0)
@@ -167,7 +170,8 @@ def config4(scan: int = 24, n: int = 128, object_px: int = 256, num_slices: int 
     rng = np.random.default_rng(3)
     true = np.asarray(ABERRATIONS_TRUE, np.float64)
     init = true * (1.0 + rng.uniform(-0.2, 0.2, size=12))
-    return AberrationWorkload(n, num_slices, dz, calib, kv, 20.0, slab, origins, shifts, true, init)
+    obj0 = (t1 * (1.0 + np.random.default_rng(4).uniform(-0.05, 0.05, size=t1.shape))).astype(np.complex64)
+    return AberrationWorkload(n, num_slices, dz, calib, kv, 20.0, slab, origins, shifts, true, init, t1, obj0)
 
 
 def projected_potential_torch(atoms: np.ndarray, grid: int, calib: float, num_slices: int, slice_thickness: float,

@@ -109,3 +109,48 @@ def test_config4_full_against_fp64_oracle_fixture():
     print(f"config4 max relative parameter error vs FP64 oracle {err.max():.2e}; residual {fit.residual_norm:.3e}"
           f" after {fit.iterations} iterations", flush=True)
     assert err.max() < 1e-3
+
+
+def _joint_data(w):
+    from oracle import joint_oracle as jo
+    prop = o.propagation_func(w.n, w.n, w.slice_thickness, w.voltage_kv, w.calib)
+    pk = o.probe_k_from_coeffs(w.n, w.calib, w.voltage_kv, w.aperture_mrad, w.coeffs_true)
+    return jo.patterns(w.object_true.astype(np.complex128), w.num_slices, pk, w.origins, w.shifts_px, prop)
+
+
+def _joint_fit(w, data, iterations, cg_iterations):
+    from ptyrodactyl_amd import jacobian
+    dev = torch.device("cuda", 0)
+    return jacobian.fit_aberrations_and_object(
+        torch.tensor(w.object_init, device=dev), w.num_slices, torch.tensor(w.origins, device=dev),
+        torch.tensor(w.shifts_px, device=dev), torch.tensor(data, dtype=torch.float32, device=dev), w.coeffs_init,
+        calib_ang=w.calib, voltage_kv=w.voltage_kv, aperture_mrad=w.aperture_mrad,
+        slice_thickness=w.slice_thickness, iterations=iterations, cg_iterations=cg_iterations)
+
+
+def _joint_errors(fit, coeffs_ref, obj_ref, w):
+    got_c = fit.coefficients.cpu().numpy()
+    err_c = np.abs(got_c - coeffs_ref) / np.abs(w.coeffs_true)
+    obj = fit.object.cpu().numpy()
+    start = w.object_init.astype(np.complex128)
+    err_o = np.linalg.norm(obj - obj_ref) / np.linalg.norm(obj_ref - start)
+    return err_c, err_o
+
+
+def test_joint_fit_matches_fp64_oracle_small():
+    """Joint aberration + object LM on a reduced workload (6 x 6 positions, 64^2 windows, 4 slices, 128^2
+    object, 6 iterations) against the FP64 oracle run of the same algorithm."""
+    from oracle import joint_oracle as jo
+    from ptyrodactyl_amd import synth
+    w = synth.config4(scan=6, n=64, object_px=128, num_slices=4)
+    data = _joint_data(w)
+    fit = _joint_fit(w, data, 6, 3)
+    ref_c, ref_o, hist = jo.joint_fit(w.object_init.astype(np.complex128), w.num_slices, w.origins, w.shifts_px,
+                                      data, w.coeffs_init, w.calib, w.voltage_kv, w.aperture_mrad,
+                                      w.slice_thickness, iterations=6, cg_iterations=3)
+    err_c, err_o = _joint_errors(fit, ref_c, ref_o, w)
+    print("joint GPU vs FP64 oracle, coefficients:", np.array2string(err_c, precision=2), flush=True)
+    print(f"joint object change rel-L2 vs oracle {err_o:.2e}; residual GPU "
+          f"{[f'{h[1]:.4e}' for h in fit.history]} oracle {[f'{h[1]:.4e}' for h in hist]}", flush=True)
+    assert np.allclose(np.array(fit.history), np.array(hist), rtol=1e-4)
+    assert err_c.max() < 1e-4 and err_o < 1e-4

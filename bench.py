@@ -31,14 +31,26 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
-# HBM bytes per step of the config-2 forward (streaming path, 512-wave batches) from two rocprofv3 --pmc passes
-# (FETCH_SIZE, then WRITE_SIZE) summed over the step's launches.  FETCH_SIZE reads 256 MiB for a 512 MiB copy
-# kernel and WRITE_SIZE 512 MiB, so FETCH is scaled by 2.0 and WRITE by 1.0.  Commands and per-kernel figures:
-# profiles/forward_256x64_streaming.md.
-MEASURED_TRAFFIC = {
-    ("stem4d_256x64_p4096_slab1024", "streaming", 512): {"fetch_raw_mib": 139765.8, "fetch_scale": 2.0,
-                                                           "write_raw_mib": 263361.0, "write_scale": 1.0},
-}
+# HBM-side bytes per step of the config-2 forward, measured by two rocprofv3 --pmc passes (FETCH_SIZE, then
+# WRITE_SIZE) and calibrated with bench.py --workload calib; the measurement records the sha256 of the kernel
+# sources it was taken on, and is reported only while those sources are unchanged.
+TRAFFIC_PROFILE = ROOT / "profiles" / "forward_256x64_traffic.json"
+
+
+def measured_traffic(workload: str, path: str, batch: int):
+    """(bytes per step, note) from the committed counter measurement, or (None, reason)."""
+    import hashlib
+    if not TRAFFIC_PROFILE.exists():
+        return None, "no counter measurement committed"
+    doc = json.loads(TRAFFIC_PROFILE.read_text())
+    if (doc.get("workload"), doc.get("path"), doc.get("batch_waves")) != (workload, path, batch):
+        return None, f"measured for {doc.get('workload')} / {doc.get('path')} / {doc.get('batch_waves')} waves"
+    h = hashlib.sha256()
+    for src in doc["source_files"]:
+        h.update((ROOT / src).read_bytes())
+    if h.hexdigest() != doc["source_sha256"]:
+        return None, "kernel sources changed since the counter measurement"
+    return int(doc["traffic_bytes"]), f"rocprofv3 FETCH_SIZE/WRITE_SIZE, {TRAFFIC_PROFILE.relative_to(ROOT)}"
 
 
 def algorithmic_bytes(n: int, s: int) -> int:
@@ -492,10 +504,7 @@ def main() -> int:
     bytes_step = algorithmic_bytes(w.n, w.num_slices) * p_count
     achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
     batch = args.batch_waves or engine.DEFAULT_BATCH_WAVES
-    pmc = MEASURED_TRAFFIC.get((w.name, "persistent" if persistent else "streaming", batch))
-    traffic = None
-    if pmc is not None:
-        traffic = int((pmc["fetch_raw_mib"] * pmc["fetch_scale"] + pmc["write_raw_mib"] * pmc["write_scale"]) * 2**20)
+    traffic, traffic_source = measured_traffic(w.name, "persistent" if persistent else "streaming", batch)
     flops_step = algorithmic_flops(w.n, w.num_slices) * p_count
     result = {
         "metric": "4D-STEM patterns/s at 256^2 x 64 slices" if args.workload == "config2" else
@@ -521,6 +530,7 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": bytes_step,
                      "traffic_gbs": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_source": traffic_source,
                      "kernel": "multislice forward (all launches of one step)", "kernel_ms": round(kernel_ms, 3),
                      "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
                      "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},

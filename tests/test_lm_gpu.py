@@ -154,3 +154,25 @@ def test_joint_fit_matches_fp64_oracle_small():
           f"{[f'{h[1]:.4e}' for h in fit.history]} oracle {[f'{h[1]:.4e}' for h in hist]}", flush=True)
     assert np.allclose(np.array(fit.history), np.array(hist), rtol=1e-4)
     assert err_c.max() < 1e-4 and err_o < 1e-4
+
+
+def test_config4_joint_full_against_fp64_oracle_fixture():
+    """Config 4 with both unknowns: 12 aberrations (seeded +-20 % offsets) and the 256^2 object slice (seeded
+    +-5 % amplitude noise), 24 x 24 positions, 128^2 windows, 8 slices, 30 block-coordinate LM iterations with
+    3 CG steps per object step, against the FP64 oracle run stored in tests/golden/config4_joint_fp64.npz
+    (generator tests/golden/make_config4_joint_reference.py)."""
+    from pathlib import Path
+    from ptyrodactyl_amd import synth
+    ref = np.load(Path(__file__).resolve().parent / "golden" / "config4_joint_fp64.npz")
+    w = synth.config4()
+    assert np.allclose(ref["coeffs_init"], w.coeffs_init) and np.allclose(ref["coeffs_true"], w.coeffs_true)
+    data = _joint_data(w)
+    fit = _joint_fit(w, data, int(ref["iterations"]), int(ref["cg_iterations"]))
+    err_c, err_o = _joint_errors(fit, ref["coefficients"], ref["object"].astype(np.complex128), w)
+    hist = np.array(fit.history)
+    print("config4 joint: GPU vs FP64 oracle, relative per coefficient:", np.array2string(err_c, precision=2),
+          flush=True)
+    print(f"config4 joint: max coefficient error {err_c.max():.2e}; object change rel-L2 {err_o:.2e}; residual GPU "
+          f"{hist[-1, 1]:.6e} oracle {ref['history'][-1, 1]:.6e}", flush=True)
+    assert np.allclose(hist, ref["history"], rtol=1e-5)
+    assert err_c.max() < 1e-5 and err_o < 1e-5

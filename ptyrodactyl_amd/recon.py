@@ -124,19 +124,24 @@ class _CapturedGradient:
     def __init__(self, forward, params, names, data, loss_mode):
         self.names, self.data, self.loss_mode = names, data, loss_mode
         with suspended_checks():
-            leaves = {k: params[k].detach().requires_grad_(True) for k in names}
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
                 for _ in range(2):  # warm-up on a side stream before recording; the parameters are not updated
-                    out = forward(**leaves)
-                    torch.autograd.grad(out, [leaves[k] for k in names], grad_outputs=_loss_seed(out, data, loss_mode),
+                    warm = {k: params[k].detach().requires_grad_(True) for k in names}
+                    out = forward(**warm)
+                    torch.autograd.grad(out, [warm[k] for k in names], grad_outputs=_loss_seed(out, data, loss_mode),
                                         allow_unused=True)
+                    del warm, out
             torch.cuda.current_stream().wait_stream(side)
             self.graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph):
-                self.out = forward(**leaves)
-                self.grads = torch.autograd.grad(self.out, [leaves[k] for k in names],
+                # leaves made on the capture stream: a leaf's gradient accumulator belongs to the stream that
+                # first used it, and warm-up leaves reused here made autograd warn of (and sync across) the
+                # stream mismatch between the side stream and the capture stream
+                self.leaves = {k: params[k].detach().requires_grad_(True) for k in names}
+                self.out = forward(**self.leaves)
+                self.grads = torch.autograd.grad(self.out, [self.leaves[k] for k in names],
                                                  grad_outputs=_loss_seed(self.out, data, loss_mode), allow_unused=True)
 
     def run(self):

@@ -15,12 +15,17 @@ torch = pytest.importorskip("torch")
 
 
 def _run(monkeypatch, capture, fn):
+    import warnings
     if capture:
         monkeypatch.setenv("PTYRODACTYL_CAPTURE_ITERATIONS", "1")
     else:
         monkeypatch.delenv("PTYRODACTYL_CAPTURE_ITERATIONS", raising=False)
-    out = fn()
-    torch.cuda.synchronize()
+    with warnings.catch_warnings(record=True) as seen:
+        warnings.simplefilter("always")
+        out = fn()
+        torch.cuda.synchronize()
+    # recording must not leave autograd syncing across streams (an accumulator made on another stream)
+    assert not [w for w in seen if "AccumulateGrad" in str(w.message)], [str(w.message)[:120] for w in seen]
     return [o.detach().cpu().clone() if isinstance(o, torch.Tensor) else o for o in out]
 
 

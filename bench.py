@@ -203,6 +203,10 @@ def serving_bench(count: int, qps: float):
         print(f"# serving {label} batched {loop}", file=sys.stderr, flush=True)
         out[label] = {"qps": rate, "batched": loop, "plain": plain,
                       "speedup": round(loop["patterns_per_s"] / plain["patterns_per_s"], 3)}
+    ceiling = serving.scheduling_ceiling(serving.make_request_stream(count, 1e4, seed=5))
+    print(f"# serving ceiling {json.dumps({k: v for k, v in ceiling.items() if k != 'requests'})}", file=sys.stderr,
+          flush=True)
+    out["ceiling"] = {k: v for k, v in ceiling.items() if k != "requests"}
     return out
 
 

@@ -498,3 +498,62 @@ class RequestLoop:
             if self.metrics_sink is not None:
                 self.metrics_sink(m)
         return summarize(finished, time.perf_counter() - t0, closed)
+
+
+# ---------------------------------------------------------------------------
+# Scheduling ceiling
+# ---------------------------------------------------------------------------
+
+
+def scheduling_ceiling(requests: List[Request], device=None, floor_positions: int = 4096) -> Dict:
+    """Measured bound on what any scheduler of these requests can gain over :class:`PlainExecutor`.
+
+    Each request is run alone on the plain path (warm, synchronised): its input build and its compute are
+    timed separately.  For every class (kind, n, S) of the stream one request of ``floor_positions``
+    positions is run alone as well; its time per position is the class's saturated cost, the rate at which
+    the kernels run when a launch fills the GPU.  A scheduler that hides every input build behind other
+    work and runs every position at its class's saturated cost needs ``floor_ms`` = sum of positions x
+    class cost; ``max_speedup`` = plain wall time / floor_ms bounds the speed-up of batching, overlap and
+    graph capture together, without faster kernels."""
+    dev = device or torch.device("cuda", 0)
+    plain = PlainExecutor(dev)
+
+    def timed(req, inp=None):
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        inp = inp or build_inputs(req, dev)
+        torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        plain.run_one(req, inp)
+        torch.cuda.synchronize(dev)
+        return inp, (t1 - t0) * 1e3, (time.perf_counter() - t1) * 1e3
+
+    rows = []
+    for r in sorted(requests, key=lambda q: q.arrival):
+        probe = Request(r.rid, r.kind, r.n, r.slices, r.positions, r.seed)
+        inp, build_ms, _ = timed(probe)
+        _, _, compute_ms = timed(probe, inp)          # second run: warm caches and workspaces
+        rows.append({"rid": r.rid, "kind": r.kind, "n": r.n, "slices": r.slices, "positions": r.positions,
+                     "build_ms": round(build_ms, 3), "compute_ms": round(compute_ms, 3)})
+        del inp
+    classes = {}
+    for row in rows:
+        key = (row["kind"], row["n"], row["slices"])
+        if key in classes:
+            continue
+        big = Request(-1, key[0], key[1], key[2], floor_positions, seed=7)
+        inp, _, _ = timed(big)
+        _, _, ms = timed(big, inp)
+        classes[key] = ms / floor_positions
+        del inp
+    for row in rows:
+        row["floor_ms"] = round(row["positions"] * classes[(row["kind"], row["n"], row["slices"])], 4)
+    plain_ms = sum(r["build_ms"] + r["compute_ms"] for r in rows)
+    floor_ms = sum(r["floor_ms"] for r in rows)
+    top = max(rows, key=lambda r: r["compute_ms"])
+    return {"plain_build_ms": round(sum(r["build_ms"] for r in rows), 2),
+            "plain_compute_ms": round(sum(r["compute_ms"] for r in rows), 2),
+            "floor_ms": round(floor_ms, 2), "max_speedup": round(plain_ms / floor_ms, 3),
+            "largest_request": {k: top[k] for k in ("rid", "kind", "n", "slices", "positions", "compute_ms")},
+            "class_us_per_position": {f"{k}_{n}_{s}": round(v * 1e3, 3) for (k, n, s), v in sorted(classes.items())},
+            "requests": rows}

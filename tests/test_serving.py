@@ -54,6 +54,29 @@ def test_results_same_alone_and_in_mix():
                 assert err < 1e-5, f"gradient rel-L2 {err:.2e} for request {r.rid}"
 
 
+def _bench_sized_mix():
+    """Requests of the bench mix's larger kinds: S in {32, 64}, P in {1024, 4096}, every window size."""
+    specs = [("forward", 128, 32, 1024), ("forward", 64, 64, 4096), ("gradient", 256, 32, 4096),
+             ("forward", 256, 32, 4096), ("gradient", 128, 64, 1024), ("forward", 64, 32, 1024),
+             ("gradient", 64, 64, 4096), ("forward", 256, 64, 1024)]
+    return [serving.Request(i, k, n, s, p, seed=40 + i, arrival=0.001 * i) for i, (k, n, s, p) in enumerate(specs)]
+
+
+@pytest.mark.gpu
+def test_results_same_alone_and_in_mix_at_bench_sizes():
+    """The bench loop's own chunking (8192 / 4096 / 512 positions per step at n = 64 / 128 / 256) over
+    S = 32 / 64 and P = 1024 / 4096 requests: every request's patterns are bitwise those of the request run
+    alone; losses to 1e-9 and gradients to rel-L2 1e-5."""
+    loop = serving.RequestLoop(keep_inputs=True)
+    mixed = _bench_sized_mix()
+    stats = loop.run(mixed, realtime=False)
+    assert stats["requests"] == len(mixed)
+    for r in mixed:
+        print(f"request {r.rid} {r.kind} n={r.n} S={r.slices} P={r.positions}", flush=True)
+        _assert_same_as_alone(r)
+        r.inputs = r.patterns = r.grad_slab = r.grad_probe_k = None
+
+
 @pytest.mark.gpu
 def test_admission_respects_hbm_budget():
     reqs = _mix()
@@ -102,8 +125,9 @@ def _assert_same_as_alone(r):
     assert torch.equal(r.patterns, alone.patterns), f"patterns differ for request {r.rid}"
     if r.kind == "gradient":
         assert r.loss == pytest.approx(alone.loss, rel=1e-9)
-        err = float(torch.linalg.norm(r.grad_slab - alone.grad_slab) / torch.linalg.norm(alone.grad_slab))
-        assert err < 1e-5, f"gradient rel-L2 {err:.2e} for request {r.rid}"
+        for got, ref in ((r.grad_slab, alone.grad_slab), (r.grad_probe_k, alone.grad_probe_k)):
+            err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+            assert err < 1e-5, f"gradient rel-L2 {err:.2e} for request {r.rid}"
 
 
 @pytest.mark.gpu

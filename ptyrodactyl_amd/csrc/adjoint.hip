@@ -47,41 +47,52 @@ __device__ __forceinline__ void row_ops(float2 (&v)[16], int t, int y, int s, in
     if (gy >= g.slab_h) gy -= (int)g.slab_h;
     const long long off = (long long)s * g.stride + (long long)gy * g.slab_w;
     const float2* trow = g.slab + off;
-    // groups of GP points: each group issues all of its loads before using any of them, so the loads
-    // overlap instead of waiting one by one under register pressure
-    constexpr int GP = N <= 128 ? 4 : 8;
+    const int sw = (int)g.slab_w;
+    // slab column of point q: a window row that does not wrap (uniform per wave) addresses its points at
+    // constant offsets from one base; the wrapped case computes each index
+    auto points = [&](auto col) {
+      // groups of GP points: each group issues all of its loads before using any of them, so the loads
+      // overlap instead of waiting one by one under register pressure
+      constexpr int GP = N <= 128 ? 4 : 8;
 #pragma unroll
-    for (int grp = 0; grp < 16 / GP; ++grp) {
-      float2 tv[GP], sv[GP];
-#pragma unroll
-      for (int u = 0; u < GP; ++u) {
-        const int q = GP * grp + u;
-        int x = g.ox + t + T * q;
-        if (x >= g.slab_w) x -= (int)g.slab_w;
-        tv[u] = trow[x];
-        if (pw == PW_ADJ) sv[u] = seg_row[t + T * q];
-      }
-      asm volatile("" ::: "memory");
-      if (pw == PW_FWD) {
-#pragma unroll
-        for (int u = 0; u < GP; ++u) v[GP * grp + u] = cmul(v[GP * grp + u], tv[u]);
-      } else {
-        float2* grow = a.grad_slab + g.grad_off + off;
+      for (int grp = 0; grp < 16 / GP; ++grp) {
+        float2 tv[GP], sv[GP];
 #pragma unroll
         for (int u = 0; u < GP; ++u) {
           const int q = GP * grp + u;
-          const float2 gt = cmulc(v[q], sv[u]);  // gphi * conj(psi)
-          if (gt_row) {
-            gt_row[t + T * q] = gt;  // window coordinates, gathered by k_grad_reduce
-          } else {
-            int x = g.ox + t + T * q;
-            if (x >= g.slab_w) x -= (int)g.slab_w;
-            unsafeAtomicAdd(&grow[x].x, gt.x);
-            unsafeAtomicAdd(&grow[x].y, gt.y);
+          tv[u] = trow[col(q)];
+          if (pw == PW_ADJ) sv[u] = seg_row[t + T * q];
+        }
+        asm volatile("" ::: "memory");
+        if (pw == PW_FWD) {
+#pragma unroll
+          for (int u = 0; u < GP; ++u) v[GP * grp + u] = cmul(v[GP * grp + u], tv[u]);
+        } else {
+          float2* grow = a.grad_slab + g.grad_off + off;
+#pragma unroll
+          for (int u = 0; u < GP; ++u) {
+            const int q = GP * grp + u;
+            const float2 gt = cmulc(v[q], sv[u]);  // gphi * conj(psi)
+            if (gt_row) {
+              gt_row[t + T * q] = gt;  // window coordinates, gathered by k_grad_reduce
+            } else {
+              const int x = col(q);
+              unsafeAtomicAdd(&grow[x].x, gt.x);
+              unsafeAtomicAdd(&grow[x].y, gt.y);
+            }
+            v[q] = cmulc(v[q], tv[u]);  // conj(T) * gphi
           }
-          v[q] = cmulc(v[q], tv[u]);  // conj(T) * gphi
         }
       }
+    };
+    const int x0 = g.ox + t;
+    if (g.ox + N <= sw) {
+      points([&](int q) { return x0 + T * q; });
+    } else {
+      points([&](int q) {
+        const int x = x0 + T * q;
+        return x >= sw ? x - sw : x;
+      });
     }
   }
   if constexpr (POST == POST_PROP) {
@@ -167,6 +178,8 @@ struct ColBufRes {
   int rs;
   __device__ __forceinline__ float2 ld(int e) const { return col[pad16(e) * rs]; }
   __device__ __forceinline__ void st(int e, float2 v) const { col[pad16(e) * rs] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return col[pe * rs]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { col[pe * rs] = v; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
@@ -350,14 +363,14 @@ __global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, Wave
   constexpr int T = N / 16, RPB = 256 / T, RS = N + N / 16;
   __shared__ float2 rows[RPB * RS];
   constexpr bool DF = N <= DF_MAX_LINE;
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[twiddle_entries<N>()];
   __shared__ float2 px[N];
-  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
   __shared__ float2 pxf[DF ? N : 1];
   const int tid = threadIdx.x;
-  const int wl = blockIdx.x % nb, rblk = blockIdx.x / nb;
+  const int wl = blockIdx.x, rblk = blockIdx.y;  // grid (nb, N / RPB)
   const long long w = w0 + wl;
-  const int p = (int)(w / a.f.M);
+  const int p = a.f.M == 1 ? (int)w : (int)(w / a.f.M);
   const int t = tid % T, lr = tid / T, y = rblk * RPB + lr;
   const WaveCtx g = resolve_wave(a.f, p);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
@@ -384,6 +397,8 @@ struct ColBufBlk {
   float2* base;
   __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
   __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return base[pe * NC]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { base[pe * NC] = v; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
@@ -392,16 +407,21 @@ __global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, Wave
   constexpr int T = N / 16, NC = 256 / T, PS = N + N / 16;
   __shared__ float2 buf[PS * NC];
   constexpr bool DF = N <= DF_MAX_LINE;
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[twiddle_entries<N>()];
   __shared__ float2 py[N];
-  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
   __shared__ float2 pyf[DF ? N : 1];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
   int wl, cblk;
-  column_block<NC>((int)blockIdx.x, nb, N / NC, wl, cblk);
+  if constexpr (NC > 2) {  // grid (nb, N / NC), as column_block maps it
+    wl = blockIdx.x;
+    cblk = blockIdx.y;
+  } else {
+    column_block<NC>((int)blockIdx.x, nb, N / NC, wl, cblk);
+  }
   const long long w = w0 + wl;
-  const int p = (int)(w / a.f.M);
+  const int p = a.f.M == 1 ? (int)w : (int)(w / a.f.M);
   const int x = cblk * NC + gc;
   const WaveCtx g = resolve_wave(a.f, p);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
@@ -741,7 +761,7 @@ static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_
     a.contrib_slices = k;
     a.gt_contrib = gathered ? base + (size_t)nb * (k + 2) * NN : nullptr;
     a.gp_contrib = gathered && (a0.grad_probe_k || a0.grad_shifts) ? base + (size_t)nb * (2 * k + 2) * NN : nullptr;
-    const dim3 rg(nb * (N / RPB)), cg(nb * (N / NC)), blk(256);
+    const dim3 rg(nb, N / RPB), cg = NC > 2 ? dim3(nb, N / NC) : dim3(nb * (N / NC)), blk(256);
     for (int c = C - 1; c >= 0; --c) {
       const int s0 = c * k, s1 = min(S, s0 + k);
       // psi_s of the segment: slot s - s0 of seg, written by the column pass that produces it; psi_{s0} of a

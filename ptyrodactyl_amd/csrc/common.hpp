@@ -46,10 +46,32 @@ __device__ __forceinline__ void column_block(int b, int nb, int blocks_per_unit,
   }
 }
 
+// v[q] *= row[(ox + t + (N / 16) q) mod w] for the 16 points a thread holds of one window row: a window
+// row that does not wrap (the common case, uniform per workgroup) reads with constant offsets from one
+// base, the wrapped case computes each index.
+template <int N>
+__device__ __forceinline__ void multiply_window_row(float2 (&v)[16], const float2* row, int ox, int t, int w) {
+  constexpr int T = N / 16;
+  if (ox + N <= w) {
+    const float2* r = row + ox + t;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], r[T * q]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      int x = ox + t + T * q;
+      if (x >= w) x -= w;
+      v[q] = cmul(v[q], row[x]);
+    }
+  }
+}
+
 struct RowBufWave {
   float2* row;
   __device__ __forceinline__ float2 ld(int e) const { return row[pad16(e)]; }
   __device__ __forceinline__ void st(int e, float2 v) const { row[pad16(e)] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return row[pe]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { row[pe] = v; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
@@ -59,6 +81,8 @@ struct RowBufBlock {
   float2* row;
   __device__ __forceinline__ float2 ld(int e) const { return row[pad16(e)]; }
   __device__ __forceinline__ void st(int e, float2 v) const { row[pad16(e)] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return row[pe]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { row[pe] = v; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 

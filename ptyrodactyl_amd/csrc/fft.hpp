@@ -62,7 +62,16 @@ PTY_HD float2 cmul_df(float2 z, float crh, float crl, float cih, float cil) {
   return make_float2(re, im);
 }
 
-// Multiply by W16^E = exp(-2 pi i E / 16) (forward) or its conjugate.
+// Lines up to this length (the LDS-resident kernels) carry the modulus-correcting low parts of their
+// twiddle and propagator tables (two more float2 tables in LDS, two more FMAs per multiply).  The
+// streaming kernels (n >= 256) keep plain float32 tables: there the extra LDS lowers occupancy from
+// four to three workgroups per CU and the extra work costs 15 % of the 256^2 x 64-slice forward,
+// while their patterns already stay within 3.2e-6 of the complex128 oracle.
+constexpr int DF_MAX_LINE = 128;
+
+// Multiply by W16^E = exp(-2 pi i E / 16) (forward) or its conjugate.  The constants are double-float
+// at every line length: with float32 constants alone the 256^2 x 64-slice patterns move from 3.2e-6 to
+// 1.3e-5 of the complex128 oracle (measured), since their modulus error, about 5e-8, is systematic.
 template <int E, bool INV>
 PTY_HD float2 tw16(float2 z) {
   constexpr int e = ((E % 16) + 16) % 16;
@@ -230,19 +239,41 @@ struct Plan<4096> {
   static constexpr int R[3] = {16, 16, 16};
 };
 
+// Twiddle table of a line of length L: every pass after the first, with NS points before it and radix R,
+// multiplies point (j, r) (j < NS, r < R) by W_L^(j r L / (NS R)), W_L^m = exp(-2 pi i m / L).  The table
+// holds these factors pass by pass, at twiddle_slot<L, NS>() + r NS + j: the pass after the first 16-point
+// one (NS = 16) from 0, the third pass of lines of 512 and more (NS = 256) from 256.  A thread's factors of
+// one pass then sit at constant offsets from one address.
+template <int L>
+constexpr int twiddle_entries() {
+  return L <= 256 ? L : 256 + L;
+}
+template <int L, int NS>
+constexpr int twiddle_slot() {
+  return NS == 16 ? 0 : 256;
+}
+
 // One Stockham pass.  NS = product of the radices of earlier passes.
-// Buf provides ld(e), st(e, v) and sync() for the group's scratch; TW::apply<INV>(x, m)
-// returns x * W_L^m (x * conj(W_L^m) when INV), W_L^m = exp(-2 pi i m / L).
+// Buf provides ldp(pe), stp(pe, v) and sync() for the group's scratch, where pe = pad16(e) is the padded
+// index of element e (one pad slot after every 16 elements); TW::apply<INV, SLOT, NS>(x, j, r) returns
+// x * the table factor of point (j, r) (x * its conjugate when INV).  Every element a thread touches in
+// one pass sits at a constant padded distance from its first one (the strides below are multiples of 16,
+// or the run stays inside one 16-element block), so each access is one address plus an immediate offset.
 template <int L, int E, int R, int NS, bool INV, bool FIRST, bool LAST, class Buf, class TW>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[E], int t, Buf& buf, const TW& tw) {
   constexpr int T = L / E;
   constexpr int NB = E / R;
+  static_assert(FIRST || (L / R) % 16 == 0, "load stride must be a multiple of 16");
+  static_assert(LAST || NS % 16 == 0 || (NS == 1 && R == 16), "store run must stay padded-contiguous");
+  constexpr int LD_STEP = (L / R) + (L / R) / 16;           // padded distance of e and e + L / R
+  constexpr int ST_STEP = NS % 16 == 0 ? NS + NS / 16 : NS;  // padded distance of idx and idx + NS
   float2 x[NB][R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + T * b;
+    const int pj = j + (j >> 4);
 #pragma unroll
-    for (int r = 0; r < R; ++r) x[b][r] = FIRST ? v[b + NB * r] : buf.ld(j + (L / R) * r);
+    for (int r = 0; r < R; ++r) x[b][r] = FIRST ? v[b + NB * r] : buf.ldp(pj + LD_STEP * r);
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -253,7 +284,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[E], int t, Buf& buf, c
       int jm = j % NS;
       asm volatile("" : "+v"(jm));
 #pragma unroll
-      for (int r = 1; r < R; ++r) x[b][r] = tw.template apply<INV>(x[b][r], jm * r * (L / (NS * R)));
+      for (int r = 1; r < R; ++r) x[b][r] = tw.template apply<INV, twiddle_slot<L, NS>(), NS>(x[b][r], jm, r);
     }
     Dft<R, INV>::run(x[b]);
   }
@@ -268,8 +299,9 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[E], int t, Buf& buf, c
     for (int b = 0; b < NB; ++b) {
       const int j = t + T * b;
       const int idx = (j / NS) * NS * R + (j % NS);
+      const int pidx = idx + (idx >> 4);
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf.st(idx + NS * r, x[b][r]);
+      for (int r = 0; r < R; ++r) buf.stp(pidx + ST_STEP * r, x[b][r]);
     }
     buf.sync();  // writes visible to the group
   }
@@ -293,38 +325,46 @@ __device__ __forceinline__ void line_fft(float2 (&v)[16], int t, Buf& buf, const
 PTY_HD float2 cmul_pair(float2 z, float2 hi, float2 lo) { return cmul_df(z, hi.x, lo.x, hi.y, lo.y); }
 PTY_HD float2 cmulc_pair(float2 z, float2 hi, float2 lo) { return cmul_df(z, hi.x, lo.x, -hi.y, -lo.y); }
 
-// Lines up to this length (the LDS-resident kernels) carry the modulus-correcting low parts of their
-// twiddle and propagator tables (two more float2 tables in LDS, two more FMAs per multiply).  The
-// streaming kernels (n >= 256) keep plain float32 tables: there the extra LDS lowers occupancy from
-// four to three workgroups per CU and the extra work costs 15 % of the 256^2 x 64-slice forward,
-// while their patterns already stay within 3.2e-6 of the complex128 oracle.
-constexpr int DF_MAX_LINE = 128;
-
-// Twiddle multiply from LDS tables of W_L^m (float32) and, when DF, the modulus-correcting low parts.
+// Twiddle multiply from the LDS tables (float32 factors laid out as twiddle_slot describes) and, when
+// DF, their modulus-correcting low parts.
 template <bool DF>
 struct TwTab {
   const float2* tab;
   const float2* lo;
-  template <bool INV>
-  __device__ __forceinline__ float2 apply(float2 x, int m) const {
-    if constexpr (DF) return INV ? cmulc_pair(x, tab[m], lo[m]) : cmul_pair(x, tab[m], lo[m]);
-    else return INV ? cmulc(x, tab[m]) : cmul(x, tab[m]);
+  template <bool INV, int SLOT, int NS>
+  __device__ __forceinline__ float2 apply(float2 x, int j, int r) const {
+    const float2* t = tab + SLOT + j;
+    if constexpr (DF) {
+      const float2* l = lo + SLOT + j;
+      return INV ? cmulc_pair(x, t[r * NS], l[r * NS]) : cmul_pair(x, t[r * NS], l[r * NS]);
+    } else {
+      return INV ? cmulc(x, t[r * NS]) : cmul(x, t[r * NS]);
+    }
   }
 };
 using TwTable = TwTab<true>;
 
-// Fill LDS tables with W_L^m = exp(-2 pi i m / L), m in [0, L), and (DF) the low parts that make
-// each rounded entry unit-modulus.
+// Fill the twiddle tables of a line of length L (twiddle_entries<L>() entries each) and (DF) the low
+// parts that make each rounded entry unit-modulus.
 template <bool DF = true>
 __device__ __forceinline__ void fill_twiddles(float2* tab, float2* lo, int L, int tid, int nthreads) {
-  for (int m = tid; m < L; m += nthreads) {
+  const int entries = L <= 256 ? L : 256 + L;
+  for (int e = tid; e < entries; e += nthreads) {
+    int m;  // exponent of W_L, below L
+    if (L <= 256) {
+      m = (e % 16) * (e / 16);  // two passes: NS = 16, R = L / 16
+    } else if (e < 256) {
+      m = (e % 16) * (e / 16) * (L / 256);  // second of three passes: NS = 16, R = 16
+    } else {
+      m = ((e - 256) % 256) * ((e - 256) / 256);  // third pass: NS = 256
+    }
     float s, c;
     sincospif(-2.0f * (float)m / (float)L, &s, &c);
     const float2 w = make_float2(c, s);
-    tab[m] = w;
+    tab[e] = w;
     if constexpr (DF) {
       const float d = modulus_fix(w, 1.0);
-      lo[m] = make_float2(w.x * d, w.y * d);
+      lo[e] = make_float2(w.x * d, w.y * d);
     }
   }
 }

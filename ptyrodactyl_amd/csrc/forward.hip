@@ -44,6 +44,8 @@ struct ColBufResident {
   int rs;
   __device__ __forceinline__ float2 ld(int e) const { return col[pad16(e) * rs]; }
   __device__ __forceinline__ void st(int e, float2 v) const { col[pad16(e) * rs] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return col[pe * rs]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { col[pe * rs] = v; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 
@@ -105,12 +107,7 @@ __global__ __launch_bounds__(N* N / 16) void k_forward_resident(FwdParams a) {
         int gy = g.oy + ry;
         if (gy >= g.slab_h) gy -= (int)g.slab_h;
         const float2* trow = g.slab + (long long)s * g.stride + (long long)gy * g.slab_w;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          int x = g.ox + rt + T * q;
-          if (x >= g.slab_w) x -= (int)g.slab_w;
-          v[q] = cmul(v[q], trow[x]);
-        }
+        multiply_window_row<N>(v, trow, g.ox, rt, (int)g.slab_w);
         line_fft<N, false>(v, rt, rb, twt);
         if (s < a.S - 1) {
 #pragma unroll
@@ -208,15 +205,16 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   constexpr int RS = N + N / 16;
   __shared__ float2 rows[RPB * RS];
   constexpr bool DF = N <= DF_MAX_LINE;
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[twiddle_entries<N>()];
   __shared__ float2 px[N];
-  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
   __shared__ float2 pxf[DF ? N : 1];
   const int tid = threadIdx.x;
-  const int wl = blockIdx.x % nb;  // consecutive blocks -> different waves (XCD round robin)
-  const int rblk = blockIdx.x / nb;
+  // grid (nb, N / RPB): consecutive blocks -> different waves (XCD round robin)
+  const int wl = blockIdx.x;
+  const int rblk = blockIdx.y;
   const int w = w0 + wl;
-  const int p = w / a.M;
+  const int p = a.M == 1 ? w : w / a.M;
   const int t = tid % T, lr = tid / T;
   const int y = rblk * RPB + lr;
   const WaveCtx g = resolve_wave(a, p);
@@ -241,12 +239,7 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   int gy = g.oy + y;
   if (gy >= g.slab_h) gy -= (int)g.slab_h;
   const float2* trow = g.slab + (long long)s * g.stride + (long long)gy * g.slab_w;
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    int x = g.ox + t + T * q;
-    if (x >= g.slab_w) x -= (int)g.slab_w;
-    v[q] = cmul(v[q], trow[x]);
-  }
+  multiply_window_row<N>(v, trow, g.ox, t, (int)g.slab_w);
   line_fft<N, false>(v, t, rb, twt);
   if (!(FLAGS & ROW_LAST)) {
 #pragma unroll
@@ -263,6 +256,8 @@ struct ColBufBlock {
   float2* base;
   __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
   __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return base[pe * NC]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { base[pe * NC] = v; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
@@ -275,17 +270,22 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
   constexpr int PS = N + N / 16;
   __shared__ float2 buf[PS * NC];
   constexpr bool DF = N <= DF_MAX_LINE;
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[twiddle_entries<N>()];
   __shared__ float2 py[N];
-  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
   __shared__ float2 pyf[DF ? N : 1];
   __shared__ double red[4];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
   int unit, cblk;  // wave (INIT/MID) or position (FINAL) inside the batch, column block
-  column_block<NC>((int)blockIdx.x, nb, N / NC, unit, cblk);
+  if constexpr (NC > 2) {  // grid (nb, N / NC), as column_block maps it
+    unit = blockIdx.x;
+    cblk = blockIdx.y;
+  } else {
+    column_block<NC>((int)blockIdx.x, nb, N / NC, unit, cblk);
+  }
   const int x = cblk * NC + gc;
-  const int pos = MODE >= COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M;
+  const int pos = a.M == 1 ? w0 + unit : (MODE >= COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M);
   const WaveCtx g = resolve_wave(a, pos);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
   if (MODE == COL_MID) stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
@@ -402,16 +402,24 @@ static hipError_t launch_resident(const FwdParams& a, hipStream_t st) {
   return a.det_count > 0 ? launch_resident_variant<N, true>(a, st) : launch_resident_variant<N, false>(a, st);
 }
 
+// Grid of the forward column passes over `units` waves or positions: (units, N / NC) while a workgroup
+// holds more than two columns, else the one-dimensional XCD-aware order of column_block.
+template <int N>
+static dim3 column_grid(int units) {
+  constexpr int NC = 256 / (N / 16);
+  return NC > 2 ? dim3(units, N / NC) : dim3(units * (N / NC));
+}
+
 template <int N>
 static hipError_t launch_streaming_batch(const FwdParams& a, float2* ws, long long p0, int np, hipStream_t st) {
   constexpr int T = N / 16;
-  constexpr int RPB = 256 / T, NC = 256 / T;
+  constexpr int RPB = 256 / T;
   const int nb = np * a.M;
   const int w0 = (int)(p0 * a.M);
-  hipLaunchKernelGGL((k_cols<N, COL_INIT>), dim3(nb * (N / NC)), dim3(256), 0, st, a, ws, w0, nb);
+  hipLaunchKernelGGL((k_cols<N, COL_INIT>), column_grid<N>(nb), dim3(256), 0, st, a, ws, w0, nb);
   for (int s = 0; s < a.S; ++s) {
     const bool first = s == 0, last = s == a.S - 1;
-    const dim3 grid(nb * (N / RPB));
+    const dim3 grid(nb, N / RPB);
     if (first && last)
       hipLaunchKernelGGL((k_rows<N, ROW_FIRST | ROW_LAST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
     else if (first)
@@ -420,14 +428,13 @@ static hipError_t launch_streaming_batch(const FwdParams& a, float2* ws, long lo
       hipLaunchKernelGGL((k_rows<N, ROW_LAST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
     else
       hipLaunchKernelGGL((k_rows<N, 0>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
-    if (!last)
-      hipLaunchKernelGGL((k_cols<N, COL_MID>), dim3(nb * (N / NC)), dim3(256), 0, st, a, ws, w0, nb);
+    if (!last) hipLaunchKernelGGL((k_cols<N, COL_MID>), column_grid<N>(nb), dim3(256), 0, st, a, ws, w0, nb);
   }
   if (a.det_count > 0) {
-    hipLaunchKernelGGL((k_cols<N, COL_FINAL_DET>), dim3(np * (N / NC)), dim3(256), 0, st, a, ws, w0, np);
+    hipLaunchKernelGGL((k_cols<N, COL_FINAL_DET>), column_grid<N>(np), dim3(256), 0, st, a, ws, w0, np);
     hipLaunchKernelGGL(k_det_finalize<N>, dim3((np * a.det_count + 255) / 256), dim3(256), 0, st, a, ws, p0, np);
   } else {
-    hipLaunchKernelGGL((k_cols<N, COL_FINAL>), dim3(np * (N / NC)), dim3(256), 0, st, a, ws, w0, np);
+    hipLaunchKernelGGL((k_cols<N, COL_FINAL>), column_grid<N>(np), dim3(256), 0, st, a, ws, w0, np);
   }
   return hipGetLastError();
 }

@@ -66,6 +66,7 @@ struct WaveCtx {
 };
 
 __device__ __forceinline__ int wrap_index(long long a, long long n) {
+  if (a >= 0 && a < n) return (int)a;
   long long r = a % n;
   return (int)(r < 0 ? r + n : r);
 }

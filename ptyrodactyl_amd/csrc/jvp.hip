@@ -37,9 +37,9 @@ __global__ __launch_bounds__(JvpShape<N>::THREADS) void k_jvp_rows(JvpParams a, 
   constexpr int T = SH::T, RPB = SH::LINES, RS = N + N / 16, NT = SH::THREADS;
   constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 rows[RPB * RS];
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[twiddle_entries<N>()];
   __shared__ float2 px[N];
-  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
   __shared__ float2 pxf[DF ? N : 1];
   const int tid = threadIdx.x;
   const int wl = blockIdx.x % nb, rblk = blockIdx.x / nb;
@@ -72,14 +72,24 @@ __global__ __launch_bounds__(JvpShape<N>::THREADS) void k_jvp_rows(JvpParams a, 
   const long long off = (long long)s * g.stride + (long long)gy * g.slab_w;
   const float2* srow = g.slab + off;
   const float2* dsrow = a.d_slab ? a.d_slab + off : nullptr;
+  const int sw = (int)g.slab_w, x0 = g.ox + t;
+  auto points = [&](auto col) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    int x = g.ox + t + T * q;
-    if (x >= g.slab_w) x -= (int)g.slab_w;
-    const float2 tv = srow[x];
-    dv[q] = cmul(dv[q], tv);
-    if (dsrow) dv[q] = cadd(dv[q], cmul(v[q], dsrow[x]));
-    v[q] = cmul(v[q], tv);
+    for (int q = 0; q < 16; ++q) {
+      const int x = col(q);
+      const float2 tv = srow[x];
+      dv[q] = cmul(dv[q], tv);
+      if (dsrow) dv[q] = cadd(dv[q], cmul(v[q], dsrow[x]));
+      v[q] = cmul(v[q], tv);
+    }
+  };
+  if (g.ox + N <= sw) {  // the window row does not wrap: constant offsets from one base
+    points([&](int q) { return x0 + T * q; });
+  } else {
+    points([&](int q) {
+      const int x = x0 + T * q;
+      return x >= sw ? x - sw : x;
+    });
   }
   line_fft<N, false>(v, t, rb, twt);
   line_fft<N, false>(dv, t, rb, twt);
@@ -104,6 +114,8 @@ struct JvpColBuf {
   float2* base;
   __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
   __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return base[pe * NC]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { base[pe * NC] = v; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
@@ -116,9 +128,9 @@ __global__ __launch_bounds__(JvpShape<N>::THREADS) void k_jvp_cols(JvpParams a, 
   constexpr int T = SH::T, NC = SH::LINES, PS = N + N / 16, NT = SH::THREADS;
   constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 buf[PS * NC];
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[twiddle_entries<N>()];
   __shared__ float2 py[N];
-  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
   __shared__ float2 pyf[DF ? N : 1];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;

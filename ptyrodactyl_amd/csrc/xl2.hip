@@ -56,6 +56,8 @@ struct ColBufX {
   float2* base;
   __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
   __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
+  __device__ __forceinline__ float2 ldp(int pe) const { return base[pe * NC]; }
+  __device__ __forceinline__ void stp(int pe, float2 v) const { base[pe * NC] = v; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
@@ -70,11 +72,11 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
   constexpr int RS = N + N / 16;
   constexpr int BLK = N / RPB;          // tasks per wave and phase
   __shared__ float2 scratch[(RPB * RS > (N + N / 16) * NC) ? RPB * RS : (N + N / 16) * NC];
-  __shared__ float2 tw[N];
+  __shared__ float2 tw[twiddle_entries<N>()];
   __shared__ float2 pvx[N];
   __shared__ float2 pvy[N];
   constexpr bool DF = N <= DF_MAX_LINE;  // the same tables as the per-pass kernels of this size
-  __shared__ float2 twf[DF ? N : 1];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
   __shared__ float2 pvxf[DF ? N : 1];
   __shared__ float2 pvyf[DF ? N : 1];
   __shared__ long long s_ticket;
@@ -165,12 +167,7 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
         int gy = c.oy + y;
         if (gy >= c.slab_h) gy -= (int)c.slab_h;
         const float2* trow = c.slab + (long long)s * c.stride + (long long)gy * c.slab_w;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          int xx = c.ox + t16 + T * q;
-          if (xx >= c.slab_w) xx -= (int)c.slab_w;
-          v[q] = cmul(v[q], trow[xx]);
-        }
+        multiply_window_row<N>(v, trow, c.ox, t16, (int)c.slab_w);
         line_fft<N, false>(v, t16, rb, twt);
         if (!last) {
 #pragma unroll

@@ -112,7 +112,7 @@ def _as_tensor(x, dtype, device=None) -> torch.Tensor:
 
 
 def _scalar(value, name: str) -> float:
-    t = torch.as_tensor(value, dtype=torch.float64)
+    t = torch.as_tensor(value.detach() if isinstance(value, torch.Tensor) else value, dtype=torch.float64)
     if t.ndim != 0:
         raise ValueError(f"{name} must be a scalar")
     return float(t)

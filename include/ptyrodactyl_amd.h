@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 6
+#define PTY_ABI_VERSION 7
 
 #define PTY_OK 0
 #define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
@@ -326,6 +326,20 @@ int pty_atom_potential_grid(int parameterization, int num_types, const double* r
                             int width, int supersampling, const double* centers, double* out, void* stream);
 
 /*
+ * Centre gradient of pty_atom_potential_grid: grad_centers[t] = (dL/dx_t, dL/dy_t) for L = sum_ij grad_out[t][i][j]
+ * out[t][i][j] (overwritten, not accumulated), from dV/dr by the Bessel identities K0' = -K1 and (x K1)' = -x K0 on
+ * the same A&S evaluations.  The reference takes this gradient with jax.grad of single_atom_potential in
+ * center_coords (src/ptyrodactyl/multislice/atom_potentials.py:588-681, tested in
+ * tests/test_ptyrodactyl/test_multislice/test_atom_potentials.py:91-151).  The sum has a fixed order (per-block
+ * partial sums in `workspace`, at least pty_atom_center_grad_workspace_bytes(num_types) bytes, added in order).
+ */
+size_t pty_atom_center_grad_workspace_bytes(int num_types);
+int pty_atom_potential_grid_center_grad(int parameterization, int num_types, const double* rows, double pixel_size,
+                                        int height, int width, int supersampling, const double* centers,
+                                        const double* grad_out, double* grad_centers, void* workspace,
+                                        size_t workspace_bytes, void* stream);
+
+/*
  * Fourier-space potential slices: for every slice s,
  *   out_k[s] = sum_t kernel_k[t] * sum_{a in [segments[s*T+t], segments[s*T+t+1])} exp(2 pi i (kx sx_a + ky sy_a))
  * with (sx_a, sy_a) = shifts[a] in pixels and kx, ky numpy's fftfreq(width), fftfreq(height).  Atoms are sorted by
@@ -337,6 +351,21 @@ int pty_atom_potential_grid(int parameterization, int num_types, const double* r
 int pty_potential_slices_fourier(int num_slices, int num_types, int height, int width, long long num_atoms,
                                  const double* shifts, const long long* segments, const void* kernel_k, void* out_k,
                                  void* stream);
+
+/*
+ * Shift gradient of pty_potential_slices_fourier.  adj_k [S][H][W] complex128 is ifft2 (numpy's normalisation) of
+ * dL/dV_s, the loss gradient w.r.t. the real slice Re ifft2(out_k[s]) (zero where the slice is cropped away).  For
+ * every atom a of the given slices (element t, slice s):
+ *   grad_shifts[a] = Re sum_k 2 pi i kernel_k[t](k) adj_k[s](k) (kx, ky) exp(2 pi i (kx sx_a + ky sy_a)),
+ * i.e. (dL/dsx_a, dL/dsy_a) in pixels; entries of atoms outside the slices are left untouched.  The reference gets
+ * this gradient from jax.grad through _process_all_slices (src/ptyrodactyl/multislice/atom_potentials.py:826-993).
+ * Per-tile partial sums go to `workspace` (pty_potential_shift_grad_workspace_bytes) and are added in tile order.
+ */
+size_t pty_potential_shift_grad_workspace_bytes(int height, int width, long long num_atoms);
+int pty_potential_slices_shift_grad(int num_slices, int num_types, int height, int width, long long num_atoms,
+                                    const double* shifts, const long long* segments, const void* kernel_k,
+                                    const void* adj_k, double* grad_shifts, void* workspace, size_t workspace_bytes,
+                                    void* stream);
 
 /* Device bytes of the state block and workspace of pty_forward_persistent. */
 size_t pty_persistent_state_bytes(void);

@@ -15,7 +15,9 @@ A float64 numpy restatement (paths relative to the reference root) of:
 * ``multislice/form_factors.py:204-584``: Lobato and Kirkland form factors,
   projected potentials and the band-limited Lobato peak;
 * ``multislice/atom_potentials.py:537-683``: ``single_atom_potential``
-  (supersampled grid, average down-sampling);
+  (supersampled grid, average down-sampling) and its gradient in the centre
+  (``projected_dr``, ``single_atom_center_grad``), which the reference takes
+  with jax.grad;
 * ``multislice/atom_potentials.py:684-1640``: ``crystal_potential_slices``
   (lattice repeats, z slicing, grid from the coordinate range plus padding,
   per-atom Fourier shift of the element's centred potential accumulated in
@@ -28,7 +30,10 @@ f_H(0) = a_0 and the Lobato Bethe normalisation; the A&S approximations
 against scipy's K0/K1 within the published error bounds; both projected
 potentials against the Hankel transform of their form factors; Lobato and
 Kirkland agreement; the slice builder against integer-pixel rolls of the
-centred potential and against the structure-factor sum.
+centred potential and against the structure-factor sum; ``projected_dr``
+and ``single_atom_center_grad`` against central differences of ``projected``
+and ``single_atom_potential`` (the reference's own check of this gradient,
+``test_atom_potentials.py:91-151``, on its Si case).
 """
 
 from __future__ import annotations
@@ -223,6 +228,47 @@ def single_atom_potential(kind, row, pixel_size, grid_shape, center=(0.0, 0.0), 
     return v.reshape(h, ss, w, ss).mean(axis=(1, 3))
 
 
+def projected_dr(kind, row, r):
+    """d/dr of ``projected`` by the Bessel identities K0'(x) = -K1(x) and (x K1(x))' = -x K0(x), applied to the A&S
+    evaluations (the reference differentiates those evaluations with jax.grad; the two derivatives differ by the
+    approximations' own error).  Zero below the 1e-10 radius floor."""
+    r = np.asarray(r, np.float64)
+    rr = np.maximum(r, 1e-10)[..., None]
+    k0 = lambda t: bessel_kv(0.0, t)  # noqa: E731
+    k1 = lambda t: bessel_kv(1.0, t)  # noqa: E731
+    if kind == "lobato":
+        a, b = lobato_split(row)
+        sb = np.sqrt(b)
+        arg = 2.0 * np.pi * rr / sb
+        terms = -(a / b) * (2.0 * np.pi / sb) * k1(arg) - a * np.pi / b ** 1.5 * arg * k0(arg)
+        d = 2.0 * np.pi * MOTT_BETHE * np.sum(terms, axis=-1)
+    else:
+        la, lb, ga, gb = kirkland_split(row)
+        sb = np.sqrt(lb)
+        lt = -la * 2.0 * np.pi * sb * k1(2.0 * np.pi * rr * sb)
+        gt = -ga / gb * np.exp(-(np.pi ** 2) * rr * rr / gb) * (2.0 * np.pi ** 2 * rr / gb)
+        d = (4.0 * np.pi ** 2 * KIRKLAND_A0 * KIRKLAND_EK * np.sum(lt, axis=-1)
+             + 2.0 * np.pi ** 2 * KIRKLAND_A0 * KIRKLAND_EK * np.sum(gt, axis=-1))
+    return np.where(r < 1e-10, 0.0, d)
+
+
+def single_atom_center_grad(kind, row, pixel_size, grid_shape, center, supersampling, grad_out):
+    """(dL/dx, dL/dy) of L = sum grad_out * single_atom_potential(..., center): the chain rule through
+    r = sqrt((x - cx)^2 + (y - cy)^2 + 1e-10) and the block average of ``single_atom_potential``."""
+    ss = int(supersampling)
+    h, w = grid_shape
+    step = pixel_size / ss
+    gh, gw = h * ss, w * ss
+    y = (np.arange(gh) - gh // 2) * step
+    x = (np.arange(gw) - gw // 2) * step
+    ya, xa = np.meshgrid(y, x, indexing="ij")
+    dx, dy = xa - center[0], ya - center[1]
+    r = np.sqrt(dx * dx + dy * dy + 1e-10)
+    dv = projected_dr(kind, row, r) / r
+    g = np.repeat(np.repeat(np.asarray(grad_out, np.float64), ss, axis=0), ss, axis=1) / (ss * ss)
+    return np.array([np.sum(g * dv * -dx), np.sum(g * dv * -dy)])
+
+
 def tile_repeats(positions, numbers, lattice, repeats):
     """``atom_potentials.py:1126-1241``: every lattice shift (ix, iy, iz) in C order, each copying all atoms."""
     positions = np.asarray(positions, np.float64)
@@ -255,7 +301,8 @@ def grid_dimensions(x, y, padding, pixel_size, grid_hw=None):
 
 
 def crystal_potential_slices(kind, rows, positions, numbers, lattice, pixel_size, slice_thickness=1.0,
-                             repeats=(1, 1, 1), padding=4.0, supersampling=4, grid_shape=None):
+                             repeats=(1, 1, 1), padding=4.0, supersampling=4, grid_shape=None,
+                             accumulate=np.float32):
     """``atom_potentials.py:1364-1633``: (H, W, S) float64 slices (float32-accumulated values).
 
     ``rows`` maps atomic number -> coefficient row.  Each atom adds
@@ -271,14 +318,62 @@ def crystal_potential_slices(kind, rows, positions, numbers, lattice, pixel_size
     kern_k = {z: np.fft.fft2(v) for z, v in kernels.items()}
     ky = np.fft.fftfreq(h, d=1.0).reshape(-1, 1)
     kx = np.fft.fftfreq(w, d=1.0).reshape(1, -1)
-    out = np.zeros((h, w, n_slices), np.float32)
+    out = np.zeros((h, w, n_slices), accumulate)
     for s in range(n_slices):
-        acc = np.zeros((h, w), np.float32)
+        acc = np.zeros((h, w), accumulate)
         for a in np.nonzero(sidx == s)[0]:
             sx = (x[a] - x_min) / pixel_size - w / 2.0
             sy = (y[a] - y_min) / pixel_size - h / 2.0
             phase = np.exp(2j * np.pi * (kx * sx + ky * sy))
-            acc = (acc + np.real(np.fft.ifft2(kern_k[int(num[a])] * phase)).astype(np.float32)).astype(np.float32)
+            acc = (acc + np.real(np.fft.ifft2(kern_k[int(num[a])] * phase)).astype(accumulate)).astype(accumulate)
         out[:, :, s] = acc
     crop = round(float(padding) / float(pixel_size))
     return out[crop:h - crop, crop:w - crop, :].astype(np.float64), (x_min, y_min, w, h)
+
+
+def crystal_potential_slices_position_grad(kind, rows, positions, numbers, lattice, pixel_size, grad_slices,
+                                           slice_thickness=1.0, repeats=(1, 1, 1), padding=4.0, supersampling=4,
+                                           grid_shape=None):
+    """(N, 3) dL/dpositions of L = sum grad_slices * crystal_potential_slices(...), the gradient jax.grad takes
+    through ``atom_potentials.py:1364-1633``.  Each atom enters through its Fourier shift
+    (sx, sy) = ((x - x_min) / pixel, (y - y_min) / pixel) - (W/2, H/2):
+        dL/dsx_a = Re sum_k 2 pi i kx Khat(k) A_s(k) exp(2 pi i (kx sx_a + ky sy_a)),  A_s = ifft2(padded dL/dV_s),
+    and x_min = min(x) - padding passes -sum_a dL/dsx_a / pixel to the atoms at the minimum, split evenly between
+    ties (jax's min rule); lattice copies add into their source atom; z enters only through the slice index
+    (piecewise constant), so its column is zero."""
+    positions = np.asarray(positions, np.float64)
+    n_in = positions.shape[0]
+    pos, num = tile_repeats(positions, numbers, lattice, repeats)
+    z = pos[:, 2]
+    idx = np.floor((z - z.min()) / float(slice_thickness)).astype(np.int64)
+    order = np.argsort(idx, kind="stable")
+    x, y, sidx, num = pos[order, 0], pos[order, 1], idx[order], np.asarray(num)[order]
+    gh = None if grid_shape is None else grid_shape[:2]
+    x_min, y_min, w, h = grid_dimensions(x, y, padding, pixel_size, gh)
+    n_slices = int(grid_shape[2]) if grid_shape is not None else int(sidx.max()) + 1
+    kern_k = {int(zz): np.fft.fft2(single_atom_potential(kind, rows[int(zz)], pixel_size, (h, w), (0.0, 0.0),
+                                                         supersampling)) for zz in np.unique(num)}
+    ky = np.fft.fftfreq(h, d=1.0).reshape(-1, 1)
+    kx = np.fft.fftfreq(w, d=1.0).reshape(1, -1)
+    crop = round(float(padding) / float(pixel_size))
+    gsx = np.zeros(x.shape[0])
+    gsy = np.zeros(x.shape[0])
+    for s in range(n_slices):
+        pad = np.zeros((h, w))
+        pad[crop:h - crop, crop:w - crop] = np.asarray(grad_slices, np.float64)[:, :, s]
+        adj = np.fft.ifft2(pad)
+        for a in np.nonzero(sidx == s)[0]:
+            sx = (x[a] - x_min) / pixel_size - w / 2.0
+            sy = (y[a] - y_min) / pixel_size - h / 2.0
+            term = 2j * np.pi * kern_k[int(num[a])] * adj * np.exp(2j * np.pi * (kx * sx + ky * sy))
+            gsx[a] = np.real(np.sum(term * kx))
+            gsy[a] = np.real(np.sum(term * ky))
+    gx, gy = gsx / pixel_size, gsy / pixel_size
+    at_min_x = x == x.min()
+    at_min_y = y == y.min()
+    gx = gx - at_min_x * gx.sum() / at_min_x.sum()
+    gy = gy - at_min_y * gy.sum() / at_min_y.sum()
+    tiled = np.zeros((x.shape[0], 3))
+    tiled[order, 0] = gx
+    tiled[order, 1] = gy
+    return tiled.reshape(-1, n_in, 3).sum(axis=0)

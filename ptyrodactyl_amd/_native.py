@@ -19,7 +19,7 @@ from .build import LIB_PATH
 PTY_OK = 0
 PTY_ERR_INVALID = 1
 PTY_ERR_DEVICE = 2
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 _lock = threading.Lock()
 _lib = None
@@ -64,6 +64,12 @@ SIGNATURES = {
     "pty_projected_potential": (c_int, [c_int, c_vp, c_ll, c_vp, c_vp, c_vp]),
     "pty_atom_potential_grid": (c_int, [c_int, c_int, c_vp, c_dbl, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "pty_potential_slices_fourier": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pty_atom_center_grad_workspace_bytes": (c_size, [c_int]),
+    "pty_potential_shift_grad_workspace_bytes": (c_size, [c_int, c_int, c_ll]),
+    "pty_potential_slices_shift_grad": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                                c_size, c_vp]),
+    "pty_atom_potential_grid_center_grad": (c_int, [c_int, c_int, c_vp, c_dbl, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                                    c_vp, c_size, c_vp]),
     "pty_optimizer_update": (c_int, [c_int, c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_ll, c_dbl, c_vp]),
     "pty_persistent_state_bytes": (c_size, []),
     "pty_persistent_workspace_bytes": (c_size, [c_int, c_int]),

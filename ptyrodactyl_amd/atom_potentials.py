@@ -137,16 +137,63 @@ def _atom_grids(rows: np.ndarray, parameterization: str, pixel_size: float, heig
     return out
 
 
+def _atom_center_grad(rows_t: torch.Tensor, parameterization: str, pixel_size: float, height: int, width: int,
+                      supersampling: int, centers: torch.Tensor, grad_out: torch.Tensor) -> torch.Tensor:
+    """(T, 2) float64 d(sum grad_out * grids)/d(centers) on the GPU (``pty_atom_potential_grid_center_grad``)."""
+    dev = grad_out.device
+    lib = _native.load()
+    t = rows_t.shape[0]
+    ws = torch.empty(max(1, int(lib.pty_atom_center_grad_workspace_bytes(t))), dtype=torch.uint8, device=dev)
+    out = torch.empty((t, 2), dtype=torch.float64, device=dev)
+    g = grad_out.to(torch.float64).contiguous()
+    with torch.cuda.device(dev):
+        status = lib.pty_atom_potential_grid_center_grad(_CODES[parameterization], t, _native.ptr(rows_t),
+                                                         float(pixel_size), int(height), int(width),
+                                                         int(supersampling), _native.ptr(centers), _native.ptr(g),
+                                                         _native.ptr(out), _native.ptr(ws), ws.numel(),
+                                                         _native.stream_handle(dev))
+    _native.check(status, "pty_atom_potential_grid_center_grad")
+    return out
+
+
+class _AtomGridFunction(torch.autograd.Function):
+    """Element potentials on the grid as a function of their centres: forward ``pty_atom_potential_grid``,
+    backward ``pty_atom_potential_grid_center_grad`` (the reference differentiates ``single_atom_potential`` in
+    ``center_coords`` with jax.grad, ``atom_potentials.py:588-681``)."""
+
+    @staticmethod
+    def forward(centers, rows, parameterization, pixel_size, height, width, supersampling):
+        return _atom_grids(rows, parameterization, pixel_size, height, width, supersampling, centers, centers.device)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        centers, rows, parameterization, pixel_size, height, width, supersampling = inputs
+        ctx.save_for_backward(centers.detach())
+        ctx.args = (rows, parameterization, pixel_size, height, width, supersampling)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (centers,) = ctx.saved_tensors
+        rows, parameterization, pixel_size, height, width, supersampling = ctx.args
+        rows_t = torch.as_tensor(rows, dtype=torch.float64).reshape(-1, 12).to(grad_out.device).contiguous()
+        g = _atom_center_grad(rows_t, parameterization, pixel_size, height, width, supersampling,
+                              centers.to(grad_out.device, torch.float64).contiguous(), grad_out)
+        return g.to(centers.dtype), None, None, None, None, None, None
+
+
 def single_atom_potential(atom_no, pixel_size, grid_shape: Tuple[int, int], center_coords=None,
                           supersampling: int = 4, *, parameterization: str = "lobato") -> torch.Tensor:
     """(H, W) float64 projected potential (V A) of one atom: the supersampled grid (step pixel_size /
     supersampling, origin at sample (H ss/2, W ss/2)) averaged over supersampling^2 blocks; centred at
-    ``center_coords`` = (x, y) A, default the origin."""
+    ``center_coords`` = (x, y) A, default the origin.  Differentiable in ``center_coords``."""
     _check_parameterization(parameterization)
     h, w = (int(v) for v in grid_shape)
     dev = _gpu(center_coords.device if isinstance(center_coords, torch.Tensor) else None)
-    ctr = None if center_coords is None else _as_tensor(center_coords, torch.float64).reshape(1, 2)
     row = coefficient_row(atom_no, parameterization)
+    if isinstance(center_coords, torch.Tensor) and center_coords.requires_grad:
+        ctr = center_coords.to(device=dev, dtype=torch.float64).reshape(1, 2)
+        return _AtomGridFunction.apply(ctr, row, parameterization, float(pixel_size), h, w, int(supersampling))[0]
+    ctr = None if center_coords is None else _as_tensor(center_coords, torch.float64).reshape(1, 2)
     return _atom_grids(row, parameterization, float(pixel_size), h, w, int(supersampling), ctr, dev)[0]
 
 
@@ -182,6 +229,7 @@ def _slice_atoms(coords: torch.Tensor, atom_numbers: torch.Tensor, slice_thickne
 def _grid_dimensions(x: torch.Tensor, y: torch.Tensor, padding: float, pixel_size: float,
                      grid_hw: Optional[Tuple[int, int]]):
     """(x_min, y_min, width, height) (``atom_potentials.py:748-823``)."""
+    x, y = x.detach(), y.detach()
     x_min = float(x.min()) - padding
     y_min = float(y.min()) - padding
     if grid_hw is not None:
@@ -225,6 +273,12 @@ def plan_slices(crystal_data: CrystalData, pixel_size, slice_thickness=1.0, repe
     anum = sliced[:, 3].to(torch.int64)
     x_min, y_min, width, height = _grid_dimensions(x, y, pad, pixel, None if grid_shape is None
                                                    else (grid_shape[0], grid_shape[1]))
+    if pos.requires_grad:
+        # the grid origin follows the lowest atom, as the reference's x_min = min(x) - padding does under
+        # jax.grad (amin splits the gradient evenly between tied minima, as jax's min does)
+        x_min_t, y_min_t = torch.amin(x) - pad, torch.amin(y) - pad
+    else:
+        x_min_t, y_min_t = x_min, y_min
     n_slices = int(grid_shape[2]) if grid_shape is not None else int(sidx.max()) + 1
     elements = torch.unique(anum)
     n_types = elements.numel()
@@ -232,8 +286,8 @@ def plan_slices(crystal_data: CrystalData, pixel_size, slice_thickness=1.0, repe
     keep = sidx < n_slices
     key = (sidx * n_types + tidx)[keep]
     order = torch.argsort(key, stable=True)
-    sx = ((x - x_min) / pixel - width / 2.0)[keep][order]
-    sy = ((y - y_min) / pixel - height / 2.0)[keep][order]
+    sx = ((x - x_min_t) / pixel - width / 2.0)[keep][order]
+    sy = ((y - y_min_t) / pixel - height / 2.0)[keep][order]
     counts = torch.bincount(key, minlength=n_slices * n_types)
     segments = torch.zeros(n_slices * n_types + 1, dtype=torch.int64, device=dev)
     segments[1:] = torch.cumsum(counts, 0)
@@ -242,6 +296,76 @@ def plan_slices(crystal_data: CrystalData, pixel_size, slice_thickness=1.0, repe
         raise ValueError("padding removes the whole grid")
     return SlicePlan(x_min, y_min, width, height, n_slices, crop, elements,
                      torch.stack([sx, sy], dim=1).contiguous(), segments)
+
+
+def _slices_from_plan(plan: SlicePlan, shifts: torch.Tensor, kern_k: torch.Tensor, n_types: int) -> torch.Tensor:
+    """(H', W', S) float64 slices of ``plan`` with the given atom shifts (pixels): ``pty_potential_slices_fourier``
+    per chunk of slices, then the inverse FFT and the crop."""
+    h, w, n_slices, crop = plan.height, plan.width, plan.num_slices, plan.crop
+    dev = kern_k.device
+    out_h, out_w = h - 2 * crop, w - 2 * crop
+    result = torch.empty((out_h, out_w, n_slices), dtype=torch.float64, device=dev)
+    chunk = max(1, min(n_slices, _FOURIER_BUDGET_BYTES // (16 * h * w)))
+    lib = _native.load()
+    for s0 in range(0, n_slices, chunk):
+        ns = min(chunk, n_slices - s0)
+        seg = plan.segments[s0 * n_types:(s0 + ns) * n_types + 1].contiguous()
+        fk = torch.empty((ns, h, w), dtype=torch.complex128, device=dev)
+        with torch.cuda.device(dev):
+            status = lib.pty_potential_slices_fourier(ns, n_types, h, w, shifts.shape[0], _native.ptr(shifts),
+                                                      _native.ptr(seg), _native.ptr(kern_k), _native.ptr(fk),
+                                                      _native.stream_handle(dev))
+        _native.check(status, "pty_potential_slices_fourier")
+        real = torch.fft.ifft2(fk).real
+        result[:, :, s0:s0 + ns] = real[:, crop:crop + out_h, crop:crop + out_w].permute(1, 2, 0)
+    return result
+
+
+class _SlicesFunction(torch.autograd.Function):
+    """Crystal slices as a function of the atoms' pixel shifts.  Backward: A_s = ifft2 of the padded slice
+    gradient, then ``pty_potential_slices_shift_grad`` (dL/dsx, dL/dsy per atom) — the gradient the reference
+    gets from jax.grad through its per-atom Fourier shifts (``atom_potentials.py:826-993``)."""
+
+    @staticmethod
+    def forward(shifts, plan, kern_k, n_types):
+        return _slices_from_plan(plan, shifts.detach().contiguous(), kern_k, n_types)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        shifts, plan, kern_k, n_types = inputs
+        ctx.save_for_backward(shifts.detach())
+        ctx.plan, ctx.kern_k, ctx.n_types = plan, kern_k, n_types
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (shifts,) = ctx.saved_tensors
+        plan, kern_k, n_types = ctx.plan, ctx.kern_k, ctx.n_types
+        h, w, n_slices, crop = plan.height, plan.width, plan.num_slices, plan.crop
+        dev = kern_k.device
+        shifts = shifts.contiguous()
+        n_atoms = shifts.shape[0]
+        grad = torch.zeros((n_atoms, 2), dtype=torch.float64, device=dev)
+        if n_atoms == 0:
+            return grad, None, None, None
+        lib = _native.load()
+        ws = torch.empty(max(1, int(lib.pty_potential_shift_grad_workspace_bytes(h, w, n_atoms))), dtype=torch.uint8,
+                         device=dev)
+        g = grad_out.to(device=dev, dtype=torch.float64)
+        out_h, out_w = h - 2 * crop, w - 2 * crop
+        chunk = max(1, min(n_slices, _FOURIER_BUDGET_BYTES // (16 * h * w)))
+        for s0 in range(0, n_slices, chunk):
+            ns = min(chunk, n_slices - s0)
+            pad = torch.zeros((ns, h, w), dtype=torch.float64, device=dev)
+            pad[:, crop:crop + out_h, crop:crop + out_w] = g[:, :, s0:s0 + ns].permute(2, 0, 1)
+            adj = torch.fft.ifft2(pad).contiguous()
+            seg = plan.segments[s0 * n_types:(s0 + ns) * n_types + 1].contiguous()
+            with torch.cuda.device(dev):
+                status = lib.pty_potential_slices_shift_grad(ns, n_types, h, w, n_atoms, _native.ptr(shifts),
+                                                             _native.ptr(seg), _native.ptr(kern_k), _native.ptr(adj),
+                                                             _native.ptr(grad), _native.ptr(ws), ws.numel(),
+                                                             _native.stream_handle(dev))
+            _native.check(status, "pty_potential_slices_shift_grad")
+        return grad.to(shifts.dtype), None, None, None
 
 
 def crystal_potential_slices(crystal_data: CrystalData, pixel_size, slice_thickness=1.0, repeats=default_repeats,
@@ -259,22 +383,10 @@ def crystal_potential_slices(crystal_data: CrystalData, pixel_size, slice_thickn
     n_types = rows.shape[0]
     kern_k = torch.fft.fft2(_atom_grids(rows, parameterization, pixel, h, w, int(supersampling), None, dev))
     kern_k = kern_k.contiguous()
-    out_h, out_w, crop = h - 2 * plan.crop, w - 2 * plan.crop, plan.crop
-    result = torch.empty((out_h, out_w, n_slices), dtype=torch.float64, device=dev)
-    chunk = max(1, min(n_slices, _FOURIER_BUDGET_BYTES // (16 * h * w)))
-    lib = _native.load()
-    for s0 in range(0, n_slices, chunk):
-        ns = min(chunk, n_slices - s0)
-        seg = plan.segments[s0 * n_types:(s0 + ns) * n_types + 1].contiguous()
-        fk = torch.empty((ns, h, w), dtype=torch.complex128, device=dev)
-        with torch.cuda.device(dev):
-            status = lib.pty_potential_slices_fourier(ns, n_types, h, w, plan.shifts.shape[0],
-                                                      _native.ptr(plan.shifts), _native.ptr(seg),
-                                                      _native.ptr(kern_k), _native.ptr(fk),
-                                                      _native.stream_handle(dev))
-        _native.check(status, "pty_potential_slices_fourier")
-        real = torch.fft.ifft2(fk).real
-        result[:, :, s0:s0 + ns] = real[:, crop:crop + out_h, crop:crop + out_w].permute(1, 2, 0)
+    if plan.shifts.requires_grad:  # differentiable in the atom positions
+        result = _SlicesFunction.apply(plan.shifts, plan, kern_k, n_types)
+    else:
+        result = _slices_from_plan(plan, plan.shifts, kern_k, n_types)
     return create_potential_slices(result, float(slice_thickness), pixel)
 
 

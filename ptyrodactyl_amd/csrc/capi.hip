@@ -508,6 +508,30 @@ int pty_atom_potential_grid(int parameterization, int num_types, const double* r
                        "pty_atom_potential_grid");
 }
 
+size_t pty_atom_center_grad_workspace_bytes(int num_types) { return pty::atom_center_grad_workspace_bytes(num_types); }
+
+int pty_atom_potential_grid_center_grad(int parameterization, int num_types, const double* rows, double pixel_size,
+                                        int height, int width, int supersampling, const double* centers,
+                                        const double* grad_out, double* grad_centers, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  if (parameterization != PTY_PARAM_LOBATO && parameterization != PTY_PARAM_KIRKLAND)
+    return fail(PTY_ERR_INVALID, "parameterization must be PTY_PARAM_LOBATO or PTY_PARAM_KIRKLAND");
+  if (num_types < 0 || num_types > 65535) return fail(PTY_ERR_INVALID, "num_types must be in [0, 65535]");
+  if (height < 1 || width < 1 || height > 32768 || width > 32768)
+    return fail(PTY_ERR_INVALID, "height and width must be in [1, 32768]");
+  if (supersampling < 1 || supersampling > 64) return fail(PTY_ERR_INVALID, "supersampling must be in [1, 64]");
+  if (!(pixel_size > 0.0) || !isfinite(pixel_size)) return fail(PTY_ERR_INVALID, "pixel_size must be positive");
+  if (num_types > 0 && (!rows || !grad_out || !grad_centers || !workspace))
+    return fail(PTY_ERR_INVALID, "null pointer");
+  if (workspace_bytes < pty::atom_center_grad_workspace_bytes(num_types))
+    return fail(PTY_ERR_INVALID, "workspace smaller than pty_atom_center_grad_workspace_bytes(num_types)");
+  return device_status(pty::launch_atom_center_grad(parameterization, num_types, rows, pixel_size, height, width,
+                                                    supersampling, centers, grad_out, grad_centers, workspace,
+                                                    (hipStream_t)stream),
+                       "pty_atom_potential_grid_center_grad");
+}
+
 int pty_potential_slices_fourier(int num_slices, int num_types, int height, int width, long long num_atoms,
                                  const double* shifts, const long long* segments, const void* kernel_k, void* out_k,
                                  void* stream) {
@@ -523,6 +547,32 @@ int pty_potential_slices_fourier(int num_slices, int num_types, int height, int 
                                                    segments, (const double2*)kernel_k, (double2*)out_k,
                                                    (hipStream_t)stream),
                        "pty_potential_slices_fourier");
+}
+
+size_t pty_potential_shift_grad_workspace_bytes(int height, int width, long long num_atoms) {
+  if (height < 1 || width < 1 || num_atoms < 0) return 0;
+  return pty::slice_shift_grad_workspace_bytes(height, width, num_atoms);
+}
+
+int pty_potential_slices_shift_grad(int num_slices, int num_types, int height, int width, long long num_atoms,
+                                    const double* shifts, const long long* segments, const void* kernel_k,
+                                    const void* adj_k, double* grad_shifts, void* workspace, size_t workspace_bytes,
+                                    void* stream) {
+  g_err[0] = '\0';
+  if (num_slices < 0 || num_slices > 65535) return fail(PTY_ERR_INVALID, "num_slices must be in [0, 65535]");
+  if (num_types < 0 || num_types > 4096) return fail(PTY_ERR_INVALID, "num_types must be in [0, 4096]");
+  if (height < 1 || width < 1 || height > 32768 || width > 32768)
+    return fail(PTY_ERR_INVALID, "height and width must be in [1, 32768]");
+  if (num_atoms < 0) return fail(PTY_ERR_INVALID, "num_atoms must be non-negative");
+  if (num_slices > 0 && num_atoms > 0 &&
+      (!segments || !shifts || !grad_shifts || !adj_k || (num_types > 0 && !kernel_k) || !workspace))
+    return fail(PTY_ERR_INVALID, "null pointer");
+  if (num_slices > 0 && num_atoms > 0 && workspace_bytes < pty::slice_shift_grad_workspace_bytes(height, width, num_atoms))
+    return fail(PTY_ERR_INVALID, "workspace smaller than pty_potential_shift_grad_workspace_bytes");
+  return device_status(pty::launch_slice_shift_grad(num_slices, num_types, height, width, num_atoms, shifts,
+                                                    segments, (const double2*)kernel_k, (const double2*)adj_k,
+                                                    grad_shifts, workspace, (hipStream_t)stream),
+                       "pty_potential_slices_shift_grad");
 }
 
 size_t pty_persistent_state_bytes(void) { return pty::xl2_state_bytes(); }

@@ -103,6 +103,33 @@ __device__ __forceinline__ double projected_potential(int kind, const double* ro
          2.0 * kPi * kPi * kKirklandA0Ek * ((gau[0] + gau[1]) + gau[2]);
 }
 
+// dV/dr of projected_potential: the Bessel identities K0'(x) = -K1(x) and (x K1(x))' = -x K0(x) applied to the
+// same A&S evaluations.  Below the radius floor the potential does not depend on r.
+__device__ __forceinline__ double projected_potential_dr(int kind, const double* row, double r) {
+  if (r < 1e-10) return 0.0;
+  if (kind == PARAM_LOBATO) {
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const double a = row[2 * i], b = row[2 * i + 1];
+      const double sb = sqrt(b);
+      const double arg = 2.0 * kPi * r / sb;
+      // d/dr [a/b K0(arg)] = -a/b (2 pi / sqrt b) K1(arg);  d/dr [a pi r / b^1.5 K1(arg)] = -a pi / b^1.5 arg K0(arg)
+      sum -= a / b * (2.0 * kPi / sb) * bessel_k1(arg) + a * kPi / (b * sb) * arg * bessel_k0(arg);
+    }
+    return 2.0 * kPi * kMottBethe * sum;
+  }
+  double lor = 0.0, gau = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double sb = sqrt(row[2 * i + 1]);
+    lor -= row[2 * i] * 2.0 * kPi * sb * bessel_k1(2.0 * kPi * r * sb);
+    const double c = row[6 + 2 * i], d = row[7 + 2 * i];
+    gau -= c / d * exp(-(kPi * kPi) * r * r / d) * (2.0 * kPi * kPi * r / d);
+  }
+  return 4.0 * kPi * kPi * kKirklandA0Ek * lor + 2.0 * kPi * kPi * kKirklandA0Ek * gau;
+}
+
 __global__ __launch_bounds__(256) void k_projected_radii(int kind, const double* __restrict__ row, long long count,
                                                          const double* __restrict__ r, double* __restrict__ out) {
   double p[PARAM_STRIDE];
@@ -143,6 +170,73 @@ __global__ __launch_bounds__(256) void k_atom_grid(int kind, const double* __res
   }
 }
 
+// Centre gradient of k_atom_grid: block b of element t sums, over its pixels (grid stride) and their
+// supersamples, grad_out[t][i][j] * dV/dr * dr/dc with dr/dc = -(x - c) / r, and writes the (x, y) pair to
+// partial[t][b] after a fixed-order tree reduction; k_atom_center_finish adds the blocks in order.
+constexpr int CG_BLOCKS = 512;
+
+__global__ __launch_bounds__(256) void k_atom_center_grad(int kind, const double* __restrict__ rows,
+                                                          double pixel_size, int height, int width, int ss,
+                                                          const double* __restrict__ centers,
+                                                          const double* __restrict__ grad_out,
+                                                          double2* __restrict__ partial) {
+  __shared__ double2 red[256];
+  const int t = blockIdx.y;
+  double p[PARAM_STRIDE];
+#pragma unroll
+  for (int i = 0; i < PARAM_STRIDE; ++i) p[i] = rows[(long long)t * PARAM_STRIDE + i];
+  const double cx = centers ? centers[2 * t] : 0.0;
+  const double cy = centers ? centers[2 * t + 1] : 0.0;
+  const double step = pixel_size / ss;
+  const int gh2 = (height * ss) / 2, gw2 = (width * ss) / 2;
+  const long long pix = (long long)height * width;
+  const double* g = grad_out + (long long)t * pix;
+  double gx = 0.0, gy = 0.0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < pix;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / width), j = (int)(e % width);
+    const double ge = g[e];
+    double sx = 0.0, sy = 0.0;
+    for (int a = 0; a < ss; ++a) {
+      const double dy = (double)(i * ss + a - gh2) * step - cy;
+      for (int b = 0; b < ss; ++b) {
+        const double dx = (double)(j * ss + b - gw2) * step - cx;
+        const double r = sqrt(dx * dx + dy * dy + 1e-10);
+        const double w = projected_potential_dr(kind, p, r) / r;
+        sx -= w * dx;
+        sy -= w * dy;
+      }
+    }
+    gx = fma(ge, sx, gx);
+    gy = fma(ge, sy, gy);
+  }
+  red[threadIdx.x] = make_double2(gx, gy);
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      const double2 o = red[threadIdx.x + h];
+      red[threadIdx.x].x += o.x;
+      red[threadIdx.x].y += o.y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(long long)t * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_atom_center_finish(int num_types, int blocks, double scale,
+                                                            const double2* __restrict__ partial,
+                                                            double* __restrict__ grad_centers) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num_types) return;
+  double sx = 0.0, sy = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    sx += partial[(long long)t * blocks + b].x;
+    sy += partial[(long long)t * blocks + b].y;
+  }
+  grad_centers[2 * t] = sx * scale;
+  grad_centers[2 * t + 1] = sy * scale;
+}
+
 // ---------------------------------------------------------------------------
 // Structure-factor slices.  A workgroup owns a 64 x 128 tile of one slice's
 // Fourier grid; thread (tr, tc) (tc = lane % 16, tr = 4 wave + lane / 16)
@@ -180,6 +274,35 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 
+// Phase factors of atoms c0 .. c0 + n - 1 for one 64 x 128 tile: u[a][row] = exp(2 pi i ky sy_a) and
+// v[a][col] = exp(2 pi i kx sx_a), runs of 16 frequencies per thread (see above).
+__device__ __forceinline__ void stage_phases(double2 (*u_lds)[SF_ROWS], double2 (*v_lds)[SF_COLS],
+                                             const double* __restrict__ shifts, long long c0, int n, int height,
+                                             int width, int y0, int x0) {
+  if ((int)threadIdx.x < n * SF_RUNS_PER_ATOM) {
+    const int a = threadIdx.x / SF_RUNS_PER_ATOM, run = threadIdx.x % SF_RUNS_PER_ATOM;
+    const double* sh = shifts + 2 * (c0 + a);
+    const bool is_row = run < SF_ROWS / SF_RUN;
+    const int dim = is_row ? height : width;
+    const double shift = is_row ? sh[1] : sh[0];
+    const int q0 = is_row ? run * SF_RUN : (run - SF_ROWS / SF_RUN) * SF_RUN;
+    const int g0 = (is_row ? y0 : x0) + q0;
+    double2* dst = is_row ? &u_lds[a][q0] : &v_lds[a][q0];
+    double ts = shift / (double)dim;
+    ts -= rint(ts);
+    double ss, cs;
+    sincospi(2.0 * ts, &ss, &cs);
+    const double2 step = make_double2(cs, ss);
+    const int wrap = (dim + 1) / 2;
+    double2 val = phase_factor(g0, dim, shift);
+    dst[0] = val;
+    for (int q = 1; q < SF_RUN; ++q) {
+      val = (g0 + q == wrap) ? phase_factor(g0 + q, dim, shift) : cmul(val, step);
+      dst[q] = val;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_slice_structure(int num_types, int height, int width, long long num_atoms,
                                                          const double* __restrict__ shifts,
                                                          const long long* __restrict__ segments,
@@ -209,28 +332,7 @@ __global__ __launch_bounds__(256) void k_slice_structure(int num_types, int heig
     for (long long c0 = a0; c0 < a1; c0 += SF_CHUNK) {
       const int n = (int)((a1 - c0) < SF_CHUNK ? (a1 - c0) : SF_CHUNK);
       __syncthreads();
-      if ((int)threadIdx.x < n * SF_RUNS_PER_ATOM) {
-        const int a = threadIdx.x / SF_RUNS_PER_ATOM, run = threadIdx.x % SF_RUNS_PER_ATOM;
-        const double* sh = shifts + 2 * (c0 + a);
-        const bool is_row = run < SF_ROWS / SF_RUN;
-        const int dim = is_row ? height : width;
-        const double shift = is_row ? sh[1] : sh[0];
-        const int q0 = is_row ? run * SF_RUN : (run - SF_ROWS / SF_RUN) * SF_RUN;
-        const int g0 = (is_row ? y0 : x0) + q0;
-        double2* dst = is_row ? &u_lds[a][q0] : &v_lds[a][q0];
-        double ts = shift / (double)dim;
-        ts -= rint(ts);
-        double ss, cs;
-        sincospi(2.0 * ts, &ss, &cs);
-        const double2 step = make_double2(cs, ss);
-        const int wrap = (dim + 1) / 2;
-        double2 val = phase_factor(g0, dim, shift);
-        dst[0] = val;
-        for (int q = 1; q < SF_RUN; ++q) {
-          val = (g0 + q == wrap) ? phase_factor(g0 + q, dim, shift) : cmul(val, step);
-          dst[q] = val;
-        }
-      }
+      stage_phases(u_lds, v_lds, shifts, c0, n, height, width, y0, x0);
       __syncthreads();
 #pragma unroll 2
       for (int a = 0; a < n; ++a) {
@@ -283,6 +385,129 @@ __global__ __launch_bounds__(256) void k_slice_structure(int num_types, int heig
   }
 }
 
+// Shift gradient of the structure-factor slices.  With A_s = ifft2 of dL/dV_s (zero outside the cropped
+// region) and W_st = 2 pi i Khat_t A_s, atom a of element t in slice s has
+//   dL/dsx_a = Re sum_k W_st(k) kx exp(2 pi i (kx sx_a + ky sy_a)),   dL/dsy_a likewise with ky,
+// kx, ky in cycles per pixel (numpy's fftfreq).  A workgroup owns a 64 x 128 tile as k_slice_structure does; per
+// element it holds W for its 32 pixels, stages the phases of 16 atoms at a time, sums its pixels for each atom,
+// reduces the workgroup in a fixed order and writes the tile's pair to partial[tile][atom];
+// k_shift_grad_finish adds the tiles of each atom in order.
+__global__ __launch_bounds__(256) void k_slice_shift_grad(int num_types, int height, int width, long long num_atoms,
+                                                          const double* __restrict__ shifts,
+                                                          const long long* __restrict__ segments,
+                                                          const double2* __restrict__ kernel_k,
+                                                          const double2* __restrict__ adj_k,
+                                                          double2* __restrict__ partial) {
+  __shared__ double2 u_lds[SF_CHUNK][SF_ROWS];
+  __shared__ double2 v_lds[SF_CHUNK][SF_COLS];
+  __shared__ double2 wsum[4][SF_CHUNK];
+  const int s = blockIdx.z;
+  const int x0 = blockIdx.x * SF_COLS, y0 = blockIdx.y * SF_ROWS;
+  const long long tile = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int tc = lane & 15, tr = wid * 4 + (lane >> 4);
+  const long long plane = (long long)height * width;
+  const double2* as = adj_k + (long long)s * plane;
+  double kx[SF_CPT], ky[SF_RPT];
+#pragma unroll
+  for (int j = 0; j < SF_CPT; ++j) {
+    const int col = x0 + tc + 16 * j;
+    kx[j] = (double)(col < (width + 1) / 2 ? col : col - width) / (double)width;
+  }
+#pragma unroll
+  for (int i = 0; i < SF_RPT; ++i) {
+    const int row = y0 + tr + 16 * i;
+    ky[i] = (double)(row < (height + 1) / 2 ? row : row - height) / (double)height;
+  }
+  for (int t = 0; t < num_types; ++t) {
+    long long a0 = segments[(long long)s * num_types + t];
+    long long a1 = segments[(long long)s * num_types + t + 1];
+    a0 = a0 < 0 ? 0 : (a0 > num_atoms ? num_atoms : a0);
+    a1 = a1 < 0 ? 0 : (a1 > num_atoms ? num_atoms : a1);
+    if (a1 <= a0) continue;
+    const double2* kt = kernel_k + (long long)t * plane;
+    double2 wk[SF_RPT][SF_CPT];  // 2 pi i Khat_t A_s, zero outside the grid
+#pragma unroll
+    for (int i = 0; i < SF_RPT; ++i) {
+      const int row = y0 + tr + 16 * i;
+#pragma unroll
+      for (int j = 0; j < SF_CPT; ++j) {
+        const int col = x0 + tc + 16 * j;
+        double2 w = make_double2(0.0, 0.0);
+        if (row < height && col < width) {
+          const long long e = (long long)row * width + col;
+          const double2 z = cmul(kt[e], as[e]);
+          w = make_double2(-2.0 * kPi * z.y, 2.0 * kPi * z.x);
+        }
+        wk[i][j] = w;
+      }
+    }
+    for (long long c0 = a0; c0 < a1; c0 += SF_CHUNK) {
+      const int n = (int)((a1 - c0) < SF_CHUNK ? (a1 - c0) : SF_CHUNK);
+      __syncthreads();
+      stage_phases(u_lds, v_lds, shifts, c0, n, height, width, y0, x0);
+      __syncthreads();
+      for (int a = 0; a < n; ++a) {
+        double gx = 0.0, gy = 0.0;
+#pragma unroll
+        for (int i = 0; i < SF_RPT; ++i) {
+          double2 sum = make_double2(0.0, 0.0), sumx = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int j = 0; j < SF_CPT; ++j) {
+            const double2 c = cmul(wk[i][j], v_lds[a][tc + 16 * j]);
+            sum.x += c.x;
+            sum.y += c.y;
+            sumx.x = fma(c.x, kx[j], sumx.x);
+            sumx.y = fma(c.y, kx[j], sumx.y);
+          }
+          const double2 u = u_lds[a][tr + 16 * i];
+          gx += u.x * sumx.x - u.y * sumx.y;
+          gy += ky[i] * (u.x * sum.x - u.y * sum.y);
+        }
+        // fixed-order reduction: butterfly within the wave, then the four waves in order
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          gx += __shfl_xor(gx, off, 64);
+          gy += __shfl_xor(gy, off, 64);
+        }
+        if (lane == 0) wsum[wid][a] = make_double2(gx, gy);
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < n) {
+        const int a = threadIdx.x;
+        double2 r = wsum[0][a];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+          r.x += wsum[w][a].x;
+          r.y += wsum[w][a].y;
+        }
+        partial[tile * num_atoms + c0 + a] = r;
+      }
+    }
+  }
+}
+
+// Atoms [segments[0], segments[num_segments]) (clamped) are the atoms of the slices just reduced.
+__global__ __launch_bounds__(256) void k_shift_grad_finish(long long num_atoms, const long long* __restrict__ segments,
+                                                           int num_segments, int tiles,
+                                                           const double2* __restrict__ partial,
+                                                           double* __restrict__ grad_shifts) {
+  long long a_begin = segments[0], a_end = segments[num_segments];
+  a_begin = a_begin < 0 ? 0 : (a_begin > num_atoms ? num_atoms : a_begin);
+  a_end = a_end < 0 ? 0 : (a_end > num_atoms ? num_atoms : a_end);
+  for (long long a = a_begin + blockIdx.x * (long long)blockDim.x + threadIdx.x; a < a_end;
+       a += (long long)gridDim.x * blockDim.x) {
+    double gx = 0.0, gy = 0.0;
+    for (int t = 0; t < tiles; ++t) {
+      const double2 p = partial[(long long)t * num_atoms + a];
+      gx += p.x;
+      gy += p.y;
+    }
+    grad_shifts[2 * a] = gx;
+    grad_shifts[2 * a + 1] = gy;
+  }
+}
+
 long long grid_blocks(long long work) {
   long long b = (work + 255) / 256;
   if (b > 65536) b = 65536;
@@ -310,6 +535,25 @@ hipError_t launch_atom_grid(int kind, int num_types, const double* rows, double 
   return hipGetLastError();
 }
 
+size_t atom_center_grad_workspace_bytes(int num_types) {
+  return (size_t)(num_types > 0 ? num_types : 0) * CG_BLOCKS * sizeof(double2);
+}
+
+hipError_t launch_atom_center_grad(int kind, int num_types, const double* rows, double pixel_size, int height,
+                                   int width, int supersampling, const double* centers, const double* grad_out,
+                                   double* grad_centers, void* ws, hipStream_t st) {
+  const long long pix = (long long)height * width;
+  if (num_types == 0) return hipSuccess;
+  long long bx = grid_blocks(pix);
+  if (bx > CG_BLOCKS) bx = CG_BLOCKS;
+  double2* partial = (double2*)ws;
+  hipLaunchKernelGGL(k_atom_center_grad, dim3((unsigned)bx, (unsigned)num_types), dim3(256), 0, st, kind, rows,
+                     pixel_size, height, width, supersampling, centers, grad_out, partial);
+  hipLaunchKernelGGL(k_atom_center_finish, dim3((unsigned)((num_types + 255) / 256)), dim3(256), 0, st, num_types,
+                     (int)bx, 1.0 / ((double)supersampling * supersampling), partial, grad_centers);
+  return hipGetLastError();
+}
+
 hipError_t launch_slice_structure(int num_slices, int num_types, int height, int width, long long num_atoms,
                                   const double* shifts, const long long* segments, const double2* kernel_k,
                                   double2* out_k, hipStream_t st) {
@@ -318,6 +562,26 @@ hipError_t launch_slice_structure(int num_slices, int num_types, int height, int
                   (unsigned)num_slices);
   hipLaunchKernelGGL(k_slice_structure, grid, dim3(256), 0, st, num_types, height, width, num_atoms, shifts,
                      segments, kernel_k, out_k);
+  return hipGetLastError();
+}
+
+size_t slice_shift_grad_workspace_bytes(int height, int width, long long num_atoms) {
+  const long long tiles = (long long)((width + SF_COLS - 1) / SF_COLS) * ((height + SF_ROWS - 1) / SF_ROWS);
+  return (size_t)(tiles * (num_atoms > 0 ? num_atoms : 0)) * sizeof(double2);
+}
+
+hipError_t launch_slice_shift_grad(int num_slices, int num_types, int height, int width, long long num_atoms,
+                                   const double* shifts, const long long* segments, const double2* kernel_k,
+                                   const double2* adj_k, double* grad_shifts, void* ws, hipStream_t st) {
+  if (num_slices == 0 || num_atoms == 0) return hipSuccess;
+  const dim3 grid((unsigned)((width + SF_COLS - 1) / SF_COLS), (unsigned)((height + SF_ROWS - 1) / SF_ROWS),
+                  (unsigned)num_slices);
+  double2* partial = (double2*)ws;
+  hipLaunchKernelGGL(k_slice_shift_grad, grid, dim3(256), 0, st, num_types, height, width, num_atoms, shifts,
+                     segments, kernel_k, adj_k, partial);
+  const int tiles = (int)(grid.x * grid.y);
+  hipLaunchKernelGGL(k_shift_grad_finish, dim3((unsigned)grid_blocks(num_atoms)), dim3(256), 0, st, num_atoms,
+                     segments, num_slices * num_types, tiles, (const double2*)partial, grad_shifts);
   return hipGetLastError();
 }
 

@@ -13,6 +13,14 @@ hipError_t launch_projected_radii(int kind, const double* row, long long count, 
                                   hipStream_t st);
 hipError_t launch_atom_grid(int kind, int num_types, const double* rows, double pixel_size, int height, int width,
                             int supersampling, const double* centers, double* out, hipStream_t st);
+size_t atom_center_grad_workspace_bytes(int num_types);
+hipError_t launch_atom_center_grad(int kind, int num_types, const double* rows, double pixel_size, int height,
+                                   int width, int supersampling, const double* centers, const double* grad_out,
+                                   double* grad_centers, void* ws, hipStream_t st);
+size_t slice_shift_grad_workspace_bytes(int height, int width, long long num_atoms);
+hipError_t launch_slice_shift_grad(int num_slices, int num_types, int height, int width, long long num_atoms,
+                                   const double* shifts, const long long* segments, const double2* kernel_k,
+                                   const double2* adj_k, double* grad_shifts, void* ws, hipStream_t st);
 hipError_t launch_slice_structure(int num_slices, int num_types, int height, int width, long long num_atoms,
                                   const double* shifts, const long long* segments, const double2* kernel_k,
                                   double2* out_k, hipStream_t st);

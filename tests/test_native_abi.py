@@ -81,6 +81,13 @@ def test_potential_builders_validate_on_host():
     assert b"num_atoms" in lib.pty_last_error()
     assert lib.pty_potential_slices_fourier(2, 1, 8, 8, 3, None, None, None, None, None) == _native.PTY_ERR_INVALID
     assert b"null pointer" in lib.pty_last_error()
+    assert lib.pty_atom_center_grad_workspace_bytes(3) == 3 * 512 * 16
+    assert lib.pty_atom_potential_grid_center_grad(5, 1, None, 0.1, 8, 8, 1, None, None, None, None, 0,
+                                                   None) == _native.PTY_ERR_INVALID
+    assert b"parameterization" in lib.pty_last_error()
+    assert lib.pty_atom_potential_grid_center_grad(0, 1, None, 0.1, 8, 8, 1, None, None, None, None, 0,
+                                                   None) == _native.PTY_ERR_INVALID
+    assert b"null pointer" in lib.pty_last_error()
 
 
 def test_grid_entry_points_validate_on_host():

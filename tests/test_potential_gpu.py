@@ -78,6 +78,64 @@ def test_single_atom_potential_matches_oracle(kind, z, shape, ss, center):
     np.testing.assert_allclose(got, want, rtol=1e-12)
 
 
+@pytest.mark.parametrize("kind, z, shape, ss, center", [("lobato", 14, (16, 16), 2, (0.13, -0.17)),
+                                                        ("lobato", 79, (33, 20), 3, (0.37, -0.21)),
+                                                        ("kirkland", 6, (25, 31), 1, (1.0, 0.5)),
+                                                        ("kirkland", 14, (64, 48), 4, (-0.4, 0.25))])
+def test_single_atom_center_gradient_matches_oracle(kind, z, shape, ss, center):
+    """Autograd through ``single_atom_potential`` in the centre (``pty_atom_potential_grid_center_grad``)
+    against the oracle's chain rule: the same FP64 formulas, rel 1e-10."""
+    print(f"centre gradient {kind} Z={z} shape={shape} ss={ss}", flush=True)
+    rng = np.random.default_rng(z)
+    weights = rng.standard_normal(shape)
+    ctr = torch.tensor(center, dtype=torch.float64, device="cuda", requires_grad=True)
+    v = pa.single_atom_potential(z, 0.15, shape, ctr, ss, parameterization=kind)
+    (v * torch.as_tensor(weights, device="cuda")).sum().backward()
+    want = po.single_atom_center_grad(kind, TABLES[kind][z], 0.15, shape, center, ss, weights)
+    np.testing.assert_allclose(ctr.grad.cpu().numpy(), want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("kind", ["lobato", "kirkland"])
+def test_single_atom_center_gradient_reference_check(kind):
+    """``test_atom_potentials.py:91-151`` through the drop-in: Si, 0.2 A, 16 x 16, supersampling 2, centre
+    (0.13, -0.17), loss mean(weights * V); the shift changes the potential, and grad . (0.6, -0.8) equals
+    central differences of the GPU forward at rtol 2e-5, atol 1e-7."""
+    axis = torch.linspace(-1.0, 1.0, 16, dtype=torch.float64, device="cuda")
+    yw, xw = torch.meshgrid(axis, axis, indexing="ij")
+    weights = 0.37 * xw - 0.61 * yw + 0.23 * xw * yw
+
+    def loss(c):
+        return torch.mean(weights * pa.single_atom_potential(14, 0.2, (16, 16), c, 2, parameterization=kind))
+
+    center = torch.tensor([0.13, -0.17], dtype=torch.float64, device="cuda")
+    centered = pa.single_atom_potential(14, 0.2, (16, 16), torch.zeros(2, dtype=torch.float64, device="cuda"), 2,
+                                        parameterization=kind)
+    shifted = pa.single_atom_potential(14, 0.2, (16, 16), center, 2, parameterization=kind)
+    assert float(torch.linalg.norm(shifted - centered) / torch.linalg.norm(centered)) > 1e-3
+    c = center.clone().requires_grad_(True)
+    (grad,) = torch.autograd.grad(loss(c), c)
+    assert torch.all(torch.isfinite(grad)) and float(torch.linalg.norm(grad)) > 1e-6
+    d = torch.tensor([0.6, -0.8], dtype=torch.float64, device="cuda")
+    step = 1e-5
+    fd = (loss(center + step * d) - loss(center - step * d)) / (2.0 * step)
+    np.testing.assert_allclose(float(grad @ d), float(fd), rtol=2e-5, atol=1e-7)
+
+
+def test_center_gradient_entry_point_rejects_a_short_workspace():
+    from ptyrodactyl_amd import _native
+    lib = _native.load()
+    rows = torch.zeros((1, 12), dtype=torch.float64, device="cuda")
+    rows[0, :10] = torch.as_tensor(TABLES["lobato"][14], dtype=torch.float64)
+    g = torch.ones((1, 8, 8), dtype=torch.float64, device="cuda")
+    out = torch.empty((1, 2), dtype=torch.float64, device="cuda")
+    need = int(lib.pty_atom_center_grad_workspace_bytes(1))
+    ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+    st = lib.pty_atom_potential_grid_center_grad(0, 1, _native.ptr(rows), 0.2, 8, 8, 1, None, _native.ptr(g),
+                                                 _native.ptr(out), _native.ptr(ws), need - 1,
+                                                 _native.stream_handle(torch.device("cuda", 0)))
+    assert st != 0 and b"workspace" in lib.pty_last_error()
+
+
 def test_default_parameterization_is_lobato():
     """``test_atom_potentials.py:715-761``."""
     a = pa.single_atom_potential(14, 0.2, (16, 16), supersampling=2)
@@ -122,6 +180,35 @@ def test_crystal_potential_slices_match_oracle(kind, seed, n_atoms, span, pixel,
     print(f"{kind} seed {seed}: shape {g.shape} rel-L2 {_rel_l2(g, want):.2e}")
     assert _rel_l2(g, want) < 2e-7
     assert np.abs(g - want).max() < 2e-6 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("kind, seed, n_atoms, span, pixel, pad, ss, grid, reps", [
+    ("lobato", 0, 12, 4.0, 0.2, 2.0, 2, None, (1, 1, 1)),
+    ("kirkland", 1, 20, 5.0, 0.15, 1.0, 3, None, (1, 1, 1)),
+    ("lobato", 2, 6, 3.0, 0.25, 1.0, 2, (40, 36, 3), (2, 2, 1)),
+    ("kirkland", 3, 9, 2.5, 0.3, 0.6, 1, None, (2, 1, 2)),
+    ("lobato", 4, 300, 12.0, 0.1, 1.0, 2, None, (1, 1, 1)),
+])
+def test_crystal_slices_position_gradient_matches_oracle(kind, seed, n_atoms, span, pixel, pad, ss, grid, reps):
+    """Autograd through ``crystal_potential_slices`` in the atom positions (``pty_potential_slices_shift_grad``,
+    grid origin through amin) against the oracle's gradient, which central differences pin: FP64 on both sides,
+    rel 1e-9.  Repeats (2, 2, 1) and (2, 1, 2) put ties at the minimum (lattice copies), split evenly on both
+    sides."""
+    print(f"slice position gradient {kind} seed {seed} atoms {n_atoms} repeats {reps}", flush=True)
+    pos, nums = _crystal_case(seed, n_atoms, span, [1, 6, 14, 79], 2.6)
+    lattice = np.asarray([[3.1, 0.0, 0.0], [0.4, 2.9, 0.0], [0.0, 0.0, 1.7]])
+    p = torch.tensor(pos, dtype=torch.float64, device="cuda", requires_grad=True)
+    crystal = pa.create_crystal_data(p, nums, lattice=lattice)
+    got = pa.crystal_potential_slices(crystal, pixel, 1.0, repeats=reps, padding=pad, supersampling=ss,
+                                      grid_shape=grid, parameterization=kind)
+    g = np.random.default_rng(seed + 100).standard_normal(tuple(got.slices.shape))
+    (got.slices * torch.as_tensor(g, device="cuda")).sum().backward()
+    want = po.crystal_potential_slices_position_grad(kind, TABLES[kind], pos, nums, lattice, pixel, g, 1.0, reps,
+                                                     pad, ss, grid)
+    err = _rel_l2(p.grad.cpu().numpy(), want)
+    print(f"  rel-L2 {err:.2e}", flush=True)
+    assert err < 1e-9
+    assert torch.all(p.grad[:, 2] == 0)
 
 
 def test_reference_slice_occupancy_and_atomic_number_ordering():

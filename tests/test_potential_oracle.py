@@ -20,6 +20,7 @@ from oracle import potential_oracle as po
 ROWS = json.loads((Path(__file__).parent / "golden" / "form_factor_rows.json").read_text())
 LOB = {int(k): np.asarray(v) for k, v in ROWS["lobato"].items()}
 KIR = {int(k): np.asarray(v) for k, v in ROWS["kirkland"].items()}
+TABLES = {"lobato": LOB, "kirkland": KIR}
 
 
 def test_fixture_rows_match_the_reference_tests_anchors():
@@ -209,3 +210,76 @@ def test_repeats_equal_the_explicitly_listed_supercell():
               for k in range(2)]
     np.testing.assert_allclose(p1, np.asarray(listed), rtol=0, atol=1e-15)
     assert n1.tolist() == [6, 14] * 12
+
+
+@pytest.mark.parametrize("kind", ["lobato", "kirkland"])
+@pytest.mark.parametrize("z", [1, 14, 79])
+def test_projected_dr_matches_central_differences(kind, z):
+    row = TABLES[kind][z]
+    r = np.geomspace(1e-3, 12.0, 300)
+    scale = 2.0 * np.pi / np.sqrt(row[1:10:2]) if kind == "lobato" else 2.0 * np.pi * np.sqrt(row[1:6:2])
+    # keep clear of the A&S branch point x = 2, where the two approximations differ by ~1e-8
+    r = r[np.abs(r[:, None] * scale[None] - 2.0).min(axis=1) > 1e-3]
+    h = 1e-6 * r
+    fd = (po.projected(kind, row, r + h) - po.projected(kind, row, r - h)) / (2 * h)
+    got = po.projected_dr(kind, row, r)
+    np.testing.assert_allclose(got, fd, rtol=2e-6, atol=1e-9 * np.abs(fd).max())
+    assert np.all(got < 0)  # the screened potential falls off monotonically
+
+
+@pytest.mark.parametrize("kind", ["lobato", "kirkland"])
+def test_single_atom_center_grad_matches_the_reference_check(kind):
+    """The reference's case (Si, 0.2 A, 16 x 16, supersampling 2, centre (0.13, -0.17), loss mean(weights * V))
+    along its direction (0.6, -0.8): gradient . direction against central differences at rtol 2e-5."""
+    row = TABLES[kind][14]
+    axis = np.linspace(-1.0, 1.0, 16)
+    yw, xw = np.meshgrid(axis, axis, indexing="ij")
+    weights = 0.37 * xw - 0.61 * yw + 0.23 * xw * yw
+    center = np.array([0.13, -0.17])
+    loss = lambda c: np.mean(weights * po.single_atom_potential(kind, row, 0.2, (16, 16), c, 2))  # noqa: E731
+    grad = po.single_atom_center_grad(kind, row, 0.2, (16, 16), center, 2, weights / weights.size)
+    d = np.array([0.6, -0.8])
+    step = 1e-5
+    fd = (loss(center + step * d) - loss(center - step * d)) / (2 * step)
+    assert np.linalg.norm(grad) > 1e-6
+    np.testing.assert_allclose(grad @ d, fd, rtol=2e-5, atol=1e-7)
+    # both components separately, and a second centre off the grid's symmetry lines
+    for c in (center, np.array([-0.31, 0.07])):
+        g = po.single_atom_center_grad(kind, row, 0.2, (16, 16), c, 2, weights / weights.size)
+        for i in range(2):
+            e = np.eye(2)[i]
+            fd_i = (loss(c + step * e) - loss(c - step * e)) / (2 * step)
+            np.testing.assert_allclose(g[i], fd_i, rtol=2e-5, atol=1e-7)
+
+
+def _crystal_case(seed):
+    rng = np.random.default_rng(seed)
+    pos = np.column_stack([rng.uniform(0.0, 3.0, 6), rng.uniform(0.0, 2.5, 6), rng.uniform(0.0, 1.9, 6)])
+    nums = np.array([6, 14, 6, 14, 14, 6])
+    lattice = np.array([[3.4, 0.35, 0.0], [0.2, 3.1, 0.0], [0.0, 0.0, 2.0]])
+    rows = {6: LOB[6], 14: LOB[14]}
+    return rng, pos, nums, lattice, rows
+
+
+@pytest.mark.parametrize("repeats", [(1, 1, 1), (2, 1, 1)])
+def test_crystal_slices_position_grad_matches_central_differences(repeats):
+    """dL/dpositions of L = sum G * slices against central differences of the FP64-accumulated oracle slices, for
+    every coordinate of three atoms, among them the atoms that set x_min and y_min (whose shift moves the grid
+    origin); lattice copies add into their source atom; z has no gradient."""
+    rng, pos, nums, lattice, rows = _crystal_case(11)
+    kw = dict(slice_thickness=1.0, repeats=repeats, padding=1.0, supersampling=2)
+
+    def slices(p):
+        return po.crystal_potential_slices("lobato", rows, p, nums, lattice, 0.25, accumulate=np.float64, **kw)[0]
+
+    g = rng.standard_normal(slices(pos).shape)
+    grad = po.crystal_potential_slices_position_grad("lobato", rows, pos, nums, lattice, 0.25, g, **kw)
+    atoms = sorted({int(np.argmin(pos[:, 0])), int(np.argmin(pos[:, 1])), 2})
+    h = 1e-5
+    for a in atoms:
+        for c in range(3):
+            dp = np.zeros_like(pos)
+            dp[a, c] = h
+            fd = (np.sum(g * slices(pos + dp)) - np.sum(g * slices(pos - dp))) / (2 * h)
+            np.testing.assert_allclose(grad[a, c], fd, rtol=1e-6, atol=1e-8 * np.abs(grad).max())
+    assert np.all(grad[:, 2] == 0.0) and np.abs(grad[:, :2]).max() > 1e-3

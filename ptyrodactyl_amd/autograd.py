@@ -249,3 +249,80 @@ def multislice_amplitudes(slab_t: torch.Tensor, probe_k: torch.Tensor, origins: 
     amp, _ = MultisliceAmplitudeFunction.apply(slab_t, probe_k, origins, shifts_px, prop_y, prop_x,
                                                int(ckpt_every), torch.is_grad_enabled())
     return amp
+
+
+# ---------------------------------------------------------------------------
+# Batched waves: one descriptor launch over many (wave, slab, propagator) triples
+# ---------------------------------------------------------------------------
+
+
+def _wave_descs(n: int, slabs: torch.Tensor, probe_k: torch.Tensor, props: torch.Tensor, group,
+                patterns: torch.Tensor, base: torch.Tensor, grad_slabs=None, grad_probe=None) -> torch.Tensor:
+    """Descriptor records for waves i = 0 .. W-1: slab ``group[i]`` of ``slabs`` (E, S, n, n), probe_k[i],
+    propagator row ``group[i]`` of ``props`` (E, 2n), pattern (or dL/dI) i of ``patterns`` (W, n, n); origin 0,
+    no shift.  For the adjoint, the slab gradient of wave i goes to grad_slabs[group[i]] and its probe gradient
+    to grad_probe[i]."""
+    import numpy as np
+    from . import _native
+    count = probe_k.shape[0]
+    s_count = slabs.shape[1]
+    rec = np.zeros(count, dtype=np.dtype(_native.POSITION_DESC_FIELDS))
+    grp = np.asarray(group, dtype=np.int64)
+    idx = np.arange(count, dtype=np.int64)
+    rec["slab_offset"] = engine.element_offset(slabs, base) + grp * (s_count * n * n)
+    rec["slab_h"] = n
+    rec["slab_w"] = n
+    rec["pattern_offset"] = engine.element_offset(patterns, base) + idx * n * n
+    rec["probe_offset"] = engine.element_offset(probe_k, base) + idx * n * n
+    rec["prop_offset"] = engine.element_offset(props, base) + grp * (2 * n)
+    if grad_slabs is not None:
+        rec["grad_offset"] = engine.element_offset(grad_slabs, base) + grp * (s_count * n * n)
+        rec["gprobe_offset"] = engine.element_offset(grad_probe, base) + idx * n * n
+    return torch.from_numpy(rec.view(np.uint8)).to(slabs.device)
+
+
+class BatchedWavesFunction(torch.autograd.Function):
+    """Intensities (W, n, n) float32 (fftshifted) of W waves, each with its own probe_k and one of E (slab,
+    propagator) pairs, in one ``pty_forward_batched`` launch with checkpoints; differentiable w.r.t. the slabs
+    and every probe_k through one ``pty_adjoint_batched`` launch (float32 atomics add the waves that share a
+    slab).  The ensemble axes use it for every (position, sample) wave: the reference maps cbed_image over
+    them (src/ptyrodactyl/multislice/simulations.py:769-887)."""
+
+    @staticmethod
+    def forward(slabs, probe_k, props, group, ckpt_every):
+        n, s_count = probe_k.shape[-1], slabs.shape[1]
+        base = engine.arena_base(slabs.device)
+        slabs_c, pk_c, props_c = slabs.detach().contiguous(), probe_k.detach().contiguous(), props.contiguous()
+        patterns = torch.empty(probe_k.shape, dtype=torch.float32, device=slabs.device)
+        nc = (s_count - 1) // ckpt_every  # checkpoints for the backward's segment recompute
+        ckpt = torch.empty((probe_k.shape[0], nc, n, n), dtype=torch.complex64, device=slabs.device) if nc else None
+        descs = _wave_descs(n, slabs_c, pk_c, props_c, group, patterns, base)
+        engine.forward_batched(n, s_count, descs, base, ckpt, ckpt_every if nc else 0)
+        return patterns, (ckpt if ckpt is not None else _empty_like_ckpt(slabs))
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        slabs, probe_k, props, group, ckpt_every = inputs
+        ctx.save_for_backward(slabs.detach().contiguous(), probe_k.detach().contiguous(), props.contiguous(),
+                              output[1])
+        ctx.group, ctx.ckpt_every = group, ckpt_every
+        ctx.mark_non_differentiable(output[1])
+
+    @staticmethod
+    def backward(ctx, grad_patterns, _grad_ckpt):
+        slabs, probe_k, props, ckpt = ctx.saved_tensors
+        n, s_count = probe_k.shape[-1], slabs.shape[1]
+        base = engine.arena_base(slabs.device)
+        gi = grad_patterns.to(torch.float32).contiguous()
+        grad_slabs = torch.zeros_like(slabs)
+        grad_probe = torch.zeros_like(probe_k)
+        descs = _wave_descs(n, slabs, probe_k, props, ctx.group, gi, base, grad_slabs, grad_probe)
+        nc = ckpt.shape[1] if ckpt.ndim == 4 else 0
+        engine.adjoint_batched(n, s_count, descs, base, ckpt if nc else None, ctx.ckpt_every)
+        return grad_slabs, grad_probe, None, None, None
+
+
+def batched_wave_patterns(slabs: torch.Tensor, probe_k: torch.Tensor, props: torch.Tensor, group,
+                          ckpt_every: int = DEFAULT_CKPT_EVERY) -> torch.Tensor:
+    """(W, n, n) float32 intensities of the waves (see ``BatchedWavesFunction``)."""
+    return BatchedWavesFunction.apply(slabs, probe_k, props, group, int(ckpt_every))[0]

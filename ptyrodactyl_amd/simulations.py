@@ -194,14 +194,18 @@ def cbed_image(pot_slices: PotentialSlices, beam: ProbeModes, microscope: Micros
 
     With ensemble axes on ``microscope.ensemble`` the pattern is the reduction of every sample's amplitude
     (``bind_cbed_axes`` + ``apply_distributions``, float64 result): all samples run in one batched HIP
-    launch, or sample by sample through the differentiable :func:`bind_cbed_axes` when an input requires
-    grad."""
+    launch.  When an input requires grad, the batched launch keeps the checkpoints and one batched adjoint
+    returns the gradients (square power-of-two grids up to 512, incoherent axes); other grids and coherent axes
+    go sample by sample through the differentiable :func:`bind_cbed_axes`."""
     dev = _device_for(pot_slices.slices, beam.modes)
     if not microscope.ensemble.is_empty():
         calib = min(float(pot_slices.calib), float(beam.calib))
         n = beam.modes.shape[0]
-        if torch.is_grad_enabled() and (pot_slices.slices.requires_grad or beam.modes.requires_grad
-                                        or _ensemble_requires_grad(microscope.ensemble)):
+        needs_grad = torch.is_grad_enabled() and (pot_slices.slices.requires_grad or beam.modes.requires_grad
+                                                  or _ensemble_requires_grad(microscope.ensemble))
+        if needs_grad and _batched_gradient_ok(pot_slices, beam, microscope):
+            pattern = _ensemble_patterns(pot_slices, beam, microscope, None, calib, dev, grad=True)[0]
+        elif needs_grad:
             from .types import create_detector_config
             axes = _ensemble_axes_for_cbed(microscope.ensemble, beam)
             det = create_detector_config(real_space_calib_ang=calib, probe_calibration_pm=calib * 100.0)
@@ -236,8 +240,14 @@ def stem_4d(pot_slice: PotentialSlices, beam: ProbeModes, microscope: Microscope
     if not microscope.ensemble.is_empty():
         n = beam.modes.shape[0]
         calib = min(float(pot_slice.calib), float(beam.calib))
-        if torch.is_grad_enabled() and (pot_slice.slices.requires_grad or beam.modes.requires_grad
-                                        or positions.requires_grad or _ensemble_requires_grad(microscope.ensemble)):
+        needs_grad = torch.is_grad_enabled() and (pot_slice.slices.requires_grad or beam.modes.requires_grad
+                                                  or positions.requires_grad
+                                                  or _ensemble_requires_grad(microscope.ensemble))
+        if needs_grad and _batched_gradient_ok(pot_slice, beam, microscope):
+            # differentiable: every (position, sample) wave in batched launches with one batched adjoint
+            patterns = _ensemble_patterns(pot_slice, beam, microscope, positions, detector.real_space_calib_ang,
+                                          dev, grad=True)
+        elif needs_grad:
             # differentiable: per position, the shifted modes through the ensemble cbed_image (its per-sample
             # binder), as the reference maps cbed_image over the positions (simulations.py:1395-1427)
             modes = beam.modes if beam.modes.ndim == 3 else beam.modes[..., None]
@@ -265,7 +275,7 @@ def stem_4d(pot_slice: PotentialSlices, beam: ProbeModes, microscope: Microscope
 def probe_modes_to_distribution(probe: ProbeModes) -> Distribution:
     """Incoherent axis over the mode indices with the probe's weights (``simulations.py:682-711``)."""
     m = probe.modes.shape[2] if probe.modes.ndim == 3 else 1
-    return create_distribution(torch.arange(m, dtype=torch.float64)[:, None], probe.weights.detach().cpu(),
+    return create_distribution(torch.arange(m, dtype=torch.float64)[:, None], probe.weights.cpu(),
                                ReductionMode.INCOHERENT, AXIS_PROBE_MODES)
 
 
@@ -422,8 +432,18 @@ _ENSEMBLE_CHUNK_BYTES = 4 << 30
 _DESC_DTYPE = np.dtype(_native.POSITION_DESC_FIELDS)
 
 
+def _batched_gradient_ok(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig) -> bool:
+    """Whether the differentiable ensemble can run as one batched launch per chunk: a square power-of-two grid of
+    the batched kernels (n <= 512) and incoherent axes only (the batched adjoint is seeded by dL/dI)."""
+    h, w = _grid_shape(beam.modes.shape[0], beam.modes.shape[1])
+    if not (engine.is_fast_grid(h, w) and h <= 512):
+        return False
+    axes = _ensemble_axes_for_cbed(microscope.ensemble, beam)
+    return not any(a.reduction is ReductionMode.COHERENT for a in axes)
+
+
 def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope: MicroscopeConfig,
-                       positions: Optional[torch.Tensor], det_calib: float, dev) -> torch.Tensor:
+                       positions: Optional[torch.Tensor], det_calib: float, dev, grad: bool = False) -> torch.Tensor:
     """(P, h, w) float64 ensemble-reduced patterns; P = 1 without positions (``cbed_image``).
 
     Every (position, ensemble sample) pair becomes one wave of a batched HIP launch: its incident wave is
@@ -431,7 +451,12 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
     shift and tilt at the cbed calibration), its transmission and propagator are those of the sample's
     energy.  Incoherent axes are summed from the kernel's intensities, coherent axes from its amplitudes.
     Grids other than the square powers of two run each energy's waves as the modes of one runtime-size
-    launch (amplitudes per wave)."""
+    launch (amplitudes per wave).
+
+    ``grad``: keep the potential, the modes, the positions and the axes' samples and weights in the autograd
+    graph and run each chunk's waves through ``autograd.BatchedWavesFunction`` (forward with checkpoints, one
+    batched adjoint); only for the grids and axes ``_batched_gradient_ok`` accepts."""
+    from . import autograd
     h, w = _grid_shape(beam.modes.shape[0], beam.modes.shape[1])
     fast = engine.is_fast_grid(h, w)
     if tuple(pot_slices.slices.shape[:2]) != (h, w):
@@ -452,22 +477,27 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
         tilt.append(upd.tilt_delta_mrad)
     dpos = torch.stack(dpos).to(dev)
     tilt = torch.stack(tilt).to(dev)
+    if not grad:
+        dpos, tilt = dpos.detach(), tilt.detach()
     energies = sorted(set(energy))
     group = [energies.index(e) for e in energy]
     coherent = any(a.reduction is ReductionMode.COHERENT for a in axes)
 
     v = pot_slices.slices if pot_slices.slices.ndim == 3 else pot_slices.slices[..., None]
-    v = v.detach().to(device=dev, dtype=torch.float64)
+    v = v.to(device=dev, dtype=torch.float64)
+    if not grad:
+        v = v.detach()
     s_count = v.shape[2]
-    slabs = torch.empty((len(energies), s_count, h, w), dtype=torch.complex64, device=dev)
     props = torch.empty((len(energies), h + w), dtype=torch.complex64, device=dev)
     kvs = [float(microscope.voltage_kv) + e / 1000.0 for e in energies]
+    slab_list = []
     for g, kv in enumerate(kvs):
         t = torch.polar(torch.ones_like(v), v * physics.interaction_sigma(kv))
-        slabs[g] = t.permute(2, 0, 1).to(torch.complex64)
+        slab_list.append(t.permute(2, 0, 1).to(torch.complex64))
         fy, fx = physics.propagator_pair(h, w, pot_slices.slice_thickness, kv, calib, device=dev)
         props[g, :h] = fy
         props[g, h:] = fx
+    slabs = torch.stack(slab_list).contiguous()
     qy = physics.fftfreq(h, calib, device=dev)
     qx = physics.fftfreq(w, calib, device=dev)
     jitter = None
@@ -481,16 +511,20 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
     weights = None
     if not coherent:
         grids = torch.meshgrid(*[torch.arange(c) for c in counts], indexing="ij")
-        weights = torch.ones(k_count, dtype=torch.float64)
+        weights = torch.ones(k_count, dtype=torch.float64, device=dev)
         for a, gi in zip(axes, grids):
-            weights = weights * a.weights.detach().cpu()[gi.reshape(-1)]
-        weights = weights.to(dev)
+            aw = a.weights.to(device=dev, dtype=torch.float64)
+            weights = weights * (aw if grad else aw.detach())[gi.reshape(-1).to(dev)]
 
     modes = beam.modes if beam.modes.ndim == 3 else beam.modes[..., None]
-    modes = modes.detach().to(device=dev, dtype=torch.complex128)
+    modes = modes.to(device=dev, dtype=torch.complex128)
+    if not grad:
+        modes = modes.detach()
     p_count = 1 if positions is None else positions.shape[0]
-    chunk = max(1, min(p_count, _ENSEMBLE_CHUNK_BYTES // (k_count * h * w * 80)))
+    per_wave = h * w * (80 + (8 * ((s_count - 1) // autograd.DEFAULT_CKPT_EVERY) + 24 if grad else 0))
+    chunk = max(1, min(p_count, _ENSEMBLE_CHUNK_BYTES // (k_count * per_wave)))
     out = torch.empty((p_count, h, w), dtype=torch.float64, device=dev)
+    pieces = []
     base = engine.arena_base(dev)
     sel_idx = torch.as_tensor(mode_idx, device=dev)
     # The position shift (shift_beam_fourier at det_calib) and the jitter shift are both Fourier phase ramps, so
@@ -504,7 +538,9 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
         if positions is None:
             kp = modes_k[None]
         else:
-            pp = positions[p0:p0 + pc].detach().to(device=dev, dtype=torch.float64)
+            pp = positions[p0:p0 + pc].to(device=dev, dtype=torch.float64)
+            if not grad:
+                pp = pp.detach()
             phase = -2.0 * math.pi * (qdy[None, :, None] * pp[:, 0, None, None]
                                       + qdx[None, None, :] * pp[:, 1, None, None])
             kp = modes_k[None] * torch.polar(torch.ones_like(phase), phase)[..., None]
@@ -516,6 +552,10 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
         probe_k = waves_k.to(torch.complex64).reshape(pc * k_count, h, w).contiguous()
         del waves_k, kp
         wave_group = np.tile(np.asarray(group, dtype=np.int64), pc)
+        if grad:
+            patterns = autograd.batched_wave_patterns(slabs, probe_k, props, wave_group.tolist())
+            pieces.append(torch.einsum("k,pkhw->phw", weights, patterns.to(torch.float64).reshape(pc, k_count, h, w)))
+            continue
         if fast:
             n = h
             patterns = torch.empty((pc * k_count, n, n), dtype=torch.float32, device=dev)
@@ -549,7 +589,7 @@ def _ensemble_patterns(pot_slices: PotentialSlices, beam: ProbeModes, microscope
         else:
             out[p0:p0 + pc] = torch.einsum("k,pkhw->phw", weights,
                                            patterns.to(torch.float64).reshape(pc, k_count, h, w))
-    return out
+    return torch.cat(pieces) if grad else out
 
 
 def _reduce_one(axes, amplitudes: torch.Tensor) -> torch.Tensor:

@@ -113,7 +113,7 @@ def test_zero_width_axes_equal_plain_path():
 
 
 def test_ensemble_cbed_image_gradient_wrt_potential():
-    """With grad enabled the ensemble runs sample by sample through the differentiable binder."""
+    """With grad enabled the ensemble runs its waves in one batched launch and one batched adjoint."""
     import ptyrodactyl_amd as pa
     from ptyrodactyl_amd import producers
     pot, modes, w, calib, kv = _case(seed=5, modes=1)
@@ -197,8 +197,8 @@ def test_cbed_image_gradient_wrt_jitter_width():
 
 
 def test_stem4d_ensemble_gradient_wrt_positions_and_potential():
-    """stem_4d with a jitter axis and grad enabled: per position through the differentiable ensemble cbed_image;
-    the loss and its gradients agree with the batched forward and with oracle central differences."""
+    """stem_4d with a jitter axis and grad enabled (batched forward with checkpoints, batched adjoint): the loss
+    and its gradients agree with the no-grad forward and with oracle central differences."""
     import ptyrodactyl_amd as pa
     from ptyrodactyl_amd import producers
     pot, modes, w, calib, kv = _case(seed=9, modes=1)
@@ -236,3 +236,48 @@ def test_stem4d_ensemble_gradient_wrt_positions_and_potential():
     pm[idx] -= 1e-3
     fd = (oracle_loss(pp, pos0) - oracle_loss(pm, pos0)) / 2e-3
     assert abs(float(g_v[idx]) - fd) <= 1e-2 * abs(fd) + 1e-3 * float(g_v.abs().max())
+
+
+def test_batched_ensemble_gradients_equal_the_per_sample_binder(monkeypatch):
+    """The batched differentiable path (BatchedWavesFunction) against the per-sample binder (one launch
+    sequence per (position, sample)) for the same stem_4d loss: gradients w.r.t. the potential, the modes, the
+    mode weights, the scan positions and the jitter width, with two modes, a 2 x 2 jitter and 2^3 samples of
+    partial coherence (energy and tilts).  rel-L2 1e-5 (the batched adjoint adds waves with float32 atomics);
+    5e-5 for the position and width gradients, and the weight gradient on the loss's scale (see below)."""
+    import ptyrodactyl_amd as pa
+    from ptyrodactyl_amd import producers, simulations
+    pot, modes, w, calib, kv = _case(seed=12, modes=2)
+    rng = np.random.default_rng(13)
+    pos0 = np.array([[0.4, -0.3], [1.1, 0.8], [-0.6, 0.2]])
+    g = torch.tensor(rng.standard_normal((3, 32, 32)), device=DEV)
+
+    def run():
+        v = torch.tensor(pot, device=DEV, requires_grad=True)
+        m = torch.tensor(modes, device=DEV, requires_grad=True)
+        wt = torch.tensor(w, device=DEV, requires_grad=True)
+        pos = torch.tensor(pos0, device=DEV, requires_grad=True)
+        sigma = torch.tensor(0.25, dtype=torch.float64, requires_grad=True)
+        ens = pa.create_ensemble_axes(position_jitter=producers.position_jitter_to_distribution(sigma, 2),
+                                      coherence=producers.coherence_to_distribution(0.8, 0.3, 2))
+        mic = pa.create_microscope_config(kv, 22.0, ensemble=ens)
+        det = pa.create_detector_config(real_space_calib_ang=calib, scan_positions_px=pos)
+        out = pa.stem_4d(pa.create_potential_slices(v, 1.2, calib), pa.ProbeModes(m, wt, calib), mic, det)
+        loss = torch.sum(out.data.double() * g)
+        grads = torch.autograd.grad(loss, (v, m, wt, pos, sigma))
+        return float(loss), [t.detach().cpu().numpy() for t in (out.data, *grads)]
+
+    loss_b, batched = run()
+    monkeypatch.setattr(simulations, "_batched_gradient_ok", lambda *a: False)
+    loss_p, per_sample = run()
+    assert loss_b == pytest.approx(loss_p, rel=1e-6)
+    for name, a, b in zip(("patterns", "potential", "modes", "weights", "positions", "sigma"), batched, per_sample):
+        err = rel_l2(a, b)
+        print(f"batched vs per-sample {name}: rel-L2 {err:.2e}", flush=True)
+        if name == "weights":
+            # dL/dw_m = (L_m - L) / sum(w) subtracts two nearly equal float32 sums: compare on the loss's scale
+            assert np.abs(a - b).max() < 1e-5 * abs(loss_p) / min(w), name
+        elif name in ("positions", "sigma"):
+            # sums of phase-ramp derivatives whose terms cancel; the oracle FD tests above hold them to 1e-3
+            assert err < 5e-5, name
+        else:
+            assert err < 1e-5, name

@@ -185,6 +185,34 @@ def gradient_bench(steps: int, warmup: int):
                          "formula": "8 n^2 (4 S + 2 floor((S-1)/k) + 2) + 8 n^2"}}
 
 
+def gradient256_bench(steps: int, warmup: int):
+    """The request loop's largest request as a workload of its own: a gradient at 256^2 windows, 32 slices, 4096
+    positions over a 512^2 synthetic crystal slab (the serving input builder, seed 11): checkpointed forward, the
+    sqrt-intensity loss gradient and the adjoint w.r.t. the slab and the probe, through the drop-in autograd path."""
+    import torch
+    from ptyrodactyl_amd import serving
+    ex = serving.PlainExecutor()
+    req = serving.Request(0, "gradient", 256, 32, 4096, seed=11)
+    inp = serving.build_inputs(req, ex.device)
+    for _ in range(warmup):
+        ex.run_one(req, inp)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * steps)]
+    for i in range(steps):
+        ev[2 * i].record()
+        ex.run_one(req, inp)
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    ms = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)]))
+    b = gradient_algorithmic_bytes(256, 32, serving.CKPT_EVERY)
+    achieved = b * req.positions / (ms * 1e-3) / 1e9
+    return {"workload": "gradient_256x32_p4096_slab512_ckpt4", "patterns_per_s": round(req.positions / (ms * 1e-3), 1),
+            "ms_per_step": round(ms, 3), "steps": steps,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_pattern": b,
+                         "formula": "8 n^2 (4 S + 2 floor((S-1)/k) + 2) + 8 n^2"}}
+
+
 def serving_bench(count: int, qps: float):
     """Mixed request stream (70 % forward / 30 % gradient, n in {64,128,256}, S in {8,16,32,64},
     P in {16,256,1024,4096}, Poisson arrivals): continuous batching vs one request at a time.
@@ -403,7 +431,8 @@ def main() -> int:
     ap.add_argument("--only-forward", action="store_true",
                     help="time the headline forward only (no detector, graph, gradient, potential, serving or CPU "
                          "legs): the run rocprofv3 counter passes profile")
-    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "membw", "potential", "grid",
+    ap.add_argument("--workload", choices=("config2", "config1", "serving", "gradient", "gradient256", "membw",
+                                           "potential", "grid",
                                            "calib"),
                     default="config2",
                     help="config2: 256^2 x 64 slices from a 1024^2 slab, 4096 positions (the headline); "
@@ -448,6 +477,12 @@ def main() -> int:
                           "higher_is_better": True, "membw": res}), flush=True)
         return 0
 
+    if args.workload == "gradient256":
+        res = gradient256_bench(args.steps, args.warmup)
+        print(json.dumps({"metric": "gradient patterns/s at 256^2 x 32 slices", "value": res["patterns_per_s"],
+                          "unit": "patterns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": res["ms_per_step"], "higher_is_better": True, "gradient": res}), flush=True)
+        return 0
     if args.workload == "gradient":
         res = gradient_bench(args.steps, args.warmup)
         print(json.dumps({"metric": "gradient patterns/s at 128^2 x 16 slices", "value": res["patterns_per_s"],

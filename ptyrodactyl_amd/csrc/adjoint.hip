@@ -462,6 +462,7 @@ __device__ __forceinline__ int frame_offset(int i, int o, int m) {
 template <int N, int TH>
 __global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, int cnt, int s_first) {
   constexpr int RPT = TH / 8;  // rows per thread: 8 rows per sweep of the 256 threads
+  constexpr int HU = 16 / RPT;  // hits loaded together: sixteen loads in flight per thread
   __shared__ int3 hits[256];   // (wave index in the chunk, oy, ox) of the windows meeting this tile
   __shared__ int wave_hits[4];
   const int sb = blockIdx.z;
@@ -501,15 +502,15 @@ __global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, 
     }
     if (hit) hits[offset + before] = make_int3(i, oy, ox);
     __syncthreads();
-    // 2. ascending wave order over the hits: a fixed summation order.  Four hits at a time: all their
+    // 2. ascending wave order over the hits: a fixed summation order.  HU hits at a time: all their
     // loads are issued first (clamped to in-window addresses, always in bounds) and the values are
     // then added through selects, so the loads stay unconditional and overlap.
     if (tx < xlen) {
-      for (int h0 = 0; h0 < total; h0 += 4) {
-        float2 val[4][RPT];
-        bool use[4][RPT];
+      for (int h0 = 0; h0 < total; h0 += HU) {
+        float2 val[HU][RPT];
+        bool use[HU][RPT];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < HU; ++u) {
           const int3 e = hits[min(h0 + u, total - 1)];
           const int dx = frame_offset(X, e.z, W);
           const float2* cw = a.gt_contrib + ((size_t)e.x * a.contrib_slices + sb) * N * N + min(dx, N - 1);
@@ -522,7 +523,7 @@ __global__ __launch_bounds__(256) void k_grad_reduce(AdjParams a, long long w0, 
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < HU; ++u)
 #pragma unroll
           for (int r = 0; r < RPT; ++r)
             acc[r] = cadd(acc[r], use[u][r] ? val[u][r] : make_float2(0.f, 0.f));

@@ -71,7 +71,9 @@ int pty_forward(int n, int num_slices, long long num_positions, int num_modes, c
  * Same multislice as pty_forward, with the patterns and/or the amplitudes
  * (each may be NULL, not both); the reference has no checkpointing (XLA
  * keeps every slice's residuals under jax.value_and_grad,
- * src/ptyrodactyl/multislice/multislice_recon.py:1915-1951).
+ * src/ptyrodactyl/multislice/multislice_recon.py:1915-1951).  With k = 1 and
+ * n >= 256 the checkpoints are the forward's own wave buffers (the column pass
+ * writes into them), so storing every slice adds no memory traffic.
  */
 int pty_forward_checkpointed(int n, int num_slices, long long num_positions, int num_modes, const void* slab,
                              long long slab_h, long long slab_w, const void* probe_k, const float* mode_weights,

@@ -19,12 +19,31 @@ are piecewise constant and get no gradient.
 from __future__ import annotations
 
 import contextlib
+import os
+from typing import Optional
 
 import torch
 
 from . import engine
 
 DEFAULT_CKPT_EVERY = 4
+# Device bytes a checkpoint at every slice may take before the differentiable forward falls back to every 4th.
+CHECKPOINT_STORE_BYTES = int(os.environ.get("PTY_CHECKPOINT_STORE_BYTES", str(64 << 30)))
+
+
+def checkpoint_interval(slab_t: torch.Tensor, probe_k: torch.Tensor, origins: torch.Tensor) -> int:
+    """Checkpoint interval of the differentiable forward when the caller gives none.  Square power-of-two windows
+    of 256 and more (the streaming kernels) checkpoint every slice when the P·M·(S−1) stored waves fit
+    ``CHECKPOINT_STORE_BYTES``: their column pass then writes each wave straight into its slot, and the adjoint
+    recomputes nothing (256² × 32 slices × 4096 positions: 163 ms against 190 ms at 4, for 62 GiB of store).
+    Otherwise, and for the LDS-resident windows (where every 4th costs the same time), every 4th slice."""
+    h, w = int(probe_k.shape[-2]), int(probe_k.shape[-1])
+    s_count = int(slab_t.shape[0])
+    waves = int(origins.shape[0]) * (int(probe_k.shape[0]) if probe_k.dim() == 3 else 1)
+    if h == w and h >= 256 and engine.is_fast_grid(h, w) and s_count > 1:
+        if waves * (s_count - 1) * h * w * 8 <= CHECKPOINT_STORE_BYTES:
+            return 1
+    return DEFAULT_CKPT_EVERY
 
 
 def _pair(prop):
@@ -234,18 +253,24 @@ class MultisliceAmplitudeFunction(torch.autograd.Function):
 
 
 def multislice_patterns(slab_t: torch.Tensor, probe_k: torch.Tensor, weights: torch.Tensor, origins: torch.Tensor,
-                        shifts_px: torch.Tensor, prop, ckpt_every: int = DEFAULT_CKPT_EVERY):
-    """Differentiable (P, n, n) float32 patterns of the windowed multislice (reverse and forward mode)."""
+                        shifts_px: torch.Tensor, prop, ckpt_every: Optional[int] = None):
+    """Differentiable (P, n, n) float32 patterns of the windowed multislice (reverse and forward mode); checkpoints
+    every ``ckpt_every`` slices (default ``checkpoint_interval``)."""
     prop_y, prop_x = _pair(prop)
+    if ckpt_every is None:
+        ckpt_every = checkpoint_interval(slab_t, probe_k, origins)
     patterns, _ = MultisliceFunction.apply(slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x,
                                            int(ckpt_every), torch.is_grad_enabled())
     return patterns
 
 
 def multislice_amplitudes(slab_t: torch.Tensor, probe_k: torch.Tensor, origins: torch.Tensor,
-                          shifts_px: torch.Tensor, prop, ckpt_every: int = DEFAULT_CKPT_EVERY):
-    """Differentiable (P, M, n, n) complex64 fftshifted amplitudes of the windowed multislice."""
+                          shifts_px: torch.Tensor, prop, ckpt_every: Optional[int] = None):
+    """Differentiable (P, M, n, n) complex64 fftshifted amplitudes of the windowed multislice (checkpoint
+    interval as for ``multislice_patterns``)."""
     prop_y, prop_x = _pair(prop)
+    if ckpt_every is None:
+        ckpt_every = checkpoint_interval(slab_t, probe_k, origins)
     amp, _ = MultisliceAmplitudeFunction.apply(slab_t, probe_k, origins, shifts_px, prop_y, prop_x,
                                                int(ckpt_every), torch.is_grad_enabled())
     return amp

@@ -198,8 +198,11 @@ static size_t resident_lds_bytes() {
 
 enum : int { ROW_FIRST = 1, ROW_LAST = 2 };
 
+// The row pass reads wave wl from src + wl * src_stride (the batch's workspace, or with checkpoints at every slice
+// the checkpoint store) and writes it to the workspace.
 template <int N, int FLAGS>
-__global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0, int nb, int s) {
+__global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, const float2* src, long long src_stride,
+                                              int w0, int nb, int s) {
   constexpr int T = N / 16;
   constexpr int RPB = 256 / T;  // rows per workgroup
   constexpr int RS = N + N / 16;
@@ -226,10 +229,12 @@ __global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, int w0
   __syncthreads();
 
   float2* wrow = waves + ((size_t)wl * N + y) * N;
+  const float2* srow = src + (size_t)wl * src_stride + (size_t)y * N;
   float2 v[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = wrow[t + T * q];
-  if (a.ckpt && s > 0 && s % a.ckpt_every == 0) {  // checkpoint the wave entering slice s
+  for (int q = 0; q < 16; ++q) v[q] = srow[t + T * q];
+  // checkpoint the wave entering slice s (every slice: the column pass wrote it to its store slot already)
+  if (a.ckpt && a.ckpt_every > 1 && s > 0 && s % a.ckpt_every == 0) {
     const int nc = num_checkpoints(a.S, a.ckpt_every);
     float2* ck = a.ckpt + ((size_t)w * nc + (s / a.ckpt_every - 1)) * N * N + (size_t)y * N;
 #pragma unroll
@@ -264,7 +269,8 @@ struct ColBufBlock {
 enum : int { COL_INIT = 0, COL_MID = 1, COL_FINAL = 2, COL_FINAL_DET = 3 };
 
 template <int N, int MODE>
-__global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0, int nb) {
+__global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, float2* dst, long long dst_stride, int w0,
+                                              int nb) {
   constexpr int T = N / 16;
   constexpr int NC = 256 / T;  // columns per workgroup
   constexpr int PS = N + N / 16;
@@ -308,8 +314,9 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
     float2* wcol = waves + (size_t)unit * N * N + x;
 #pragma unroll
     for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
-  } else if constexpr (MODE == COL_MID) {
-    float2* wcol = waves + (size_t)unit * N * N + x;
+  } else if constexpr (MODE == COL_MID) {  // reads the workspace, writes dst (the workspace or a store slot)
+    const float2* wcol = waves + (size_t)unit * N * N + x;
+    float2* dcol = dst + (size_t)unit * dst_stride + x;
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, int w0
     for (int q = 0; q < 16; ++q) v[q] = pyt.mul(v[q], t + T * q);
     line_fft<N, true>(v, t, cb, twt);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
+    for (int q = 0; q < 16; ++q) dcol[(size_t)(t + T * q) * N] = v[q];
   } else {
     const int pl = unit;  // position inside the batch
     const int p = pos;
@@ -416,25 +423,36 @@ static hipError_t launch_streaming_batch(const FwdParams& a, float2* ws, long lo
   constexpr int RPB = 256 / T;
   const int nb = np * a.M;
   const int w0 = (int)(p0 * a.M);
-  hipLaunchKernelGGL((k_cols<N, COL_INIT>), column_grid<N>(nb), dim3(256), 0, st, a, ws, w0, nb);
+  const long long NN = (long long)N * N;
+  // With a checkpoint at every slice the column pass writes the wave entering slice s + 1 straight into its
+  // checkpoint slot and the next row pass reads it from there: the checkpoints cost no extra traffic.
+  const bool store = a.ckpt && a.ckpt_every == 1;
+  const int nc = store ? num_checkpoints(a.S, 1) : 0;
+  auto slot = [&](int c) { return a.ckpt + ((size_t)w0 * nc + c) * NN; };
+  hipLaunchKernelGGL((k_cols<N, COL_INIT>), column_grid<N>(nb), dim3(256), 0, st, a, ws, ws, NN, w0, nb);
   for (int s = 0; s < a.S; ++s) {
     const bool first = s == 0, last = s == a.S - 1;
     const dim3 grid(nb, N / RPB);
+    const float2* src = store && !first ? slot(s - 1) : ws;
+    const long long src_stride = store && !first ? nc * NN : NN;
     if (first && last)
-      hipLaunchKernelGGL((k_rows<N, ROW_FIRST | ROW_LAST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
+      hipLaunchKernelGGL((k_rows<N, ROW_FIRST | ROW_LAST>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb,
+                         s);
     else if (first)
-      hipLaunchKernelGGL((k_rows<N, ROW_FIRST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
+      hipLaunchKernelGGL((k_rows<N, ROW_FIRST>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb, s);
     else if (last)
-      hipLaunchKernelGGL((k_rows<N, ROW_LAST>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
+      hipLaunchKernelGGL((k_rows<N, ROW_LAST>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb, s);
     else
-      hipLaunchKernelGGL((k_rows<N, 0>), grid, dim3(256), 0, st, a, ws, w0, nb, s);
-    if (!last) hipLaunchKernelGGL((k_cols<N, COL_MID>), column_grid<N>(nb), dim3(256), 0, st, a, ws, w0, nb);
+      hipLaunchKernelGGL((k_rows<N, 0>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb, s);
+    if (!last)
+      hipLaunchKernelGGL((k_cols<N, COL_MID>), column_grid<N>(nb), dim3(256), 0, st, a, ws,
+                         store ? slot(s) : ws, store ? nc * NN : NN, w0, nb);
   }
   if (a.det_count > 0) {
-    hipLaunchKernelGGL((k_cols<N, COL_FINAL_DET>), column_grid<N>(np), dim3(256), 0, st, a, ws, w0, np);
+    hipLaunchKernelGGL((k_cols<N, COL_FINAL_DET>), column_grid<N>(np), dim3(256), 0, st, a, ws, ws, NN, w0, np);
     hipLaunchKernelGGL(k_det_finalize<N>, dim3((np * a.det_count + 255) / 256), dim3(256), 0, st, a, ws, p0, np);
   } else {
-    hipLaunchKernelGGL((k_cols<N, COL_FINAL>), column_grid<N>(np), dim3(256), 0, st, a, ws, w0, np);
+    hipLaunchKernelGGL((k_cols<N, COL_FINAL>), column_grid<N>(np), dim3(256), 0, st, a, ws, ws, NN, w0, np);
   }
   return hipGetLastError();
 }

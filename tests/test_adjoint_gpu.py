@@ -76,6 +76,7 @@ def _oracle_grads(slab, probes, origins, shifts, g, w, n, dz=1.5, kv=300.0, cali
     (256, 5, 2, 3, (256, 280), 1),
     (256, 3, 8, 2, (300, 256), 2),
     (512, 2, 1, 2, (512, 512), 1),
+    (256, 6, 1, 3, (280, 256), 2),
 ])
 def test_adjoint_matches_oracle_vjp(n, s_count, k, p_count, slab_hw, modes):
     print(f"adjoint n={n} S={s_count} k={k} P={p_count} M={modes}", flush=True)
@@ -93,6 +94,31 @@ def test_checkpoint_interval_does_not_change_gradient():
     _, a, pa, _ = _gpu_grads(slab, probes, origins, shifts, g, n, 1)
     _, b, pb, _ = _gpu_grads(slab, probes, origins, shifts, g, n, 8)
     assert rel_l2(a, b) < 1e-6 and rel_l2(pa, pb) < 1e-6
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_checkpoint_store_at_every_slice(n):
+    """With a checkpoint at every slice the streaming column pass writes each wave into its slot and the next row
+    pass reads it there: the patterns equal the plain forward bitwise, and slot 3 (the wave entering slice 4)
+    equals slot 0 of a forward that checkpoints every 4th slice."""
+    from ptyrodactyl_amd import engine
+    print(f"checkpoint store n={n}", flush=True)
+    s_count = 9
+    slab, probes, origins, shifts, _ = _case(n, s_count, 5, (n + 40, n + 24), seed=n + 3, modes=1)
+    dev = torch.device("cuda", 0)
+    sl = torch.tensor(slab, device=dev)
+    pk = torch.fft.fft2(torch.tensor(probes, device=dev)).to(torch.complex64).contiguous()
+    w = torch.ones(1, dtype=torch.float32, device=dev)
+    org = torch.tensor(origins, dtype=torch.int32, device=dev)
+    sh = torch.tensor(shifts, device=dev)
+    from ptyrodactyl_amd import physics
+    prop = physics.propagator_factors(n, 1.2, 200.0, 0.2, device=dev)
+    plain, _ = engine.multislice_forward(sl, pk, w, org, sh, prop)
+    p1, ck1 = engine.multislice_forward_checkpointed(sl, pk, w, org, sh, prop, 1)
+    p4, ck4 = engine.multislice_forward_checkpointed(sl, pk, w, org, sh, prop, 4)
+    assert ck1.shape[1] == s_count - 1 and ck4.shape[1] == 2
+    assert torch.equal(p1, plain) and torch.equal(p4, plain)
+    assert torch.equal(ck1[:, 3], ck4[:, 0]) and torch.equal(ck1[:, 7], ck4[:, 1])
 
 
 def test_autograd_function_matches_oracle():

@@ -12,18 +12,19 @@
 // apply_distribution (multislice/reduce.py:81-86).
 //
 // The Fresnel propagator exp(-i pi lambda dz (qx^2 + qy^2)) factors into
-// Py(qy) Px(qx), so fft2 -> *P -> ifft2 is applied as a row pass
-// (fft_x, *Px, ifft_x) followed by a column pass (fft_y, *Py, ifft_y); the
-// transmission multiply is fused into the row pass and |.|^2 into the last
-// column pass.
+// Py(qy) Px(qx), so fft2 -> *P -> ifft2 is applied as a row operator
+// (fft_x, *Px, ifft_x) and a column operator (fft_y, *Py, ifft_y).
 //
 // Two execution strategies:
 //  * n <= 128 (32, 64, 128): one workgroup per scan position keeps the whole wave resident
 //    in LDS across all slices (128^2 c64 = 128 KiB of the 160 KiB LDS) and
-//    streams only the transmission window from HBM.
-//  * n >= 256: the wave exceeds LDS, so each slice is one row-pass launch
-//    and one column-pass launch over a batch of waves held in a device
-//    workspace sized to stay in the L2/Infinity-Cache working set.
+//    streams only the transmission window from HBM; the transmission multiply
+//    is fused into the row phase and |.|^2 into the last column phase.
+//  * n >= 256: the wave exceeds LDS, so the slices run as passes over a batch
+//    of waves held in a device workspace sized to stay in the L2/Infinity-Cache
+//    working set.  The passes alternate axis and each applies two slices'
+//    operators along its axis, so a slice costs one trip of the wave through
+//    memory.
 #include <stdlib.h>
 
 #include "common.hpp"
@@ -193,63 +194,115 @@ static size_t resident_lds_bytes() {
 }
 
 // ---------------------------------------------------------------------------
-// Streaming row / column passes (n = 256 ... 4096)
+// Streaming passes (n = 256 ... 4096)
 // ---------------------------------------------------------------------------
+//
+// The propagator's row operator Rx = ifft_x Px fft_x and column operator Cy = ifft_y Py fft_y act on different
+// axes, so they commute and slice s may apply them in either order: psi_{s+1} = Rx Cy (psi_s T_s) = Cy Rx (psi_s T_s).
+// The passes therefore alternate axis, and each pass applies everything along its own axis between two trips of
+// the wave through memory: the operator that completes slice s - 1, the transmission T_s, and the first operator
+// of slice s.  A slice costs one read and one write of the wave instead of two of each.
+//
+//   pass 0            probe_k x shift ramp, inverse transform along the first axis
+//   pass 1 (s = 0)    inverse transform along this axis (psi_0 complete), x T_0, first operator of slice 0
+//   pass s + 1        second operator of slice s - 1 (psi_s complete), checkpoint, x T_s, first operator of slice s
+//   last slice        ... x T_{S-1}, forward transform along this axis (no propagation after the last slice)
+//   pass S + 1        forward transform along columns, |.|^2 and the fftshift store
+//
+// The first axis is chosen from the parity of S so that pass S + 1 always runs on columns.
 
-enum : int { ROW_FIRST = 1, ROW_LAST = 2 };
+enum : int { PH_INIT = 1, PH_FIRST = 2, PH_START = 4 };
 
-// The row pass reads wave wl from src + wl * src_stride (the batch's workspace, or with checkpoints at every slice
-// the checkpoint store) and writes it to the workspace.
-template <int N, int FLAGS>
-__global__ __launch_bounds__(256) void k_rows(FwdParams a, float2* waves, const float2* src, long long src_stride,
-                                              int w0, int nb, int s) {
+// Window column x of slice s times the 16 points a thread holds of it (rows t + (N / 16) q): a window whose rows
+// do not wrap (uniform per workgroup) reads from one base with a fixed stride.
+template <int N>
+__device__ __forceinline__ void multiply_window_col(float2 (&v)[16], const WaveCtx& g, int s, int x, int t) {
+  constexpr int T = N / 16;
+  int gx = g.ox + x;
+  if (gx >= g.slab_w) gx -= (int)g.slab_w;
+  const float2* col = g.slab + (long long)s * g.stride + gx;
+  const long long w = g.slab_w;
+  if (g.oy + N <= g.slab_h) {
+    const float2* c = col + (long long)(g.oy + t) * w;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], c[(long long)T * q * w]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      int gy = g.oy + t + T * q;
+      if (gy >= g.slab_h) gy -= (int)g.slab_h;
+      v[q] = cmul(v[q], col[(long long)gy * w]);
+    }
+  }
+}
+
+// Row pass: each group of N / 16 threads holds one row of a wave (v[q] = row[t + (N / 16) q]).
+template <int N, int F>
+__global__ __launch_bounds__(256) void k_row_phase(FwdParams a, float2* waves, int w0, int s) {
   constexpr int T = N / 16;
   constexpr int RPB = 256 / T;  // rows per workgroup
   constexpr int RS = N + N / 16;
+  constexpr bool INIT = F & PH_INIT, FIRST = F & PH_FIRST, START = F & PH_START;
+  constexpr bool PROP = !INIT && (!FIRST || START);
   __shared__ float2 rows[RPB * RS];
   constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 tw[twiddle_entries<N>()];
-  __shared__ float2 px[N];
+  __shared__ float2 px[PROP ? N : 1];
   __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
-  __shared__ float2 pxf[DF ? N : 1];
+  __shared__ float2 pxf[DF && PROP ? N : 1];
   const int tid = threadIdx.x;
   // grid (nb, N / RPB): consecutive blocks -> different waves (XCD round robin)
   const int wl = blockIdx.x;
-  const int rblk = blockIdx.y;
   const int w = w0 + wl;
   const int p = a.M == 1 ? w : w / a.M;
   const int t = tid % T, lr = tid / T;
-  const int y = rblk * RPB + lr;
+  const int y = blockIdx.y * RPB + lr;
   const WaveCtx g = resolve_wave(a, p);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
-  if (!(FLAGS & ROW_LAST)) stage_factors<DF>(px, pxf, g.prop_x, N, tid, 256);
+  if constexpr (PROP) stage_factors<DF>(px, pxf, g.prop_x, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
   const FixTab<DF> pxt{px, pxf};
   const typename RowBufFor<(T <= 64)>::type rb{rows + lr * RS};
   __syncthreads();
 
   float2* wrow = waves + ((size_t)wl * N + y) * N;
-  const float2* srow = src + (size_t)wl * src_stride + (size_t)y * N;
   float2 v[16];
+  if constexpr (INIT) {  // probe_k row * shift ramp, inverse FFT along x
+    const float2 ry = shift_ramp(y, N, g.my, g.dy);
+    const float2* pk = g.probe_k + (size_t)(w % a.M) * N * N + (size_t)y * N;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = srow[t + T * q];
-  // checkpoint the wave entering slice s (every slice: the column pass wrote it to its store slot already)
-  if (a.ckpt && a.ckpt_every > 1 && s > 0 && s % a.ckpt_every == 0) {
-    const int nc = num_checkpoints(a.S, a.ckpt_every);
-    float2* ck = a.ckpt + ((size_t)w * nc + (s / a.ckpt_every - 1)) * N * N + (size_t)y * N;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) ck[t + T * q] = v[q];
-  }
-  if (FLAGS & ROW_FIRST) line_fft<N, true>(v, t, rb, twt);
-  int gy = g.oy + y;
-  if (gy >= g.slab_h) gy -= (int)g.slab_h;
-  const float2* trow = g.slab + (long long)s * g.stride + (long long)gy * g.slab_w;
-  multiply_window_row<N>(v, trow, g.ox, t, (int)g.slab_w);
-  line_fft<N, false>(v, t, rb, twt);
-  if (!(FLAGS & ROW_LAST)) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = pxt.mul(v[q], t + T * q);
+    for (int q = 0; q < 16; ++q) {
+      const int kx = t + T * q;
+      v[q] = cmul(cmul(pk[kx], shift_ramp(kx, N, g.mx, g.dx)), ry);
+    }
     line_fft<N, true>(v, t, rb, twt);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = wrow[t + T * q];
+    if constexpr (FIRST) {
+      line_fft<N, true>(v, t, rb, twt);  // completes psi_0
+    } else {  // second operator of slice s - 1
+      line_fft<N, false>(v, t, rb, twt);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = pxt.mul(v[q], t + T * q);
+      line_fft<N, true>(v, t, rb, twt);
+      if (a.ckpt && s % a.ckpt_every == 0) {  // checkpoint the wave entering slice s
+        const int nc = num_checkpoints(a.S, a.ckpt_every);
+        float2* ck = a.ckpt + ((size_t)w * nc + (s / a.ckpt_every - 1)) * N * N + (size_t)y * N;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) ck[t + T * q] = v[q];
+      }
+    }
+    int gy = g.oy + y;
+    if (gy >= g.slab_h) gy -= (int)g.slab_h;
+    const float2* trow = g.slab + (long long)s * g.stride + (long long)gy * g.slab_w;
+    multiply_window_row<N>(v, trow, g.ox, t, (int)g.slab_w);
+    line_fft<N, false>(v, t, rb, twt);
+    if constexpr (START) {  // first operator of slice s
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = pxt.mul(v[q], t + T * q);
+      line_fft<N, true>(v, t, rb, twt);
+    }
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q) wrow[t + T * q] = v[q];
@@ -266,106 +319,145 @@ struct ColBufBlock {
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
-enum : int { COL_INIT = 0, COL_MID = 1, COL_FINAL = 2, COL_FINAL_DET = 3 };
-
-template <int N, int MODE>
-__global__ __launch_bounds__(256) void k_cols(FwdParams a, float2* waves, float2* dst, long long dst_stride, int w0,
-                                              int nb) {
+// Column pass: each workgroup holds NC columns of one wave, thread (gc, t) the points t + (N / 16) q of column gc.
+template <int N, int F>
+__global__ __launch_bounds__(256) void k_col_phase(FwdParams a, float2* waves, int w0, int nb, int s) {
   constexpr int T = N / 16;
   constexpr int NC = 256 / T;  // columns per workgroup
   constexpr int PS = N + N / 16;
+  constexpr bool INIT = F & PH_INIT, FIRST = F & PH_FIRST, START = F & PH_START;
+  constexpr bool PROP = !INIT && (!FIRST || START);
   __shared__ float2 buf[PS * NC];
   constexpr bool DF = N <= DF_MAX_LINE;
   __shared__ float2 tw[twiddle_entries<N>()];
-  __shared__ float2 py[N];
+  __shared__ float2 py[PROP ? N : 1];
   __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
-  __shared__ float2 pyf[DF ? N : 1];
-  __shared__ double red[4];
+  __shared__ float2 pyf[DF && PROP ? N : 1];
   const int tid = threadIdx.x;
   const int gc = tid % NC, t = tid / NC;
-  int unit, cblk;  // wave (INIT/MID) or position (FINAL) inside the batch, column block
+  int wl, cblk;
   if constexpr (NC > 2) {  // grid (nb, N / NC), as column_block maps it
-    unit = blockIdx.x;
+    wl = blockIdx.x;
     cblk = blockIdx.y;
   } else {
-    column_block<NC>((int)blockIdx.x, nb, N / NC, unit, cblk);
+    column_block<NC>((int)blockIdx.x, nb, N / NC, wl, cblk);
   }
   const int x = cblk * NC + gc;
-  const int pos = a.M == 1 ? w0 + unit : (MODE >= COL_FINAL ? w0 / a.M + unit : (w0 + unit) / a.M);
-  const WaveCtx g = resolve_wave(a, pos);
+  const int w = w0 + wl;
+  const int p = a.M == 1 ? w : w / a.M;
+  const WaveCtx g = resolve_wave(a, p);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
-  if (MODE == COL_MID) stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
+  if constexpr (PROP) stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
   const FixTab<DF> pyt{py, pyf};
   const ColBufBlock<NC> cb{buf + gc};
   __syncthreads();
 
-  if constexpr (MODE == COL_INIT) {
-    const int m = (w0 + unit) % a.M;
+  float2* wcol = waves + (size_t)wl * N * N + x;
+  float2 v[16];
+  if constexpr (INIT) {  // probe_k column * shift ramp, inverse FFT along y
     const float2 rx = shift_ramp(x, N, g.mx, g.dx);
-    const float2* pk = g.probe_k + (size_t)m * N * N + x;
-    float2 v[16];
+    const float2* pk = g.probe_k + (size_t)(w % a.M) * N * N + x;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int ky = t + T * q;
       v[q] = cmul(cmul(pk[(size_t)ky * N], shift_ramp(ky, N, g.my, g.dy)), rx);
     }
     line_fft<N, true>(v, t, cb, twt);
-    float2* wcol = waves + (size_t)unit * N * N + x;
+  } else {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
-  } else if constexpr (MODE == COL_MID) {  // reads the workspace, writes dst (the workspace or a store slot)
-    const float2* wcol = waves + (size_t)unit * N * N + x;
-    float2* dcol = dst + (size_t)unit * dst_stride + x;
+    for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
+    if constexpr (FIRST) {
+      line_fft<N, true>(v, t, cb, twt);  // completes psi_0
+    } else {  // second operator of slice s - 1
+      line_fft<N, false>(v, t, cb, twt);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = pyt.mul(v[q], t + T * q);
+      line_fft<N, true>(v, t, cb, twt);
+      if (a.ckpt && s % a.ckpt_every == 0) {  // checkpoint the wave entering slice s
+        const int nc = num_checkpoints(a.S, a.ckpt_every);
+        float2* ck = a.ckpt + ((size_t)w * nc + (s / a.ckpt_every - 1)) * N * N + x;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) ck[(size_t)(t + T * q) * N] = v[q];
+      }
+    }
+    multiply_window_col<N>(v, g, s, x, t);
+    line_fft<N, false>(v, t, cb, twt);
+    if constexpr (START) {  // first operator of slice s
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = pyt.mul(v[q], t + T * q);
+      line_fft<N, true>(v, t, cb, twt);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
+}
+
+// Last pass: the forward transform along columns, |.|^2 summed over the modes of a position with their weights,
+// and the fftshifted store.  One workgroup runs every mode of its position, so each pixel has one writer.
+template <int N, bool DET>
+__global__ __launch_bounds__(256) void k_col_final(FwdParams a, float2* waves, int w0, int np) {
+  constexpr int T = N / 16;
+  constexpr int NC = 256 / T;
+  constexpr int PS = N + N / 16;
+  __shared__ float2 buf[PS * NC];
+  constexpr bool DF = N <= DF_MAX_LINE;
+  __shared__ float2 tw[twiddle_entries<N>()];
+  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  const int gc = tid % NC, t = tid / NC;
+  int pl, cblk;  // position inside the batch, column block
+  if constexpr (NC > 2) {
+    pl = blockIdx.x;
+    cblk = blockIdx.y;
+  } else {
+    column_block<NC>((int)blockIdx.x, np, N / NC, pl, cblk);
+  }
+  const int x = cblk * NC + gc;
+  const int p = a.M == 1 ? w0 + pl : w0 / a.M + pl;
+  const WaveCtx g = resolve_wave(a, p);
+  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  const TwTab<DF> twt{tw, twf};
+  const ColBufBlock<NC> cb{buf + gc};
+  __syncthreads();
+
+  float acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+  const int xs = (x + N / 2) & (N - 1);
+  for (int m = 0; m < a.M; ++m) {
+    const float2* wcol = waves + ((size_t)pl * a.M + m) * N * N + x;
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
     line_fft<N, false>(v, t, cb, twt);
+    const float wm = (a.weights ? a.weights[m] : 1.f);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = pyt.mul(v[q], t + T * q);
-    line_fft<N, true>(v, t, cb, twt);
+    for (int q = 0; q < 16; ++q) acc[q] += wm * cabs2(v[q]);
+    if (a.amplitude) {
+      float2* amp = a.amplitude + ((size_t)p * a.M + m) * N * N;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) dcol[(size_t)(t + T * q) * N] = v[q];
-  } else {
-    const int pl = unit;  // position inside the batch
-    const int p = pos;
-    float acc[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
-    const int xs = (x + N / 2) & (N - 1);
-    for (int m = 0; m < a.M; ++m) {
-      const float2* wcol = waves + ((size_t)pl * a.M + m) * N * N + x;
-      float2 v[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
-      line_fft<N, false>(v, t, cb, twt);
-      const float wm = (a.weights ? a.weights[m] : 1.f);
-#pragma unroll
-      for (int q = 0; q < 16; ++q) acc[q] += wm * cabs2(v[q]);
-      if (a.amplitude) {
-        float2* amp = a.amplitude + ((size_t)p * a.M + m) * N * N;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) amp[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = v[q];
-      }
+      for (int q = 0; q < 16; ++q) amp[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = v[q];
     }
-    if (a.patterns) {
-      float* out = a.patterns + g.pat_off;
+  }
+  if (a.patterns) {
+    float* out = a.patterns + g.pat_off;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) out[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = acc[q];
-    }
-    // virtual detectors: this tile's masked sums go to row 0, columns cblk * NC + d of the position's
-    // first wave in the workspace (only this workgroup reads those columns; its reads are complete
-    // after the reduction's barrier) and k_det_finalize adds the tiles in order
-    for (int d = 0; MODE == COL_FINAL_DET && d < a.det_count; ++d) {
-      const unsigned char* mk = a.det_masks + (size_t)d * N * N + xs;
-      double part = 0.0;
+    for (int q = 0; q < 16; ++q) out[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = acc[q];
+  }
+  // virtual detectors: this tile's masked sums go to row 0, columns cblk * NC + d of the position's
+  // first wave in the workspace (only this workgroup reads those columns; its reads are complete
+  // after the reduction's barrier) and k_det_finalize adds the tiles in order
+  for (int d = 0; DET && d < a.det_count; ++d) {
+    const unsigned char* mk = a.det_masks + (size_t)d * N * N + xs;
+    double part = 0.0;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) part += mk[(size_t)((t + T * q + N / 2) & (N - 1)) * N] ? (double)acc[q] : 0.0;
-      const double total = block_sum_fp64<256>(part, red);
-      if (tid == 0) {
-        double* slot = (double*)(waves + (size_t)pl * a.M * N * N + cblk * NC + d);
-        *slot = total;
-      }
+    for (int q = 0; q < 16; ++q) part += mk[(size_t)((t + T * q + N / 2) & (N - 1)) * N] ? (double)acc[q] : 0.0;
+    const double total = block_sum_fp64<256>(part, red);
+    if (tid == 0) {
+      double* slot = (double*)(waves + (size_t)pl * a.M * N * N + cblk * NC + d);
+      *slot = total;
     }
   }
 }
@@ -417,42 +509,45 @@ static dim3 column_grid(int units) {
   return NC > 2 ? dim3(units, N / NC) : dim3(units * (N / NC));
 }
 
+// Pass of slice s on rows or columns (flags PH_FIRST when s = 0, PH_START while s < S - 1).
+template <int N>
+static void launch_phase(bool col, bool first, bool start, const FwdParams& a, float2* ws, int w0, int nb, int s,
+                         hipStream_t st) {
+  constexpr int RPB = 256 / (N / 16);
+  const dim3 rg(nb, N / RPB), cg = column_grid<N>(nb), b(256);
+  if (col) {
+    if (first && start) hipLaunchKernelGGL((k_col_phase<N, PH_FIRST | PH_START>), cg, b, 0, st, a, ws, w0, nb, s);
+    else if (first) hipLaunchKernelGGL((k_col_phase<N, PH_FIRST>), cg, b, 0, st, a, ws, w0, nb, s);
+    else if (start) hipLaunchKernelGGL((k_col_phase<N, PH_START>), cg, b, 0, st, a, ws, w0, nb, s);
+    else hipLaunchKernelGGL((k_col_phase<N, 0>), cg, b, 0, st, a, ws, w0, nb, s);
+  } else {
+    if (first && start) hipLaunchKernelGGL((k_row_phase<N, PH_FIRST | PH_START>), rg, b, 0, st, a, ws, w0, s);
+    else if (first) hipLaunchKernelGGL((k_row_phase<N, PH_FIRST>), rg, b, 0, st, a, ws, w0, s);
+    else if (start) hipLaunchKernelGGL((k_row_phase<N, PH_START>), rg, b, 0, st, a, ws, w0, s);
+    else hipLaunchKernelGGL((k_row_phase<N, 0>), rg, b, 0, st, a, ws, w0, s);
+  }
+}
+
 template <int N>
 static hipError_t launch_streaming_batch(const FwdParams& a, float2* ws, long long p0, int np, hipStream_t st) {
-  constexpr int T = N / 16;
-  constexpr int RPB = 256 / T;
+  constexpr int RPB = 256 / (N / 16);
   const int nb = np * a.M;
   const int w0 = (int)(p0 * a.M);
-  const long long NN = (long long)N * N;
-  // With a checkpoint at every slice the column pass writes the wave entering slice s + 1 straight into its
-  // checkpoint slot and the next row pass reads it from there: the checkpoints cost no extra traffic.
-  const bool store = a.ckpt && a.ckpt_every == 1;
-  const int nc = store ? num_checkpoints(a.S, 1) : 0;
-  auto slot = [&](int c) { return a.ckpt + ((size_t)w0 * nc + c) * NN; };
-  hipLaunchKernelGGL((k_cols<N, COL_INIT>), column_grid<N>(nb), dim3(256), 0, st, a, ws, ws, NN, w0, nb);
+  // first axis from the parity of S: pass S + 1, the final column transform, then falls on columns
+  bool col = (a.S % 2) == 1;
+  if (col)
+    hipLaunchKernelGGL((k_col_phase<N, PH_INIT>), column_grid<N>(nb), dim3(256), 0, st, a, ws, w0, nb, 0);
+  else
+    hipLaunchKernelGGL((k_row_phase<N, PH_INIT>), dim3(nb, N / RPB), dim3(256), 0, st, a, ws, w0, 0);
   for (int s = 0; s < a.S; ++s) {
-    const bool first = s == 0, last = s == a.S - 1;
-    const dim3 grid(nb, N / RPB);
-    const float2* src = store && !first ? slot(s - 1) : ws;
-    const long long src_stride = store && !first ? nc * NN : NN;
-    if (first && last)
-      hipLaunchKernelGGL((k_rows<N, ROW_FIRST | ROW_LAST>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb,
-                         s);
-    else if (first)
-      hipLaunchKernelGGL((k_rows<N, ROW_FIRST>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb, s);
-    else if (last)
-      hipLaunchKernelGGL((k_rows<N, ROW_LAST>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb, s);
-    else
-      hipLaunchKernelGGL((k_rows<N, 0>), grid, dim3(256), 0, st, a, ws, src, src_stride, w0, nb, s);
-    if (!last)
-      hipLaunchKernelGGL((k_cols<N, COL_MID>), column_grid<N>(nb), dim3(256), 0, st, a, ws,
-                         store ? slot(s) : ws, store ? nc * NN : NN, w0, nb);
+    col = !col;
+    launch_phase<N>(col, s == 0, s < a.S - 1, a, ws, w0, nb, s, st);
   }
   if (a.det_count > 0) {
-    hipLaunchKernelGGL((k_cols<N, COL_FINAL_DET>), column_grid<N>(np), dim3(256), 0, st, a, ws, ws, NN, w0, np);
+    hipLaunchKernelGGL((k_col_final<N, true>), column_grid<N>(np), dim3(256), 0, st, a, ws, w0, np);
     hipLaunchKernelGGL(k_det_finalize<N>, dim3((np * a.det_count + 255) / 256), dim3(256), 0, st, a, ws, p0, np);
   } else {
-    hipLaunchKernelGGL((k_cols<N, COL_FINAL>), column_grid<N>(np), dim3(256), 0, st, a, ws, ws, NN, w0, np);
+    hipLaunchKernelGGL((k_col_final<N, false>), column_grid<N>(np), dim3(256), 0, st, a, ws, w0, np);
   }
   return hipGetLastError();
 }

@@ -98,9 +98,9 @@ def test_checkpoint_interval_does_not_change_gradient():
 
 @pytest.mark.parametrize("n", [256, 512])
 def test_checkpoint_store_at_every_slice(n):
-    """With a checkpoint at every slice the streaming column pass writes each wave into its slot and the next row
-    pass reads it there: the patterns equal the plain forward bitwise, and slot 3 (the wave entering slice 4)
-    equals slot 0 of a forward that checkpoints every 4th slice."""
+    """Checkpoints are side writes of the streaming passes: the patterns of a forward with a checkpoint at every
+    slice or every 4th equal the plain forward bitwise, and slot 3 (the wave entering slice 4) equals slot 0 of
+    the forward that checkpoints every 4th slice."""
     from ptyrodactyl_amd import engine
     print(f"checkpoint store n={n}", flush=True)
     s_count = 9

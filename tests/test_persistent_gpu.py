@@ -1,8 +1,10 @@
 """GPU checks of the single-launch persistent forward (n = 256, one probe mode).
 
-It runs the same arithmetic as the per-pass streaming forward, so the
-patterns must be bitwise equal to ``engine.multislice_forward``; the oracle
-comparison uses the forward path's tolerance (relative L2 <= 1e-5).
+It applies each slice's row operator before its column operator, while the
+streaming forward alternates the order slice by slice (the two operators
+commute), so the two paths agree to float32 rounding (AGREE below) rather
+than bitwise; the oracle comparison uses the forward path's tolerance
+(relative L2 <= 1e-5).
 """
 
 import numpy as np
@@ -14,6 +16,13 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 TOL = 1e-5
+AGREE = 2e-6
+
+
+def _agree(got, ref, what):
+    err = float(torch.linalg.vector_norm(got - ref) / torch.linalg.vector_norm(ref))
+    print(f"{what}: persistent vs streaming rel-L2 {err:.3e}", flush=True)
+    assert err < AGREE
 
 
 def _case(s_count, p_count, slab_hw, seed):
@@ -37,14 +46,14 @@ def _device_args(slab, probe, origins, shifts):
 
 
 @pytest.mark.parametrize("s_count,p_count,group_waves", [(1, 3, 2), (3, 37, 2), (2, 20, 1), (4, 50, 3)])
-def test_persistent_equals_per_pass(s_count, p_count, group_waves):
+def test_persistent_agrees_with_streaming(s_count, p_count, group_waves):
     from ptyrodactyl_amd import engine
     print(f"persistent S={s_count} P={p_count} g={group_waves}", flush=True)
     args = _device_args(*_case(s_count, p_count, (300, 280), seed=s_count * 100 + p_count))
     ref, _ = engine.multislice_forward(*args)
     got = engine.multislice_forward_persistent(*args, group_waves=group_waves)
     torch.cuda.synchronize()
-    assert torch.equal(got, ref)
+    _agree(got, ref, f"S={s_count}")
 
 
 def test_persistent_matches_oracle():
@@ -59,7 +68,8 @@ def test_persistent_matches_oracle():
 
 
 def test_persistent_reuses_state_and_workspace():
-    """Two calls in a row on shared state and workspace, the second smaller, each equal to the per-pass path."""
+    """Two calls in a row on shared state and workspace, the second smaller: the second call is bitwise equal to
+    the same call on fresh state, and each agrees with the streaming path."""
     from ptyrodactyl_amd import engine
     big = _device_args(*_case(2, 40, (256, 256), seed=11))
     small = _device_args(*_case(2, 9, (256, 256), seed=12))
@@ -67,12 +77,13 @@ def test_persistent_reuses_state_and_workspace():
     ws = torch.empty(engine._native.load().pty_persistent_workspace_bytes(256, 2), dtype=torch.uint8, device="cuda")
     a = engine.multislice_forward_persistent(*big, state=state, workspace=ws)
     b = engine.multislice_forward_persistent(*small, state=state, workspace=ws)
-    assert torch.equal(a, engine.multislice_forward(*big)[0])
-    assert torch.equal(b, engine.multislice_forward(*small)[0])
+    _agree(a, engine.multislice_forward(*big)[0], "first call")
+    _agree(b, engine.multislice_forward(*small)[0], "second call")
+    assert torch.equal(b, engine.multislice_forward_persistent(*small))
 
 
 def test_persistent_config2_slice_count():
-    """64 slices on the 1024^2 slab of the headline workload, 256 positions: equal to the per-pass path."""
+    """64 slices on the 1024^2 slab of the headline workload, 256 positions: agrees with the streaming path."""
     from ptyrodactyl_amd import engine, synth
     w = synth.config2(scan=16)
     args = _device_args(w.slab_t, w.probe, w.origins, w.shifts_px)
@@ -81,7 +92,7 @@ def test_persistent_config2_slice_count():
     print("persistent config2 subset", flush=True)
     ref, _ = engine.multislice_forward(*args)
     got = engine.multislice_forward_persistent(*args)
-    assert torch.equal(got, ref)
+    _agree(got, ref, "config 2, 64 slices")
 
 
 def test_persistent_rejects_unsupported_shapes():

@@ -8,6 +8,28 @@
 
 namespace pty {
 
+// Buffer view of `bytes` (below 4 GiB) of global memory at a wave-uniform base.  An access takes the lane's byte
+// offset in a VGPR and a wave-uniform byte offset in an SGPR, so the 16 strided accesses a thread makes to one
+// column (rows t + (n / 16) q) cost no vector address arithmetic; a plain pointer needs a 64-bit add per access.
+// Word 3 of the resource is the gfx9 raw-buffer format (32-bit data, no swizzle).
+struct BufView {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ BufView(const void* base, unsigned long long bytes)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                             (int)(unsigned)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull),
+                                             0x00020000)) {}
+  __device__ __forceinline__ float2 ld2(unsigned voff, unsigned soff) const {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+  }
+  __device__ __forceinline__ void st2(float2 v, unsigned voff, unsigned soff) const {
+    typedef __attribute__((__vector_size__(2 * sizeof(unsigned)))) unsigned u2;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, voff, soff, 0);
+  }
+  __device__ __forceinline__ void st1(float v, unsigned voff, unsigned soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, voff, soff, 0);
+  }
+};
+
 // Padded index inside a line: one spare slot every 16 points keeps the
 // Stockham exchange free of LDS bank conflicts.
 __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }

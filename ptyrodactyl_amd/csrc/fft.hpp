@@ -47,11 +47,9 @@ PTY_HD float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 PTY_HD float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 PTY_HD float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
 
-// Complex multiply by a constant given as double-float parts (hi + lo);
-// the small terms are accumulated first so the product carries the
-// constant's full precision.  Rounding the radix-16 constants to float32
-// alone is a systematic error that accumulates coherently over the ~100
-// chained transforms of a thick specimen; this keeps it below float32 noise.
+// Complex multiply by a factor given as double-float parts (hi + lo); the
+// small terms are accumulated first so the product carries the factor's
+// full precision (the modulus-corrected tables below).
 PTY_HD float2 cmul_df(float2 z, float crh, float crl, float cih, float cil) {
   float re = fmaf(z.x, crl, -z.y * cil);
   re = fmaf(-z.y, cih, re);
@@ -69,11 +67,50 @@ PTY_HD float2 cmul_df(float2 z, float crh, float crl, float cih, float cil) {
 // while their patterns already stay within 3.2e-6 of the complex128 oracle.
 constexpr int DF_MAX_LINE = 128;
 
-// Multiply by W16^E = exp(-2 pi i E / 16) (forward) or its conjugate.  The constants are double-float
-// at every line length: with float32 constants alone the 256^2 x 64-slice patterns move from 3.2e-6 to
+// z * exp(i theta) as three shears (lifting steps; A. W. Paeth, "A fast algorithm for general raster
+// rotation", Graphics Interface 1986; the lifting form of FFT twiddles in S. Oraintara, Y.-J. Chen and
+// T. Q. Nguyen, "Integer fast Fourier transform", IEEE Trans. Signal Process. 50(3), 2002):
+//   x1 = x - t y,  y' = y + s x1,  x' = x1 - t y',  t = tan(theta / 2), s = sin(theta).
+// Each step is one FMA, and the map has determinant exactly 1 for any rounded t and s: rounding the constants
+// to float32 leaves no systematic error in the modulus, only a phase error of the constants' size.  A complex
+// multiply by rounded (cos, sin) scales every point it touches by the same |W| != 1, an error that adds up
+// coherently over the ~100 chained transforms of a thick specimen.
+PTY_HD float2 rot_shear(float2 z, float t, float s) {
+  const float x1 = fmaf(-t, z.y, z.x);
+  const float y = fmaf(s, x1, z.y);
+  return make_float2(fmaf(-t, y, x1), y);
+}
+
+// Multiply by W16^E = exp(-2 pi i E / 16) (forward) or its conjugate: W16^E = (-i)^k W16^r with E = 4 k + r,
+// r in {-1, 0, 1, 2}; the rotation by -r pi / 8 runs as three shears (tan(pi / 16), sin(pi / 8) or
+// tan(pi / 8), sin(pi / 4)) and the quarter turn is a swap with a sign change.  Three FMAs against the
+// eight of tw16_df; the phase error of the float32 constants (about 3e-8) stays.
+template <int E, bool INV>
+PTY_HD float2 tw16_shear(float2 z) {
+  constexpr int e = ((E % 16) + 16) % 16;
+  constexpr int r = (e % 4 == 3) ? -1 : e % 4;
+  constexpr int k = ((e - r) / 4) % 4;
+  constexpr float T1 = 0.19891236737965800691f, S1 = 0.38268343236508977173f;  // tan(pi/16), sin(pi/8)
+  constexpr float T2 = 0.41421356237309504880f, S2 = 0.70710678118654752440f;  // tan(pi/8), sin(pi/4)
+  // rotation angle of W16^r: -r pi / 8 (forward), +r pi / 8 (inverse)
+  constexpr float sg = (INV ? 1.f : -1.f) * (r < 0 ? -1.f : 1.f);
+  float2 w = z;
+  if constexpr (r == 1 || r == -1) w = rot_shear(z, sg * T1, sg * S1);
+  else if constexpr (r == 2) w = rot_shear(z, sg * T2, sg * S2);
+  // (-i)^k forward, (+i)^k inverse
+  constexpr int kk = INV ? (4 - k) % 4 : k;
+  if constexpr (kk == 0) return w;
+  else if constexpr (kk == 1) return mul_mi(w);
+  else if constexpr (kk == 2) return make_float2(-w.x, -w.y);
+  else return mul_pi(w);
+}
+
+// Multiply by W16^E = exp(-2 pi i E / 16) (forward) or its conjugate with double-float constants: exact to
+// float32 rounding of the product, in modulus and phase (the LDS-resident kernels, whose gradients carry the
+// tightest parity bound).  With float32 constants alone the 256^2 x 64-slice patterns move from 3.2e-6 to
 // 1.3e-5 of the complex128 oracle (measured), since their modulus error, about 5e-8, is systematic.
 template <int E, bool INV>
-PTY_HD float2 tw16(float2 z) {
+PTY_HD float2 tw16_df(float2 z) {
   constexpr int e = ((E % 16) + 16) % 16;
   // cos(pi/8), sin(pi/8), 1/sqrt(2) split into float32 hi + lo
   constexpr float C1H = 0.9238795042037964f, C1L = 2.830748968563057e-08f;
@@ -113,6 +150,13 @@ PTY_HD float2 tw16(float2 z) {
   }
 }
 
+// W16^E by the double-float constants (EXACT) or by three shears.
+template <int E, bool INV, bool EXACT>
+PTY_HD float2 tw16(float2 z) {
+  if constexpr (EXACT) return tw16_df<E, INV>(z);
+  else return tw16_shear<E, INV>(z);
+}
+
 template <bool INV>
 PTY_HD void dft2(float2& a, float2& b) {
   float2 t = a;
@@ -130,31 +174,31 @@ PTY_HD void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
   a3 = csub(t1, t3);
 }
 
-// Natural-order in-register DFT of size R (R in {2, 4, 8, 16}).
-template <int R, bool INV>
+// Natural-order in-register DFT of size R (R in {2, 4, 8, 16}); EXACT selects the double-float W16 constants.
+template <int R, bool INV, bool EXACT>
 struct Dft;
 
-template <bool INV>
-struct Dft<2, INV> {
+template <bool INV, bool EXACT>
+struct Dft<2, INV, EXACT> {
   PTY_HD static void run(float2* a) { dft2<INV>(a[0], a[1]); }
 };
 
-template <bool INV>
-struct Dft<4, INV> {
+template <bool INV, bool EXACT>
+struct Dft<4, INV, EXACT> {
   PTY_HD static void run(float2* a) { dft4<INV>(a[0], a[1], a[2], a[3]); }
 };
 
 // 8 = 2 x 4 (decimation in time): X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k].
-template <bool INV>
-struct Dft<8, INV> {
+template <bool INV, bool EXACT>
+struct Dft<8, INV, EXACT> {
   PTY_HD static void run(float2* a) {
     float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
     float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
     dft4<INV>(e0, e1, e2, e3);
     dft4<INV>(o0, o1, o2, o3);
-    o1 = tw16<2, INV>(o1);
-    o2 = tw16<4, INV>(o2);
-    o3 = tw16<6, INV>(o3);
+    o1 = tw16<2, INV, EXACT>(o1);
+    o2 = tw16<4, INV, EXACT>(o2);
+    o3 = tw16<6, INV, EXACT>(o3);
     a[0] = cadd(e0, o0);
     a[4] = csub(e0, o0);
     a[1] = cadd(e1, o1);
@@ -167,21 +211,21 @@ struct Dft<8, INV> {
 };
 
 // 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.
-template <bool INV>
-struct Dft<16, INV> {
+template <bool INV, bool EXACT>
+struct Dft<16, INV, EXACT> {
   PTY_HD static void run(float2* a) {
 #pragma unroll
     for (int n2 = 0; n2 < 4; ++n2) dft4<INV>(a[n2], a[4 + n2], a[8 + n2], a[12 + n2]);
     // a[4*k1 + n2] now holds the inner DFT; apply W16^(n2*k1)
-    a[5] = tw16<1, INV>(a[5]);
-    a[6] = tw16<2, INV>(a[6]);
-    a[7] = tw16<3, INV>(a[7]);
-    a[9] = tw16<2, INV>(a[9]);
-    a[10] = tw16<4, INV>(a[10]);
-    a[11] = tw16<6, INV>(a[11]);
-    a[13] = tw16<3, INV>(a[13]);
-    a[14] = tw16<6, INV>(a[14]);
-    a[15] = tw16<9, INV>(a[15]);
+    a[5] = tw16<1, INV, EXACT>(a[5]);
+    a[6] = tw16<2, INV, EXACT>(a[6]);
+    a[7] = tw16<3, INV, EXACT>(a[7]);
+    a[9] = tw16<2, INV, EXACT>(a[9]);
+    a[10] = tw16<4, INV, EXACT>(a[10]);
+    a[11] = tw16<6, INV, EXACT>(a[11]);
+    a[13] = tw16<3, INV, EXACT>(a[13]);
+    a[14] = tw16<6, INV, EXACT>(a[14]);
+    a[15] = tw16<9, INV, EXACT>(a[15]);
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) dft4<INV>(a[4 * k1], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
     // a[4*k1 + k2] = X[k1 + 4*k2]: transpose the 4x4 index grid into natural order
@@ -286,7 +330,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[E], int t, Buf& buf, c
 #pragma unroll
       for (int r = 1; r < R; ++r) x[b][r] = tw.template apply<INV, twiddle_slot<L, NS>(), NS>(x[b][r], jm, r);
     }
-    Dft<R, INV>::run(x[b]);
+    Dft<R, INV, TW::EXACT>::run(x[b]);
   }
   if constexpr (LAST) {
 #pragma unroll
@@ -329,6 +373,8 @@ PTY_HD float2 cmulc_pair(float2 z, float2 hi, float2 lo) { return cmul_df(z, hi.
 // DF, their modulus-correcting low parts.
 template <bool DF>
 struct TwTab {
+  // the double-float tables come with the double-float W16 constants, plain float32 tables with the shears
+  static constexpr bool EXACT = DF;
   const float2* tab;
   const float2* lo;
   template <bool INV, int SLOT, int NS>

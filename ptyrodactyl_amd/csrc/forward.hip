@@ -213,25 +213,29 @@ static size_t resident_lds_bytes() {
 
 enum : int { PH_INIT = 1, PH_FIRST = 2, PH_START = 4 };
 
-// Window column x of slice s times the 16 points a thread holds of it (rows t + (N / 16) q): a window whose rows
-// do not wrap (uniform per workgroup) reads from one base with a fixed stride.
+// Window column x of slice s times the 16 points a thread holds of it (rows t + (N / 16) q).  A window whose rows
+// do not wrap (uniform per workgroup) reads the slice through a buffer view, one lane offset plus a uniform row
+// offset per point; a slice of 4 GiB or more, or a window that wraps, computes each address.
 template <int N>
 __device__ __forceinline__ void multiply_window_col(float2 (&v)[16], const WaveCtx& g, int s, int x, int t) {
   constexpr int T = N / 16;
   int gx = g.ox + x;
   if (gx >= g.slab_w) gx -= (int)g.slab_w;
-  const float2* col = g.slab + (long long)s * g.stride + gx;
+  const float2* slice = g.slab + (long long)s * g.stride;
   const long long w = g.slab_w;
-  if (g.oy + N <= g.slab_h) {
-    const float2* c = col + (long long)(g.oy + t) * w;
+  const unsigned long long slice_bytes = (unsigned long long)g.stride * sizeof(float2);
+  if (g.oy + N <= g.slab_h && slice_bytes <= 0xFFFFFFFFull) {
+    const BufView tv(slice, slice_bytes);
+    const unsigned lo = (unsigned)(((long long)(g.oy + t) * w + gx) * sizeof(float2));
+    const unsigned qs = (unsigned)(T * w * sizeof(float2));
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], c[(long long)T * q * w]);
+    for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], tv.ld2(lo, qs * q));
   } else {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       int gy = g.oy + t + T * q;
       if (gy >= g.slab_h) gy -= (int)g.slab_h;
-      v[q] = cmul(v[q], col[(long long)gy * w]);
+      v[q] = cmul(v[q], slice[(long long)gy * w + gx]);
     }
   }
 }
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(256) void k_row_phase(FwdParams a, float2* waves, i
   const int p = a.M == 1 ? w : w / a.M;
   const int t = tid % T, lr = tid / T;
   const int y = blockIdx.y * RPB + lr;
-  const WaveCtx g = resolve_wave(a, p);
+  const WaveCtx g = resolve_wave<INIT>(a, p);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
   if constexpr (PROP) stage_factors<DF>(px, pxf, g.prop_x, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(256) void k_col_phase(FwdParams a, float2* waves, i
   const int x = cblk * NC + gc;
   const int w = w0 + wl;
   const int p = a.M == 1 ? w : w / a.M;
-  const WaveCtx g = resolve_wave(a, p);
+  const WaveCtx g = resolve_wave<INIT>(a, p);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
   if constexpr (PROP) stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
@@ -353,20 +357,21 @@ __global__ __launch_bounds__(256) void k_col_phase(FwdParams a, float2* waves, i
   const ColBufBlock<NC> cb{buf + gc};
   __syncthreads();
 
-  float2* wcol = waves + (size_t)wl * N * N + x;
+  // point q of the thread's column sits lo + q * QS bytes into an n x n array
+  constexpr unsigned QS = (unsigned)(T * N * sizeof(float2));
+  constexpr unsigned long long NN8 = (unsigned long long)N * N * sizeof(float2);
+  const unsigned lo = (unsigned)((t * N + x) * sizeof(float2));
+  const BufView wv(waves + (size_t)wl * N * N, NN8);
   float2 v[16];
   if constexpr (INIT) {  // probe_k column * shift ramp, inverse FFT along y
     const float2 rx = shift_ramp(x, N, g.mx, g.dx);
-    const float2* pk = g.probe_k + (size_t)(w % a.M) * N * N + x;
+    const BufView pk(g.probe_k + (size_t)(w % a.M) * N * N, NN8);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int ky = t + T * q;
-      v[q] = cmul(cmul(pk[(size_t)ky * N], shift_ramp(ky, N, g.my, g.dy)), rx);
-    }
+    for (int q = 0; q < 16; ++q) v[q] = cmul(cmul(pk.ld2(lo, QS * q), shift_ramp(t + T * q, N, g.my, g.dy)), rx);
     line_fft<N, true>(v, t, cb, twt);
   } else {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
+    for (int q = 0; q < 16; ++q) v[q] = wv.ld2(lo, QS * q);
     if constexpr (FIRST) {
       line_fft<N, true>(v, t, cb, twt);  // completes psi_0
     } else {  // second operator of slice s - 1
@@ -376,9 +381,9 @@ __global__ __launch_bounds__(256) void k_col_phase(FwdParams a, float2* waves, i
       line_fft<N, true>(v, t, cb, twt);
       if (a.ckpt && s % a.ckpt_every == 0) {  // checkpoint the wave entering slice s
         const int nc = num_checkpoints(a.S, a.ckpt_every);
-        float2* ck = a.ckpt + ((size_t)w * nc + (s / a.ckpt_every - 1)) * N * N + x;
+        const BufView ck(a.ckpt + ((size_t)w * nc + (s / a.ckpt_every - 1)) * N * N, NN8);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) ck[(size_t)(t + T * q) * N] = v[q];
+        for (int q = 0; q < 16; ++q) ck.st2(v[q], lo, QS * q);
       }
     }
     multiply_window_col<N>(v, g, s, x, t);
@@ -390,7 +395,7 @@ __global__ __launch_bounds__(256) void k_col_phase(FwdParams a, float2* waves, i
     }
   }
 #pragma unroll
-  for (int q = 0; q < 16; ++q) wcol[(size_t)(t + T * q) * N] = v[q];
+  for (int q = 0; q < 16; ++q) wv.st2(v[q], lo, QS * q);
 }
 
 // Last pass: the forward transform along columns, |.|^2 summed over the modes of a position with their weights,
@@ -416,35 +421,40 @@ __global__ __launch_bounds__(256) void k_col_final(FwdParams a, float2* waves, i
   }
   const int x = cblk * NC + gc;
   const int p = a.M == 1 ? w0 + pl : w0 / a.M + pl;
-  const WaveCtx g = resolve_wave(a, p);
+  const WaveCtx g = resolve_wave<false>(a, p);
   fill_twiddles<DF>(tw, twf, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
   const ColBufBlock<NC> cb{buf + gc};
   __syncthreads();
 
+  constexpr unsigned QS = (unsigned)(T * N * sizeof(float2));
+  constexpr unsigned long long NN8 = (unsigned long long)N * N * sizeof(float2);
+  const unsigned lo = (unsigned)((t * N + x) * sizeof(float2));
+  const int xs = (x + N / 2) & (N - 1);
+  // fftshifted row of point q: t + (N / 16) ((q + 8) mod 16)
+  const unsigned lo_s = (unsigned)((t * N + xs) * sizeof(float2));
   float acc[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) acc[q] = 0.f;
-  const int xs = (x + N / 2) & (N - 1);
   for (int m = 0; m < a.M; ++m) {
-    const float2* wcol = waves + ((size_t)pl * a.M + m) * N * N + x;
+    const BufView wv(waves + ((size_t)pl * a.M + m) * N * N, NN8);
     float2 v[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = wcol[(size_t)(t + T * q) * N];
+    for (int q = 0; q < 16; ++q) v[q] = wv.ld2(lo, QS * q);
     line_fft<N, false>(v, t, cb, twt);
     const float wm = (a.weights ? a.weights[m] : 1.f);
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] += wm * cabs2(v[q]);
     if (a.amplitude) {
-      float2* amp = a.amplitude + ((size_t)p * a.M + m) * N * N;
+      const BufView amp(a.amplitude + ((size_t)p * a.M + m) * N * N, NN8);
 #pragma unroll
-      for (int q = 0; q < 16; ++q) amp[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = v[q];
+      for (int q = 0; q < 16; ++q) amp.st2(v[q], lo_s, QS * ((q + 8) & 15));
     }
   }
   if (a.patterns) {
-    float* out = a.patterns + g.pat_off;
+    const BufView out(a.patterns + g.pat_off, NN8 / 2);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) out[(size_t)((t + T * q + N / 2) & (N - 1)) * N + xs] = acc[q];
+    for (int q = 0; q < 16; ++q) out.st1(acc[q], lo_s / 2, (QS / 2) * ((q + 8) & 15));
   }
   // virtual detectors: this tile's masked sums go to row 0, columns cblk * NC + d of the position's
   // first wave in the workspace (only this workgroup reads those columns; its reads are complete

@@ -78,6 +78,8 @@ __device__ __forceinline__ void split_shift_px(double s, int n, int& m, float& d
   m = wrap_index((long long)fmod(r, (double)n), n);
 }
 
+// SHIFTS = false leaves my, mx, dy, dx unset (passes that do not build the incident wave skip the FP64 split).
+template <bool SHIFTS = true>
 __device__ __forceinline__ WaveCtx resolve_wave(const FwdParams& a, long long p) {
   WaveCtx c;
   const int n = a.n;
@@ -116,8 +118,13 @@ __device__ __forceinline__ WaveCtx resolve_wave(const FwdParams& a, long long p)
   c.stride = c.slab_h * c.slab_w;
   c.oy = wrap_index(oy, c.slab_h);
   c.ox = wrap_index(ox, c.slab_w);
-  split_shift_px(sy, n, c.my, c.dy);
-  split_shift_px(sx, n, c.mx, c.dx);
+  if constexpr (SHIFTS) {
+    split_shift_px(sy, n, c.my, c.dy);
+    split_shift_px(sx, n, c.mx, c.dx);
+  } else {
+    (void)sy;
+    (void)sx;
+  }
   return c;
 }
 

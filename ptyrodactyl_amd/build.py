@@ -16,7 +16,7 @@ PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR / "csrc"
 LIB_NAME = "libptyrodactyl_amd.so"
 LIB_PATH = PKG_DIR / LIB_NAME
-SOURCES = ("forward.hip", "adjoint.hip", "aux_kernels.hip", "optim.hip", "potential.hip", "xl2.hip", "grid.hip", "jvp.hip", "capi.hip")
+SOURCES = ("forward.hip", "adjoint.hip", "adjoint_stream.hip", "aux_kernels.hip", "optim.hip", "potential.hip", "xl2.hip", "grid.hip", "jvp.hip", "capi.hip")
 ARCH = os.environ.get("PTY_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -347,103 +347,6 @@ static size_t resident_adj_lds_bytes() {
 }
 
 // ---------------------------------------------------------------------------
-// Streaming adjoint (n = 256 ... 4096): per-slice row / column passes over a
-// batch of waves in global scratch.
-// ---------------------------------------------------------------------------
-
-struct WaveBuf {  // per-wave buffers: base + wl * stride
-  float2* base;
-  long long stride;
-  __device__ __forceinline__ float2* at(int wl) const { return base + (size_t)wl * stride; }
-};
-
-template <int N, int PRE, int POST>
-__global__ __launch_bounds__(256) void k_adj_rows(AdjParams a, WaveBuf src, WaveBuf dst, WaveBuf segb, long long w0,
-                                                  int nb, int s, int pw, int seg_idx, int seg_store) {
-  constexpr int T = N / 16, RPB = 256 / T, RS = N + N / 16;
-  __shared__ float2 rows[RPB * RS];
-  constexpr bool DF = N <= DF_MAX_LINE;
-  __shared__ float2 tw[twiddle_entries<N>()];
-  __shared__ float2 px[N];
-  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
-  __shared__ float2 pxf[DF ? N : 1];
-  const int tid = threadIdx.x;
-  const int wl = blockIdx.x, rblk = blockIdx.y;  // grid (nb, N / RPB)
-  const long long w = w0 + wl;
-  const int p = a.f.M == 1 ? (int)w : (int)(w / a.f.M);
-  const int t = tid % T, lr = tid / T, y = rblk * RPB + lr;
-  const WaveCtx g = resolve_wave(a.f, p);
-  fill_twiddles<DF>(tw, twf, N, tid, 256);
-  stage_factors<DF>(px, pxf, g.prop_x, N, tid, 256);
-  const TwTab<DF> twt{tw, twf};
-  const FixTab<DF> pxt{px, pxf};
-  const typename RowBufFor<(T <= 64)>::type rb{rows + lr * RS};
-  __syncthreads();
-  const float2* srow = src.at(wl) + (size_t)y * N;
-  float2 v[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = srow[t + T * q];
-  float2* seg_row = segb.base ? segb.at(wl) + ((size_t)seg_idx * N + y) * N : nullptr;
-  // contributions of the k slices of the current segment: [wave][slice - segment start][n][n]
-  float2* gt_row = a.gt_contrib ? a.gt_contrib + (((size_t)wl * a.contrib_slices + seg_idx) * N + y) * N : nullptr;
-  row_ops<N, PRE, POST>(v, t, y, s, pw, a, g, seg_row, seg_store != 0, gt_row, rb, twt, pxt);
-  float2* drow = dst.at(wl) + (size_t)y * N;
-#pragma unroll
-  for (int q = 0; q < 16; ++q) drow[t + T * q] = v[q];
-}
-
-template <int NC>
-struct ColBufBlk {
-  float2* base;
-  __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
-  __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
-  __device__ __forceinline__ float2 ldp(int pe) const { return base[pe * NC]; }
-  __device__ __forceinline__ void stp(int pe, float2 v) const { base[pe * NC] = v; }
-  __device__ __forceinline__ void sync() const { __syncthreads(); }
-};
-
-template <int N, int OP>
-__global__ __launch_bounds__(256) void k_adj_cols(AdjParams a, WaveBuf src, WaveBuf dst, long long w0, int nb) {
-  constexpr int T = N / 16, NC = 256 / T, PS = N + N / 16;
-  __shared__ float2 buf[PS * NC];
-  constexpr bool DF = N <= DF_MAX_LINE;
-  __shared__ float2 tw[twiddle_entries<N>()];
-  __shared__ float2 py[N];
-  __shared__ float2 twf[DF ? twiddle_entries<N>() : 1];
-  __shared__ float2 pyf[DF ? N : 1];
-  const int tid = threadIdx.x;
-  const int gc = tid % NC, t = tid / NC;
-  int wl, cblk;
-  if constexpr (NC > 2) {  // grid (nb, N / NC), as column_block maps it
-    wl = blockIdx.x;
-    cblk = blockIdx.y;
-  } else {
-    column_block<NC>((int)blockIdx.x, nb, N / NC, wl, cblk);
-  }
-  const long long w = w0 + wl;
-  const int p = a.f.M == 1 ? (int)w : (int)(w / a.f.M);
-  const int x = cblk * NC + gc;
-  const WaveCtx g = resolve_wave(a.f, p);
-  fill_twiddles<DF>(tw, twf, N, tid, 256);
-  stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
-  const TwTab<DF> twt{tw, twf};
-  const FixTab<DF> pyt{py, pyf};
-  const ColBufBlk<NC> cb{buf + gc};
-  __syncthreads();
-  float2 v[16];
-  if constexpr (OP != C_IFFT_PROBE) {
-    const float2* scol = src.at(wl) + x;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = scol[(size_t)(t + T * q) * N];
-  }
-  if (col_ops<N, OP>(v, t, x, w, a, g, cb, twt, pyt, nullptr, nullptr)) {
-    float2* dcol = dst.at(wl) + x;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) dcol[(size_t)(t + T * q) * N] = v[q];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Deterministic gathers of the per-wave contributions
 // ---------------------------------------------------------------------------
 
@@ -630,6 +533,19 @@ static void launch_reduces(const AdjParams& a, long long w0, int cnt, int s_firs
   }
 }
 
+hipError_t launch_adj_reduces(const AdjParams& a, long long w0, int cnt, int s_first, int s_count, bool probe,
+                              hipStream_t st) {
+  switch (a.f.n) {
+    case 256: launch_reduces<256>(a, w0, cnt, s_first, s_count, probe, st); break;
+    case 512: launch_reduces<512>(a, w0, cnt, s_first, s_count, probe, st); break;
+    case 1024: launch_reduces<1024>(a, w0, cnt, s_first, s_count, probe, st); break;
+    case 2048: launch_reduces<2048>(a, w0, cnt, s_first, s_count, probe, st); break;
+    case 4096: launch_reduces<4096>(a, w0, cnt, s_first, s_count, probe, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -649,7 +565,7 @@ static long long resident_slots(int n) {
 
 // Waves per streaming batch: 512 of 256^2, 128 of 512^2, then 16 / 4 / 1 of 1024^2 / 2048^2 / 4096^2
 // (about 1.4 GiB of segment, work and contribution buffers at k = 4).
-static long long streaming_batch(int n) {
+long long adjoint_streaming_batch(int n) {
   if (n <= 256) return 512;
   if (n == 512) return 128;
   const long long q = n / 1024;  // 1, 2, 4
@@ -680,7 +596,7 @@ size_t adjoint_workspace_bytes(int n, int S, int k, long long waves) {
     if (chunk > waves) chunk = waves;
     return (size_t)slots * (k + 1) * wave_bytes + (size_t)chunk * per_wave;
   }
-  long long nb = streaming_batch(n);
+  long long nb = adjoint_streaming_batch(n);
   if (nb > waves) nb = waves;
   return (size_t)nb * (2 * k + 3) * wave_bytes;  // seg[k], X, G, k slices of contributions, probe
 }
@@ -736,87 +652,6 @@ static hipError_t launch_adj_resident(const AdjParams& a0, void* ws, size_t ws_b
   return hipSuccess;
 }
 
-template <int N>
-static hipError_t launch_adj_streaming(const AdjParams& a0, void* ws, size_t ws_bytes, hipStream_t st) {
-  constexpr int T = N / 16, RPB = 256 / T, NC = 256 / T;
-  const long long NN = (long long)N * N;
-  const int S = a0.f.S, k = a0.k, nc = num_checkpoints(S, k), C = nc + 1;
-  const long long waves = a0.f.P * a0.f.M;
-  // gathered (deterministic) when there are no descriptors and the workspace holds the contributions
-  // of one k-slice segment and the probe: k + 1 more waves per wave in flight
-  const bool gathered = !a0.f.desc && ws_bytes >= (size_t)(2 * k + 3) * NN * sizeof(float2);
-  long long nbmax = (long long)(ws_bytes / ((size_t)(gathered ? 2 * k + 3 : k + 2) * NN * sizeof(float2)));
-  if (nbmax > waves) nbmax = waves;
-  if (nbmax < 1 || (a0.grad_shifts && !gathered)) return hipErrorInvalidValue;
-  if (nbmax >= 8) nbmax -= nbmax % 8;
-  float2* base = (float2*)ws;
-  for (long long w0 = 0; w0 < waves; w0 += nbmax) {
-    const int nb = (int)((waves - w0) < nbmax ? (waves - w0) : nbmax);
-    const WaveBuf seg{base, (long long)k * NN};
-    const WaveBuf X{base + (size_t)nb * k * NN, NN};
-    const WaveBuf Gb{base + (size_t)nb * (k + 1) * NN, NN};
-    const WaveBuf none{nullptr, 0};
-    AdjParams a = a0;
-    a.w_begin = w0;
-    a.w_end = w0 + nb;
-    a.contrib_slices = k;
-    a.gt_contrib = gathered ? base + (size_t)nb * (k + 2) * NN : nullptr;
-    a.gp_contrib = gathered && (a0.grad_probe_k || a0.grad_shifts) ? base + (size_t)nb * (2 * k + 2) * NN : nullptr;
-    const dim3 rg(nb, N / RPB), cg = NC > 2 ? dim3(nb, N / NC) : dim3(nb * (N / NC)), blk(256);
-    for (int c = C - 1; c >= 0; --c) {
-      const int s0 = c * k, s1 = min(S, s0 + k);
-      // psi_s of the segment: slot s - s0 of seg, written by the column pass that produces it; psi_{s0} of a
-      // segment with c > 0 is read from its checkpoint in place, and psi_0 is stored by the first row pass
-      // (the probe's inverse transform ends there)
-      const WaveBuf ck = c > 0 ? WaveBuf{const_cast<float2*>(a.ckpt) + ((size_t)w0 * nc + (c - 1)) * NN,
-                                         (long long)nc * NN}
-                               : none;
-      auto slot = [&](int s) { return WaveBuf{seg.base + (size_t)(s - s0) * NN, seg.stride}; };
-      auto psi = [&](int s) { return (s == s0 && c > 0) ? ck : slot(s); };
-      if (c == 0) hipLaunchKernelGGL((k_adj_cols<N, C_IFFT_PROBE>), cg, blk, 0, st, a, none, X, w0, nb);
-      for (int s = s0; s < s1; ++s) {
-        const bool more = s < s1 - 1;
-        if (s == 0) {
-          if (more)
-            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_PROP>), rg, blk, 0, st, a, X, X, seg, w0, nb, s,
-                               (int)PW_FWD, 0, 1);
-          else
-            hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, X, X, seg, w0, nb, s,
-                               (int)PW_NONE, 0, 1);
-        } else if (more) {
-          hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_PROP>), rg, blk, 0, st, a, psi(s), X, none, w0, nb, s,
-                             (int)PW_FWD, s - s0, 0);
-        }
-        if (more) hipLaunchKernelGGL((k_adj_cols<N, C_PROP>), cg, blk, 0, st, a, X, slot(s + 1), w0, nb);
-      }
-      int s = s1 - 1;
-      if (c == C - 1) {
-        hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, psi(S - 1), X, none, w0, nb,
-                           S - 1, (int)PW_FWD, 0, 0);
-        hipLaunchKernelGGL((k_adj_cols<N, C_FINAL_ADJ>), cg, blk, 0, st, a, X, Gb, w0, nb);
-        hipLaunchKernelGGL((k_adj_rows<N, PRE_IFFT, POST_NONE>), rg, blk, 0, st, a, Gb, Gb,
-                           WaveBuf{psi(S - 1).base - (size_t)(S - 1 - s0) * NN, psi(S - 1).stride}, w0, nb, S - 1,
-                           (int)PW_ADJ, S - 1 - s0, 0);
-        s = S - 2;
-      }
-      for (; s >= s0; --s) {
-        hipLaunchKernelGGL((k_adj_cols<N, C_ADJPROP>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
-        hipLaunchKernelGGL((k_adj_rows<N, PRE_ADJPROP, POST_NONE>), rg, blk, 0, st, a, Gb, Gb,
-                           WaveBuf{psi(s).base - (size_t)(s - s0) * NN, psi(s).stride}, w0, nb, s, (int)PW_ADJ,
-                           s - s0, 0);
-      }
-      if (gathered) launch_reduces<N>(a, w0, nb, s0, s1 - s0, false, st);  // the segment's slices
-    }
-    hipLaunchKernelGGL((k_adj_rows<N, PRE_NONE, POST_FFT>), rg, blk, 0, st, a, Gb, Gb, none, w0, nb, 0,
-                       (int)PW_NONE, 0, 0);
-    hipLaunchKernelGGL((k_adj_cols<N, C_PROBE_GRAD>), cg, blk, 0, st, a, Gb, Gb, w0, nb);
-    if (gathered && (a0.grad_probe_k || a0.grad_shifts)) launch_reduces<N>(a, w0, nb, 0, 0, true, st);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-
 hipError_t launch_adjoint(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st) {
   switch (a.f.n) {
     case 32:
@@ -826,15 +661,11 @@ hipError_t launch_adjoint(const AdjParams& a, void* ws, size_t ws_bytes, hipStre
     case 128:
       return launch_adj_resident<128>(a, ws, ws_bytes, st);
     case 256:
-      return launch_adj_streaming<256>(a, ws, ws_bytes, st);
     case 512:
-      return launch_adj_streaming<512>(a, ws, ws_bytes, st);
     case 1024:
-      return launch_adj_streaming<1024>(a, ws, ws_bytes, st);
     case 2048:
-      return launch_adj_streaming<2048>(a, ws, ws_bytes, st);
     case 4096:
-      return launch_adj_streaming<4096>(a, ws, ws_bytes, st);
+      return launch_adj_streaming(a, ws, ws_bytes, st);
     default:
       return hipErrorInvalidValue;
   }

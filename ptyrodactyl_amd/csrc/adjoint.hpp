@@ -32,4 +32,13 @@ size_t adjoint_workspace_bytes(int n, int S, int k, long long waves);
 size_t adjoint_min_workspace_bytes(int n, int k);
 hipError_t launch_adjoint(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st);
 
+// The streaming adjoint (n = 256 ... 4096, adjoint_stream.hip) and the deterministic gathers it shares with the
+// resident adjoint: slab contributions of slices s_first .. s_first + s_count - 1 and (probe) the probe and shift
+// gradients of waves w0 .. w0 + cnt - 1.
+hipError_t launch_adj_streaming(const AdjParams& a, void* ws, size_t ws_bytes, hipStream_t st);
+hipError_t launch_adj_reduces(const AdjParams& a, long long w0, int cnt, int s_first, int s_count, bool probe,
+                              hipStream_t st);
+// Waves of one streaming batch (n >= 256).
+long long adjoint_streaming_batch(int n);
+
 }  // namespace pty

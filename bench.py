@@ -124,7 +124,7 @@ def captured_small_bench(w, slab, pk, weights, origins, shifts, prop, positions:
         torch.cuda.synchronize()
         res[f"{name}_ms"] = round((time.perf_counter() - t0) * 1000.0 / reps, 3)
     res["positions"] = positions
-    res["launches_per_call"] = 2 * w.num_slices + 2 if w.n >= 256 else 1
+    res["launches_per_call"] = w.num_slices + 2 if w.n >= 256 else 1
     return res
 
 

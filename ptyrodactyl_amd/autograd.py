@@ -128,6 +128,9 @@ class MultisliceFunction(torch.autograd.Function):
         slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x, ckpt_every, want_ckpt = inputs
         _, ckpt = output
         ctx.mark_non_differentiable(ckpt)
+        # no zero tensor for the checkpoints' (absent) gradient: at 256^2 x 32 slices x 4096 positions the
+        # checkpoint store is 62 GiB, and materialising its zero gradient cost 10 ms of fills per step
+        ctx.set_materialize_grads(False)
         slab_c, probe_c = slab_t.detach().contiguous(), probe_k.detach().contiguous()
         w_c = weights.detach().to(torch.float32).contiguous()
         ctx.save_for_backward(slab_c, probe_c, w_c, origins, shifts_px.detach(), prop_y.detach(), prop_x.detach(),
@@ -144,6 +147,8 @@ class MultisliceFunction(torch.autograd.Function):
             slab_t, probe_k, weights, origins, shifts_px, prop_y, prop_x, ckpt = saved
             if not ctx.has_ckpt:
                 raise RuntimeError("multislice_patterns ran without checkpoints (grad mode was off): no reverse mode")
+            if grad_patterns is None:  # the patterns do not reach the loss
+                return (None,) * 9
             prop = (prop_y, prop_x)
             g = _plain(grad_patterns).to(torch.float32).contiguous()
             need_slab, need_probe, need_w = ctx.needs_input_grad[0], ctx.needs_input_grad[1], ctx.needs_input_grad[2]
@@ -172,6 +177,9 @@ class MultisliceFunction(torch.autograd.Function):
             if not _jvp_supported(probe_k):
                 raise ValueError("forward-mode multislice (pty_jvp) takes square windows n = 32 ... 512")
             prop = (prop_y, prop_x)
+            if d_slab is None and d_probe_k is None and d_weights is None and d_shifts is None:
+                n = probe_k.shape[-1]
+                return torch.zeros((origins.shape[0], n, n), dtype=torch.float32, device=probe_k.device), None
             want_amp = d_weights is not None
             _, amp, d_pat, _ = engine.multislice_jvp(
                 slab_t, probe_k, weights, origins, shifts_px, prop,
@@ -209,6 +217,7 @@ class MultisliceAmplitudeFunction(torch.autograd.Function):
         slab_t, probe_k, origins, shifts_px, prop_y, prop_x, ckpt_every, want_ckpt = inputs
         _, ckpt = output
         ctx.mark_non_differentiable(ckpt)
+        ctx.set_materialize_grads(False)  # no zero gradient for the checkpoint store
         slab_c, probe_c = slab_t.detach().contiguous(), probe_k.detach().contiguous()
         ones = torch.ones(probe_c.shape[0], dtype=torch.float32, device=probe_c.device)
         ctx.save_for_backward(slab_c, probe_c, ones, origins, shifts_px.detach(), prop_y.detach(), prop_x.detach(),
@@ -224,6 +233,8 @@ class MultisliceAmplitudeFunction(torch.autograd.Function):
             grad_amp = _plain(grad_amp)
             if not ctx.has_ckpt:
                 raise RuntimeError("multislice_amplitudes ran without checkpoints (grad mode was off): no reverse mode")
+            if grad_amp is None:  # the amplitudes do not reach the loss
+                return (None,) * 8
             prop = (prop_y, prop_x)
             g_shift = None
             if ctx.needs_input_grad[3]:
@@ -243,6 +254,9 @@ class MultisliceAmplitudeFunction(torch.autograd.Function):
             d_slab, d_probe_k, d_shifts = (_plain(t) for t in (d_slab, d_probe_k, d_shifts))
             if not _jvp_supported(probe_k):
                 raise ValueError("forward-mode multislice (pty_jvp) takes square windows n = 32 ... 512")
+            if d_slab is None and d_probe_k is None and d_shifts is None:
+                return torch.zeros((origins.shape[0],) + tuple(probe_k.shape), dtype=torch.complex64,
+                                   device=probe_k.device), None
             _, _, _, d_amp = engine.multislice_jvp(
                 slab_t, probe_k, ones, origins, shifts_px, (prop_y, prop_x),
                 d_slab.to(torch.complex64).contiguous() if d_slab is not None else None,
@@ -332,9 +346,12 @@ class BatchedWavesFunction(torch.autograd.Function):
                               output[1])
         ctx.group, ctx.ckpt_every = group, ckpt_every
         ctx.mark_non_differentiable(output[1])
+        ctx.set_materialize_grads(False)  # no zero gradient for the checkpoints
 
     @staticmethod
     def backward(ctx, grad_patterns, _grad_ckpt):
+        if grad_patterns is None:  # the patterns do not reach the loss
+            return None, None, None, None, None
         slabs, probe_k, props, ckpt = ctx.saved_tensors
         n, s_count = probe_k.shape[-1], slabs.shape[1]
         base = engine.arena_base(slabs.device)

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_adj_pass(AdjParams a, WaveBuf src, Wave
   const int M = a.f.M;
   const int p = M == 1 ? (int)w : (int)(w / M), m = M == 1 ? 0 : (int)(w % M);
   const WaveCtx g = resolve_wave<SHIFTS>(a.f, p);
-  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  stage_twiddles<N, DF>(tw, twf, tid, 256);
   if constexpr (PROP) stage_factors<DF>(pf, pff, COL ? g.prop_y : g.prop_x, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
   const FixTab<DF> pt{pf, pff};

@@ -415,6 +415,84 @@ __device__ __forceinline__ void fill_twiddles(float2* tab, float2* lo, int L, in
   }
 }
 
+// Twiddle tables of the streaming line lengths, built at compile time: entry e holds exp(-2 pi i m / L) with m as
+// fill_twiddles maps it, evaluated in double by octant reduction and a Taylor series (|angle| <= pi / 4, 14 terms:
+// truncation below 1e-20) and rounded to float32.  The streaming passes copy these into LDS instead of evaluating
+// one sincospif per entry in every workgroup.
+struct TwPair {
+  float x, y;
+};
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// sin and cos of 2 pi r / L for 0 <= r <= L / 8 (Taylor series in double).
+constexpr void cx_sincos_octant(long long r, long long L, double& s, double& c) {
+  const double x = kTwoPi * (double)r / (double)L;
+  double term_s = x, term_c = 1.0, sum_s = x, sum_c = 1.0;
+  const double x2 = x * x;
+  for (int k = 1; k <= 14; ++k) {
+    term_s *= -x2 / ((double)(2 * k) * (double)(2 * k + 1));
+    term_c *= -x2 / ((double)(2 * k - 1) * (double)(2 * k));
+    sum_s += term_s;
+    sum_c += term_c;
+  }
+  s = sum_s;
+  c = sum_c;
+}
+
+// (cos, -sin)(2 pi m / L) for L a multiple of 8, by exact integer reduction to the first octant.
+constexpr TwPair cx_twiddle(long long m, long long L) {
+  m %= L;
+  const long long q = m / (L / 4), rq = m % (L / 4);  // quadrant and offset in it
+  double s = 0.0, c = 0.0;
+  if (rq <= L / 8) {
+    cx_sincos_octant(rq, L, s, c);
+  } else {  // angle = pi / 2 - 2 pi (L / 4 - rq) / L
+    double s2 = 0.0, c2 = 0.0;
+    cx_sincos_octant(L / 4 - rq, L, s2, c2);
+    s = c2;
+    c = s2;
+  }
+  double cs = c, sn = s;  // rotate by q quarter turns
+  for (long long i = 0; i < q; ++i) {
+    const double t = cs;
+    cs = -sn;
+    sn = t;
+  }
+  return TwPair{(float)cs, (float)-sn};
+}
+
+template <int L>
+struct TwiddleData {
+  TwPair e[twiddle_entries<L>()];
+  constexpr TwiddleData() : e{} {
+    for (int i = 0; i < twiddle_entries<L>(); ++i) {
+      long long m;
+      if (L <= 256) m = (long long)(i % 16) * (i / 16);
+      else if (i < 256) m = (long long)(i % 16) * (i / 16) * (L / 256);
+      else m = (long long)((i - 256) % 256) * ((i - 256) / 256);
+      e[i] = cx_twiddle(m, L);
+    }
+  }
+};
+
+template <int L>
+__device__ const TwiddleData<L> kTwiddles{};
+
+// Stage the twiddle table of a line of length L in LDS (and, when DF, the modulus-correcting low parts):
+// streaming lengths copy the compile-time table, the LDS-resident ones evaluate and correct each entry.
+template <int L, bool DF>
+__device__ __forceinline__ void stage_twiddles(float2* tab, float2* lo, int tid, int nthreads) {
+  if constexpr (DF || L < 256) {
+    fill_twiddles<DF>(tab, lo, L, tid, nthreads);
+  } else {
+    for (int e = tid; e < twiddle_entries<L>(); e += nthreads) {
+      const TwPair w = kTwiddles<L>.e[e];
+      tab[e] = make_float2(w.x, w.y);
+    }
+  }
+}
+
 // A complex64 factor table of modulus r (the separable Fresnel propagator, r = 1/n) staged in LDS
 // with (DF) the modulus-correcting low parts of its entries.
 template <bool DF>

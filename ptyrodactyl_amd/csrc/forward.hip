@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_row_phase(FwdParams a, float2* waves, i
   const int t = tid % T, lr = tid / T;
   const int y = blockIdx.y * RPB + lr;
   const WaveCtx g = resolve_wave<INIT>(a, p);
-  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  stage_twiddles<N, DF>(tw, twf, tid, 256);
   if constexpr (PROP) stage_factors<DF>(px, pxf, g.prop_x, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
   const FixTab<DF> pxt{px, pxf};
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_col_phase(FwdParams a, float2* waves, i
   const int w = w0 + wl;
   const int p = a.M == 1 ? w : w / a.M;
   const WaveCtx g = resolve_wave<INIT>(a, p);
-  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  stage_twiddles<N, DF>(tw, twf, tid, 256);
   if constexpr (PROP) stage_factors<DF>(py, pyf, g.prop_y, N, tid, 256);
   const TwTab<DF> twt{tw, twf};
   const FixTab<DF> pyt{py, pyf};
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void k_col_final(FwdParams a, float2* waves, i
   const int x = cblk * NC + gc;
   const int p = a.M == 1 ? w0 + pl : w0 / a.M + pl;
   const WaveCtx g = resolve_wave<false>(a, p);
-  fill_twiddles<DF>(tw, twf, N, tid, 256);
+  stage_twiddles<N, DF>(tw, twf, tid, 256);
   const TwTab<DF> twt{tw, twf};
   const ColBufBlock<NC> cb{buf + gc};
   __syncthreads();

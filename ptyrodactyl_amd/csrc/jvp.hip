@@ -48,7 +48,7 @@ __global__ __launch_bounds__(JvpShape<N>::THREADS) void k_jvp_rows(JvpParams a, 
   const int t = tid % T, lr = tid / T;
   const int y = rblk * RPB + lr;
   const WaveCtx g = resolve_wave(a.f, p);
-  fill_twiddles<DF>(tw, twf, N, tid, NT);
+  stage_twiddles<N, DF>(tw, twf, tid, NT);
   if (!(FLAGS & JROW_LAST)) stage_factors<DF>(px, pxf, g.prop_x, N, tid, NT);
   const TwTab<DF> twt{tw, twf};
   const FixTab<DF> pxt{px, pxf};
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(JvpShape<N>::THREADS) void k_jvp_cols(JvpParams a, 
   const int x = cblk * NC + gc;
   const long long pos = MODE == JCOL_FINAL ? a.w0 / a.f.M + unit : (a.w0 + unit) / a.f.M;
   const WaveCtx g = resolve_wave(a.f, pos);
-  fill_twiddles<DF>(tw, twf, N, tid, NT);
+  stage_twiddles<N, DF>(tw, twf, tid, NT);
   if (MODE == JCOL_MID) stage_factors<DF>(py, pyf, g.prop_y, N, tid, NT);
   const TwTab<DF> twt{tw, twf};
   const FixTab<DF> pyt{py, pyf};

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(XL2_THREADS, 2) void k_forward_xl2(FwdParams a, flo
   int* ticket = st;
   int* done = st + XL2_LINE;  // done[(slot * XL2_MAX_GROUP + wl) * XL2_LINE]: completed tasks of one wave
   int* err = state;
-  fill_twiddles<DF>(tw, twf, N, tid, XL2_THREADS);
+  stage_twiddles<N, DF>(tw, twf, tid, XL2_THREADS);
   // without position descriptors every wave shares one propagator
   stage_factors<DF>(pvx, pvxf, a.prop_x, N, tid, XL2_THREADS);
   stage_factors<DF>(pvy, pvyf, a.prop_y, N, tid, XL2_THREADS);

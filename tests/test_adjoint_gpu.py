@@ -77,6 +77,8 @@ def _oracle_grads(slab, probes, origins, shifts, g, w, n, dz=1.5, kv=300.0, cali
     (256, 3, 8, 2, (300, 256), 2),
     (512, 2, 1, 2, (512, 512), 1),
     (256, 6, 1, 3, (280, 256), 2),
+    (256, 1, 1, 2, (256, 270), 1),
+    (256, 7, 3, 2, (260, 300), 2),
 ])
 def test_adjoint_matches_oracle_vjp(n, s_count, k, p_count, slab_hw, modes):
     print(f"adjoint n={n} S={s_count} k={k} P={p_count} M={modes}", flush=True)

@@ -70,7 +70,9 @@ def test_library_loads_on_gpu():
     (64, 4, 7, (64, 80)),
     (128, 5, 6, (160, 128)),
     (256, 3, 9, (256, 300)),
+    (256, 4, 5, (300, 260)),
     (512, 2, 3, (512, 512)),
+    (512, 3, 2, (520, 512)),
 ])
 def test_forward_matches_oracle(n, s_count, p_count, slab_hw):
     slab, probes, origins, shifts, prop = _random_case(n, s_count, p_count, slab_hw, seed=n + s_count)

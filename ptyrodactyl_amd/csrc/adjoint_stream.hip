@@ -12,6 +12,7 @@
 
 #include "adjoint.hpp"
 #include "common.hpp"
+#include "passes.hpp"
 
 namespace pty {
 
@@ -32,94 +33,10 @@ struct WaveBuf {  // per-wave buffers: base + wl * stride
   __device__ __forceinline__ float2* at(int wl) const { return base + (size_t)wl * stride; }
 };
 
-template <int NC>
-struct ColBufBlk {
-  float2* base;
-  __device__ __forceinline__ float2 ld(int e) const { return base[pad16(e) * NC]; }
-  __device__ __forceinline__ void st(int e, float2 v) const { base[pad16(e) * NC] = v; }
-  __device__ __forceinline__ float2 ldp(int pe) const { return base[pe * NC]; }
-  __device__ __forceinline__ void stp(int pe, float2 v) const { base[pe * NC] = v; }
-  __device__ __forceinline__ void sync() const { __syncthreads(); }
-};
-
 // What a pass does before, at and after the slice's pointwise step.
 enum : int { A_PRE_NONE = 0, A_PRE_PROBE = 1, A_PRE_IFFT = 2, A_PRE_FOP = 3, A_PRE_AOP = 4, A_PRE_SEED = 5 };
 enum : int { A_MID_NONE = 0, A_MID_T = 1, A_MID_ADJ = 2 };
 enum : int { A_POST_NONE = 0, A_POST_FOP = 1, A_POST_AOP = 2, A_POST_FFT = 3, A_POST_PROBE = 4 };
-
-// Line geometry of a pass: rows (COL = false; line li is row y, point q at x = t + T q) or columns (line li is
-// column x, point q at y = t + T q).  Point q of the line sits lo + q * QS bytes into an n x n array.
-template <int N, bool COL>
-struct LineGeo {
-  static constexpr int T = N / 16;
-  static constexpr unsigned QS = COL ? (unsigned)(T * N * sizeof(float2)) : (unsigned)(T * sizeof(float2));
-  int t, li;
-  unsigned lo;
-  __device__ __forceinline__ LineGeo(int t_, int li_)
-      : t(t_), li(li_), lo(COL ? (unsigned)((t_ * N + li_) * sizeof(float2)) : (unsigned)((li_ * N + t_) * sizeof(float2))) {}
-  __device__ __forceinline__ int ky(int q) const { return COL ? t + T * q : li; }
-  __device__ __forceinline__ int kx(int q) const { return COL ? li : t + T * q; }
-  __device__ __forceinline__ void load(float2 (&v)[16], const float2* arr) const {
-    const BufView b(arr, (unsigned long long)N * N * sizeof(float2));
-#pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = b.ld2(lo, QS * q);
-  }
-  __device__ __forceinline__ void store(const float2 (&v)[16], float2* arr) const {
-    const BufView b(arr, (unsigned long long)N * N * sizeof(float2));
-#pragma unroll
-    for (int q = 0; q < 16; ++q) b.st2(v[q], lo, QS * q);
-  }
-  // slab offset (within a slice) of point q of the window at (oy, ox), wrapped periodically
-  __device__ __forceinline__ long long slab_offset(const WaveCtx& g, int q) const {
-    int gy = g.oy + ky(q), gx = g.ox + kx(q);
-    if (gy >= g.slab_h) gy -= (int)g.slab_h;
-    if (gx >= g.slab_w) gx -= (int)g.slab_w;
-    return (long long)gy * g.slab_w + gx;
-  }
-};
-
-// The transmission window of slice s along the thread's line.
-template <int N, bool COL>
-__device__ __forceinline__ void load_window(float2 (&tv)[16], const WaveCtx& g, int s, const LineGeo<N, COL>& L) {
-  constexpr int T = N / 16;
-  const float2* slice = g.slab + (long long)s * g.stride;
-  if constexpr (!COL) {
-    int gy = g.oy + L.li;
-    if (gy >= g.slab_h) gy -= (int)g.slab_h;
-    const float2* row = slice + (long long)gy * g.slab_w;
-    if (g.ox + N <= g.slab_w) {
-      const float2* r = row + g.ox + L.t;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) tv[q] = r[T * q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        int x = g.ox + L.t + T * q;
-        if (x >= g.slab_w) x -= (int)g.slab_w;
-        tv[q] = row[x];
-      }
-    }
-  } else {
-    int gx = g.ox + L.li;
-    if (gx >= g.slab_w) gx -= (int)g.slab_w;
-    const long long w = g.slab_w;
-    const unsigned long long slice_bytes = (unsigned long long)g.stride * sizeof(float2);
-    if (g.oy + N <= g.slab_h && slice_bytes <= 0xFFFFFFFFull) {
-      const BufView tb(slice, slice_bytes);
-      const unsigned lo = (unsigned)(((long long)(g.oy + L.t) * w + gx) * sizeof(float2));
-      const unsigned qs = (unsigned)(T * w * sizeof(float2));
-#pragma unroll
-      for (int q = 0; q < 16; ++q) tv[q] = tb.ld2(lo, qs * q);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        int gy = g.oy + L.t + T * q;
-        if (gy >= g.slab_h) gy -= (int)g.slab_h;
-        tv[q] = slice[(long long)gy * w + gx];
-      }
-    }
-  }
-}
 
 template <int N, bool COL, int PRE, int MID, int POST>
 __global__ __launch_bounds__(256) void k_adj_pass(AdjParams a, WaveBuf src, WaveBuf dst, WaveBuf psi_out,
@@ -207,14 +124,14 @@ __global__ __launch_bounds__(256) void k_adj_pass(AdjParams a, WaveBuf src, Wave
   // ---- the slice's pointwise step
   if constexpr (MID == A_MID_T) {
     float2 tv[16];
-    load_window<N, COL>(tv, g, s, L);
+    load_window<N, COL>(tv, g.slab, g, s, L);
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], tv[q]);
   } else if constexpr (MID == A_MID_ADJ) {
     // two groups of eight points: each group issues its transmission and psi loads before using them, and
     // fewer values stay live than with all sixteen at once (128 VGPRs, four waves per SIMD)
     float2 tv[16];
-    load_window<N, COL>(tv, g, s, L);
+    load_window<N, COL>(tv, g.slab, g, s, L);
     const BufView pv(psi_in.at(wl), (unsigned long long)N * N * sizeof(float2));
     const BufView gv(a.gt_contrib ? a.gt_contrib + ((size_t)wl * a.contrib_slices + seg_idx) * N * N : a.grad_slab,
                      (unsigned long long)N * N * sizeof(float2));

@@ -139,26 +139,29 @@ __device__ __forceinline__ void small_dft(float2* v) {
     v[2] = make_float2(t.x - u.x, t.y - u.y);
   } else {
     static_assert(R == 5, "radix");
-    // cos and sin of 2 pi m / 5, m = 0..4
-    constexpr float C[5] = {1.0f, 0.30901699437494745f, -0.80901699437494745f, -0.80901699437494745f,
-                            0.30901699437494745f};
-    constexpr float S[5] = {0.0f, 0.95105651629515357f, 0.58778525229247314f, -0.58778525229247314f,
-                            -0.95105651629515357f};
-    float2 y[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      float re = 0.f, im = 0.f;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        const int m = (q * k) % 5;
-        const float c = C[m], sn = INV ? S[m] : -S[m];
-        re += v[q].x * c - v[q].y * sn;
-        im += v[q].x * sn + v[q].y * c;
-      }
-      y[k] = make_float2(re, im);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) v[k] = y[k];
+    // symmetric form: with s1 = x1 + x4, d1 = x1 - x4, s2 = x2 + x3, d2 = x2 - x3,
+    //   y0 = x0 + s1 + s2,  y1,4 = t1 -+ i u1,  y2,3 = t2 -+ i u2  (forward; the inverse swaps the signs)
+    //   t1 = x0 + c1 s1 + c2 s2,  t2 = x0 + c2 s1 + c1 s2,  u1 = n1 d1 + n2 d2,  u2 = n2 d1 - n1 d2
+    // with c1, c2 = cos(2 pi / 5), cos(4 pi / 5) and n1, n2 = sin(2 pi / 5), sin(4 pi / 5)
+    constexpr float C1 = 0.30901699437494745f, C2 = -0.80901699437494745f;
+    constexpr float N1 = 0.95105651629515357f, N2 = 0.58778525229247314f;
+    const float2 x0 = v[0];
+    const float2 s1 = make_float2(v[1].x + v[4].x, v[1].y + v[4].y);
+    const float2 d1 = make_float2(v[1].x - v[4].x, v[1].y - v[4].y);
+    const float2 s2 = make_float2(v[2].x + v[3].x, v[2].y + v[3].y);
+    const float2 d2 = make_float2(v[2].x - v[3].x, v[2].y - v[3].y);
+    const float2 t1 = make_float2(fmaf(C2, s2.x, fmaf(C1, s1.x, x0.x)), fmaf(C2, s2.y, fmaf(C1, s1.y, x0.y)));
+    const float2 t2 = make_float2(fmaf(C1, s2.x, fmaf(C2, s1.x, x0.x)), fmaf(C1, s2.y, fmaf(C2, s1.y, x0.y)));
+    const float2 u1 = make_float2(fmaf(N2, d2.x, N1 * d1.x), fmaf(N2, d2.y, N1 * d1.y));
+    const float2 u2 = make_float2(fmaf(-N1, d2.x, N2 * d1.x), fmaf(-N1, d2.y, N2 * d1.y));
+    // -i u = (u.y, -u.x)
+    const float2 iu1 = INV ? make_float2(-u1.y, u1.x) : make_float2(u1.y, -u1.x);
+    const float2 iu2 = INV ? make_float2(-u2.y, u2.x) : make_float2(u2.y, -u2.x);
+    v[0] = make_float2(x0.x + s1.x + s2.x, x0.y + s1.y + s2.y);
+    v[1] = make_float2(t1.x + iu1.x, t1.y + iu1.y);
+    v[4] = make_float2(t1.x - iu1.x, t1.y - iu1.y);
+    v[2] = make_float2(t2.x + iu2.x, t2.y + iu2.y);
+    v[3] = make_float2(t2.x - iu2.x, t2.y - iu2.y);
   }
 }
 

@@ -182,10 +182,12 @@ __device__ void butterfly_stage(const float2* a, float2* b, int nl, int L, int l
     divmod(j, ns, rns, jb, lo);
     const float2* src = a + line * ls + j;
     float2 v[R];
+    v[0] = src[0];  // the twiddle of input 0 is 1
+    const int tq = lo * tstep;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
+    for (int q = 1; q < R; ++q) {
       const float2 x = src[q * nb];
-      const float2 w = tw[q * lo * tstep];
+      const float2 w = tw[q * tq];
       const float wi = INV ? -w.y : w.y;
       v[q] = make_float2(x.x * w.x - x.y * wi, x.x * wi + x.y * w.x);
     }
@@ -216,9 +218,9 @@ __device__ void generic_stage(const float2* a, float2* b, int nl, int L, int ls,
     const int j = jr * ns + lo;
     const int step = lo + k * ns;
     const float2* src = a + line * ls + j;
-    float re = 0.f, im = 0.f;
-    int ex = 0;
-    for (int q = 0; q < r; ++q) {
+    float re = src[0].x, im = src[0].y;  // the q = 0 term has the factor 1
+    int ex = step;
+    for (int q = 1; q < r; ++q) {
       const float2 x = src[q * lr];
       const float2 t = tw[ex * tstep];
       const float ti = inv ? -t.y : t.y;

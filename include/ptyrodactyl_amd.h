@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 7
+#define PTY_ABI_VERSION 8
 
 #define PTY_OK 0
 #define PTY_ERR_INVALID 1 /* bad argument: maps to ValueError */
@@ -194,6 +194,22 @@ int pty_adjoint_amplitude(int n, int num_slices, long long num_positions, int nu
  * (src/ptyrodactyl/multislice/simulations.py:105-159) for a whole slab at once.
  */
 int pty_transmission(const double* potential, double sigma, void* transmission, long long count, void* stream);
+
+/* FP64 scratch for pty_sqrt_intensity_loss (one partial sum per workgroup of its fixed grid). */
+size_t pty_sqrt_intensity_loss_workspace_bytes(void);
+
+/*
+ * The square-root-intensity loss of the gradient workloads and its gradient, in one pass over `count` pixels:
+ *   loss = sum (sqrt(I + eps) - sqrt(I_meas))^2        (FP64, fixed summation order; loss may be NULL)
+ *   grad = (sqrt(I + eps) - sqrt(I_meas)) / sqrt(I + eps)   (float32, may be NULL)
+ * The reference's reconstructors reduce MAE / MSE / RMSE losses (_reduce_loss,
+ * src/ptyrodactyl/multislice/multislice_recon.py:127-163); this sqrt-intensity loss has no counterpart there.
+ * It replaces the elementwise tensor evaluation of that loss between the forward and the adjoint of a gradient
+ * step.  patterns, measured and grad are device float32 arrays; loss is one device double; workspace holds
+ * pty_sqrt_intensity_loss_workspace_bytes() bytes.
+ */
+int pty_sqrt_intensity_loss(long long count, const float* patterns, const float* measured, float eps, float* grad,
+                            double* loss, void* workspace, size_t workspace_bytes, void* stream);
 
 #define PTY_MAX_NORMAL_PARAMS 21
 

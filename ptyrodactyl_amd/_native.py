@@ -19,7 +19,7 @@ from .build import LIB_PATH
 PTY_OK = 0
 PTY_ERR_INVALID = 1
 PTY_ERR_DEVICE = 2
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 _lock = threading.Lock()
 _lib = None
@@ -50,6 +50,8 @@ SIGNATURES = {
     "pty_adjoint_amplitude": (c_int, [c_int, c_int, c_ll, c_int, c_vp, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp,
                                       c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "pty_transmission": (c_int, [c_vp, c_dbl, c_vp, c_ll, c_vp]),
+    "pty_sqrt_intensity_loss_workspace_bytes": (c_size, []),
+    "pty_sqrt_intensity_loss": (c_int, [c_ll, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, c_vp, c_size, c_vp]),
     "pty_normal_equations_workspace_bytes": (c_size, [c_ll, c_int, c_int]),
     "pty_normal_equations": (c_int, [c_vp, c_vp, c_ll, c_int, c_int, c_vp, c_size, c_vp, c_vp, c_vp, c_vp]),
     "pty_damped_solve": (c_int, [c_vp, c_vp, c_dbl, c_int, c_vp, c_vp, c_vp, c_vp]),

@@ -29,6 +29,43 @@ hipError_t launch_transmission(const double* pot, double sigma, float2* out, lon
 }
 
 // ---------------------------------------------------------------------------
+// Square-root-intensity loss of the gradient workloads, in one pass:
+//   L = sum_i (sqrt(I_i + eps) - sqrt(M_i))^2   (FP64)
+//   dL/dI_i = (sqrt(I_i + eps) - sqrt(M_i)) / sqrt(I_i + eps)   (float32)
+// A fixed grid of SQRT_LOSS_BLOCKS workgroups writes one FP64 partial each (fixed-order tree inside the
+// workgroup), and one workgroup adds the partials in order, so the loss is bitwise reproducible.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_sqrt_loss(const float* pat, const float* meas, float eps, long long count,
+                                                   float* grad, double* partials) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (long long)SQRT_LOSS_BLOCKS * 256) {
+    const float root = sqrtf(pat[i] + eps);
+    const float diff = root - sqrtf(meas[i]);
+    if (grad) grad[i] = diff / root;
+    acc += (double)diff * (double)diff;
+  }
+  const double total = block_sum_fp64<256>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double* partials, int n, double* out) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += partials[i];
+  const double total = block_sum_fp64<256>(acc, red);
+  if (threadIdx.x == 0) *out = total;
+}
+
+hipError_t launch_sqrt_loss(const float* pat, const float* meas, float eps, long long count, float* grad,
+                            double* loss, double* partials, hipStream_t st) {
+  hipLaunchKernelGGL(k_sqrt_loss, dim3(SQRT_LOSS_BLOCKS), dim3(256), 0, st, pat, meas, eps, count, grad, partials);
+  if (loss) hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partials, SQRT_LOSS_BLOCKS, loss);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Normal equations.  For every detector pixel of every position:
 //   r   = |A|^2 - data                      (A = amp[p][0])
 //   J_j = 2 Re(conj(A) dA_j)                (dA_j = amp[p][1 + j])

@@ -6,6 +6,9 @@
 namespace pty {
 
 hipError_t launch_transmission(const double* pot, double sigma, float2* out, long long count, hipStream_t st);
+constexpr int SQRT_LOSS_BLOCKS = 1024;  // workgroups (and FP64 partials) of the square-root-intensity loss
+hipError_t launch_sqrt_loss(const float* pat, const float* meas, float eps, long long count, float* grad,
+                            double* loss, double* partials, hipStream_t st);
 int normal_partial_blocks(long long positions, int pix);
 hipError_t launch_normal_equations(const float2* amp, const float* data, long long positions, int pix, int Q,
                                    double* partials, double* jtj, double* jtr, double* rss, hipStream_t st);

@@ -403,6 +403,21 @@ int pty_transmission(const double* potential, double sigma, void* transmission, 
                        "pty_transmission");
 }
 
+size_t pty_sqrt_intensity_loss_workspace_bytes(void) { return (size_t)pty::SQRT_LOSS_BLOCKS * sizeof(double); }
+
+int pty_sqrt_intensity_loss(long long count, const float* patterns, const float* measured, float eps, float* grad,
+                            double* loss, void* workspace, size_t workspace_bytes, void* stream) {
+  g_err[0] = '\0';
+  if (!patterns || !measured) return fail(PTY_ERR_INVALID, "null pointer");
+  if (count < 0) return fail(PTY_ERR_INVALID, "count must be >= 0");
+  if (!(eps >= 0.f)) return fail(PTY_ERR_INVALID, "eps must be >= 0");
+  if (!workspace || workspace_bytes < pty_sqrt_intensity_loss_workspace_bytes())
+    return fail(PTY_ERR_INVALID, "workspace smaller than pty_sqrt_intensity_loss_workspace_bytes()");
+  return device_status(pty::launch_sqrt_loss(patterns, measured, eps, count, grad, loss, (double*)workspace,
+                                             (hipStream_t)stream),
+                       "pty_sqrt_intensity_loss");
+}
+
 size_t pty_normal_equations_workspace_bytes(long long num_positions, int n, int num_params) {
   if (num_positions < 1 || n < 1 || num_params < 1 || num_params > PTY_MAX_NORMAL_PARAMS) return 0;
   const int nprod = num_params * (num_params + 1) / 2 + num_params + 1;

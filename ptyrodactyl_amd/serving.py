@@ -147,12 +147,47 @@ def build_inputs(req: Request, device) -> RequestInputs:
                          torch.from_numpy(shifts).to(device), measured, origins, shifts)
 
 
+def _sqrt_loss_kernel(patterns: torch.Tensor, measured: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    """One ``pty_sqrt_intensity_loss`` launch: the FP64 loss (0-dim tensor) and dL/dI (float32)."""
+    if not (patterns.is_cuda and measured.is_cuda) or patterns.shape != measured.shape:
+        raise ValueError("patterns and measured must be device tensors of the same shape")
+    pat = patterns.detach().to(torch.float32).contiguous()
+    meas = measured.detach().to(torch.float32).contiguous()
+    dev = pat.device
+    lib = _native.load()
+    grad = torch.empty_like(pat)
+    loss = torch.empty((), dtype=torch.float64, device=dev)
+    work = torch.empty(int(lib.pty_sqrt_intensity_loss_workspace_bytes()) // 8, dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        status = lib.pty_sqrt_intensity_loss(pat.numel(), _native.ptr(pat), _native.ptr(meas), EPS_SQRT,
+                                             _native.ptr(grad), _native.ptr(loss), _native.ptr(work),
+                                             work.numel() * 8, _native.stream_handle(dev))
+    _native.check(status, "pty_sqrt_intensity_loss")
+    return loss, grad
+
+
+class _SqrtIntensityLoss(torch.autograd.Function):
+    """The kernel's loss as a differentiable scalar of the patterns; backward multiplies the kernel's dL/dI by the
+    incoming gradient."""
+
+    @staticmethod
+    def forward(ctx, patterns, loss, grad):
+        ctx.save_for_backward(grad)
+        return loss.clone()
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        (grad,) = ctx.saved_tensors
+        return (grad * grad_loss.to(grad.dtype)).to(grad.dtype), None, None
+
+
 def loss_and_grad_intensity(patterns: torch.Tensor, measured: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
-    """L = sum (sqrt(I + eps) - sqrt(I_meas))^2 (FP64 reduction) and dL/dI (float32)."""
-    root = torch.sqrt(patterns + EPS_SQRT)
-    diff = root - torch.sqrt(measured)
-    loss = torch.sum(diff.to(torch.float64) ** 2)
-    return loss, (diff / root).to(torch.float32).contiguous()
+    """L = sum (sqrt(I + eps) - sqrt(I_meas))^2 (FP64, fixed summation order) and dL/dI (float32), from one fused
+    ``pty_sqrt_intensity_loss`` launch.  The loss is differentiable w.r.t. the patterns when they require grad."""
+    loss, grad = _sqrt_loss_kernel(patterns, measured)
+    if patterns.requires_grad and torch.is_grad_enabled():
+        loss = _SqrtIntensityLoss.apply(patterns, loss, grad)
+    return loss, grad
 
 
 def make_request_stream(count: int, qps: float, seed: int = 0, sizes=(64, 128, 256), slices=(8, 16, 32, 64),

@@ -135,3 +135,17 @@ def test_grid_workspace_holds_every_mode_of_a_position():
     assert fwd >= twiddles + 2 * wave
     adj = engine.grid_workspace_bytes(h, w, s_count, k, 1, 2, adjoint=True)
     assert adj >= twiddles + 256 + 2 * (k + 2) * wave
+
+
+def test_sqrt_loss_validates_on_host():
+    from ptyrodactyl_amd import _native
+    if not _native.LIB_PATH.exists():
+        pytest.skip("HIP library not built")
+    lib = _native.load()
+    assert lib.pty_sqrt_intensity_loss_workspace_bytes() == 1024 * 8
+    assert lib.pty_sqrt_intensity_loss(4, None, None, 1e-6, None, None, None, 0, None) == _native.PTY_ERR_INVALID
+    assert b"null pointer" in lib.pty_last_error()
+    fake = 1 << 20  # never dereferenced: the workspace check fails first
+    rc = lib.pty_sqrt_intensity_loss(4, fake, fake, 1e-6, None, None, fake, 8, None)
+    assert rc == _native.PTY_ERR_INVALID and b"workspace" in lib.pty_last_error()
+    assert lib.pty_sqrt_intensity_loss(-1, fake, fake, 1e-6, None, None, fake, 8192, None) == _native.PTY_ERR_INVALID

@@ -109,7 +109,64 @@ __device__ __forceinline__ float2 ramp1(int kidx, int L, int m, float d) {
   return make_float2(c, s);
 }
 
-// In-register DFT of R = 2, 3, 4 or 5 points, y_k = sum_q x_q exp(-+2 pi i q k / R) (INV: +).
+// cos and sin of 2 pi j k / R for j, k = 1 .. (R - 1) / 2, R odd, evaluated in double at compile time (integer
+// reduction of j k to [0, R / 2], then the Taylor series of cx_sincos_octant, accurate to 1e-16 up to pi).
+template <int R>
+struct OddDftConsts {
+  static constexpr int H = (R - 1) / 2;
+  float c[H][H], s[H][H];  // [k - 1][j - 1]
+  constexpr OddDftConsts() : c{}, s{} {
+    for (int k = 1; k <= H; ++k)
+      for (int j = 1; j <= H; ++j) {
+        long long m = (long long)j * k % R;
+        double sign = 1.0;
+        if (2 * m > R) {
+          m = R - m;
+          sign = -1.0;
+        }
+        double sn = 0.0, cs = 0.0;
+        cx_sincos_octant(m, R, sn, cs);
+        c[k - 1][j - 1] = (float)cs;
+        s[k - 1][j - 1] = (float)(sign * sn);
+      }
+  }
+};
+
+// R-point DFT for an odd R, by the symmetric pairs s_j = x_j + x_{R-j}, d_j = x_j - x_{R-j} (j = 1 .. H):
+//   y_0 = x_0 + sum_j s_j,  y_k, y_{R-k} = t_k -+ i u_k  (forward; the inverse swaps the signs)
+//   t_k = x_0 + sum_j cos(2 pi j k / R) s_j,  u_k = sum_j sin(2 pi j k / R) d_j.
+// 2 H^2 complex multiply-adds instead of the (R - 1)^2 complex products of the direct sum.
+template <int R, bool INV>
+__device__ __forceinline__ void small_dft_odd(float2* v) {
+  constexpr int H = (R - 1) / 2;
+  constexpr OddDftConsts<R> K{};
+  const float2 x0 = v[0];
+  float2 sp[H], dm[H];
+  float2 y0 = x0;
+#pragma unroll
+  for (int j = 1; j <= H; ++j) {
+    sp[j - 1] = make_float2(v[j].x + v[R - j].x, v[j].y + v[R - j].y);
+    dm[j - 1] = make_float2(v[j].x - v[R - j].x, v[j].y - v[R - j].y);
+    y0 = make_float2(y0.x + sp[j - 1].x, y0.y + sp[j - 1].y);
+  }
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    float2 t = x0;
+    float2 u = make_float2(K.s[k - 1][0] * dm[0].x, K.s[k - 1][0] * dm[0].y);
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      t = make_float2(fmaf(K.c[k - 1][j - 1], sp[j - 1].x, t.x), fmaf(K.c[k - 1][j - 1], sp[j - 1].y, t.y));
+      if (j > 1) u = make_float2(fmaf(K.s[k - 1][j - 1], dm[j - 1].x, u.x), fmaf(K.s[k - 1][j - 1], dm[j - 1].y, u.y));
+    }
+    // -i u = (u.y, -u.x)
+    const float2 iu = INV ? make_float2(-u.y, u.x) : make_float2(u.y, -u.x);
+    v[k] = make_float2(t.x + iu.x, t.y + iu.y);
+    v[R - k] = make_float2(t.x - iu.x, t.y - iu.y);
+  }
+  v[0] = y0;
+}
+
+// In-register DFT of R = 2, 3, 4, 5, 7, 11 or 13 points, y_k = sum_q x_q exp(-+2 pi i q k / R) (INV: +).
 template <int R, bool INV>
 __device__ __forceinline__ void small_dft(float2* v) {
   if constexpr (R == 2) {
@@ -137,6 +194,8 @@ __device__ __forceinline__ void small_dft(float2* v) {
     v[0] = make_float2(v[0].x + s12.x, v[0].y + s12.y);
     v[1] = make_float2(t.x + u.x, t.y + u.y);
     v[2] = make_float2(t.x - u.x, t.y - u.y);
+  } else if constexpr (R == 7 || R == 11 || R == 13) {
+    small_dft_odd<R, INV>(v);
   } else {
     static_assert(R == 5, "radix");
     // symmetric form: with s1 = x1 + x4, d1 = x1 - x4, s2 = x2 + x3, d2 = x2 - x3,
@@ -165,7 +224,7 @@ __device__ __forceinline__ void small_dft(float2* v) {
   }
 }
 
-// One Stockham stage of radix R in {2, 3, 4, 5}: a thread takes a butterfly (line, j), j < L / R,
+// One Stockham stage of radix R in {2, 3, 4, 5, 7, 11, 13}: a thread takes a butterfly (line, j), j < L / R,
 // twiddles its R inputs by w^(q lo) (w = exp(-+2 pi i / (ns R)), lo = j mod ns), runs the R-point
 // DFT and writes outputs (j / ns) ns R + lo + k ns.
 template <int R, bool INV>
@@ -235,7 +294,7 @@ __device__ void generic_stage(const float2* a, float2* b, int nl, int L, int ls,
 
 // Transforms `nl` lines held at stride ax.ls in `a` (ping-pong with `b`); returns the buffer that
 // holds the result.  Ends with a barrier.
-template <bool INV>
+template <bool INV, bool WIDE>
 __device__ float2* fft_lines_dir(float2* a, float2* b, int nl, const Axis& ax) {
   const int L = ax.L, ls = ax.ls;
   int ns = 1;
@@ -246,7 +305,23 @@ __device__ float2* fft_lines_dir(float2* a, float2* b, int nl, const Axis& ax) {
       case 3: butterfly_stage<3, INV>(a, b, nl, L, ls, ns, ax.tw); break;
       case 4: butterfly_stage<4, INV>(a, b, nl, L, ls, ns, ax.tw); break;
       case 5: butterfly_stage<5, INV>(a, b, nl, L, ls, ns, ax.tw); break;
-      default: generic_stage(a, b, nl, L, ls, ns, r, ax.tw, INV); break;
+      default:
+        if constexpr (WIDE) {
+          if (r == 7) {
+            butterfly_stage<7, INV>(a, b, nl, L, ls, ns, ax.tw);
+            break;
+          }
+          if (r == 11) {
+            butterfly_stage<11, INV>(a, b, nl, L, ls, ns, ax.tw);
+            break;
+          }
+          if (r == 13) {
+            butterfly_stage<13, INV>(a, b, nl, L, ls, ns, ax.tw);
+            break;
+          }
+        }
+        generic_stage(a, b, nl, L, ls, ns, r, ax.tw, INV);
+        break;
     }
     __syncthreads();
     float2* t = a;
@@ -257,8 +332,9 @@ __device__ float2* fft_lines_dir(float2* a, float2* b, int nl, const Axis& ax) {
   return a;
 }
 
+template <bool WIDE>
 __device__ float2* fft_lines(float2* a, float2* b, int nl, const Axis& ax, bool inv) {
-  return inv ? fft_lines_dir<true>(a, b, nl, ax) : fft_lines_dir<false>(a, b, nl, ax);
+  return inv ? fft_lines_dir<true, WIDE>(a, b, nl, ax) : fft_lines_dir<false, WIDE>(a, b, nl, ax);
 }
 
 enum { NONE = 0, FWD = 1, INV = 2 };
@@ -267,7 +343,7 @@ enum { NONE = 0, FWD = 1, INV = 2 };
 // element index is the row), and global accesses put consecutive threads on consecutive columns;
 // otherwise a line is a row.  load(line, e) -> value; transform fft1; mid(line, e, value) -> value
 // (if has_mid); transform fft2; store(line, e, value).
-template <bool COLS, class Load, class Mid, class Store>
+template <bool COLS, bool WIDE, class Load, class Mid, class Store>
 __device__ void line_pass(int l_begin, int l_end, int g_max, const Axis& ax, float2* bufa, float2* bufb, int fft1,
                           bool has_mid, int fft2, Load load, Mid mid, Store store) {
   const int L = ax.L, ls = ax.ls;
@@ -286,7 +362,7 @@ __device__ void line_pass(int l_begin, int l_end, int g_max, const Axis& ax, flo
     float2* cur = bufa;
     float2* oth = bufb;
     if (fft1 != NONE) {
-      cur = fft_lines(cur, oth, g, ax, fft1 == INV);
+      cur = fft_lines<WIDE>(cur, oth, g, ax, fft1 == INV);
       oth = cur == bufa ? bufb : bufa;
     }
     if (has_mid) {
@@ -298,7 +374,7 @@ __device__ void line_pass(int l_begin, int l_end, int g_max, const Axis& ax, flo
       }
       __syncthreads();
     }
-    if (fft2 != NONE) cur = fft_lines(cur, oth, g, ax, fft2 == INV);
+    if (fft2 != NONE) cur = fft_lines<WIDE>(cur, oth, g, ax, fft2 == INV);
     {
       Walk wk(threadIdx.x, COLS ? g : L);
       for (int o = threadIdx.x; o < total; o += GT, wk.step()) {
@@ -380,12 +456,13 @@ __device__ __forceinline__ long long slab_index(const GridParams& a, const Posit
 
 // Incident wave of mode m: rows of probe_k times the shift ramp (with the 1/(h w) of the inverse
 // transform), inverse transform along x.
+template <bool WIDE>
 __device__ void incident_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
                               const Position& q, int m, float2* wb) {
   const int h = a.h, w = a.w;
   const float inv_hw = 1.0f / ((float)h * (float)w);
   const float2* pk = a.probe_k + (long long)m * h * w;
-  line_pass<false>(
+  line_pass<false, WIDE>(
       l0, l1, a.gx, ax, ba, bb, INV, false, NONE,
       [&](int ky, int kx) {
         const float2 r = gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx));
@@ -395,11 +472,11 @@ __device__ void incident_rows(const GridParams& a, const Axis& ax, float2* ba, f
 }
 
 // Rows: forward along x, times Py Px (or its conjugate), inverse along x.
-template <bool CONJ>
+template <bool CONJ, bool WIDE>
 __device__ void propagate_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
                                float2* wb) {
   const int w = a.w;
-  line_pass<false>(
+  line_pass<false, WIDE>(
       l0, l1, a.gx, ax, ba, bb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
       [&](int ky, int kx, float2 v) {
         const float2 f = gmul(a.prop_y[ky], a.prop_x[kx]);
@@ -409,11 +486,12 @@ __device__ void propagate_rows(const GridParams& a, const Axis& ax, float2* ba, 
 }
 
 // Columns of slice s: to real space along y, checkpoint (if ck), times T_s, back along y.
+template <bool WIDE>
 __device__ void slice_cols(const GridParams& a, const Axis& ay, float2* ba, float2* bb, int l0, int l1,
                            const Position& q, int s, float2* wb, float2* ck) {
   const int w = a.w;
   const float2* ts = a.slab + (long long)s * a.slab_h * a.slab_w;
-  line_pass<true>(
+  line_pass<true, WIDE>(
       l0, l1, a.gy, ay, ba, bb, INV, true, FWD, [&](int x, int ky) { return wb[(long long)ky * w + x]; },
       [&](int x, int y, float2 v) {
         if (ck) ck[(long long)y * w + x] = v;
@@ -424,6 +502,7 @@ __device__ void slice_cols(const GridParams& a, const Axis& ay, float2* ba, floa
 
 // Rows after the last slice: forward along x gives the detector amplitude A(ky, kx) of mode m at
 // position p; writes it fftshifted and adds w_m |A|^2 to the pattern (stores it for m = 0).
+template <bool WIDE>
 __device__ void final_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
                            long long p, int m, float2* wb) {
   const int h = a.h, w = a.w;
@@ -431,7 +510,7 @@ __device__ void final_rows(const GridParams& a, const Axis& ax, float2* ba, floa
   const float wm = a.weights[m];
   float* pat = a.patterns ? a.patterns + p * hw : nullptr;
   float2* amp = a.amplitude ? a.amplitude + (p * a.M + m) * hw : nullptr;
-  line_pass<false>(
+  line_pass<false, WIDE>(
       l0, l1, a.gx, ax, ba, bb, FWD, false, NONE, [&](int ky, int x) { return wb[(long long)ky * w + x]; }, Ident{},
       [&](int ky, int kx, float2 v) {
         const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
@@ -449,6 +528,7 @@ __device__ __forceinline__ float2* checkpoint_of(const GridParams& a, long long 
   return a.ckpt + ((p * a.M + m) * nc + (s / a.ckpt_every - 1)) * (long long)a.h * a.w;
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_grid_forward(GridParams a, float2* ws) {
   extern __shared__ float2 lds[];
   __shared__ int s_rad[2 * GRID_MAX_STAGES];
@@ -466,13 +546,13 @@ __global__ __launch_bounds__(GT) void k_grid_forward(GridParams a, float2* ws) {
     const int m1 = per_wave ? m0 + 1 : a.M;
     const Position q = position(a, p);
     for (int m = m0; m < m1; ++m) {
-      incident_rows(a, u.ax, u.bufa, u.bufb, 0, h, q, m, wb);
+      incident_rows<WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, q, m, wb);
       for (int s = 0; s < a.S; ++s) {
-        slice_cols(a, u.ay, u.bufa, u.bufb, 0, w, q, s, wb, checkpoint_of(a, p, m, s));
+        slice_cols<WIDE>(a, u.ay, u.bufa, u.bufb, 0, w, q, s, wb, checkpoint_of(a, p, m, s));
         if (s + 1 < a.S)
-          propagate_rows<false>(a, u.ax, u.bufa, u.bufb, 0, h, wb);
+          propagate_rows<false, WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, wb);
         else
-          final_rows(a, u.ax, u.bufa, u.bufb, 0, h, p, m, wb);
+          final_rows<WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, p, m, wb);
       }
     }
   }
@@ -512,6 +592,7 @@ __device__ Axis stream_axis(const GridParams& a, const float2* twg, float2* lds,
 }
 
 // Blocks b = group * nw + wave; wave wl lives at waves + wl * stride.
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gs_incident(GridParams a, const float2* twg, float2* waves, long long stride,
                                                      long long p0, int nw) {
   extern __shared__ float2 lds[];
@@ -520,9 +601,10 @@ __global__ __launch_bounds__(GT) void k_gs_incident(GridParams a, const float2* 
   const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
   const long long p = p0 + wl / a.M;
-  incident_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p), wl % a.M, waves + (long long)wl * stride);
+  incident_rows<WIDE>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p), wl % a.M, waves + (long long)wl * stride);
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gs_cols(GridParams a, const float2* twg, float2* waves, long long p0, int nw,
                                                  int s) {
   extern __shared__ float2 lds[];
@@ -532,11 +614,11 @@ __global__ __launch_bounds__(GT) void k_gs_cols(GridParams a, const float2* twg,
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gy;
   const long long p = p0 + wl / a.M;
   const int m = wl % a.M;
-  slice_cols(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p), s, waves + (long long)wl * a.h * a.w,
+  slice_cols<WIDE>(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p), s, waves + (long long)wl * a.h * a.w,
              checkpoint_of(a, p, m, s));
 }
 
-template <bool CONJ>
+template <bool CONJ, bool WIDE>
 __global__ __launch_bounds__(GT) void k_gs_rows(GridParams a, const float2* twg, float2* waves, long long stride,
                                                  int nw) {
   extern __shared__ float2 lds[];
@@ -544,11 +626,12 @@ __global__ __launch_bounds__(GT) void k_gs_rows(GridParams a, const float2* twg,
   float2 *ba, *bb;
   const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
-  propagate_rows<CONJ>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), waves + (long long)wl * stride);
+  propagate_rows<CONJ, WIDE>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), waves + (long long)wl * stride);
 }
 
 // Blocks b = group * np + position; each block runs the modes of its position in order, so the pattern
 // sum over modes has one writer per pixel.
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gs_final(GridParams a, const float2* twg, float2* waves, long long p0,
                                                   int np) {
   extern __shared__ float2 lds[];
@@ -557,7 +640,7 @@ __global__ __launch_bounds__(GT) void k_gs_final(GridParams a, const float2* twg
   const Axis ax = stream_axis<false>(a, twg, lds, s_rad, ba, bb);
   const int pl = blockIdx.x % np, l0 = (blockIdx.x / np) * a.gx;
   for (int m = 0; m < a.M; ++m)
-    final_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), p0 + pl, m,
+    final_rows<WIDE>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), p0 + pl, m,
                waves + ((long long)pl * a.M + m) * a.h * a.w);
 }
 
@@ -566,12 +649,13 @@ __global__ __launch_bounds__(GT) void k_gs_final(GridParams a, const float2* twg
 // Recompute pass over columns of slice s: the wave entering slice s, from a checkpoint (real space) or
 // from wb ((ky, x) layout, inverse along y), is stored in sg; if `advance`, times T_s and forward along y
 // into wb.
+template <bool WIDE>
 __device__ void recompute_cols(const GridParams& a, const Axis& ay, float2* ba, float2* bb, int l0, int l1,
                                const Position& q, int s, const float2* src, bool from_ckpt, bool advance, float2* sg,
                                float2* wb) {
   const int w = a.w;
   const float2* ts = a.slab + (long long)s * a.slab_h * a.slab_w;
-  line_pass<true>(
+  line_pass<true, WIDE>(
       l0, l1, a.gy, ay, ba, bb, from_ckpt ? NONE : INV, true, advance ? FWD : NONE,
       [&](int x, int e) { return src[(long long)e * w + x]; },
       [&](int x, int y, float2 v) {
@@ -585,6 +669,7 @@ __device__ void recompute_cols(const GridParams& a, const Axis& ay, float2* ba, 
 
 // Rows after the last slice: detector amplitude, seed dL/dA (from dL/dI or given), adjoint of the forward
 // x transform into gb.
+template <bool WIDE>
 __device__ void seed_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1, long long p,
                           int m, const float2* wb, float2* gb) {
   const int h = a.h, w = a.w;
@@ -592,7 +677,7 @@ __device__ void seed_rows(const GridParams& a, const Axis& ax, float2* ba, float
   const float wm = a.weights[m];
   const float* gpat = a.grad_patterns ? a.grad_patterns + p * hw : nullptr;
   const float2* gamp = a.grad_amplitude ? a.grad_amplitude + (p * a.M + m) * hw : nullptr;
-  line_pass<false>(
+  line_pass<false, WIDE>(
       l0, l1, a.gx, ax, ba, bb, FWD, true, INV, [&](int ky, int x) { return wb[(long long)ky * w + x]; },
       [&](int ky, int kx, float2 amp) {
         const long long o = (long long)shifted_index(ky, h) * w + shifted_index(kx, w);
@@ -604,13 +689,14 @@ __device__ void seed_rows(const GridParams& a, const Axis& ax, float2* ba, float
 
 // Back through slice s over columns: dL/dphi (inverse along y), dL/dT_s += conj(psi_s) dL/dphi (float32
 // atomics), dL/dpsi_s = conj(T_s) dL/dphi, forward along y into gb.
+template <bool WIDE>
 __device__ void back_cols(const GridParams& a, const Axis& ay, float2* ba, float2* bb, int l0, int l1,
                           const Position& q, int s, const float2* sg, float2* gb) {
   const int w = a.w;
   const long long sstride = a.slab_h * a.slab_w;
   const float2* ts = a.slab + (long long)s * sstride;
   float2* gs = a.grad_slab + (long long)s * sstride;
-  line_pass<true>(
+  line_pass<true, WIDE>(
       l0, l1, a.gy, ay, ba, bb, INV, true, FWD, [&](int x, int ky) { return gb[(long long)ky * w + x]; },
       [&](int x, int y, float2 g) {
         const long long si = slab_index(a, q, y, x);
@@ -624,6 +710,7 @@ __device__ void back_cols(const GridParams& a, const Axis& ay, float2* ba, float
 
 // Rows of dL/dpsi_0, forward along x: dL/d(probe_k) += conj(R) dL/du (float32 atomics) and this thread's
 // share of dL/dshift = Re sum conj(dL/du) du/dshift, u = probe_k R.
+template <bool WIDE>
 __device__ void probe_rows(const GridParams& a, const Axis& ax, float2* ba, float2* bb, int l0, int l1,
                            const Position& q, int m, const float2* gb, double& gsy, double& gsx) {
   const int h = a.h, w = a.w;
@@ -633,7 +720,7 @@ __device__ void probe_rows(const GridParams& a, const Axis& ax, float2* ba, floa
   const float2* pk = a.probe_k + (long long)m * hw;
   float2* gp = a.grad_probe_k ? a.grad_probe_k + (long long)m * hw : nullptr;
   const bool want_shift = a.grad_shifts != nullptr;
-  line_pass<false>(
+  line_pass<false, WIDE>(
       l0, l1, a.gx, ax, ba, bb, FWD, true, NONE, [&](int ky, int x) { return gb[(long long)ky * w + x]; },
       [&](int ky, int kx, float2 gu) {
         const float2 r = gscale(gmul(ramp1(ky, h, q.my, q.dy), ramp1(kx, w, q.mx, q.dx)), inv_hw);
@@ -659,6 +746,7 @@ __device__ __forceinline__ const float2* checkpoint_in(const GridParams& a, long
   return a.ckpt + ((p * a.M + m) * nc + (c - 1)) * (long long)a.h * a.w;
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
   __shared__ int s_rad[2 * GRID_MAX_STAGES];
   __shared__ double s_red[GT / 64];
@@ -679,24 +767,24 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
         const int s0 = c * k;
         const int s1 = min(a.S, s0 + k);
         // recompute the waves entering slices s0 .. s1-1 (and the detector seed in the last segment)
-        if (c == 0) incident_rows(a, u.ax, u.bufa, u.bufb, 0, h, q, m, wb);
+        if (c == 0) incident_rows<WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, q, m, wb);
         for (int s = s0; s < s1; ++s) {
           const bool from_ckpt = c > 0 && s == s0;
           const bool advance = s + 1 < s1 || s == a.S - 1;
-          recompute_cols(a, u.ay, u.bufa, u.bufb, 0, w, q, s, from_ckpt ? checkpoint_in(a, p, m, c) : wb, from_ckpt,
-                         advance, seg + (long long)(s - s0) * hw, wb);
+          recompute_cols<WIDE>(a, u.ay, u.bufa, u.bufb, 0, w, q, s, from_ckpt ? checkpoint_in(a, p, m, c) : wb,
+                               from_ckpt, advance, seg + (long long)(s - s0) * hw, wb);
           if (s + 1 < s1)
-            propagate_rows<false>(a, u.ax, u.bufa, u.bufb, 0, h, wb);
+            propagate_rows<false, WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, wb);
           else if (s == a.S - 1)
-            seed_rows(a, u.ax, u.bufa, u.bufb, 0, h, p, m, wb, gb);
+            seed_rows<WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, p, m, wb, gb);
         }
         // back through slices s1-1 .. s0
         for (int s = s1 - 1; s >= s0; --s) {
-          back_cols(a, u.ay, u.bufa, u.bufb, 0, w, q, s, seg + (long long)(s - s0) * hw, gb);
+          back_cols<WIDE>(a, u.ay, u.bufa, u.bufb, 0, w, q, s, seg + (long long)(s - s0) * hw, gb);
           if (s > 0)
-            propagate_rows<true>(a, u.ax, u.bufa, u.bufb, 0, h, gb);
+            propagate_rows<true, WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, gb);
           else
-            probe_rows(a, u.ax, u.bufa, u.bufb, 0, h, q, m, gb, gsy, gsx);
+            probe_rows<WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, q, m, gb, gsy, gsx);
         }
       }
     }
@@ -715,6 +803,7 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
 // Streaming adjoint: per wave the workspace holds (k + 2) h w elements (forward wave, gradient wave, k
 // segment waves); every pass is one launch over (wave, line group) workgroups.  Blocks b = group * nw + wave.
 
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gsa_recompute(GridParams a, const float2* twg, float2* base, long long p0,
                                                        int nw, int s, int c) {
   extern __shared__ float2 lds[];
@@ -730,10 +819,11 @@ __global__ __launch_bounds__(GT) void k_gsa_recompute(GridParams a, const float2
   const bool from_ckpt = c > 0 && s == s0;
   const bool advance = s + 1 < s1 || s == a.S - 1;
   float2* wb = base + (long long)wl * (k + 2) * hw;
-  recompute_cols(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p), s,
+  recompute_cols<WIDE>(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p), s,
                  from_ckpt ? checkpoint_in(a, p, m, c) : wb, from_ckpt, advance, wb + (2 + s - s0) * hw, wb);
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gsa_seed(GridParams a, const float2* twg, float2* base, long long p0,
                                                   int nw) {
   extern __shared__ float2 lds[];
@@ -743,9 +833,10 @@ __global__ __launch_bounds__(GT) void k_gsa_seed(GridParams a, const float2* twg
   const long long hw = (long long)a.h * a.w;
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
   float2* wb = base + (long long)wl * (a.ckpt_every + 2) * hw;
-  seed_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), p0 + wl / a.M, wl % a.M, wb, wb + hw);
+  seed_rows<WIDE>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), p0 + wl / a.M, wl % a.M, wb, wb + hw);
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gsa_back(GridParams a, const float2* twg, float2* base, long long p0, int nw,
                                                   int s, int s0) {
   extern __shared__ float2 lds[];
@@ -755,12 +846,13 @@ __global__ __launch_bounds__(GT) void k_gsa_back(GridParams a, const float2* twg
   const long long hw = (long long)a.h * a.w;
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gy;
   float2* wb = base + (long long)wl * (a.ckpt_every + 2) * hw;
-  back_cols(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p0 + wl / a.M), s, wb + (2 + s - s0) * hw,
+  back_cols<WIDE>(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p0 + wl / a.M), s, wb + (2 + s - s0) * hw,
             wb + hw);
 }
 
 // Probe gradient rows; with shift gradients each block writes its FP64 partial sums to
 // partials[2 b], partials[2 b + 1].
+template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gsa_probe(GridParams a, const float2* twg, float2* base, long long p0, int nw,
                                                    double* partials) {
   extern __shared__ float2 lds[];
@@ -772,7 +864,7 @@ __global__ __launch_bounds__(GT) void k_gsa_probe(GridParams a, const float2* tw
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gx;
   float2* wb = base + (long long)wl * (a.ckpt_every + 2) * hw;
   double gsy = 0.0, gsx = 0.0;
-  probe_rows(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p0 + wl / a.M), wl % a.M, wb + hw, gsy, gsx);
+  probe_rows<WIDE>(a, ax, ba, bb, l0, min(a.h, l0 + a.gx), position(a, p0 + wl / a.M), wl % a.M, wb + hw, gsy, gsx);
   if (a.grad_shifts) {
     const double ty = block_sum_fp64<GT>(gsy, s_red);
     const double tx = block_sum_fp64<GT>(gsx, s_red);
@@ -814,6 +906,17 @@ int factor_edge(int L, int* r) {
   return n;
 }
 
+// Whether an edge has a radix-7, -11 or -13 stage.  The per-position kernels come in two builds: with the
+// symmetric butterflies of those radices (which take more registers: three waves per SIMD instead of four) and
+// without them (those stages then run as generic one-output-per-thread stages).  The streaming kernels always
+// have them: their LDS use holds them at four waves per SIMD either way.
+bool wide_radices(const GridParams& a) {
+  for (int i = 0; i < GRID_MAX_STAGES; ++i)
+    for (int r : {a.ry[i], a.rx[i]})
+      if (r == 7 || r == 11 || r == 13) return true;
+  return false;
+}
+
 int lds_stride(int L) { return L % 2 == 0 ? L + 1 : L; }
 
 size_t prepare(GridParams& a) {
@@ -849,6 +952,92 @@ long long blocks_for(long long P, size_t per_block, size_t ws_bytes) {
   return std::min(std::min(P, (long long)MAX_BLOCKS), fit);
 }
 
+// Streaming forward: one launch per pass over (wave, line group) workgroups, batches of bpos positions.
+template <bool WIDE>
+hipError_t stream_forward(const GridParams& a, float2* tw, float2* waves, long long bpos, hipStream_t st) {
+  const long long hw = (long long)a.h * a.w;
+  const size_t lds_rows = (size_t)(a.w + 2 * a.gx * a.lsx) * sizeof(float2);
+  const size_t lds_cols = (size_t)(a.h + 2 * a.gy * a.lsy) * sizeof(float2);
+  hipError_t e = set_lds((const void*)k_gs_incident<WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<false, WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gs_final<WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gs_cols<WIDE>, lds_cols);
+  if (e != hipSuccess) return e;
+  const long long row_groups = (a.h + a.gx - 1) / a.gx;
+  const long long col_groups = (a.w + a.gy - 1) / a.gy;
+  hipLaunchKernelGGL(k_grid_twiddles, dim3((unsigned)((a.h + a.w + GT - 1) / GT)), dim3(GT), 0, st, a.h, a.w, tw);
+  for (long long p0 = 0; p0 < a.P; p0 += bpos) {
+    const int np = (int)std::min(bpos, a.P - p0);
+    const int nw = np * a.M;
+    hipLaunchKernelGGL(k_gs_incident<WIDE>, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves,
+                       hw, p0, nw);
+    for (int s = 0; s < a.S; ++s) {
+      hipLaunchKernelGGL(k_gs_cols<WIDE>, dim3((unsigned)(nw * col_groups)), dim3(GT), lds_cols, st, a, tw, waves, p0,
+                         nw, s);
+      if (s + 1 < a.S)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gs_rows<false, WIDE>), dim3((unsigned)(nw * row_groups)), dim3(GT),
+                           lds_rows, st, a, tw, waves, hw, nw);
+      else
+        hipLaunchKernelGGL(k_gs_final<WIDE>, dim3((unsigned)(np * row_groups)), dim3(GT), lds_rows, st, a, tw, waves,
+                           p0, np);
+    }
+  }
+  return hipGetLastError();
+}
+
+// Streaming adjoint: per batch of bpos positions, the segment recompute and the reverse passes, one launch per
+// pass over (wave, line group) workgroups.
+template <bool WIDE>
+hipError_t stream_adjoint(const GridParams& a, float2* tw, double* partials, float2* base, long long bpos,
+                          hipStream_t st) {
+  const long long hw = (long long)a.h * a.w;
+  const int k = a.ckpt_every;
+  const long long row_groups = (a.h + a.gx - 1) / a.gx;
+  const long long col_groups = (a.w + a.gy - 1) / a.gy;
+  const long long stride = (long long)(k + 2) * hw;
+  const size_t lds_rows = (size_t)(a.w + 2 * a.gx * a.lsx) * sizeof(float2);
+  const size_t lds_cols = (size_t)(a.h + 2 * a.gy * a.lsy) * sizeof(float2);
+  hipError_t e = set_lds((const void*)k_gs_incident<WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<false, WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<true, WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gsa_seed<WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gsa_probe<WIDE>, lds_rows);
+  if (e == hipSuccess) e = set_lds((const void*)k_gsa_recompute<WIDE>, lds_cols);
+  if (e == hipSuccess) e = set_lds((const void*)k_gsa_back<WIDE>, lds_cols);
+  if (e != hipSuccess) return e;
+  const int nc = num_checkpoints(a.S, k);
+  hipLaunchKernelGGL(k_grid_twiddles, dim3((unsigned)((a.h + a.w + GT - 1) / GT)), dim3(GT), 0, st, a.h, a.w, tw);
+  for (long long p0 = 0; p0 < a.P; p0 += bpos) {
+    const int np = (int)std::min(bpos, a.P - p0);
+    const int nw = np * a.M;
+    const dim3 rows((unsigned)(nw * row_groups)), cols((unsigned)(nw * col_groups));
+    for (int c = nc; c >= 0; --c) {
+      const int s0 = c * k, s1 = std::min(a.S, s0 + k);
+      if (c == 0) hipLaunchKernelGGL(k_gs_incident<WIDE>, rows, dim3(GT), lds_rows, st, a, tw, base, stride, p0, nw);
+      for (int s = s0; s < s1; ++s) {
+        hipLaunchKernelGGL(k_gsa_recompute<WIDE>, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, c);
+        if (s + 1 < s1)
+          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gs_rows<false, WIDE>), rows, dim3(GT), lds_rows, st, a, tw, base,
+                             stride, nw);
+        else if (s == a.S - 1)
+          hipLaunchKernelGGL(k_gsa_seed<WIDE>, rows, dim3(GT), lds_rows, st, a, tw, base, p0, nw);
+      }
+      for (int s = s1 - 1; s >= s0; --s) {
+        hipLaunchKernelGGL(k_gsa_back<WIDE>, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, s0);
+        if (s > 0)
+          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gs_rows<true, WIDE>), rows, dim3(GT), lds_rows, st, a, tw, base + hw,
+                             stride, nw);
+        else
+          hipLaunchKernelGGL(k_gsa_probe<WIDE>, rows, dim3(GT), lds_rows, st, a, tw, base, p0, nw, partials);
+      }
+    }
+    if (a.grad_shifts)
+      hipLaunchKernelGGL(k_gsa_shift_reduce, dim3((unsigned)((np + GT - 1) / GT)), dim3(GT), 0, st, a,
+                         (const double*)partials, p0, np, (int)row_groups);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
 
 size_t grid_block_bytes(int h, int w, int ckpt_every, bool adjoint) {
@@ -879,40 +1068,19 @@ hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStrea
     float2* tw = (float2*)ws;
     float2* waves = (float2*)((char*)ws + twb);
     const long long bpos = std::min(a.P, (long long)((ws_bytes - twb) / per_pos));
-    const size_t lds_rows = (size_t)(a.w + 2 * a.gx * a.lsx) * sizeof(float2);
-    const size_t lds_cols = (size_t)(a.h + 2 * a.gy * a.lsy) * sizeof(float2);
-    hipError_t e = set_lds((const void*)k_gs_incident, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<false>, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gs_final, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gs_cols, lds_cols);
-    if (e != hipSuccess) return e;
-    const long long row_groups = (a.h + a.gx - 1) / a.gx;
-    const long long col_groups = (a.w + a.gy - 1) / a.gy;
-    hipLaunchKernelGGL(k_grid_twiddles, dim3((unsigned)((a.h + a.w + GT - 1) / GT)), dim3(GT), 0, st, a.h, a.w, tw);
-    for (long long p0 = 0; p0 < a.P; p0 += bpos) {
-      const int np = (int)std::min(bpos, a.P - p0);
-      const int nw = np * a.M;
-      hipLaunchKernelGGL(k_gs_incident, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, hw,
-                         p0, nw);
-      for (int s = 0; s < a.S; ++s) {
-        hipLaunchKernelGGL(k_gs_cols, dim3((unsigned)(nw * col_groups)), dim3(GT), lds_cols, st, a, tw, waves, p0,
-                           nw, s);
-        if (s + 1 < a.S)
-          hipLaunchKernelGGL(k_gs_rows<false>, dim3((unsigned)(nw * row_groups)), dim3(GT), lds_rows, st, a, tw, waves,
-                             hw, nw);
-        else
-          hipLaunchKernelGGL(k_gs_final, dim3((unsigned)(np * row_groups)), dim3(GT), lds_rows, st, a, tw, waves, p0,
-                             np);
-      }
-    }
-    return hipGetLastError();
+    return wide_radices(a) ? stream_forward<true>(a, tw, waves, bpos, st)
+                           : stream_forward<false>(a, tw, waves, bpos, st);
   }
   const long long tasks = a.patterns ? a.P : a.P * a.M;
   const long long nb = blocks_for(tasks, grid_block_bytes(a.h, a.w, 1, false), ws_bytes);
   if (nb < 1) return hipErrorInvalidValue;
-  hipError_t e = set_lds((const void*)k_grid_forward, lds);
+  const void* kern = wide_radices(a) ? (const void*)k_grid_forward<true> : (const void*)k_grid_forward<false>;
+  hipError_t e = set_lds(kern, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_grid_forward, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  if (wide_radices(a))
+    hipLaunchKernelGGL(k_grid_forward<true>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  else
+    hipLaunchKernelGGL(k_grid_forward<false>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
   return hipGetLastError();
 }
 
@@ -922,7 +1090,6 @@ hipError_t launch_grid_adjoint(GridParams a, void* ws, size_t ws_bytes, hipStrea
   const int k = a.ckpt_every;
   const long long hw = (long long)a.h * a.w;
   const long long row_groups = (a.h + a.gx - 1) / a.gx;
-  const long long col_groups = (a.w + a.gy - 1) / a.gy;
   const size_t twb = twiddle_bytes(a.h, a.w);
   // per position: M waves of (k + 2) h w elements and M x row_groups pairs of shift partial sums
   const size_t per_pos = (size_t)a.M * ((size_t)(k + 2) * (size_t)hw * sizeof(float2) + (size_t)row_groups * 16);
@@ -932,52 +1099,18 @@ hipError_t launch_grid_adjoint(GridParams a, void* ws, size_t ws_bytes, hipStrea
     double* partials = (double*)((char*)ws + twb);
     const size_t waves_off = (twb + (size_t)bpos * a.M * row_groups * 16 + 255) & ~(size_t)255;
     float2* base = (float2*)((char*)ws + waves_off);
-    const long long stride = (long long)(k + 2) * hw;
-    const size_t lds_rows = (size_t)(a.w + 2 * a.gx * a.lsx) * sizeof(float2);
-    const size_t lds_cols = (size_t)(a.h + 2 * a.gy * a.lsy) * sizeof(float2);
-    hipError_t e = set_lds((const void*)k_gs_incident, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<false>, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gs_rows<true>, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gsa_seed, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gsa_probe, lds_rows);
-    if (e == hipSuccess) e = set_lds((const void*)k_gsa_recompute, lds_cols);
-    if (e == hipSuccess) e = set_lds((const void*)k_gsa_back, lds_cols);
-    if (e != hipSuccess) return e;
-    const int nc = num_checkpoints(a.S, k);
-    hipLaunchKernelGGL(k_grid_twiddles, dim3((unsigned)((a.h + a.w + GT - 1) / GT)), dim3(GT), 0, st, a.h, a.w, tw);
-    for (long long p0 = 0; p0 < a.P; p0 += bpos) {
-      const int np = (int)std::min(bpos, a.P - p0);
-      const int nw = np * a.M;
-      const dim3 rows((unsigned)(nw * row_groups)), cols((unsigned)(nw * col_groups));
-      for (int c = nc; c >= 0; --c) {
-        const int s0 = c * k, s1 = std::min(a.S, s0 + k);
-        if (c == 0) hipLaunchKernelGGL(k_gs_incident, rows, dim3(GT), lds_rows, st, a, tw, base, stride, p0, nw);
-        for (int s = s0; s < s1; ++s) {
-          hipLaunchKernelGGL(k_gsa_recompute, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, c);
-          if (s + 1 < s1)
-            hipLaunchKernelGGL(k_gs_rows<false>, rows, dim3(GT), lds_rows, st, a, tw, base, stride, nw);
-          else if (s == a.S - 1)
-            hipLaunchKernelGGL(k_gsa_seed, rows, dim3(GT), lds_rows, st, a, tw, base, p0, nw);
-        }
-        for (int s = s1 - 1; s >= s0; --s) {
-          hipLaunchKernelGGL(k_gsa_back, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, s0);
-          if (s > 0)
-            hipLaunchKernelGGL(k_gs_rows<true>, rows, dim3(GT), lds_rows, st, a, tw, base + hw, stride, nw);
-          else
-            hipLaunchKernelGGL(k_gsa_probe, rows, dim3(GT), lds_rows, st, a, tw, base, p0, nw, partials);
-        }
-      }
-      if (a.grad_shifts)
-        hipLaunchKernelGGL(k_gsa_shift_reduce, dim3((unsigned)((np + GT - 1) / GT)), dim3(GT), 0, st, a,
-                           (const double*)partials, p0, np, (int)row_groups);
-    }
-    return hipGetLastError();
+    return wide_radices(a) ? stream_adjoint<true>(a, tw, partials, base, bpos, st)
+                           : stream_adjoint<false>(a, tw, partials, base, bpos, st);
   }
   const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, k, true), ws_bytes);
   if (nb < 1) return hipErrorInvalidValue;
-  hipError_t e = set_lds((const void*)k_grid_adjoint, lds);
+  const void* kern = wide_radices(a) ? (const void*)k_grid_adjoint<true> : (const void*)k_grid_adjoint<false>;
+  hipError_t e = set_lds(kern, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_grid_adjoint, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  if (wide_radices(a))
+    hipLaunchKernelGGL(k_grid_adjoint<true>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
+  else
+    hipLaunchKernelGGL(k_grid_adjoint<false>, dim3((unsigned)nb), dim3(GT), lds, st, a, (float2*)ws);
   return hipGetLastError();
 }
 

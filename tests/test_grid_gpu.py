@@ -20,7 +20,10 @@ torch = pytest.importorskip("torch")
 TOL = 1e-5
 DEV = "cuda"
 
-SHAPES = [(8, 8), (15, 32), (31, 32), (48, 40), (1, 12), (9, 1), (22, 33), (97, 61), (100, 100), (243, 200)]
+# 22, 33: radix 11; 91 = 7 * 13, 77 = 7 * 11, 143 = 11 * 13 (the symmetric odd butterflies); 97, 61: generic
+# prime stages
+SHAPES = [(8, 8), (15, 32), (31, 32), (48, 40), (1, 12), (9, 1), (22, 33), (97, 61), (100, 100), (243, 200),
+          (91, 77), (26, 143)]
 
 
 def rel_l2(a, b):
@@ -120,7 +123,7 @@ def test_grid_checkpointed_forward_equals_plain_and_stores_entering_waves():
 
 
 @pytest.mark.parametrize("shape,k,modes", [((8, 8), 2, 1), ((15, 32), 3, 2), ((31, 32), 4, 1), ((1, 12), 2, 1),
-                                           ((97, 61), 2, 1), ((100, 100), 4, 2)])
+                                           ((97, 61), 2, 1), ((100, 100), 4, 2), ((91, 77), 2, 1), ((26, 143), 3, 2)])
 def test_grid_adjoint_matches_oracle(shape, k, modes):
     from ptyrodactyl_amd import engine
     h, w = shape
@@ -245,12 +248,13 @@ def test_ensemble_cbed_image_on_non_square_grid():
     assert got.shape == (h, w) and err < TOL
 
 
-def test_streaming_forward_matches_oracle_and_per_position_kernel(monkeypatch):
+@pytest.mark.parametrize("h,w", [(257, 256), (143, 130)])
+def test_streaming_forward_matches_oracle_and_per_position_kernel(monkeypatch, h, w):
     """Grids of 2^14 pixels or more run one launch per pass over (wave, line group) workgroups.  Each line is
     transformed by the same code with the same grouping as in the per-position kernel, so the two agree
-    bitwise; both match the oracle."""
+    bitwise; both match the oracle.  143 = 11 * 13 and 130 = 2 * 5 * 13 run the radix-11 and -13 butterflies."""
     from ptyrodactyl_amd import engine
-    h, w, s_count, k = 257, 256, 4, 3
+    s_count, k = 4, 3
     slab, probes, origins, shifts, weights = _case(h, w, s_count, 3, seed=41, modes=2)
     ops = _operands(slab, probes, origins, shifts, weights)
     pat_s, amp_s = engine.multislice_forward(*ops, want_amplitude=True)
@@ -270,11 +274,13 @@ def test_streaming_forward_matches_oracle_and_per_position_kernel(monkeypatch):
     assert err < TOL
 
 
-def test_streaming_adjoint_matches_oracle_and_per_position_kernel(monkeypatch):
+@pytest.mark.parametrize("h,w", [(150, 130), (154, 121)])
+def test_streaming_adjoint_matches_oracle_and_per_position_kernel(monkeypatch, h, w):
     """The adjoint on grids of 2^14 pixels or more runs one launch per pass; it matches the oracle's VJP and the
-    per-position adjoint (float32 atomics in both: equal to rounding)."""
+    per-position adjoint (float32 atomics in both: equal to rounding).  130 = 2 * 5 * 13, 154 = 2 * 7 * 11 and
+    121 = 11 * 11 run the radix-7, -11 and -13 butterflies."""
     from ptyrodactyl_amd import engine
-    h, w, s_count, k, p_count, modes = 150, 130, 5, 2, 3, 2
+    s_count, k, p_count, modes = 5, 2, 3, 2
     slab, probes, origins, shifts, weights = _case(h, w, s_count, p_count, seed=43, modes=modes)
     ops = _operands(slab, probes, origins, shifts, weights)
     g = torch.tensor(np.random.default_rng(9).standard_normal((p_count, h, w)), dtype=torch.float32, device=DEV)

@@ -467,7 +467,7 @@ struct TwiddleData {
   TwPair e[twiddle_entries<L>()];
   constexpr TwiddleData() : e{} {
     for (int i = 0; i < twiddle_entries<L>(); ++i) {
-      long long m = 0;
+      long long m;
       if (L <= 256) m = (long long)(i % 16) * (i / 16);
       else if (i < 256) m = (long long)(i % 16) * (i / 16) * (L / 256);
       else m = (long long)((i - 256) % 256) * ((i - 256) / 256);

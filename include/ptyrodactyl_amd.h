@@ -411,13 +411,17 @@ int pty_forward_persistent(int n, int num_slices, long long num_positions, const
  * factorisation (odd, prime, 15 x 32, 100 x 100, ...).  The reference's multislice takes any
  * grid shape (src/ptyrodactyl/multislice/simulations.py:501-603 with jnp.fft.fft2; its tests
  * use 8 x 8, 15 x 32 and 31 x 32 grids); the power-of-two entry points above cover only the
- * square power-of-two windows.  One workgroup per position runs a mixed-radix transform in
- * LDS; gradients accumulate with float32 atomics (not bitwise reproducible across runs).
+ * square power-of-two windows.  Grids below 2^14 pixels run one workgroup per position, with a
+ * mixed-radix transform in LDS, and their gradients accumulate with float32 atomics (not bitwise
+ * reproducible across runs).  Larger grids run one launch per pass (streaming); there the slab
+ * gradient is stored per wave and gathered in a fixed order (bitwise reproducible), and the probe
+ * gradient uses float32 atomics.
  *
  * Device scratch for pty_forward_grid (adjoint = 0) or pty_adjoint_grid (adjoint = 1) with the
  * default number of workgroups; any workspace of at least the bytes of one workgroup works
  * (h * w * 8 for the forward, h * w * 8 * (min(ckpt_every, num_slices) + 2) for the adjoint),
- * with fewer positions in flight.  Pass num_positions * num_modes: a forward without a pattern
+ * with fewer positions in flight (the streaming adjoint needs one more wave of h * w * 8 per
+ * position and mode, for the slab-gradient contributions).  Pass num_positions * num_modes: a forward without a pattern
  * output runs every (position, mode) wave as its own task, and the streaming kernels (grids of
  * 2^14 pixels or more) keep all num_modes waves of a position in flight, so a workspace sized
  * for fewer waves than num_modes falls back to one workgroup per position.

@@ -687,11 +687,12 @@ __device__ void seed_rows(const GridParams& a, const Axis& ax, float2* ba, float
       [&](int ky, int x, float2 v) { gb[(long long)ky * w + x] = v; });
 }
 
-// Back through slice s over columns: dL/dphi (inverse along y), dL/dT_s += conj(psi_s) dL/dphi (float32
-// atomics), dL/dpsi_s = conj(T_s) dL/dphi, forward along y into gb.
+// Back through slice s over columns: dL/dphi (inverse along y), the contribution conj(psi_s) dL/dphi to dL/dT_s
+// (stored at window pixel (y, x) of `contrib` for a gather, or added to the slab gradient with float32 atomics
+// when contrib is null), dL/dpsi_s = conj(T_s) dL/dphi, forward along y into gb.
 template <bool WIDE>
 __device__ void back_cols(const GridParams& a, const Axis& ay, float2* ba, float2* bb, int l0, int l1,
-                          const Position& q, int s, const float2* sg, float2* gb) {
+                          const Position& q, int s, const float2* sg, float2* gb, float2* contrib) {
   const int w = a.w;
   const long long sstride = a.slab_h * a.slab_w;
   const float2* ts = a.slab + (long long)s * sstride;
@@ -701,8 +702,12 @@ __device__ void back_cols(const GridParams& a, const Axis& ay, float2* ba, float
       [&](int x, int y, float2 g) {
         const long long si = slab_index(a, q, y, x);
         const float2 cg = gmulc(sg[(long long)y * w + x], g);
-        unsafeAtomicAdd(&gs[si].x, cg.x);
-        unsafeAtomicAdd(&gs[si].y, cg.y);
+        if (contrib) {
+          contrib[(long long)y * w + x] = cg;
+        } else {
+          unsafeAtomicAdd(&gs[si].x, cg.x);
+          unsafeAtomicAdd(&gs[si].y, cg.y);
+        }
         return gmulc(ts[si], g);
       },
       [&](int x, int ky, float2 v) { gb[(long long)ky * w + x] = v; });
@@ -780,7 +785,7 @@ __global__ __launch_bounds__(GT) void k_grid_adjoint(GridParams a, float2* ws) {
         }
         // back through slices s1-1 .. s0
         for (int s = s1 - 1; s >= s0; --s) {
-          back_cols<WIDE>(a, u.ay, u.bufa, u.bufb, 0, w, q, s, seg + (long long)(s - s0) * hw, gb);
+          back_cols<WIDE>(a, u.ay, u.bufa, u.bufb, 0, w, q, s, seg + (long long)(s - s0) * hw, gb, nullptr);
           if (s > 0)
             propagate_rows<true, WIDE>(a, u.ax, u.bufa, u.bufb, 0, h, gb);
           else
@@ -838,7 +843,7 @@ __global__ __launch_bounds__(GT) void k_gsa_seed(GridParams a, const float2* twg
 
 template <bool WIDE>
 __global__ __launch_bounds__(GT) void k_gsa_back(GridParams a, const float2* twg, float2* base, long long p0, int nw,
-                                                  int s, int s0) {
+                                                  int s, int s0, float2* contrib) {
   extern __shared__ float2 lds[];
   __shared__ int s_rad[GRID_MAX_STAGES];
   float2 *ba, *bb;
@@ -847,7 +852,98 @@ __global__ __launch_bounds__(GT) void k_gsa_back(GridParams a, const float2* twg
   const int wl = blockIdx.x % nw, l0 = (blockIdx.x / nw) * a.gy;
   float2* wb = base + (long long)wl * (a.ckpt_every + 2) * hw;
   back_cols<WIDE>(a, ay, ba, bb, l0, min(a.w, l0 + a.gy), position(a, p0 + wl / a.M), s, wb + (2 + s - s0) * hw,
-            wb + hw);
+                  wb + hw, contrib + (long long)wl * hw);
+}
+
+// dL/dT_s of one batch: every slab pixel adds the contributions (window coordinates, contrib[wl][y][x]) of the
+// waves whose windows cover it, in ascending wave order, so the sum has a fixed order and is bitwise
+// reproducible.  One workgroup per 32 x TH tile of the slab: the waves meeting the tile are compacted in order
+// with a ballot, then each thread sums its RPT pixels over them with HU windows' loads in flight (clamped to
+// in-window addresses, always in bounds, and selected afterwards).
+constexpr int GATHER_TILE = 32;
+
+__device__ __forceinline__ int frame_offset(int i, int o, int m) {
+  const int d = i - o;
+  return d < 0 ? d + m : d;
+}
+
+template <int TH>
+__global__ __launch_bounds__(GT) void k_gsa_gather(GridParams a, const float2* contrib, long long p0, int nw, int s) {
+  constexpr int RPT = TH / 8;   // rows per thread: 8 rows per sweep of the 256 threads
+  constexpr int HU = 16 / RPT;  // windows loaded together: sixteen loads in flight per thread
+  __shared__ int3 hits[GT];     // (wave in the batch, oy, ox) of the windows meeting this tile
+  __shared__ int wave_hits[GT / 64];
+  const int H = (int)a.slab_h, W = (int)a.slab_w, h = a.h, w = a.w;
+  const long long hw = (long long)h * w;
+  const int Y0 = blockIdx.y * TH, X0 = blockIdx.x * GATHER_TILE;
+  const int ylen = min(TH, H - Y0), xlen = min(GATHER_TILE, W - X0);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tx = tid % GATHER_TILE, ty = tid / GATHER_TILE;
+  const int X = X0 + tx;
+  float2 acc[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) acc[r] = make_float2(0.f, 0.f);
+  for (int base = 0; base < nw; base += GT) {
+    const int i = base + tid;
+    int oy = 0, ox = 0;
+    bool hit = false;
+    if (i < nw) {
+      const long long p = p0 + i / a.M;
+      oy = wrap_index(a.origins[2 * p], a.slab_h);
+      ox = wrap_index(a.origins[2 * p + 1], a.slab_w);
+      // the tile meets the periodic window [o, o + extent) if its first row / column lies inside it or the
+      // tile runs past the window's start
+      const int ey = frame_offset(Y0, oy, H), ex = frame_offset(X0, ox, W);
+      hit = (ey < h || ey + ylen > H) && (ex < w || ex + xlen > W);
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();  // the previous chunk's list has been consumed
+    if (lane == 0) wave_hits[wid] = __popcll(mask);
+    __syncthreads();
+    int offset = 0, total = 0;
+#pragma unroll
+    for (int v = 0; v < GT / 64; ++v) {
+      offset += v < wid ? wave_hits[v] : 0;
+      total += wave_hits[v];
+    }
+    if (hit) hits[offset + before] = make_int3(i, oy, ox);
+    __syncthreads();
+    if (tx < xlen) {
+      for (int h0 = 0; h0 < total; h0 += HU) {
+        float2 val[HU][RPT];
+        bool use[HU][RPT];
+#pragma unroll
+        for (int u = 0; u < HU; ++u) {
+          const int3 e = hits[min(h0 + u, total - 1)];
+          const int dx = frame_offset(X, e.z, W);
+          const float2* cw = contrib + (long long)e.x * hw + min(dx, w - 1);
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) {
+            const int yl = ty + 8 * r;
+            const int dy = frame_offset(Y0 + yl, e.y, H);
+            val[u][r] = cw[(long long)min(dy, h - 1) * w];
+            use[u][r] = h0 + u < total && yl < ylen && dy < h && dx < w;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < HU; ++u)
+#pragma unroll
+          for (int r = 0; r < RPT; ++r)
+            if (use[u][r]) acc[r] = make_float2(acc[r].x + val[u][r].x, acc[r].y + val[u][r].y);
+      }
+    }
+  }
+  if (tx >= xlen) return;
+  float2* gs = a.grad_slab + (long long)s * H * W;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int yl = ty + 8 * r;
+    if (yl < ylen) {
+      const long long o = (long long)(Y0 + yl) * W + X;
+      gs[o] = make_float2(gs[o].x + acc[r].x, gs[o].y + acc[r].y);
+    }
+  }
 }
 
 // Probe gradient rows; with shift gradients each block writes its FP64 partial sums to
@@ -986,10 +1082,11 @@ hipError_t stream_forward(const GridParams& a, float2* tw, float2* waves, long l
 }
 
 // Streaming adjoint: per batch of bpos positions, the segment recompute and the reverse passes, one launch per
-// pass over (wave, line group) workgroups.
+// pass over (wave, line group) workgroups.  Each reverse column pass stores its dL/dT contributions per wave in
+// `contrib` (one wave of the batch each), and a gather adds them to the slab gradient in wave order.
 template <bool WIDE>
-hipError_t stream_adjoint(const GridParams& a, float2* tw, double* partials, float2* base, long long bpos,
-                          hipStream_t st) {
+hipError_t stream_adjoint(const GridParams& a, float2* tw, double* partials, float2* base, float2* contrib,
+                          long long bpos, hipStream_t st) {
   const long long hw = (long long)a.h * a.w;
   const int k = a.ckpt_every;
   const long long row_groups = (a.h + a.gx - 1) / a.gx;
@@ -1006,6 +1103,10 @@ hipError_t stream_adjoint(const GridParams& a, float2* tw, double* partials, flo
   if (e == hipSuccess) e = set_lds((const void*)k_gsa_back<WIDE>, lds_cols);
   if (e != hipSuccess) return e;
   const int nc = num_checkpoints(a.S, k);
+  // gather tiles: 32 x 32 when there are enough of them to fill the GPU, else 32 x 8
+  const unsigned gtx = (unsigned)((a.slab_w + GATHER_TILE - 1) / GATHER_TILE);
+  const bool tall = (long long)gtx * ((a.slab_h + 31) / 32) >= 1024;
+  const dim3 gather_grid(gtx, (unsigned)(tall ? (a.slab_h + 31) / 32 : (a.slab_h + 7) / 8));
   hipLaunchKernelGGL(k_grid_twiddles, dim3((unsigned)((a.h + a.w + GT - 1) / GT)), dim3(GT), 0, st, a.h, a.w, tw);
   for (long long p0 = 0; p0 < a.P; p0 += bpos) {
     const int np = (int)std::min(bpos, a.P - p0);
@@ -1023,7 +1124,11 @@ hipError_t stream_adjoint(const GridParams& a, float2* tw, double* partials, flo
           hipLaunchKernelGGL(k_gsa_seed<WIDE>, rows, dim3(GT), lds_rows, st, a, tw, base, p0, nw);
       }
       for (int s = s1 - 1; s >= s0; --s) {
-        hipLaunchKernelGGL(k_gsa_back<WIDE>, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, s0);
+        hipLaunchKernelGGL(k_gsa_back<WIDE>, cols, dim3(GT), lds_cols, st, a, tw, base, p0, nw, s, s0, contrib);
+        if (tall)
+          hipLaunchKernelGGL(k_gsa_gather<32>, gather_grid, dim3(GT), 0, st, a, (const float2*)contrib, p0, nw, s);
+        else
+          hipLaunchKernelGGL(k_gsa_gather<8>, gather_grid, dim3(GT), 0, st, a, (const float2*)contrib, p0, nw, s);
         if (s > 0)
           hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gs_rows<true, WIDE>), rows, dim3(GT), lds_rows, st, a, tw, base + hw,
                              stride, nw);
@@ -1046,7 +1151,8 @@ size_t grid_block_bytes(int h, int w, int ckpt_every, bool adjoint) {
 }
 
 size_t grid_workspace_bytes(int h, int w, int ckpt_every, long long num_positions, bool adjoint) {
-  const size_t per = grid_block_bytes(h, w, ckpt_every, adjoint);
+  // the streaming adjoint keeps one more wave per position: its dL/dT contributions for the gather
+  const size_t per = grid_block_bytes(h, w, ckpt_every, adjoint) + (adjoint ? (size_t)h * w * sizeof(float2) : 0);
   long long b = std::min(num_positions, (long long)MAX_BLOCKS);
   b = std::min(b, std::max(1LL, (long long)(WS_BUDGET / per)));
   b = std::max(1LL, b);
@@ -1091,16 +1197,18 @@ hipError_t launch_grid_adjoint(GridParams a, void* ws, size_t ws_bytes, hipStrea
   const long long hw = (long long)a.h * a.w;
   const long long row_groups = (a.h + a.gx - 1) / a.gx;
   const size_t twb = twiddle_bytes(a.h, a.w);
-  // per position: M waves of (k + 2) h w elements and M x row_groups pairs of shift partial sums
-  const size_t per_pos = (size_t)a.M * ((size_t)(k + 2) * (size_t)hw * sizeof(float2) + (size_t)row_groups * 16);
+  // per position: M waves of (k + 3) h w elements ((k + 2) waves and the dL/dT contributions) and M x row_groups
+  // pairs of shift partial sums
+  const size_t per_pos = (size_t)a.M * ((size_t)(k + 3) * (size_t)hw * sizeof(float2) + (size_t)row_groups * 16);
   if (hw >= stream_min_pixels() && ws_bytes >= twb + 256 + per_pos) {
     const long long bpos = std::min(a.P, (long long)((ws_bytes - twb - 256) / per_pos));
     float2* tw = (float2*)ws;
     double* partials = (double*)((char*)ws + twb);
     const size_t waves_off = (twb + (size_t)bpos * a.M * row_groups * 16 + 255) & ~(size_t)255;
     float2* base = (float2*)((char*)ws + waves_off);
-    return wide_radices(a) ? stream_adjoint<true>(a, tw, partials, base, bpos, st)
-                           : stream_adjoint<false>(a, tw, partials, base, bpos, st);
+    float2* contrib = base + bpos * a.M * (k + 2) * hw;
+    return wide_radices(a) ? stream_adjoint<true>(a, tw, partials, base, contrib, bpos, st)
+                           : stream_adjoint<false>(a, tw, partials, base, contrib, bpos, st);
   }
   const long long nb = blocks_for(a.P, grid_block_bytes(a.h, a.w, k, true), ws_bytes);
   if (nb < 1) return hipErrorInvalidValue;

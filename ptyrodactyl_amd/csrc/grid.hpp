@@ -44,7 +44,8 @@ size_t grid_workspace_bytes(int h, int w, int ckpt_every, long long num_position
 
 // Forward (patterns and/or amplitudes, optional checkpoints).  The caller has validated the operands.
 hipError_t launch_grid_forward(GridParams a, void* ws, size_t ws_bytes, hipStream_t st);
-// Adjoint seeded by grad_patterns or grad_amplitude; accumulates with float32 atomics.
+// Adjoint seeded by grad_patterns or grad_amplitude.  The per-position kernel accumulates dL/dT and dL/dprobe_k
+// with float32 atomics; the streaming kernels gather dL/dT in a fixed order and add dL/dprobe_k with atomics.
 hipError_t launch_grid_adjoint(GridParams a, void* ws, size_t ws_bytes, hipStream_t st);
 
 }  // namespace pty

@@ -277,7 +277,8 @@ def test_streaming_forward_matches_oracle_and_per_position_kernel(monkeypatch, h
 @pytest.mark.parametrize("h,w", [(150, 130), (154, 121)])
 def test_streaming_adjoint_matches_oracle_and_per_position_kernel(monkeypatch, h, w):
     """The adjoint on grids of 2^14 pixels or more runs one launch per pass; it matches the oracle's VJP and the
-    per-position adjoint (float32 atomics in both: equal to rounding).  130 = 2 * 5 * 13, 154 = 2 * 7 * 11 and
+    per-position adjoint (float32 atomics there, a gather in wave order here: equal to rounding), and its slab
+    gradient is bitwise reproducible.  130 = 2 * 5 * 13, 154 = 2 * 7 * 11 and
     121 = 11 * 11 run the radix-7, -11 and -13 butterflies."""
     from ptyrodactyl_amd import engine
     s_count, k, p_count, modes = 5, 2, 3, 2
@@ -293,6 +294,8 @@ def test_streaming_adjoint_matches_oracle_and_per_position_kernel(monkeypatch, h
         return g_slab.cpu().numpy(), g_pk.cpu().numpy(), g_shift.cpu().numpy()
 
     s_slab, s_pk, s_shift = grads()
+    # the streaming slab gradient is gathered in wave order: the same bits on every run
+    assert np.array_equal(grads()[0], s_slab)
     monkeypatch.setenv("PTY_GRID_STREAM_MIN_PIXELS", str(1 << 40))   # the per-position kernel
     p_slab, p_pk, p_shift = grads()
     prop = _oracle_prop(h, w)

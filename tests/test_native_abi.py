@@ -104,12 +104,13 @@ def test_grid_entry_points_validate_on_host():
     rc = lib.pty_forward_grid(100, 100, *args, None, None, None, 1, None, 0, None)
     assert rc == _native.PTY_ERR_INVALID and b"null input pointer" in lib.pty_last_error()
     # one forward workgroup holds one h x w complex64 wave (plus the streaming passes' twiddle tables,
-    # (h + w) x 8 bytes rounded up to 256); the adjoint k + 2 waves (k clamped to S) and, for the streaming
-    # adjoint, 16 bytes of shift partials per row group (one group of 15 rows here) and 256 of alignment
+    # (h + w) x 8 bytes rounded up to 256); the adjoint k + 2 waves (k clamped to S), one more wave for the
+    # streaming adjoint's slab-gradient contributions, 16 bytes of shift partials per row group (one group of
+    # 15 rows here) and 256 of alignment
     tw = 512
     assert lib.pty_grid_workspace_bytes(15, 32, 8, 4, 1, 0) == 15 * 32 * 8 + tw
-    assert lib.pty_grid_workspace_bytes(15, 32, 8, 4, 3, 1) == 3 * (15 * 32 * 8 * 6 + 16) + tw + 256
-    assert lib.pty_grid_workspace_bytes(15, 32, 2, 4, 1, 1) == 15 * 32 * 8 * 4 + 16 + tw + 256
+    assert lib.pty_grid_workspace_bytes(15, 32, 8, 4, 3, 1) == 3 * (15 * 32 * 8 * 7 + 16) + tw + 256
+    assert lib.pty_grid_workspace_bytes(15, 32, 2, 4, 1, 1) == 15 * 32 * 8 * 5 + 16 + tw + 256
     assert lib.pty_grid_workspace_bytes(15, 32, 8, 4, 10 ** 6, 0) == 2048 * 15 * 32 * 8 + tw
     assert lib.pty_grid_workspace_bytes(4097, 32, 8, 4, 1, 0) == 0
     fake = 16  # any non-null address: validation fails before it is used

@@ -30,6 +30,8 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+# VALU issue rate: one wave64 vector instruction per CU per cycle (4 SIMDs x 16 lanes), 256 CUs at 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 2.4e9
 
 # HBM-side bytes per step of the config-2 forward, measured by two rocprofv3 --pmc passes (FETCH_SIZE, then
 # WRITE_SIZE) and calibrated with bench.py --workload calib; the measurement records the sha256 of the kernel
@@ -51,6 +53,22 @@ def measured_traffic(workload: str, path: str, batch: int):
     if h.hexdigest() != doc["source_sha256"]:
         return None, "kernel sources changed since the counter measurement"
     return int(doc["traffic_bytes"]), f"rocprofv3 FETCH_SIZE/WRITE_SIZE, {TRAFFIC_PROFILE.relative_to(ROOT)}"
+
+
+def measured_valu(workload: str, path: str, batch: int, kernel_ms: float):
+    """Wavefront VALU instructions per step (rocprofv3 SQ_INSTS_VALU, committed beside the traffic counts under
+    the same source hash) and the fraction of the device's issue rate they take in ``kernel_ms``, or None."""
+    traffic, _ = measured_traffic(workload, path, batch)
+    if traffic is None:
+        return None
+    doc = json.loads(TRAFFIC_PROFILE.read_text())
+    if "valu_instructions" not in doc:
+        return None
+    instr = float(doc["valu_instructions"])
+    issue_s = instr / VALU_ISSUE_PER_S
+    return {"instructions": instr, "issue_ms": round(issue_s * 1e3, 2), "peak_per_s": VALU_ISSUE_PER_S,
+            "frac": round(issue_s / (kernel_ms * 1e-3), 4),
+            "source": f"rocprofv3 SQ_INSTS_VALU, {TRAFFIC_PROFILE.relative_to(ROOT)}"}
 
 
 def algorithmic_bytes(n: int, s: int) -> int:
@@ -590,7 +608,9 @@ def main() -> int:
                      "traffic_source": traffic_source,
                      "kernel": "multislice forward (all launches of one step)", "kernel_ms": round(kernel_ms, 3),
                      "fp32_tflops": round(flops_step / (kernel_ms * 1e-3) / 1e12, 2),
-                     "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+                     "fp32_frac": round(flops_step / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                     "valu_issue": measured_valu(w.name, "persistent" if persistent else "streaming", batch,
+                                                 kernel_ms)},
     }
     if args.only_forward:
         print(json.dumps(result), flush=True)

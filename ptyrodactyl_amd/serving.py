@@ -286,6 +286,7 @@ class RequestLoop:
     hbm_budget_bytes: int = 64 << 30
     keep_inputs: bool = False
     admit_per_step: int = 4   # requests built per step while the GPU runs (bounds the host work between launches)
+    max_steps_ahead: int = 2  # launched steps the host may queue ahead of the GPU
     chunk_positions: Dict[int, int] = field(default_factory=lambda: {32: 8192, 64: 8192, 128: 4096, 256: 512,
                                                                      512: 128})
     device: Optional[torch.device] = None
@@ -451,7 +452,7 @@ class RequestLoop:
                 self._close(r, "rejected", f"needs {need} bytes, above the HBM budget of {self.hbm_budget_bytes}",
                             now, closed)
                 continue
-            if active and in_use + need > self.hbm_budget_bytes:
+            if in_use > 0 and in_use + need > self.hbm_budget_bytes:  # something resident (running or draining)
                 break
             pending.pop(0)
             try:
@@ -471,15 +472,23 @@ class RequestLoop:
         closed.append(r)
 
     def run(self, requests: List[Request], realtime: bool = True) -> Dict:
+        """Serve ``requests`` to completion.  Steps are launched without waiting for the GPU: the host builds
+        arriving requests and launches the next step while earlier steps run, at most ``max_steps_ahead``
+        steps ahead.  A request whose positions are all enqueued is retired once the event recorded after its
+        last step has completed; only then are its buffers (allocated on the builder stream, read on the class
+        streams) released."""
         pending = sorted(requests, key=lambda r: r.arrival)
         active: List[_Active] = []
+        draining: List[Tuple[_Active, torch.cuda.Event]] = []  # every position enqueued, GPU work outstanding
+        in_flight: List[torch.cuda.Event] = []  # one per launched step, oldest first
         in_use = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         finished: List[Request] = []
         closed: List[Request] = []
-        while pending or active:
-            if not active:
+        main = torch.cuda.current_stream(self.device)
+        while pending or active or draining:
+            if not active and not draining:
                 in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, 1, closed)
                 if not active:
                     if pending:
@@ -490,22 +499,25 @@ class RequestLoop:
             for a in active:
                 if a.next_pos < a.req.positions:
                     classes.setdefault((a.req.n, a.req.slices), []).append(a)
-            main = torch.cuda.current_stream(self.device)
             failures: Dict[int, str] = {}
             for key, members in classes.items():
                 st = self._stream(key)
-                st.wait_stream(main)
                 for a in members:
                     if a.ready is not None:
                         st.wait_event(a.ready)
                 with torch.cuda.stream(st):
                     for a, message in self._step_class(key, members):
                         failures[a.req.rid] = message
-            for key in classes:
-                main.wait_stream(self._stream(key))
-            # requests that arrived meanwhile are built while the GPU runs this step; they join the next one
+            if classes:
+                for key in classes:
+                    main.wait_stream(self._stream(key))
+                ev = torch.cuda.Event()
+                ev.record(main)
+                in_flight.append(ev)
+            # requests that arrived meanwhile are built while the GPU runs the queued steps; they join the next one
             in_use = self._admit_arrivals(pending, active, in_use, t0, realtime, self.admit_per_step, closed)
-            torch.cuda.synchronize()
+            if failures or any(a.req.rid in self._cancelled for a in active):
+                torch.cuda.synchronize()  # a dropped request's earlier chunks may still read its buffers
             now = time.perf_counter() - t0
             still = []
             for a in active:
@@ -516,22 +528,39 @@ class RequestLoop:
                     else:
                         self._close(a.req, "cancelled", None, now, closed)
                 elif a.done_pos >= a.req.positions:
-                    if a.req.kind == "gradient":
-                        a.req.loss = float(a.loss)
-                    a.req.t_done = now
-                    if self.keep_inputs:
-                        a.req.inputs = a.inp
-                    a.req.status = "done"
-                    in_use -= request_bytes(a.req)
-                    finished.append(a.req)
+                    if a.ready is not None:
+                        main.wait_event(a.ready)  # covers the build of a request with no positions
+                    done = torch.cuda.Event()
+                    done.record(main)
+                    draining.append((a, done))
                 else:
                     still.append(a)
             active = still
-            m = {"t_s": round(now, 6), "active": len(active), "pending": len(pending), "hbm_in_use": in_use,
-                 "done": len(finished), "closed": len(closed)}
+            while len(in_flight) > self.max_steps_ahead:
+                in_flight.pop(0).synchronize()
+            if draining and not active:
+                draining[0][1].synchronize()  # nothing else to launch: wait for the oldest request
+            now = time.perf_counter() - t0
+            waiting = []
+            for a, done in draining:
+                if not done.query():
+                    waiting.append((a, done))
+                    continue
+                if a.req.kind == "gradient":
+                    a.req.loss = float(a.loss)
+                a.req.t_done = now
+                if self.keep_inputs:
+                    a.req.inputs = a.inp
+                a.req.status = "done"
+                in_use -= request_bytes(a.req)
+                finished.append(a.req)
+            draining = waiting
+            m = {"t_s": round(now, 6), "active": len(active) + len(draining), "pending": len(pending),
+                 "hbm_in_use": in_use, "done": len(finished), "closed": len(closed)}
             self.metrics.append(m)
             if self.metrics_sink is not None:
                 self.metrics_sink(m)
+        torch.cuda.synchronize()
         return summarize(finished, time.perf_counter() - t0, closed)
 
 

@@ -183,3 +183,28 @@ def test_oversize_rejected_cancelled_dropped_and_metrics_recorded():
     assert doomed.status == "cancelled"
     assert stats["requests"] == 4 and stats.get("rejected") == 1 and stats.get("cancelled") == 1
     assert seen and seen == loop.metrics and seen[-1]["active"] == 0 and seen[-1]["hbm_in_use"] == 0
+
+
+@pytest.mark.gpu
+def test_queued_steps_give_the_results_of_synchronised_steps():
+    """Steps queued ahead of the GPU (max_steps_ahead 2, the default) against a loop that waits for every step
+    (max_steps_ahead 0): every request's patterns are bitwise the same.  Requests retire, and free their HBM budget,
+    at different loop iterations in the two runs, so a gradient request may be cut into different chunks: losses
+    agree to 1e-9 and gradients to rel-L2 1e-5, the bounds of the alone-versus-mix test."""
+    chunks = {64: 128, 128: 48, 256: 16}
+    runs = []
+    for ahead in (0, 2):
+        reqs = _mix()
+        loop = serving.RequestLoop(hbm_budget_bytes=256 << 20, chunk_positions=chunks, max_steps_ahead=ahead)
+        stats = loop.run(reqs, realtime=False)
+        assert stats["requests"] == len(reqs)
+        assert all(r.status == "done" for r in reqs)
+        runs.append(reqs)
+    for a, b in zip(*runs):
+        print(f"request {a.rid} {a.kind} n={a.n} S={a.slices} P={a.positions}", flush=True)
+        assert torch.equal(a.patterns, b.patterns), f"patterns differ for request {a.rid}"
+        if a.kind == "gradient":
+            assert a.loss == pytest.approx(b.loss, rel=1e-9)
+            for got, ref in ((a.grad_slab, b.grad_slab), (a.grad_probe_k, b.grad_probe_k)):
+                err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+                assert err < 1e-5, f"gradient rel-L2 {err:.2e} for request {a.rid}"

@@ -4,6 +4,7 @@ Tolerance: relative L2 error of the patterns <= 1e-5 (complex64 kernels
 against the complex128 oracle), the bar the forward path is held to.
 """
 
+import os
 from pathlib import Path
 
 import numpy as np
@@ -181,6 +182,36 @@ def test_config2_subset_full_size():
     err = rel_l2(got, ref)
     print(f"config2 subset (64 positions) rel-L2 {err:.3e}", flush=True)
     assert err < TOL
+
+
+@pytest.mark.skipif(os.environ.get("PTY_FULL_CONFIG2") != "1",
+                    reason="about 2.5 min of 16-thread CPU oracle work; set PTY_FULL_CONFIG2=1 to run it")
+def test_config2_all_positions():
+    """Headline workload at full size: all 4096 positions of the 256^2 x 64-slice scan over the 1024^2 slab
+    against the complex128 oracle, in chunks of 256 positions; rel-L2 over the whole stack and the worst
+    pattern."""
+    from ptyrodactyl_amd import synth
+    w = synth.config2()
+    prop = o.propagation_func(w.n, w.n, w.slice_thickness, w.voltage_kv, w.calib)
+    print("config2 all positions", flush=True)
+    got, _, _ = _run_hip(w.slab_t, w.probe[None], w.origins, w.shifts_px, w.n, w.slice_thickness, w.voltage_kv,
+                         w.calib)
+    slab128 = w.slab_t.astype(np.complex128)
+    num = den = 0.0
+    worst = 0.0
+    for c0 in range(0, len(w.origins), 256):
+        sel = slice(c0, c0 + 256)
+        ref = o.window_stem_batched(slab128, w.probe, w.origins[sel], w.shifts_px[sel], prop, workers=16)
+        d = got[sel].astype(np.float64) - ref
+        num += float(np.sum(d * d))
+        den += float(np.sum(ref * ref))
+        per = np.sqrt(np.sum(d * d, axis=(1, 2)) / np.sum(ref * ref, axis=(1, 2)))
+        worst = max(worst, float(per.max()))
+        print(f"positions {c0}-{c0 + 255}: running rel-L2 {np.sqrt(num / den):.3e}, worst pattern {worst:.3e}",
+              flush=True)
+    err = float(np.sqrt(num / den))
+    print(f"config2 all 4096 positions rel-L2 {err:.3e}, worst pattern {worst:.3e}", flush=True)
+    assert err < TOL and worst < TOL
 
 
 def test_config1_full_workload():
